@@ -1,0 +1,162 @@
+"""Benchmark of the plain (one mesh per call) voxelizer on one MI355X.
+
+Workload: the seeded (3,7) torus-knot tube, 2,048 segments x 256 ring
+vertices (1,048,576 FP32 triangles), scaled to 0.9 of the unit cube;
+Level1 voxelCount 256, Level2 8^3 per boundary voxel.  One step = one
+gpv_voxelize call on the device-resident soup (all stages, counts, sync).
+
+Prints ONE JSON line: meshes/s (value), triangle-box tests/s, voxels/s,
+p50/p99 step latency, per-stage device times, the roofline of the dominant
+kernel (HIP events on the library's stream) and the CPU oracle timed on this
+host on the same mesh.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+VALU_PEAK_OPS = 39.3e12        # 256 CU x 64 lanes x 2.4 GHz, one FP32 op per lane-cycle
+SAT_OPS = 198                  # FP32 ops of one full SAT evaluation (src/TriBoxIntersection.cpp:135-210)
+STAGES = ["setup", "l1_sat", "l1_parity", "compaction_bins", "l2_parity", "l2_sat"]
+
+
+def make_workload(gv):
+    v, f = gv.torus_knot(p=3, q=7, segments=2048, ring=256, tube=0.3, jitter=0.01, extent=0.9, seed=2)
+    return v, f, gv.flatten(v, f)
+
+
+def kernel_bytes(stage, info, T):
+    """Algorithmic HBM bytes of one launch of the stage's main kernel."""
+    B, n2, P = info["nb"], info["n2"], info["pairs"]
+    n23 = n2 ** 3
+    if stage == "l2_sat":
+        # bins (4 B/pair) + bin offsets and boundary ids (8 B/voxel) + every
+        # triangle once (36 B) + the Level2 boundary bitplane (n2^3/8 B/voxel)
+        return 4 * P + 8 * B + 36 * T + B * n23 // 8
+    if stage == "l2_parity":
+        return 8 * P + 8 * B + 36 * T + B * n23 // 8
+    if stage == "l1_sat":
+        return 36 * T + 8 * P + (info["n1"] + 7) // 8
+    return 36 * T + 2 * ((info["n1"] + 7) // 8)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (meshes shard "
+              "independently across GPUs; see the sharding notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import gpview_amd as gv
+    v, f, soup = make_workload(gv)
+    T = len(soup)
+    bmin, bmax, mms = gv.mesh_bbox(v)
+    ctx = gv.Context(0)
+    dsoup = ctx.upload(soup)
+
+    def step():
+        return ctx.voxelize(soup, bmin, bmax, mms, 256, 8, True, device_ptr=dsoup)
+
+    for _ in range(args.warmup):
+        step().release()
+    lat = []
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        g = step()
+        lat.append(time.perf_counter() - s0)
+        infos.append(g.info())
+        g.release()
+    wall = time.perf_counter() - t0
+
+    # one more run outside the timed region, kept for the correctness check
+    g = step()
+    info = g.info()
+    got = g.download()
+    g.release()
+    ctx.free(dsoup)
+    ctx.close()
+
+    meshes_per_s = args.steps / wall
+    tests = info["tests_l1"] + info["tests_l2"]
+    n2 = info["n2"]
+    voxels = info["n1"] + info["nb"] * n2 ** 3
+    stage_ms = {s: float(np.median([i["ms_stage"][k] for i in infos])) for k, s in enumerate(STAGES)}
+    dom = max(STAGES, key=lambda s: stage_ms[s])
+    t_dom = stage_ms[dom] / 1e3
+    bytes_dom = kernel_bytes(dom, info, T)
+    rl = {"bound": "hbm", "achieved": bytes_dom / t_dom / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+          "frac": bytes_dom / t_dom / 1e9 / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+          "algorithmic_bytes": bytes_dom, "ms": stage_ms[dom]}
+    valu = None
+    if dom in ("l2_sat", "l1_sat"):
+        ops = SAT_OPS * (info["tests_l2"] if dom == "l2_sat" else info["tests_l1"])
+        valu = {"bound": "valu", "achieved": ops / t_dom / 1e12, "peak": VALU_PEAK_OPS / 1e12,
+                "unit": "TFLOP/s", "frac": ops / t_dom / VALU_PEAK_OPS, "kernel": dom,
+                "note": "reference-equivalent ops: 198 x reference tri-box tests of the stage"}
+
+    cpu = None
+    exact = None
+    if not args.no_cpu:
+        from oracle import orc
+        threads = args.cpu_threads or os.cpu_count()
+        g0 = orc.grid_setup(bmin, bmax, mms, 256, 8)
+        c0 = time.perf_counter()
+        want = orc.voxelize(soup, g0, orc.ORC_FAST, threads)
+        cdt = time.perf_counter() - c0
+        cpu = {"value": 1.0 / cdt, "unit": "meshes/s", "cores": threads, "kind": "port",
+               "sample": "the full workload mesh (1 mesh), oracle column-factorised mode",
+               "seconds": cdt, "tests_per_s": (want["tests_l1"] + want["tests_l2"]) / cdt,
+               "stage_s": dict(zip(["l1_parity", "l1_sat", "compaction", "l2_parity", "l2_sat"],
+                                   want["t_stage"]))}
+        exact = all(np.array_equal(got[k], want[k]) for k in ("l1", "prefix", "bidx", "l2"))
+
+    out = {
+        "metric": "meshes/s & tri-box tests/s at 256^3 L1 + 8^3 L2 (bit-exact); p50/p99 latency",
+        "value": meshes_per_s,
+        "unit": "meshes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "torus-knot (3,7) tube, 2048x256, 1,048,576 tris, L1 256 nominal, L2 8^3",
+                   "triangles": T, "level1": info["n"], "level2": n2, "boundary_voxels": info["nb"],
+                   "parallelism": "single GPU"},
+        "tri_box_tests_per_s": tests / (wall / args.steps),
+        "voxels_per_s": voxels / (wall / args.steps),
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "device_ms_total": float(np.median([i["ms_total"] for i in infos])),
+        "stage_ms": stage_ms,
+        "tests_l1": info["tests_l1"], "tests_l2": info["tests_l2"], "pairs": info["pairs"],
+        "bit_exact_vs_cpu": exact,
+        "roofline": rl,
+        "roofline_valu": valu,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,339 @@
+"""gpview_amd -- Python view of libgpv, the gfx950 two-level hybrid voxelizer.
+
+The product is the C ABI in include/gpv.h (libgpv.so, built in-tree by
+gpview_amd/build.py).  This module binds it with ctypes and mirrors the
+reference's Object-level flow (ReadObject -> CreateFlatTriangleData ->
+PerformVoxelization -> SaveVoxelization, src/Object.cpp) for tests, the
+benchmark and Python callers.  There is no CPU fallback: every call goes to
+the HIP library, and a missing library is an ImportError-class failure.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpv.so")
+
+GPV_OK, GPV_ERR_INVALID, GPV_ERR_OOM, GPV_ERR_CAPACITY, GPV_ERR_HIP, GPV_ERR_NO_DEVICE, GPV_ERR_IO, \
+    GPV_ERR_STATE = range(8)
+
+
+class GpvError(RuntimeError):
+    def __init__(self, status, where, detail):
+        super().__init__(f"{where}: status {status} ({detail})")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    """gpv_params (include/gpv.h)."""
+    _fields_ = [("voxel_count", ctypes.c_int), ("voxel_count2", ctypes.c_int), ("level2", ctypes.c_int),
+                ("initial_capacity", ctypes.c_int64)]
+
+
+class GridInfo(ctypes.Structure):
+    """gpv_grid_info (include/gpv.h)."""
+    _fields_ = [
+        ("bmin", ctypes.c_float * 3), ("bmax", ctypes.c_float * 3), ("max_model_size", ctypes.c_float),
+        ("n", ctypes.c_int * 3), ("g", ctypes.c_float * 3), ("n2", ctypes.c_int), ("g2", ctypes.c_float * 3),
+        ("n1", ctypes.c_int64), ("nb", ctypes.c_int64),
+        ("l1_inside", ctypes.c_int64), ("l1_boundary", ctypes.c_int64),
+        ("l2_inside", ctypes.c_int64), ("l2_boundary", ctypes.c_int64),
+        ("tests_l1", ctypes.c_int64), ("tests_l2", ctypes.c_int64), ("pairs", ctypes.c_int64),
+        ("max_tris_per_voxel", ctypes.c_int32), ("regrows", ctypes.c_int32),
+        ("ms_stage", ctypes.c_float * 6), ("ms_total", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
+class Mesh(ctypes.Structure):
+    """gpv_mesh (include/gpv_mesh.h)."""
+    _fields_ = [("nverts", ctypes.c_int64), ("ntris", ctypes.c_int64),
+                ("verts", ctypes.POINTER(ctypes.c_float)), ("faces", ctypes.POINTER(ctypes.c_int32)),
+                ("bbox_from_faces", ctypes.c_int)]
+
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+_lib = None
+
+# every symbol include/gpv.h and include/gpv_mesh.h declare
+EXPORTED = [
+    "gpv_ctx_create", "gpv_ctx_destroy", "gpv_last_error", "gpv_status_string",
+    "gpv_device_upload", "gpv_device_free",
+    "gpv_mesh_bbox", "gpv_grid_derive", "gpv_voxelize", "gpv_grid_create",
+    "gpv_classify_tessellation", "gpv_classify_inout", "gpv_compact_boundary",
+    "gpv_classify_inout_level2", "gpv_classify_tessellation_level2", "gpv_grid_finish",
+    "gpv_grid_get_info", "gpv_grid_download", "gpv_grid_device_views", "gpv_save_raw",
+    "gpv_grid_release",
+    "gpv_mesh_free", "gpv_mesh_flatten", "gpv_mesh_bounds", "gpv_mesh_icosphere",
+    "gpv_mesh_torus_knot", "gpv_mesh_torus", "gpv_mesh_drilled_box", "gpv_mesh_mixed",
+    "gpv_mesh_stress", "gpv_mesh_read_obj", "gpv_mesh_read_off",
+]
+
+
+def lib():
+    """Load the in-tree libgpv.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m gpview_amd.build` (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    c_int, c_i64, c_float = ctypes.c_int, ctypes.c_int64, ctypes.c_float
+    P = ctypes.POINTER
+    L.gpv_last_error.restype = ctypes.c_char_p
+    L.gpv_status_string.restype = ctypes.c_char_p
+    L.gpv_status_string.argtypes = [c_int]
+    L.gpv_ctx_create.argtypes = [c_int, ctypes.c_uint, P(_vp)]
+    L.gpv_ctx_destroy.argtypes = [_vp]
+    L.gpv_device_upload.argtypes = [_vp, _vp, c_i64, P(_vp)]
+    L.gpv_device_free.argtypes = [_vp, _vp]
+    L.gpv_mesh_bbox.argtypes = [_f32p, c_i64, _f32p, _f32p, P(c_float)]
+    L.gpv_grid_derive.argtypes = [_f32p, _f32p, c_float, P(Params), P(GridInfo)]
+    for fn in ("gpv_voxelize", "gpv_grid_create"):
+        getattr(L, fn).argtypes = [_vp, _vp, c_i64, c_int, _f32p, _f32p, c_float, P(Params), P(_vp)]
+    L.gpv_classify_tessellation.argtypes = [_vp, _vp, P(c_i64)]
+    for fn in ("gpv_classify_inout", "gpv_compact_boundary", "gpv_classify_inout_level2",
+               "gpv_classify_tessellation_level2", "gpv_grid_finish", "gpv_grid_release"):
+        getattr(L, fn).argtypes = [_vp, _vp]
+    L.gpv_grid_get_info.argtypes = [_vp, P(GridInfo)]
+    L.gpv_grid_download.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+    L.gpv_grid_device_views.argtypes = [_vp, P(_vp), P(_vp), P(_vp), P(_vp), P(_vp)]
+    L.gpv_save_raw.argtypes = [_vp, _vp, c_int, ctypes.c_char_p]
+    L.gpv_mesh_free.argtypes = [P(Mesh)]
+    L.gpv_mesh_flatten.argtypes = [P(Mesh), _f32p]
+    L.gpv_mesh_bounds.argtypes = [P(Mesh), _f32p, _f32p, P(c_float)]
+    L.gpv_mesh_icosphere.argtypes = [c_int, c_float, _f32p, c_float, ctypes.c_uint64, P(Mesh)]
+    L.gpv_mesh_torus_knot.argtypes = [c_int, c_int, c_int, c_int, c_float, c_float, c_float, ctypes.c_uint64,
+                                      P(Mesh)]
+    L.gpv_mesh_torus.argtypes = [c_int, c_int, c_float, c_float, ctypes.c_uint64, P(Mesh)]
+    L.gpv_mesh_drilled_box.argtypes = [c_i64, c_float, ctypes.c_uint64, P(Mesh)]
+    L.gpv_mesh_mixed.argtypes = [c_int, c_i64, ctypes.c_uint64, P(Mesh)]
+    L.gpv_mesh_stress.argtypes = [P(Mesh), c_int, c_int, c_float, c_float, ctypes.c_uint64]
+    L.gpv_mesh_read_obj.argtypes = [ctypes.c_char_p, P(Mesh)]
+    L.gpv_mesh_read_off.argtypes = [ctypes.c_char_p, P(Mesh)]
+    _lib = L
+    return L
+
+
+def _check(st, where):
+    if st != GPV_OK:
+        L = lib()
+        raise GpvError(st, where, f"{L.gpv_status_string(st).decode()}: {L.gpv_last_error().decode()}")
+
+
+# ---------------------------------------------------------------------------
+# meshes (host)
+# ---------------------------------------------------------------------------
+def _take_mesh(m):
+    verts = np.ctypeslib.as_array(m.verts, (m.nverts * 3,)).reshape(-1, 3).copy() if m.nverts else \
+        np.zeros((0, 3), np.float32)
+    faces = np.ctypeslib.as_array(m.faces, (m.ntris * 3,)).reshape(-1, 3).copy() if m.ntris else \
+        np.zeros((0, 3), np.int32)
+    bff = bool(m.bbox_from_faces)
+    lib().gpv_mesh_free(ctypes.byref(m))
+    return verts, faces, bff
+
+
+def _gen(fn, *args):
+    m = Mesh()
+    _check(getattr(lib(), fn)(*args, ctypes.byref(m)), fn)
+    v, f, _ = _take_mesh(m)
+    return v, f
+
+
+def icosphere(level=4, radius=0.4, center=(0.5, 0.5, 0.5), jitter=0.02, seed=1):
+    return _gen("gpv_mesh_icosphere", level, radius, np.asarray(center, np.float32), jitter, seed)
+
+
+def torus_knot(p=3, q=7, segments=2048, ring=256, tube=0.3, jitter=0.01, extent=0.9, seed=2):
+    return _gen("gpv_mesh_torus_knot", p, q, segments, ring, tube, jitter, extent, seed)
+
+
+def torus(nu, nv, ratio, extent=0.9, seed=3):
+    return _gen("gpv_mesh_torus", nu, nv, ratio, extent, seed)
+
+
+def drilled_box(target_tris, extent=0.9, seed=4):
+    return _gen("gpv_mesh_drilled_box", int(target_tris), extent, seed)
+
+
+def mixed(kind, target_tris, seed):
+    return _gen("gpv_mesh_mixed", int(kind), int(target_tris), seed)
+
+
+def stress(verts, faces, voxel_count, voxel_count2, p_snap=0.3, f_sliver=0.1, seed=5):
+    """Lattice snapping + slivers applied to a copy of (verts, faces)."""
+    L = lib()
+    v = np.ascontiguousarray(verts, np.float32)
+    f = np.ascontiguousarray(faces, np.int32)
+    m = Mesh()
+    # the library owns (and may reallocate) the arrays: hand it malloc'd copies
+    libc = ctypes.CDLL(None)
+    libc.malloc.restype = ctypes.c_void_p
+    libc.malloc.argtypes = [ctypes.c_size_t]
+    pv = libc.malloc(max(v.nbytes, 4))
+    pf = libc.malloc(max(f.nbytes, 4))
+    ctypes.memmove(pv, v.ctypes.data, v.nbytes)
+    ctypes.memmove(pf, f.ctypes.data, f.nbytes)
+    m.nverts, m.ntris = len(v), len(f)
+    m.verts = ctypes.cast(pv, ctypes.POINTER(ctypes.c_float))
+    m.faces = ctypes.cast(pf, ctypes.POINTER(ctypes.c_int32))
+    m.bbox_from_faces = 0
+    st = L.gpv_mesh_stress(ctypes.byref(m), voxel_count, voxel_count2, p_snap, f_sliver, seed)
+    if st != GPV_OK:
+        L.gpv_mesh_free(ctypes.byref(m))
+        _check(st, "gpv_mesh_stress")
+    v2, f2, _ = _take_mesh(m)
+    return v2, f2
+
+
+def read_obj(path):
+    m = Mesh()
+    _check(lib().gpv_mesh_read_obj(os.fsencode(path), ctypes.byref(m)), "gpv_mesh_read_obj")
+    return _take_mesh(m)
+
+
+def read_off(path):
+    m = Mesh()
+    _check(lib().gpv_mesh_read_off(os.fsencode(path), ctypes.byref(m)), "gpv_mesh_read_off")
+    return _take_mesh(m)
+
+
+def flatten(verts, faces):
+    """Object::CreateFlatTriangleData (src/Object.cpp:3496-3527)."""
+    return np.ascontiguousarray(np.asarray(verts, np.float32)[np.asarray(faces)].reshape(-1, 9))
+
+
+def mesh_bbox(verts):
+    """bbox + 0.1% pad + maxModelSize (src/Object.cpp:441-450, 572-583)."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    bmin = np.zeros(3, np.float32)
+    bmax = np.zeros(3, np.float32)
+    mms = ctypes.c_float()
+    _check(lib().gpv_mesh_bbox(v, len(v) // 3, bmin, bmax, ctypes.byref(mms)), "gpv_mesh_bbox")
+    return bmin, bmax, np.float32(mms.value)
+
+
+def grid_derive(bmin, bmax, mms, voxel_count, voxel_count2, level2=True):
+    info = GridInfo()
+    p = Params(voxel_count, voxel_count2, 1 if level2 else 0, 0)
+    _check(lib().gpv_grid_derive(np.asarray(bmin, np.float32), np.asarray(bmax, np.float32), float(mms),
+                                 ctypes.byref(p), ctypes.byref(info)), "gpv_grid_derive")
+    return info
+
+
+# ---------------------------------------------------------------------------
+# device path
+# ---------------------------------------------------------------------------
+class Context:
+    """gpv_ctx: device, stream, memory pool."""
+
+    def __init__(self, device=-1):
+        self.h = _vp()
+        _check(lib().gpv_ctx_create(device, 0, ctypes.byref(self.h)), "gpv_ctx_create")
+
+    def close(self):
+        if self.h:
+            lib().gpv_ctx_destroy(self.h)
+            self.h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, array):
+        """Device copy of a host array (context pool); returns the pointer."""
+        a = np.ascontiguousarray(array)
+        d = _vp()
+        _check(lib().gpv_device_upload(self.h, a.ctypes.data, a.nbytes, ctypes.byref(d)), "gpv_device_upload")
+        return d.value
+
+    def free(self, ptr):
+        _check(lib().gpv_device_free(self.h, ptr), "gpv_device_free")
+
+    def voxelize(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0,
+                 device_ptr=None):
+        """Object::PerformVoxelization on a flat soup; returns a Grid."""
+        soup = np.ascontiguousarray(soup, np.float32).reshape(-1, 9)
+        p = Params(int(voxel_count), int(voxel_count2), 1 if level2 else 0, int(initial_capacity))
+        g = _vp()
+        ptr = device_ptr if device_ptr is not None else soup.ctypes.data
+        _check(lib().gpv_voxelize(self.h, ptr, len(soup), 1 if device_ptr is not None else 0,
+                                  np.asarray(bmin, np.float32), np.asarray(bmax, np.float32), float(mms),
+                                  ctypes.byref(p), ctypes.byref(g)), "gpv_voxelize")
+        return Grid(self, g)
+
+    def create(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0):
+        """gpv_grid_create: the staged form (call the stage methods on the Grid)."""
+        soup = np.ascontiguousarray(soup, np.float32).reshape(-1, 9)
+        p = Params(int(voxel_count), int(voxel_count2), 1 if level2 else 0, int(initial_capacity))
+        g = _vp()
+        _check(lib().gpv_grid_create(self.h, soup.ctypes.data, len(soup), 0, np.asarray(bmin, np.float32),
+                                     np.asarray(bmax, np.float32), float(mms), ctypes.byref(p),
+                                     ctypes.byref(g)), "gpv_grid_create")
+        return Grid(self, g)
+
+
+class Grid:
+    """A device-resident voxelization (gpv_grid)."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+
+    # staged entry points, in PerformVoxelization's order
+    def classify_tessellation(self):
+        need = ctypes.c_int64()
+        _check(lib().gpv_classify_tessellation(self.ctx.h, self.h, ctypes.byref(need)),
+               "gpv_classify_tessellation")
+        return need.value
+
+    def classify_inout(self):
+        _check(lib().gpv_classify_inout(self.ctx.h, self.h), "gpv_classify_inout")
+
+    def compact_boundary(self):
+        _check(lib().gpv_compact_boundary(self.ctx.h, self.h), "gpv_compact_boundary")
+
+    def classify_inout_level2(self):
+        _check(lib().gpv_classify_inout_level2(self.ctx.h, self.h), "gpv_classify_inout_level2")
+
+    def classify_tessellation_level2(self):
+        _check(lib().gpv_classify_tessellation_level2(self.ctx.h, self.h), "gpv_classify_tessellation_level2")
+
+    def finish(self):
+        _check(lib().gpv_grid_finish(self.ctx.h, self.h), "gpv_grid_finish")
+
+    def info(self):
+        i = GridInfo()
+        _check(lib().gpv_grid_get_info(self.h, ctypes.byref(i)), "gpv_grid_get_info")
+        return i.as_dict()
+
+    def download(self):
+        i = self.info()
+        n1, nb, n2 = i["n1"], i["nb"], i["n2"]
+        l1 = np.zeros(n1, np.uint8)
+        prefix = np.zeros(n1, np.int32)
+        bidx = np.zeros(max(nb, 1), np.int32)
+        l2 = np.zeros(max(nb * n2 ** 3, 1), np.uint8)
+        _check(lib().gpv_grid_download(self.ctx.h, self.h, l1.ctypes.data, prefix.ctypes.data,
+                                       bidx.ctypes.data if nb else None,
+                                       l2.ctypes.data if (nb and n2) else None), "gpv_grid_download")
+        return {"l1": l1, "prefix": prefix, "bidx": bidx[:nb], "l2": l2[:nb * n2 ** 3]}
+
+    def save_raw(self, obj_id, directory):
+        _check(lib().gpv_save_raw(self.ctx.h, self.h, int(obj_id), os.fsencode(directory)), "gpv_save_raw")
+
+    def release(self):
+        if self.h:
+            lib().gpv_grid_release(self.ctx.h, self.h)
+            self.h = _vp()

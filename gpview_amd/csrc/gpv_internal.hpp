@@ -1,0 +1,114 @@
+// gpv_internal.hpp -- shared host/device declarations of libgpv (not public).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpv {
+
+// Grid constants and per-axis center tables, passed to kernels by value.
+// Bit layout of every Level1 bitplane: word w = (z*ny + y)*nxw + x/64,
+// bit x%64 (rows padded to whole 64-bit words), so words are in the
+// reference's linear order z*ny*nx + y*nx + x and a boundary rank is a
+// popcount prefix.  Level2: w2 = ceil(n2^3/64) words per boundary voxel,
+// bit s = (r*n2 + q)*n2 + p inside the block ([b][r][q][p] order).
+struct GridDev {
+    int nx, ny, nz, nxw;
+    int n2, w2;
+    float bminx, bminy, bminz, bmaxx, bmaxy, bmaxz;
+    float h1x, h1y, h1z;   // Level1 half extents
+    float h2x, h2y, h2z;   // Level2 half extents
+    const float* c1x; const float* c1y; const float* c1z;   // [n]    Level1 SAT centers
+    const float* r1x; const float* r1y; const float* r1z;   // [n]    Level1 ray origins
+    const float* c2x; const float* c2y; const float* c2z;   // [n*n2] Level2 SAT centers
+    const float* r2x; const float* r2y; const float* r2z;   // [n*n2] Level2 ray origins
+};
+
+// Append-buffer descriptor: a device counter and a capacity.  Writers past
+// the capacity are dropped; the host sees count > cap and regrows.
+struct Append64 {
+    unsigned long long* count;
+    uint64_t* key;
+    uint32_t* val;     // optional payload (nullptr if none)
+    uint64_t cap;
+};
+
+// Device-side counters of one voxelization (one cache line each avoided:
+// small, read back once per mesh).
+struct Counters {
+    unsigned long long pairs;       // Level1 SAT (voxel, triangle) pairs
+    unsigned long long colents;     // (column, triangle, zmax) entries
+    unsigned long long hits;        // (column, triangle) ray hits, Level1
+    unsigned long long tests_l1;    // sum of clipped index-AABB volumes
+    unsigned long long pair_est;    // sum of index-AABB face areas (capacity hint)
+    unsigned long long l1_inside;
+    unsigned long long l2_inside;
+    unsigned long long l2_boundary;
+    unsigned int max_per_voxel;
+    unsigned int pad;
+};
+
+// ---- launch wrappers (gpv_kernels.hip) ------------------------------------
+void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
+
+void launch_tri_setup(const float* tris, int64_t T, const GridDev& g, int4* box0, int4* box1,
+                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, Counters* ctr,
+                      hipStream_t s);
+
+// exclusive scan of n u32 into out[0..n] (out[n] = total); tmp sized by
+// scan_tmp_elems(n).  If max_out != nullptr the max input is atomically
+// folded into it.
+size_t scan_tmp_elems(int64_t n);
+void launch_scan_u32_u64(const uint32_t* in, uint64_t* out, int64_t n, uint64_t* tmp,
+                         hipStream_t s);
+void launch_scan_u32_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
+                         unsigned int* max_out, hipStream_t s);
+
+void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1,
+                       const uint64_t* off, const GridDev& g, uint64_t* bnd, Append64 pairs,
+                       Append64 cols, int blocks, hipStream_t s);
+void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off,
+                       const GridDev& g, Append64 hits, int blocks, hipStream_t s);
+
+// counting sort helpers
+void launch_count_keys_hi(const uint64_t* key, uint64_t n,
+                          uint32_t* cnt, hipStream_t s);
+void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n,
+                          const GridDev& g, const uint32_t* off, uint32_t* cnt,
+                         float4* rec, hipStream_t s);
+void launch_scatter_cols(const uint64_t* key, const uint32_t* zmax, uint64_t n,
+                          const uint32_t* off, uint32_t* cnt, uint32_t* ct,
+                         uint32_t* cz, hipStream_t s);
+
+void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g, uint64_t* ins,
+                      hipStream_t s);
+
+void launch_popc_words(const uint64_t* bits, int64_t W, uint32_t* cnt, hipStream_t s);
+void launch_boundary_index(const uint64_t* bnd, const uint32_t* wprefix, const GridDev& g,
+                           int32_t* bidx, hipStream_t s);
+void launch_pair_rank(const uint64_t* pairs, uint64_t n,
+                      const uint64_t* bnd, const uint32_t* wprefix, uint32_t* pb, uint32_t* cntb,
+                      hipStream_t s);
+void launch_pair_scatter(const uint64_t* pairs, const uint32_t* pb, uint64_t n,
+                          const uint32_t* boff, uint32_t* cntb, uint32_t* list,
+                         hipStream_t s);
+
+void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff,
+                   const uint32_t* list, const GridDev& g, uint64_t* sat2, hipStream_t s);
+void launch_l2_parity(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* coff,
+                      const uint32_t* ct, const uint32_t* cz, const GridDev& g, uint64_t* par2,
+                      hipStream_t s);
+
+void launch_count_states(const uint64_t* bnd, const uint64_t* ins, int64_t W,
+                         const uint64_t* sat2, const uint64_t* par2, int64_t W2, Counters* ctr,
+                         hipStream_t s);
+
+void launch_export_l1(const uint64_t* bnd, const uint64_t* ins, const GridDev& g, uint8_t unit, uint8_t* out,
+                      hipStream_t s);
+void launch_export_prefix(const uint64_t* bnd, const uint32_t* wprefix, const GridDev& g,
+                          int32_t* out, hipStream_t s);
+void launch_export_l2(const uint64_t* sat2, const uint64_t* par2, int64_t B, const GridDev& g,
+                      uint8_t unit, uint8_t* out, hipStream_t s);
+
+}  // namespace gpv

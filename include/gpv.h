@@ -1,0 +1,166 @@
+/*
+ * gpv.h -- C ABI of libgpv, the gfx950 two-level hybrid voxelizer.
+ *
+ * Drop-in for GPView's voxelization path.  The reference binds three
+ * extern "C" CUDA wrappers (includes/CUDAUtilities.h:87-89, defined at
+ * cuda/CUDAClassifyTessellation.cu:507-540) from Object's host code
+ * (src/Object.cpp:2071-2654, 3077-3430); this header exposes the same stages
+ * with plain pointers and sizes, device buffers owned by the library, and a
+ * status code per call instead of the reference's unconditional `return 1`.
+ *
+ * Results follow the reference's CPU composition bit for bit: Level1 ray
+ * parity (src/Object.cpp:716-779), Level1 SAT binning (:2256-2342),
+ * boundary compaction (:3270-3279), Level2 ray parity (:1111-1207), Level2
+ * SAT (:2344-2424), .raw output (:2934-3075).
+ *
+ * Threading: one context is used by one host thread at a time.  Calls on a
+ * context are ordered on its stream; separate contexts run concurrently.
+ */
+#ifndef GPV_H
+#define GPV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+enum gpv_status {
+    GPV_OK = 0,
+    GPV_ERR_INVALID = 1,      /* bad argument (null pointer, size, grid limits) */
+    GPV_ERR_OOM = 2,          /* device or host allocation failed */
+    GPV_ERR_CAPACITY = 3,     /* bin capacity regrow did not converge */
+    GPV_ERR_HIP = 4,          /* HIP runtime error (message via gpv_last_error) */
+    GPV_ERR_NO_DEVICE = 5,    /* no usable gfx950 device */
+    GPV_ERR_IO = 6,           /* file could not be written */
+    GPV_ERR_STATE = 7         /* stage called out of order */
+};
+
+/* ---- parameters: GLParameters fields used by the path
+ *      (includes/GLParameters.h, defaults src/GLParameters.cpp:63-67) ------- */
+typedef struct gpv_params {
+    int voxel_count;          /* GLParameters::voxelCount  (Level1 nominal divisions) */
+    int voxel_count2;         /* GLParameters::voxelCount2 (Level2 divisions per axis) */
+    int level2;               /* GLParameters::level2Voxels */
+    int64_t initial_capacity; /* bin-pair capacity; <= 0 picks a default.  The
+                                 analogue of PerformVoxelization's bufferSize
+                                 (src/Object.cpp:3088-3089); overflow regrows and
+                                 reruns Level1 binning (src/Object.cpp:3198-3214) */
+} gpv_params;
+
+/* ---- grid description (VoxelData, includes/Utilities.h:79-154) ----------- */
+typedef struct gpv_grid_info {
+    float bmin[3], bmax[3];   /* padded bbox, Object::bBoxMin/Max */
+    float max_model_size;     /* Object::maxModelSize */
+    int n[3];                 /* numDivX/Y/Z */
+    float g[3];               /* gridSizeX/Y/Z */
+    int n2;                   /* numDivX2 (= Y2 = Z2), 0 when Level2 is off */
+    float g2[3];              /* gridSizeX2/Y2/Z2 */
+    int64_t n1;               /* nx*ny*nz */
+    int64_t nb;               /* boundary Level1 voxels (boundaryIndex.size()) */
+    int64_t l1_inside, l1_boundary, l2_inside, l2_boundary;  /* src/Object.cpp:3353-3378 */
+    int64_t tests_l1;         /* sum over triangles of index-AABB volume */
+    int64_t tests_l2;         /* sum over boundary voxels of |bin| * n2^3 */
+    int64_t pairs;            /* total Level1 bin entries */
+    int32_t max_tris_per_voxel; /* "Max Triangles in Voxel", src/Object.cpp:2221 */
+    int32_t regrows;          /* Level1 reruns caused by bin capacity */
+    float ms_stage[6];        /* device time per stage: setup, L1 SAT, L1 parity,
+                                 compaction+bins, L2 parity, L2 SAT */
+    float ms_total;           /* device time, first to last stage */
+} gpv_grid_info;
+
+typedef struct gpv_ctx gpv_ctx;     /* device, stream, memory pool */
+typedef struct gpv_grid gpv_grid;   /* one voxelization, device resident */
+
+/* ---- context -------------------------------------------------------------- */
+/* Replaces InitializeCUDA (src/CUDAUtilities.cpp:51-55): binds `device`
+ * (-1 = current), creates the context's stream and pool.  flags: reserved, 0. */
+int gpv_ctx_create(int device, unsigned flags, gpv_ctx **out);
+int gpv_ctx_destroy(gpv_ctx *ctx);
+/* last error text of this thread (never NULL) */
+const char *gpv_last_error(void);
+const char *gpv_status_string(int status);
+
+/* Device copy of a host buffer, from the context's pool: the cudaMalloc +
+ * cudaMemcpy of the soup in Object::ClassifyTessellationCUDA
+ * (src/Object.cpp:2096-2107), for callers that keep inputs resident and pass
+ * them to gpv_voxelize with tris_on_device = 1. */
+int gpv_device_upload(gpv_ctx *ctx, const void *host, int64_t bytes, void **dev);
+int gpv_device_free(gpv_ctx *ctx, void *dev);
+
+/* ---- host-side geometry set-up ----------------------------------------- */
+/* bbox over vertices + 0.1% pad + maxModelSize: Object::ReadObject
+ * (src/Object.cpp:441-450, 572-583).  verts: nverts*3 floats (host). */
+int gpv_mesh_bbox(const float *verts, int64_t nverts, float bmin[3], float bmax[3],
+                  float *max_model_size);
+/* Grid derivation of PerformVoxelization (src/Object.cpp:3094-3136) into
+ * info's bbox/n/g/n2/g2 fields. */
+int gpv_grid_derive(const float bmin[3], const float bmax[3], float max_model_size,
+                    const gpv_params *params, gpv_grid_info *info);
+
+/* ---- whole path ----------------------------------------------------------- */
+/* Object::PerformVoxelization (src/Object.cpp:3077-3430) on a flat triangle
+ * soup (Object::CreateFlatTriangleData layout, src/Object.cpp:3496-3527:
+ * tris[t*9 + k*3 + c]).  tris_on_device != 0 means `tris` is a device
+ * pointer that stays valid until the grid is released (no copy).  bmin/bmax/
+ * max_model_size as produced by gpv_mesh_bbox.  The grid stays on the
+ * device until gpv_grid_release. */
+int gpv_voxelize(gpv_ctx *ctx, const float *tris, int64_t ntris, int tris_on_device,
+                 const float bmin[3], const float bmax[3], float max_model_size,
+                 const gpv_params *params, gpv_grid **out);
+
+/* ---- stage entry points (same order PerformVoxelization calls them) ------- */
+/* Allocates the grid, uploads the soup and the center tables: the set-up of
+ * PerformVoxelization (src/Object.cpp:3094-3193). */
+int gpv_grid_create(gpv_ctx *ctx, const float *tris, int64_t ntris, int tris_on_device,
+                    const float bmin[3], const float bmax[3], float max_model_size,
+                    const gpv_params *params, gpv_grid **out);
+/* Level1 SAT binning: CUDAClassifyTessellation (cuda/CUDAClassifyTessellation.cu:507)
+ * as driven by Object::ClassifyTessellationCUDA (src/Object.cpp:2071-2253).
+ * *needed receives 0, or the capacity that would have been required when the
+ * bins overflowed (then call again: the reference's retry, src/Object.cpp:3198-3214). */
+int gpv_classify_tessellation(gpv_ctx *ctx, gpv_grid *grid, int64_t *needed);
+/* Level1 inside/outside by +z ray parity over all triangles:
+ * Object::ClassifyInOutCPU (src/Object.cpp:716-779). */
+int gpv_classify_inout(gpv_ctx *ctx, gpv_grid *grid);
+/* Boundary compaction + exclusive prefix (src/Object.cpp:3270-3279) and the
+ * per-boundary-voxel bins (the CSR of src/Object.cpp:2137-2219). */
+int gpv_compact_boundary(gpv_ctx *ctx, gpv_grid *grid);
+/* Level2 ray parity: CUDAClassifyInOutLevel2 (cuda/CUDAClassifyTessellation.cu:531)
+ * with the CPU candidate rule of Object::ClassifyInOutLevel2CPU (src/Object.cpp:1111-1207). */
+int gpv_classify_inout_level2(gpv_ctx *ctx, gpv_grid *grid);
+/* Level2 SAT: CUDAClassifyTessellationLevel2 (cuda/CUDAClassifyTessellation.cu:518)
+ * with the CPU centers of Object::ClassifyTessellationLevel2 (src/Object.cpp:2344-2424). */
+int gpv_classify_tessellation_level2(gpv_ctx *ctx, gpv_grid *grid);
+/* Counts (src/Object.cpp:3353-3378); waits for the grid's work. */
+int gpv_grid_finish(gpv_ctx *ctx, gpv_grid *grid);
+
+/* ---- results ---------------------------------------------------------------- */
+int gpv_grid_get_info(const gpv_grid *grid, gpv_grid_info *info);
+/* Copies results to host buffers (any may be NULL):
+ *   l1_state[n1]        0 outside / 1 inside / 2 boundary, [z][y][x]
+ *   prefix[n1]          exclusive boundary prefix sum (Level1BoundaryPrefixSum.raw)
+ *   bidx[nb]            boundary Level1 linear indices, ascending
+ *   l2_state[nb*n2^3]   0/1/2 per Level2 voxel, [b][r][q][p] */
+int gpv_grid_download(gpv_ctx *ctx, const gpv_grid *grid, uint8_t *l1_state, int32_t *prefix,
+                      int32_t *bidx, uint8_t *l2_state);
+/* Raw device views (valid until release), for zero-copy consumers.
+ * Level1 bitplanes: word (z*ny + y)*ceil(nx/64) + x/64, bit x%64 (rows padded
+ * to whole 64-bit words).  Level2 bitplanes: ceil(n2^3/64) words per boundary
+ * voxel b, bit (r*n2 + q)*n2 + p.  A voxel is boundary if its boundary bit is
+ * set, otherwise inside if its inside bit is set. */
+int gpv_grid_device_views(const gpv_grid *grid, const uint64_t **l1_boundary_bits,
+                          const uint64_t **l1_inside_bits, const uint64_t **l2_boundary_bits,
+                          const uint64_t **l2_inside_bits, const int32_t **bidx);
+/* Object::SaveVoxelization (src/Object.cpp:2934-3075): writes
+ * Obj<obj_id>VoxelConfig.txt, Obj<obj_id>Level1InOut.raw and, with Level2,
+ * Obj<obj_id>Level1BoundaryPrefixSum.raw and Obj<obj_id>Level2InOut.raw
+ * under dir (NULL = current directory). */
+int gpv_save_raw(gpv_ctx *ctx, const gpv_grid *grid, int obj_id, const char *dir);
+int gpv_grid_release(gpv_ctx *ctx, gpv_grid *grid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPV_H */

@@ -1,0 +1,900 @@
+/*
+ * gpv_oracle.c -- CPU restatement of GPView's two-level hybrid voxelizer.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path, and the CPU
+ * baseline that is timed beside it.  Nothing in the product links it.
+ *
+ * Float semantics: every expression keeps the reference's operand order and
+ * its float/double promotions (the reference mixes double literals such as
+ * 0.5, 1.0 and 1e-6 into float expressions).  Compile with
+ * -ffp-contract=off and without -ffast-math.
+ *
+ * Two deliberate departures, both where the reference's behaviour is
+ * undefined rather than a result:
+ *   - index-AABB lower bounds are clamped at 0 (the reference indexes out of
+ *     bounds for a vertex below the bbox; our bbox always contains the mesh);
+ *   - Level2 storage starts zeroed for every boundary voxel (the reference
+ *     memsets only the first Level2 block, src/Object.cpp:3284).
+ */
+#include "gpv_oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* small growable arrays                                                     */
+/* ------------------------------------------------------------------------ */
+typedef struct { int64_t *a; int64_t n, cap; } vec64;
+
+static void v64_push(vec64 *v, int64_t x)
+{
+    if (v->n == v->cap) {
+        v->cap = v->cap ? v->cap * 2 : 1024;
+        v->a = (int64_t *)realloc(v->a, (size_t)v->cap * sizeof(int64_t));
+        if (!v->a) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    }
+    v->a[v->n++] = x;
+}
+
+static void *xcalloc(size_t n, size_t sz)
+{
+    void *p = calloc(n ? n : 1, sz);
+    if (!p) { fprintf(stderr, "oracle: out of memory (%zu x %zu)\n", n, sz); abort(); }
+    return p;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SAT triangle/box overlap: src/TriBoxIntersection.cpp:64-210               */
+/* ------------------------------------------------------------------------ */
+
+/* planeBoxOverlap, src/TriBoxIntersection.cpp:64-86 */
+static int plane_box_overlap(const float nrm[3], const float vert[3], const float box[3])
+{
+    float vmin[3], vmax[3];
+    for (int q = 0; q < 3; q++) {
+        float v = vert[q];
+        if (nrm[q] > 0.0f) {
+            vmin[q] = -box[q] - v;
+            vmax[q] = box[q] - v;
+        } else {
+            vmin[q] = box[q] - v;
+            vmax[q] = -box[q] - v;
+        }
+    }
+    if (nrm[0] * vmin[0] + nrm[1] * vmin[1] + nrm[2] * vmin[2] > 0.0f) return 0;
+    if (nrm[0] * vmax[0] + nrm[1] * vmax[1] + nrm[2] * vmax[2] >= 0.0f) return 1;
+    return 0;
+}
+
+/* One edge-axis test (the AXISTEST_* macros, src/TriBoxIntersection.cpp:90-133):
+ * separated when the projected interval [min(pa,pb), max(pa,pb)] misses
+ * [-rad, rad] strictly. */
+static int separated(float pa, float pb, float rad)
+{
+    float mn, mx;
+    if (pa < pb) { mn = pa; mx = pb; } else { mn = pb; mx = pa; }
+    return (mn > rad || mx < -rad);
+}
+
+/* TriBoxOverlap, src/TriBoxIntersection.cpp:135-210 */
+int orc_tri_box_overlap(const float c[3], const float h[3], const float tv[9])
+{
+    float v0[3], v1[3], v2[3], e0[3], e1[3], e2[3], n[3];
+    float fx, fy, fz;
+    for (int q = 0; q < 3; q++) {
+        v0[q] = tv[q] - c[q];
+        v1[q] = tv[3 + q] - c[q];
+        v2[q] = tv[6 + q] - c[q];
+    }
+    for (int q = 0; q < 3; q++) {
+        e0[q] = v1[q] - v0[q];
+        e1[q] = v2[q] - v1[q];
+        e2[q] = v0[q] - v2[q];
+    }
+    /* edge e0: X01, Y02, Z12 (:163-168) */
+    fx = fabsf(e0[0]); fy = fabsf(e0[1]); fz = fabsf(e0[2]);
+    if (separated(e0[2] * v0[1] - e0[1] * v0[2], e0[2] * v2[1] - e0[1] * v2[2],
+                  fz * h[1] + fy * h[2])) return 0;
+    if (separated(-e0[2] * v0[0] + e0[0] * v0[2], -e0[2] * v2[0] + e0[0] * v2[2],
+                  fz * h[0] + fx * h[2])) return 0;
+    if (separated(e0[1] * v1[0] - e0[0] * v1[1], e0[1] * v2[0] - e0[0] * v2[1],
+                  fy * h[0] + fx * h[1])) return 0;
+    /* edge e1: X01, Y02, Z0 (:170-175) */
+    fx = fabsf(e1[0]); fy = fabsf(e1[1]); fz = fabsf(e1[2]);
+    if (separated(e1[2] * v0[1] - e1[1] * v0[2], e1[2] * v2[1] - e1[1] * v2[2],
+                  fz * h[1] + fy * h[2])) return 0;
+    if (separated(-e1[2] * v0[0] + e1[0] * v0[2], -e1[2] * v2[0] + e1[0] * v2[2],
+                  fz * h[0] + fx * h[2])) return 0;
+    if (separated(e1[1] * v0[0] - e1[0] * v0[1], e1[1] * v1[0] - e1[0] * v1[1],
+                  fy * h[0] + fx * h[1])) return 0;
+    /* edge e2: X2, Y1, Z12 (:177-182) */
+    fx = fabsf(e2[0]); fy = fabsf(e2[1]); fz = fabsf(e2[2]);
+    if (separated(e2[2] * v0[1] - e2[1] * v0[2], e2[2] * v1[1] - e2[1] * v1[2],
+                  fz * h[1] + fy * h[2])) return 0;
+    if (separated(-e2[2] * v0[0] + e2[0] * v0[2], -e2[2] * v1[0] + e2[0] * v1[2],
+                  fz * h[0] + fx * h[2])) return 0;
+    if (separated(e2[1] * v1[0] - e2[0] * v1[1], e2[1] * v2[0] - e2[0] * v2[1],
+                  fy * h[0] + fx * h[1])) return 0;
+    /* AABB axes, FINDMINMAX (:191-200) */
+    for (int q = 0; q < 3; q++) {
+        float mn = v0[q], mx = v0[q];
+        if (v1[q] < mn) mn = v1[q];
+        if (v1[q] > mx) mx = v1[q];
+        if (v2[q] < mn) mn = v2[q];
+        if (v2[q] > mx) mx = v2[q];
+        if (mn > h[q] || mx < -h[q]) return 0;
+    }
+    /* plane of the triangle, n = e0 x e1 (:205-207, CROSS :45-48) */
+    n[0] = e0[1] * e1[2] - e0[2] * e1[1];
+    n[1] = e0[2] * e1[0] - e0[0] * e1[2];
+    n[2] = e0[0] * e1[1] - e0[1] * e1[0];
+    return plane_box_overlap(n, v0, h);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Moller-Trumbore: src/TriRayIntersection.cpp:78-131 (EPSILON :38 is the   */
+/* double 1e-6; det and t are promoted to double for the compares)          */
+/* ------------------------------------------------------------------------ */
+#define MT_EPS 0.000001
+
+int orc_tri_ray(const float V1[3], const float V2[3], const float V3[3],
+                const float O[3], const float D[3], float *t_out)
+{
+    float e1[3], e2[3], P[3], Q[3], T[3];
+    float det, inv_det, u, v, t;
+    for (int q = 0; q < 3; q++) { e1[q] = V2[q] - V1[q]; e2[q] = V3[q] - V1[q]; }
+    P[0] = D[1] * e2[2] - D[2] * e2[1];
+    P[1] = D[2] * e2[0] - D[0] * e2[2];
+    P[2] = D[0] * e2[1] - D[1] * e2[0];
+    det = e1[0] * P[0] + e1[1] * P[1] + e1[2] * P[2];
+    if ((double)det > -MT_EPS && (double)det < MT_EPS) return 0;
+    inv_det = 1.f / det;
+    for (int q = 0; q < 3; q++) T[q] = O[q] - V1[q];
+    u = (T[0] * P[0] + T[1] * P[1] + T[2] * P[2]) * inv_det;
+    if (u < 0.f || u > 1.f) return 0;
+    Q[0] = T[1] * e1[2] - T[2] * e1[1];
+    Q[1] = T[2] * e1[0] - T[0] * e1[2];
+    Q[2] = T[0] * e1[1] - T[1] * e1[0];
+    v = (D[0] * Q[0] + D[1] * Q[1] + D[2] * Q[2]) * inv_det;
+    if (v < 0.f || u + v > 1.f) return 0;
+    t = (e2[0] * Q[0] + e2[1] * Q[1] + e2[2] * Q[2]) * inv_det;
+    if ((double)t > MT_EPS) {
+        if (t_out) *t_out = t;
+        return 1;
+    }
+    return 0;
+}
+
+/* Column factorisation of the +z ray (D = (0,0,1)).  For that D every term
+ * in P, det, u and v that involves O.z is multiplied by an exact zero, so the
+ * det/u/v decision is a function of (O.x, O.y) alone and only t needs O.z.
+ * mt_xy/mt_t evaluate the same float expressions as orc_tri_ray, split at
+ * that seam.  Equality with orc_tri_ray is asserted by the tests. */
+typedef struct {
+    float v1[3], e1[3], e2[3];
+    float inv_det;
+    float tx, ty;           /* T.x, T.y for the column */
+} mt_col;
+
+static const float RAY_D[3] = {0.0f, 0.0f, 1.0f};
+
+static int mt_xy(const float *tv, float ox, float oy, mt_col *s)
+{
+    float e1[3], e2[3], P[3];
+    for (int q = 0; q < 3; q++) { e1[q] = tv[3 + q] - tv[q]; e2[q] = tv[6 + q] - tv[q]; }
+    P[0] = RAY_D[1] * e2[2] - RAY_D[2] * e2[1];
+    P[1] = RAY_D[2] * e2[0] - RAY_D[0] * e2[2];
+    P[2] = RAY_D[0] * e2[1] - RAY_D[1] * e2[0];
+    float det = e1[0] * P[0] + e1[1] * P[1] + e1[2] * P[2];
+    if ((double)det > -MT_EPS && (double)det < MT_EPS) return 0;
+    float inv_det = 1.f / det;
+    float tx = ox - tv[0], ty = oy - tv[1];
+    /* T.z * P.z is an exact signed zero: it cannot change a nonzero sum, and
+     * a zero sum gives u = +-0 which passes/fails every compare identically. */
+    float u = (tx * P[0] + ty * P[1] + 0.0f * P[2]) * inv_det;
+    if (u < 0.f || u > 1.f) return 0;
+    float qz = tx * e1[1] - ty * e1[0];
+    float v = (RAY_D[0] * 0.0f + RAY_D[1] * 0.0f + RAY_D[2] * qz) * inv_det;
+    if (v < 0.f || u + v > 1.f) return 0;
+    for (int q = 0; q < 3; q++) { s->v1[q] = tv[q]; s->e1[q] = e1[q]; s->e2[q] = e2[q]; }
+    s->inv_det = inv_det;
+    s->tx = tx;
+    s->ty = ty;
+    return 1;
+}
+
+static int mt_t(const mt_col *s, float oz)
+{
+    float tz = oz - s->v1[2];
+    float qx = s->ty * s->e1[2] - tz * s->e1[1];
+    float qy = tz * s->e1[0] - s->tx * s->e1[2];
+    float qz = s->tx * s->e1[1] - s->ty * s->e1[0];
+    float t = (s->e2[0] * qx + s->e2[1] * qy + s->e2[2] * qz) * s->inv_det;
+    return (double)t > MT_EPS;
+}
+
+void orc_tri_box_batch(int64_t n, const float *c, const float *h, const float *tv, uint8_t *res)
+{
+    for (int64_t i = 0; i < n; i++) res[i] = (uint8_t)orc_tri_box_overlap(c + 3 * i, h + 3 * i, tv + 9 * i);
+}
+
+void orc_tri_ray_batch(int64_t n, const float *tv, const float *o, const float *d, uint8_t *hit,
+                       float *t)
+{
+    for (int64_t i = 0; i < n; i++) {
+        float out = 0.0f;
+        hit[i] = (uint8_t)orc_tri_ray(tv + 9 * i, tv + 9 * i + 3, tv + 9 * i + 6, o + 3 * i, d + 3 * i, &out);
+        t[i] = hit[i] ? out : 0.0f;
+    }
+}
+
+void orc_tri_zray_split_batch(int64_t n, const float *tv, const float *o, uint8_t *hit)
+{
+    for (int64_t i = 0; i < n; i++) {
+        mt_col s;
+        hit[i] = (uint8_t)(mt_xy(tv + 9 * i, o[3 * i], o[3 * i + 1], &s) && mt_t(&s, o[3 * i + 2]));
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* bbox and grid                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* src/Object.cpp:441-450 (MinFloat3/MaxFloat3 includes/FloatVector.h:344-360)
+ * and the pad at src/Object.cpp:572-583.  VectorMagnitude
+ * (includes/FloatVector.h:323-326) sums in float and takes sqrt; the
+ * float->double->float round trip of sqrt equals a float sqrt. */
+void orc_bbox(const float *verts, int64_t nv, float bmin[3], float bmax[3], float *mms)
+{
+    float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    for (int64_t i = 0; i < nv; i++) {
+        const float *p = verts + 3 * i;
+        for (int q = 0; q < 3; q++) {
+            if (i == 0) { mn[q] = p[q]; mx[q] = p[q]; continue; }
+            mn[q] = mn[q] < p[q] ? mn[q] : p[q];
+            mx[q] = mx[q] > p[q] ? mx[q] : p[q];
+        }
+    }
+    float d0 = mx[0] - mn[0], d1 = mx[1] - mn[1], d2 = mx[2] - mn[2];
+    float sq = d0 * d0 + d1 * d1 + d2 * d2;
+    float mag = (float)sqrt((double)sq);
+    double model_size = mag;
+    float offset = (float)(0.001 * model_size);
+    for (int q = 0; q < 3; q++) {
+        bmax[q] = offset + mx[q];
+        bmin[q] = mn[q] - offset;
+    }
+    float ex = bmax[0] - bmin[0], ey = bmax[1] - bmin[1], ez = bmax[2] - bmin[2];
+    float m12 = ey > ez ? ey : ez;
+    *mms = ex > m12 ? ex : m12;
+}
+
+static int next_div4(int a) { return (a % 4 == 0) ? a : a + (4 - a % 4); }
+
+/* src/Object.cpp:3094-3136 */
+void orc_grid_setup(const float bmin[3], const float bmax[3], float mms,
+                    int voxel_count, int voxel_count2, orc_grid *g)
+{
+    memset(g, 0, sizeof(*g));
+    for (int q = 0; q < 3; q++) { g->bmin[q] = bmin[q]; g->bmax[q] = bmax[q]; }
+    g->max_model_size = mms;
+    float nominal = (float)((double)mms / (1.0 * voxel_count));
+    for (int q = 0; q < 3; q++) {
+        float extent = bmax[q] - bmin[q];
+        int n = (int)(extent / nominal);
+        if (n == 0) n++;
+        n = next_div4(n);
+        g->n[q] = n;
+        g->g[q] = (float)((double)(bmax[q] - bmin[q]) / (n * 1.0));
+        g->ext1[q] = (float)((double)g->g[q] / 2.0);
+    }
+    if (voxel_count2 > 0) {
+        g->n2 = voxel_count2;
+        for (int q = 0; q < 3; q++) {
+            g->g2[q] = (float)((double)g->g[q] / (voxel_count2 * 1.0));
+            g->ext2[q] = (float)((double)g->g2[q] / 2.0);
+        }
+    }
+}
+
+/* Level1 SAT centers: boxMidPoint = (p + 0.5)*boxExtents*2 + bBoxMin
+ * (src/Object.cpp:2329-2331), evaluated in double, stored as float. */
+void orc_l1_centers(const orc_grid *g, int a, float *out)
+{
+    for (int p = 0; p < g->n[a]; p++)
+        out[p] = (float)((p + 0.5) * (double)g->ext1[a] * 2 + (double)g->bmin[a]);
+}
+
+/* Level1 ray origins: bBoxMin + (i + 0.5)*gridSize (src/Object.cpp:743-745). */
+void orc_l1_ray_origins(const orc_grid *g, int a, float *out)
+{
+    for (int i = 0; i < g->n[a]; i++)
+        out[i] = (float)((double)g->bmin[a] + (i + 0.5) * (double)g->g[a]);
+}
+
+/* Level2 SAT centers (src/Object.cpp:2369-2371, 2389-2391):
+ * mid = (i + 0.5)*g + bBoxMin; center = (p + 0.5)*g2 + mid - ext1. */
+void orc_l2_centers(const orc_grid *g, int a, float *out)
+{
+    int n2 = g->n2;
+    for (int i = 0; i < g->n[a]; i++) {
+        float mid = (float)((i + 0.5) * (double)g->g[a] + (double)g->bmin[a]);
+        for (int p = 0; p < n2; p++)
+            out[i * n2 + p] =
+                (float)((p + 0.5) * (double)g->g2[a] + (double)mid - (double)g->ext1[a]);
+    }
+}
+
+/* Level2 ray origins (src/Object.cpp:1134-1136, 1158-1160):
+ * minPoint = i*g + bBoxMin in float; origin = minPoint + (p + 0.5)*g2. */
+void orc_l2_ray_origins(const orc_grid *g, int a, float *out)
+{
+    int n2 = g->n2;
+    for (int i = 0; i < g->n[a]; i++) {
+        float minp = (float)i * g->g[a] + g->bmin[a];
+        for (int p = 0; p < n2; p++)
+            out[i * n2 + p] = (float)((double)minp + (p + 0.5) * (double)g->g2[a]);
+    }
+}
+
+/* src/Object.cpp:2272-2281: int((v - min)/(max - min) * nDiv), pulled back
+ * by one when a vertex sits exactly on the max plane. */
+int orc_vertex_index(float v, float mn, float mx, int n)
+{
+    float num = v - mn;
+    float den = mx - mn;
+    float r = num / den;
+    float s = r * (float)n;
+    int i = (int)s;
+    if (i == n && v == mx) i--;
+    return i;
+}
+
+/* Clipped index-AABB of one triangle (src/Object.cpp:2303-2318 and the loop
+ * bounds :2320-2324).  Returns the number of voxels the reference tests. */
+static int64_t tri_index_box(const float *tv, const orc_grid *g, int lo[3], int hi[3])
+{
+    int64_t vol = 1;
+    for (int q = 0; q < 3; q++) {
+        int i0 = orc_vertex_index(tv[q], g->bmin[q], g->bmax[q], g->n[q]);
+        int i1 = orc_vertex_index(tv[3 + q], g->bmin[q], g->bmax[q], g->n[q]);
+        int i2 = orc_vertex_index(tv[6 + q], g->bmin[q], g->bmax[q], g->n[q]);
+        int mn = i0 < i1 ? i0 : i1; mn = mn < i2 ? mn : i2;
+        int mx = i0 > i1 ? i0 : i1; mx = mx > i2 ? mx : i2;
+        if (mn < 0) mn = 0;
+        if (mx > g->n[q] - 1) mx = g->n[q] - 1;
+        lo[q] = mn;
+        hi[q] = mx;
+        vol *= (mx >= mn) ? (int64_t)(mx - mn + 1) : 0;
+    }
+    return vol;
+}
+
+/* ------------------------------------------------------------------------ */
+/* conservative column bounds for the all-triangle Level1 parity             */
+/* ------------------------------------------------------------------------ */
+/* For D = +z the MT acceptance (u >= 0, v >= 0, u + v <= 1 in float) is a
+ * perturbed barycentric test.  With U = 2^-24 and |T| bounded over the grid,
+ * the float u, v differ from the exact barycentric coordinates (u*, v*) of
+ * the ray point w.r.t. the float edges e1, e2 by at most
+ *    du = (E_un + E_det (1+2U)) / (|det| - E_det) + 3U      (same for v)
+ * where E_det, E_un bound the rounding of det and of the u numerator.  An
+ * accepted column therefore lies in {V1 + a e1 + b e2 : a >= -du, b >= -dv,
+ * a + b <= 1 + 2U + du + dv}; the xy bbox of that triangle is returned,
+ * widened by 'safety'.  Slivers with |det| near 1e-6 get a wide box, which
+ * is exactly where float MT accepts rays well outside the triangle. */
+int orc_ray_column_bounds(const float tv[9], const float bmin[3], const float bmax[3],
+                          double safety, double lo[2], double hi[2])
+{
+    float e1[3], e2[3], P[3];
+    for (int q = 0; q < 3; q++) { e1[q] = tv[3 + q] - tv[q]; e2[q] = tv[6 + q] - tv[q]; }
+    P[0] = RAY_D[1] * e2[2] - RAY_D[2] * e2[1];
+    P[1] = RAY_D[2] * e2[0] - RAY_D[0] * e2[2];
+    P[2] = RAY_D[0] * e2[1] - RAY_D[1] * e2[0];
+    float det = e1[0] * P[0] + e1[1] * P[1] + e1[2] * P[2];
+    if ((double)det > -MT_EPS && (double)det < MT_EPS) return 0;
+    const double U = 5.9604644775390625e-08;
+    double ad = fabs((double)det);
+    double tmx = fmax(fabs((double)bmin[0] - tv[0]), fabs((double)bmax[0] - tv[0]));
+    double tmy = fmax(fabs((double)bmin[1] - tv[1]), fabs((double)bmax[1] - tv[1]));
+    double e1x = e1[0], e1y = e1[1], e2x = e2[0], e2y = e2[1];
+    double edet = U * (fabs(e1x * e2y) + fabs(e1y * e2x) + ad);
+    double eun = U * (3.0 * (fabs(e2y) * tmx + fabs(e2x) * tmy) + 2.0 * ad);
+    double evn = U * (3.0 * (fabs(e1y) * tmx + fabs(e1x) * tmy) + 2.0 * ad);
+    double den = ad - edet;
+    if (den <= 0.5 * ad) { /* det not even sign-reliable: whole grid */
+        lo[0] = bmin[0]; lo[1] = bmin[1];
+        hi[0] = bmax[0]; hi[1] = bmax[1];
+        return 1;
+    }
+    double du = safety * ((eun + edet * (1 + 2 * U)) / den + 3 * U);
+    double dv = safety * ((evn + edet * (1 + 2 * U)) / den + 3 * U);
+    double ds = du + dv + safety * 2 * U;
+    double ca[3] = {-du, 1 + ds + dv, -du};
+    double cb[3] = {-dv, -dv, 1 + ds + du};
+    for (int ax = 0; ax < 2; ax++) {
+        double ea = ax ? e1y : e1x, eb = ax ? e2y : e2x, base = tv[ax];
+        double mn = 1e300, mx = -1e300;
+        for (int c = 0; c < 3; c++) {
+            double x = base + ca[c] * ea + cb[c] * eb;
+            mn = x < mn ? x : mn;
+            mx = x > mx ? x : mx;
+        }
+        double slop = 1e-12 * (fabs(mn) + fabs(mx) + 1.0);
+        lo[ax] = mn - slop;
+        hi[ax] = mx + slop;
+    }
+    return 1;
+}
+
+/* first index with tab[i] >= x (n if none) */
+static int lower_idx(const float *tab, int n, double x)
+{
+    int a = 0, b = n;
+    while (a < b) {
+        int m = (a + b) / 2;
+        if ((double)tab[m] < x) a = m + 1; else b = m;
+    }
+    return a;
+}
+
+/* ------------------------------------------------------------------------ */
+/* pipeline                                                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const float *tris;
+    int64_t T;
+    const orc_grid *g;
+    int nt;
+    int64_t nx, ny, nz, n1;
+    float *cx, *cy, *cz;     /* L1 SAT centers */
+    float *rx, *ry, *rz;     /* L1 ray origins */
+} ctx_t;
+
+/* Level1 ray parity over ALL triangles (src/Object.cpp:716-779), literal. */
+static void l1_parity_exact(const ctx_t *c, uint8_t *par)
+{
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(c->nt)
+    for (int64_t kj = 0; kj < nz * ny; kj++) {
+        int64_t k = kj / ny, j = kj % ny;
+        for (int64_t i = 0; i < nx; i++) {
+            float o[3] = {c->rx[i], c->ry[j], c->rz[k]};
+            int hits = 0;
+            for (int64_t t = 0; t < c->T; t++) {
+                const float *tv = c->tris + 9 * t;
+                float tp;
+                if (orc_tri_ray(tv, tv + 3, tv + 6, o, RAY_D, &tp)) hits++;
+            }
+            par[k * ny * nx + j * nx + i] = (uint8_t)(hits % 2 == 1);
+        }
+    }
+}
+
+/* Level1 ray parity, column factorised: triangles are binned to the columns
+ * their conservative bound admits, det/u/v run once per (triangle, column)
+ * and only t runs per voxel. */
+static void l1_parity_fast(const ctx_t *c, uint8_t *par)
+{
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz, ncol = nx * ny;
+    int nt = c->nt;
+    vec64 *loc = (vec64 *)xcalloc((size_t)nt, sizeof(vec64));
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = omp_get_thread_num();
+#pragma omp for schedule(static)
+        for (int64_t t = 0; t < c->T; t++) {
+            const float *tv = c->tris + 9 * t;
+            double lo[2], hi[2];
+            if (!orc_ray_column_bounds(tv, c->g->bmin, c->g->bmax, 8.0, lo, hi)) continue;
+            int ia = lower_idx(c->rx, (int)nx, lo[0]);
+            int ib = lower_idx(c->rx, (int)nx, hi[0]);  /* first > hi is <= this */
+            while (ib < nx && (double)c->rx[ib] <= hi[0]) ib++;
+            int ja = lower_idx(c->ry, (int)ny, lo[1]);
+            int jb = lower_idx(c->ry, (int)ny, hi[1]);
+            while (jb < ny && (double)c->ry[jb] <= hi[1]) jb++;
+            for (int j = ja; j < jb; j++)
+                for (int i = ia; i < ib; i++)
+                    v64_push(&loc[tid], ((int64_t)(j * nx + i) << 32) | t);
+        }
+    }
+    int64_t *cnt = (int64_t *)xcalloc((size_t)ncol + 1, sizeof(int64_t));
+    int64_t total = 0;
+    for (int th = 0; th < nt; th++) {
+        for (int64_t q = 0; q < loc[th].n; q++) cnt[(loc[th].a[q] >> 32) + 1]++;
+        total += loc[th].n;
+    }
+    for (int64_t q = 0; q < ncol; q++) cnt[q + 1] += cnt[q];
+    int32_t *lst = (int32_t *)xcalloc((size_t)total, sizeof(int32_t));
+    int64_t *fill = (int64_t *)xcalloc((size_t)ncol, sizeof(int64_t));
+    for (int th = 0; th < nt; th++) {
+        for (int64_t q = 0; q < loc[th].n; q++) {
+            int64_t col = loc[th].a[q] >> 32;
+            lst[cnt[col] + fill[col]++] = (int32_t)(loc[th].a[q] & 0xffffffff);
+        }
+        free(loc[th].a);
+    }
+    free(loc);
+    free(fill);
+#pragma omp parallel num_threads(nt)
+    {
+        mt_col *hits = NULL;
+        int64_t cap = 0;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t col = 0; col < ncol; col++) {
+            int64_t i = col % nx, j = col / nx;
+            int64_t n = cnt[col + 1] - cnt[col];
+            if (n > cap) { cap = n; hits = (mt_col *)realloc(hits, (size_t)cap * sizeof(mt_col)); }
+            int64_t nh = 0;
+            for (int64_t q = 0; q < n; q++) {
+                const float *tv = c->tris + 9 * (int64_t)lst[cnt[col] + q];
+                if (mt_xy(tv, c->rx[i], c->ry[j], &hits[nh])) nh++;
+            }
+            for (int64_t k = 0; k < nz; k++) {
+                int p = 0;
+                for (int64_t h = 0; h < nh; h++) p ^= mt_t(&hits[h], c->rz[k]);
+                par[k * ny * nx + j * nx + i] = (uint8_t)p;
+            }
+        }
+        free(hits);
+    }
+    free(cnt);
+    free(lst);
+}
+
+/* Level1 SAT sweep + bins (src/Object.cpp:2256-2342 with AddPointToVoxel
+ * :2705-2748).  Bins list triangle ids in ascending order, as the
+ * reference's triangle-outer loop appends them. */
+static void l1_sat(const ctx_t *c, orc_result *r)
+{
+    int nt = c->nt;
+    int64_t nx = c->nx, ny = c->ny;
+    vec64 *loc = (vec64 *)xcalloc((size_t)nt, sizeof(vec64));
+    int64_t tests = 0;
+    const float *h = c->g->ext1;
+#pragma omp parallel num_threads(nt) reduction(+ : tests)
+    {
+        int tid = omp_get_thread_num();
+        int nth = omp_get_num_threads();
+        int64_t t0 = c->T * tid / nth, t1 = c->T * (tid + 1) / nth;
+        for (int64_t t = t0; t < t1; t++) {
+            const float *tv = c->tris + 9 * t;
+            int lo[3], hi[3];
+            tests += tri_index_box(tv, c->g, lo, hi);
+            for (int p = lo[0]; p <= hi[0]; p++)
+                for (int q = lo[1]; q <= hi[1]; q++)
+                    for (int s = lo[2]; s <= hi[2]; s++) {
+                        float ctr[3] = {c->cx[p], c->cy[q], c->cz[s]};
+                        if (orc_tri_box_overlap(ctr, h, tv))
+                            v64_push(&loc[tid], ((int64_t)(s * ny * nx + q * nx + p) << 32) | t);
+                    }
+        }
+    }
+    r->tests_l1 = tests;
+    int64_t n1 = c->n1;
+    int64_t *off = (int64_t *)xcalloc((size_t)n1 + 1, sizeof(int64_t));
+    int64_t total = 0;
+    for (int th = 0; th < nt; th++) {
+        for (int64_t q = 0; q < loc[th].n; q++) off[(loc[th].a[q] >> 32) + 1]++;
+        total += loc[th].n;
+    }
+    int maxc = 0;
+    for (int64_t v = 0; v < n1; v++) {
+        if (off[v + 1] > maxc) maxc = (int)off[v + 1];
+        off[v + 1] += off[v];
+    }
+    int32_t *lst = (int32_t *)xcalloc((size_t)total, sizeof(int32_t));
+    int64_t *fill = (int64_t *)xcalloc((size_t)n1, sizeof(int64_t));
+    for (int th = 0; th < nt; th++) {  /* threads in order keeps ids ascending */
+        for (int64_t q = 0; q < loc[th].n; q++) {
+            int64_t v = loc[th].a[q] >> 32;
+            lst[off[v] + fill[v]++] = (int32_t)(loc[th].a[q] & 0xffffffff);
+        }
+        free(loc[th].a);
+    }
+    free(loc);
+    free(fill);
+    r->list_off = off;
+    r->list_tri = lst;
+    r->pairs = total;
+    r->max_tris_per_voxel = maxc;
+}
+
+/* Level2 ray parity, literal (src/Object.cpp:1111-1207): candidates are the
+ * bins of the column at z' >= k1, each triangle counted once. */
+static void l2_parity_exact(const ctx_t *c, const orc_result *r, const float *ox,
+                            const float *oy, const float *oz, uint8_t *par)
+{
+    int n2 = c->g->n2;
+    int64_t n23 = (int64_t)n2 * n2 * n2, nx = c->nx, ny = c->ny, nz = c->nz;
+#pragma omp parallel num_threads(c->nt)
+    {
+        uint8_t *seen = (uint8_t *)xcalloc((size_t)c->T, 1);
+        vec64 touched = {0};
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t b = 0; b < r->nb; b++) {
+            int64_t l1 = r->bidx[b];
+            int64_t k1 = l1 / (nx * ny), ij = l1 - k1 * nx * ny, j1 = ij / nx, i1 = ij % nx;
+            for (int s = 0; s < n2; s++)
+                for (int q = 0; q < n2; q++)
+                    for (int p = 0; p < n2; p++) {
+                        float o[3] = {ox[i1 * n2 + p], oy[j1 * n2 + q], oz[k1 * n2 + s]};
+                        int hits = 0;
+                        for (int64_t z = k1; z < nz; z++) {
+                            int64_t v = z * nx * ny + ij;
+                            for (int64_t e = r->list_off[v]; e < r->list_off[v + 1]; e++) {
+                                int32_t t = r->list_tri[e];
+                                if (seen[t]) continue;
+                                const float *tv = c->tris + 9 * (int64_t)t;
+                                float tp;
+                                if (orc_tri_ray(tv, tv + 3, tv + 6, o, RAY_D, &tp)) {
+                                    hits++;
+                                    seen[t] = 1;
+                                    v64_push(&touched, t);
+                                }
+                            }
+                        }
+                        for (int64_t e = 0; e < touched.n; e++) seen[touched.a[e]] = 0;
+                        touched.n = 0;
+                        par[b * n23 + ((int64_t)s * n2 + q) * n2 + p] = (uint8_t)(hits % 2 == 1);
+                    }
+        }
+        free(seen);
+        free(touched.a);
+    }
+}
+
+/* Level2 ray parity, column factorised.  For column (i1, j1) let zmax(t) be
+ * the highest bin holding t; t is a candidate of boundary voxel k1 iff
+ * zmax(t) >= k1, which is the deduplicated union of the literal loop. */
+static void l2_parity_fast(const ctx_t *c, const orc_result *r, const float *ox,
+                           const float *oy, const float *oz, uint8_t *par)
+{
+    int n2 = c->g->n2;
+    int64_t n23 = (int64_t)n2 * n2 * n2, nx = c->nx, ny = c->ny, nz = c->nz;
+    /* group boundary voxels by column: consecutive b with equal column after
+     * a sort by (column, z) */
+    int64_t nb = r->nb;
+    int64_t *order = (int64_t *)xcalloc((size_t)nb, sizeof(int64_t));
+    {
+        int64_t ncol = nx * ny;
+        int64_t *cc = (int64_t *)xcalloc((size_t)ncol + 1, sizeof(int64_t));
+        for (int64_t b = 0; b < nb; b++) cc[r->bidx[b] % ncol + 1]++;
+        for (int64_t q = 0; q < ncol; q++) cc[q + 1] += cc[q];
+        for (int64_t b = 0; b < nb; b++) order[cc[r->bidx[b] % ncol]++] = b;  /* z ascending */
+        free(cc);
+    }
+    int64_t *starts = (int64_t *)xcalloc((size_t)nb + 1, sizeof(int64_t));
+    int64_t ng = 0;
+    for (int64_t q = 0; q < nb; q++)
+        if (q == 0 || r->bidx[order[q]] % (nx * ny) != r->bidx[order[q - 1]] % (nx * ny))
+            starts[ng++] = q;
+    starts[ng] = nb;
+#pragma omp parallel num_threads(c->nt)
+    {
+        int32_t *zmax = (int32_t *)xcalloc((size_t)c->T, sizeof(int32_t));
+        for (int64_t t = 0; t < c->T; t++) zmax[t] = -1;
+        vec64 uniq = {0};
+        mt_col *hs = NULL;
+        int32_t *hz = NULL;
+        int64_t hcap = 0;
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t gi = 0; gi < ng; gi++) {
+            int64_t ij = r->bidx[order[starts[gi]]] % (nx * ny);
+            int64_t i1 = ij % nx, j1 = ij / nx;
+            for (int64_t z = 0; z < nz; z++) {
+                int64_t v = z * nx * ny + ij;
+                for (int64_t e = r->list_off[v]; e < r->list_off[v + 1]; e++) {
+                    int32_t t = r->list_tri[e];
+                    if (zmax[t] < 0) v64_push(&uniq, t);
+                    zmax[t] = (int32_t)z;
+                }
+            }
+            if (uniq.n > hcap) {
+                hcap = uniq.n;
+                hs = (mt_col *)realloc(hs, (size_t)hcap * sizeof(mt_col));
+                hz = (int32_t *)realloc(hz, (size_t)hcap * sizeof(int32_t));
+            }
+            for (int q = 0; q < n2; q++)
+                for (int p = 0; p < n2; p++) {
+                    int64_t nh = 0;
+                    for (int64_t e = 0; e < uniq.n; e++) {
+                        int32_t t = (int32_t)uniq.a[e];
+                        if (mt_xy(c->tris + 9 * (int64_t)t, ox[i1 * n2 + p], oy[j1 * n2 + q], &hs[nh]))
+                            hz[nh++] = zmax[t];
+                    }
+                    for (int64_t m = starts[gi]; m < starts[gi + 1]; m++) {
+                        int64_t b = order[m];
+                        int64_t k1 = r->bidx[b] / (nx * ny);
+                        for (int s = 0; s < n2; s++) {
+                            int pp = 0;
+                            for (int64_t h = 0; h < nh; h++)
+                                if (hz[h] >= k1) pp ^= mt_t(&hs[h], oz[k1 * n2 + s]);
+                            par[b * n23 + ((int64_t)s * n2 + q) * n2 + p] = (uint8_t)pp;
+                        }
+                    }
+                }
+            for (int64_t e = 0; e < uniq.n; e++) zmax[uniq.a[e]] = -1;
+            uniq.n = 0;
+        }
+        free(zmax);
+        free(uniq.a);
+        free(hs);
+        free(hz);
+    }
+    free(order);
+    free(starts);
+}
+
+/* Level2 SAT (src/Object.cpp:2344-2424): every triangle of the parent's bin
+ * against every Level2 voxel; a hit sets 2. */
+static int64_t l2_sat(const ctx_t *c, const orc_result *r, const float *sx, const float *sy,
+                      const float *sz, uint8_t *sat)
+{
+    int n2 = c->g->n2;
+    int64_t n23 = (int64_t)n2 * n2 * n2, nx = c->nx, ny = c->ny;
+    int64_t tests = 0;
+    const float *h = c->g->ext2;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(c->nt) reduction(+ : tests)
+    for (int64_t b = 0; b < r->nb; b++) {
+        int64_t l1 = r->bidx[b];
+        int64_t k1 = l1 / (nx * ny), ij = l1 - k1 * nx * ny, j1 = ij / nx, i1 = ij % nx;
+        tests += (r->list_off[l1 + 1] - r->list_off[l1]) * n23;
+        for (int64_t e = r->list_off[l1]; e < r->list_off[l1 + 1]; e++) {
+            const float *tv = c->tris + 9 * (int64_t)r->list_tri[e];
+            for (int p = 0; p < n2; p++)
+                for (int q = 0; q < n2; q++)
+                    for (int s = 0; s < n2; s++) {
+                        float ctr[3] = {sx[i1 * n2 + p], sy[j1 * n2 + q], sz[k1 * n2 + s]};
+                        if (orc_tri_box_overlap(ctr, h, tv))
+                            sat[b * n23 + ((int64_t)s * n2 + q) * n2 + p] = 1;
+                    }
+        }
+    }
+    return tests;
+}
+
+int orc_voxelize(const float *tris, int64_t T, const orc_grid *g, int mode, int threads,
+                 orc_result *r)
+{
+    memset(r, 0, sizeof(*r));
+    ctx_t c;
+    c.tris = tris;
+    c.T = T;
+    c.g = g;
+    c.nt = threads > 0 ? threads : omp_get_max_threads();
+    c.nx = g->n[0]; c.ny = g->n[1]; c.nz = g->n[2];
+    c.n1 = c.nx * c.ny * c.nz;
+    float *tab[6];
+    for (int q = 0; q < 3; q++) {
+        tab[q] = (float *)xcalloc((size_t)g->n[q], sizeof(float));
+        tab[3 + q] = (float *)xcalloc((size_t)g->n[q], sizeof(float));
+        orc_l1_centers(g, q, tab[q]);
+        orc_l1_ray_origins(g, q, tab[3 + q]);
+    }
+    c.cx = tab[0]; c.cy = tab[1]; c.cz = tab[2];
+    c.rx = tab[3]; c.ry = tab[4]; c.rz = tab[5];
+    r->n1 = c.n1;
+    r->n2 = g->n2;
+
+    double t0 = omp_get_wtime();
+    uint8_t *par = (uint8_t *)xcalloc((size_t)c.n1, 1);
+    if (mode == ORC_EXACT) l1_parity_exact(&c, par); else l1_parity_fast(&c, par);
+    double t1 = omp_get_wtime();
+    l1_sat(&c, r);
+    double t2 = omp_get_wtime();
+
+    r->l1 = (uint8_t *)xcalloc((size_t)c.n1, 1);
+    r->prefix = (int32_t *)xcalloc((size_t)c.n1, sizeof(int32_t));
+    int64_t nb = 0;
+    for (int64_t v = 0; v < c.n1; v++) {
+        uint8_t s = (r->list_off[v + 1] > r->list_off[v]) ? 2 : par[v];
+        r->l1[v] = s;
+        r->prefix[v] = (int32_t)nb;   /* src/Object.cpp:3270-3279 */
+        if (s == 2) nb++;
+        if (s == 1) r->l1_inside++;
+    }
+    free(par);
+    r->nb = nb;
+    r->l1_boundary = nb;
+    r->bidx = (int32_t *)xcalloc((size_t)nb, sizeof(int32_t));
+    nb = 0;
+    for (int64_t v = 0; v < c.n1; v++)
+        if (r->l1[v] == 2) r->bidx[nb++] = (int32_t)v;
+    double t3 = omp_get_wtime();
+    double t4 = t3, t5 = t3;
+
+    if (g->n2 > 0) {
+        int n2 = g->n2;
+        int64_t n23 = (int64_t)n2 * n2 * n2;
+        float *l2t[6];
+        for (int q = 0; q < 3; q++) {
+            l2t[q] = (float *)xcalloc((size_t)g->n[q] * n2, sizeof(float));
+            l2t[3 + q] = (float *)xcalloc((size_t)g->n[q] * n2, sizeof(float));
+            orc_l2_centers(g, q, l2t[q]);
+            orc_l2_ray_origins(g, q, l2t[3 + q]);
+        }
+        uint8_t *p2 = (uint8_t *)xcalloc((size_t)(r->nb * n23), 1);
+        if (mode == ORC_EXACT) l2_parity_exact(&c, r, l2t[3], l2t[4], l2t[5], p2);
+        else l2_parity_fast(&c, r, l2t[3], l2t[4], l2t[5], p2);
+        t4 = omp_get_wtime();
+        uint8_t *s2 = (uint8_t *)xcalloc((size_t)(r->nb * n23), 1);
+        r->tests_l2 = l2_sat(&c, r, l2t[0], l2t[1], l2t[2], s2);
+        t5 = omp_get_wtime();
+        r->l2 = (uint8_t *)xcalloc((size_t)(r->nb * n23), 1);
+        for (int64_t q = 0; q < r->nb * n23; q++) {
+            uint8_t s = s2[q] ? 2 : p2[q];
+            r->l2[q] = s;
+            if (s == 1) r->l2_inside++;     /* int(v) % 2 == 1, src/Object.cpp:3366 */
+            if (s == 2) r->l2_boundary++;
+        }
+        free(p2);
+        free(s2);
+        for (int q = 0; q < 6; q++) free(l2t[q]);
+    }
+    r->t_stage[0] = t1 - t0;
+    r->t_stage[1] = t2 - t1;
+    r->t_stage[2] = t3 - t2;
+    r->t_stage[3] = t4 - t3;
+    r->t_stage[4] = t5 - t4;
+    for (int q = 0; q < 6; q++) free(tab[q]);
+    return 0;
+}
+
+void orc_result_free(orc_result *r)
+{
+    free(r->l1);
+    free(r->prefix);
+    free(r->bidx);
+    free(r->l2);
+    free(r->list_off);
+    free(r->list_tri);
+    memset(r, 0, sizeof(*r));
+}
+
+/* src/Object.cpp:2934-3075 (in/out, prefix and config files; states are
+ * written as uint8(state * 127.0f)). */
+int orc_save_raw(const orc_grid *g, const orc_result *r, int obj_id, const char *dir)
+{
+    char path[4096];
+    FILE *f;
+    snprintf(path, sizeof path, "%s/Obj%dVoxelConfig.txt", dir, obj_id);
+    if (!(f = fopen(path, "w"))) return 1;
+    fprintf(f, "Obj%d\n", obj_id);
+    fprintf(f, "%g\t%g\t%g\n", g->bmin[0], g->bmin[1], g->bmin[2]);
+    fprintf(f, "%g\t%g\t%g\n", g->bmax[0], g->bmax[1], g->bmax[2]);
+    fprintf(f, "%d\t%d\t%d\n", g->n[0], g->n[1], g->n[2]);
+    fprintf(f, "%g\t%g\t%g\n", g->g[0], g->g[1], g->g[2]);
+    fprintf(f, "%lld\n%lld\n", (long long)r->l1_inside, (long long)r->l1_boundary);
+    if (g->n2 > 0) {
+        fprintf(f, "%d\t%d\t%d\n", g->n2, g->n2, g->n2);
+        fprintf(f, "%g\t%g\t%g\n", g->g2[0], g->g2[1], g->g2[2]);
+        fprintf(f, "%lld\n%lld\n", (long long)r->l2_inside, (long long)r->l2_boundary);
+    }
+    fclose(f);
+    uint8_t *buf = (uint8_t *)xcalloc((size_t)r->n1, 1);
+    for (int64_t v = 0; v < r->n1; v++) buf[v] = (uint8_t)((float)r->l1[v] * 127.0f);
+    snprintf(path, sizeof path, "%s/Obj%dLevel1InOut.raw", dir, obj_id);
+    if (!(f = fopen(path, "wb"))) { free(buf); return 1; }
+    fwrite(buf, 1, (size_t)r->n1, f);
+    fclose(f);
+    free(buf);
+    if (g->n2 > 0) {
+        snprintf(path, sizeof path, "%s/Obj%dLevel1BoundaryPrefixSum.raw", dir, obj_id);
+        if (!(f = fopen(path, "wb"))) return 1;
+        fwrite(r->prefix, sizeof(int32_t), (size_t)r->n1, f);
+        fclose(f);
+        int64_t n = r->nb * (int64_t)g->n2 * g->n2 * g->n2;
+        buf = (uint8_t *)xcalloc((size_t)n, 1);
+        for (int64_t v = 0; v < n; v++) buf[v] = (uint8_t)((float)r->l2[v] * 127.0f);
+        snprintf(path, sizeof path, "%s/Obj%dLevel2InOut.raw", dir, obj_id);
+        if (!(f = fopen(path, "wb"))) { free(buf); return 1; }
+        fwrite(buf, 1, (size_t)n, f);
+        fclose(f);
+        free(buf);
+    }
+    return 0;
+}

@@ -1,0 +1,113 @@
+/*
+ * gpv_oracle.h -- CPU restatement of GPView's two-level hybrid voxelizer.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against; the product (libgpv) never links, loads or calls it.
+ *
+ * Every function restates the reference's CPU composition (the path
+ * PerformVoxelization takes with the CUDA classifier compiled out and ray
+ * parity in place of the GL stencil fill):
+ *
+ *   bbox + offset         src/Object.cpp:441-450, 572-583 (OFF: 257-296)
+ *   grid derivation       src/Object.cpp:3094-3136, includes/Utilities.h:313-316
+ *   Level1 ray parity     src/Object.cpp:716-779  + src/TriRayIntersection.cpp:78-131
+ *   Level1 SAT + bins     src/Object.cpp:2256-2342, 2681-2748 + src/TriBoxIntersection.cpp:64-210
+ *   compaction / prefix   src/Object.cpp:3258-3284
+ *   Level2 ray parity     src/Object.cpp:1111-1207
+ *   Level2 SAT            src/Object.cpp:2344-2424
+ *   counts                src/Object.cpp:3353-3378
+ *   .raw writer           src/Object.cpp:2934-3075
+ *
+ * Built with gcc -O2 -ffp-contract=off (x86-64 SSE2 float semantics, no
+ * contraction), matching the reference's Release flags.
+ */
+#ifndef GPV_ORACLE_H
+#define GPV_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_grid {
+    float bmin[3];        /* padded object bbox (Object::bBoxMin) */
+    float bmax[3];        /* padded object bbox (Object::bBoxMax) */
+    float max_model_size; /* Object::maxModelSize */
+    int n[3];             /* Level1 divisions numDivX/Y/Z */
+    float g[3];           /* Level1 grid size gridSizeX/Y/Z */
+    float ext1[3];        /* Level1 half extents (float)(g/2.0) */
+    int n2;               /* Level2 divisions per axis, 0 = Level2 off */
+    float g2[3];          /* Level2 grid size gridSizeX2/Y2/Z2 */
+    float ext2[3];        /* Level2 half extents (float)(g2/2.0) */
+} orc_grid;
+
+typedef struct orc_result {
+    int64_t n1;           /* nx*ny*nz */
+    uint8_t *l1;          /* [n1] state 0 outside, 1 inside, 2 boundary; [z][y][x] */
+    int32_t *prefix;      /* [n1] exclusive boundary prefix sum */
+    int64_t nb;           /* number of boundary Level1 voxels */
+    int32_t *bidx;        /* [nb] boundary linear indices, ascending */
+    int n2;
+    uint8_t *l2;          /* [nb * n2^3] state per Level2 voxel, [b][r][q][p] */
+    int64_t *list_off;    /* [n1+1] CSR offsets of the Level1 SAT bins */
+    int32_t *list_tri;    /* [pairs] triangle ids per bin, ascending */
+    int64_t pairs;
+    int64_t tests_l1;     /* sum over triangles of clipped index-AABB volume */
+    int64_t tests_l2;     /* sum over boundary voxels of |bin| * n2^3 */
+    int64_t l1_inside, l1_boundary, l2_inside, l2_boundary;
+    int max_tris_per_voxel;
+    double t_stage[5];    /* seconds: L1 parity, L1 SAT, compaction, L2 parity, L2 SAT */
+} orc_result;
+
+enum { ORC_EXACT = 0, ORC_FAST = 1 };
+
+/* Reference geometric predicates, restated. */
+int orc_tri_box_overlap(const float center[3], const float half[3], const float tv[9]);
+int orc_tri_ray(const float v1[3], const float v2[3], const float v3[3],
+                const float o[3], const float d[3], float *t_out);
+
+/* Batched forms for tests: n independent cases. */
+void orc_tri_box_batch(int64_t n, const float *c, const float *h, const float *tv, uint8_t *res);
+void orc_tri_ray_batch(int64_t n, const float *tv, const float *o, const float *d,
+                       uint8_t *hit, float *t);
+/* Column-factorised +z ray test (det/u/v on O.x,O.y; t on O.z), batched. */
+void orc_tri_zray_split_batch(int64_t n, const float *tv, const float *o, uint8_t *hit);
+
+/* bbox over a vertex array (stride 3 floats) with the 0.1% pad. */
+void orc_bbox(const float *verts, int64_t nverts, float bmin[3], float bmax[3],
+              float *max_model_size);
+/* Grid derivation; voxel_count2 <= 0 switches Level2 off. */
+void orc_grid_setup(const float bmin[3], const float bmax[3], float max_model_size,
+                    int voxel_count, int voxel_count2, orc_grid *g);
+
+/* Per-axis center tables. */
+void orc_l1_centers(const orc_grid *g, int axis, float *out);      /* [n]  SAT centers   */
+void orc_l1_ray_origins(const orc_grid *g, int axis, float *out);  /* [n]  ray origins   */
+void orc_l2_centers(const orc_grid *g, int axis, float *out);      /* [n*n2] SAT centers */
+void orc_l2_ray_origins(const orc_grid *g, int axis, float *out);  /* [n*n2] ray origins */
+
+/* Integer index range of one vertex coordinate (src/Object.cpp:2272-2300). */
+int orc_vertex_index(float v, float mn, float mx, int n);
+
+/* Conservative column range for the Level1 parity fill: every (ix, iy)
+ * column whose ray the Moller-Trumbore test can accept lies inside
+ * [lo[0],hi[0]] x [lo[1],hi[1]].  Returns 0 when the triangle is rejected by
+ * the det test for every ray (then no column can count it). */
+int orc_ray_column_bounds(const float tv[9], const float bmin[3], const float bmax[3],
+                          double safety, double lo[2], double hi[2]);
+
+/* Full pipeline.  mode ORC_EXACT runs the literal all-triangle parity loops;
+ * ORC_FAST runs the result-identical column factorisation. threads<=0: all. */
+int orc_voxelize(const float *tris, int64_t ntris, const orc_grid *g, int mode,
+                 int threads, orc_result *out);
+void orc_result_free(orc_result *r);
+
+/* Writes Obj<obj_id>{VoxelConfig.txt,Level1InOut.raw,Level1BoundaryPrefixSum.raw,
+ * Level2InOut.raw} under dir.  Returns 0 on success. */
+int orc_save_raw(const orc_grid *g, const orc_result *r, int obj_id, const char *dir);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,209 @@
+"""ctypes view of the CPU oracle (oracle/liborc.so) and of the reference's own
+predicates compiled into oracle/_ref/libgpvref.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker, and by bench.py's cpu_baseline leg.  The product never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORC_EXACT, ORC_FAST = 0, 1
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class Grid(ctypes.Structure):
+    """Mirror of orc_grid (gpv_oracle.h)."""
+    _fields_ = [
+        ("bmin", ctypes.c_float * 3),
+        ("bmax", ctypes.c_float * 3),
+        ("max_model_size", ctypes.c_float),
+        ("n", ctypes.c_int * 3),
+        ("g", ctypes.c_float * 3),
+        ("ext1", ctypes.c_float * 3),
+        ("n2", ctypes.c_int),
+        ("g2", ctypes.c_float * 3),
+        ("ext2", ctypes.c_float * 3),
+    ]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("n1", ctypes.c_int64),
+        ("l1", ctypes.POINTER(ctypes.c_uint8)),
+        ("prefix", ctypes.POINTER(ctypes.c_int32)),
+        ("nb", ctypes.c_int64),
+        ("bidx", ctypes.POINTER(ctypes.c_int32)),
+        ("n2", ctypes.c_int),
+        ("l2", ctypes.POINTER(ctypes.c_uint8)),
+        ("list_off", ctypes.POINTER(ctypes.c_int64)),
+        ("list_tri", ctypes.POINTER(ctypes.c_int32)),
+        ("pairs", ctypes.c_int64),
+        ("tests_l1", ctypes.c_int64),
+        ("tests_l2", ctypes.c_int64),
+        ("l1_inside", ctypes.c_int64),
+        ("l1_boundary", ctypes.c_int64),
+        ("l2_inside", ctypes.c_int64),
+        ("l2_boundary", ctypes.c_int64),
+        ("max_tris_per_voxel", ctypes.c_int),
+        ("t_stage", ctypes.c_double * 5),
+    ]
+
+
+_lib = None
+_ref = None
+
+
+def build():
+    """Compile the oracle (and _ref when the reference tree is mounted)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liborc.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_tri_box_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _f32p, _u8p]
+        L.orc_tri_ray_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _f32p, _u8p, _f32p]
+        L.orc_tri_zray_split_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _u8p]
+        L.orc_bbox.argtypes = [_f32p, ctypes.c_int64, _f32p, _f32p, ctypes.POINTER(ctypes.c_float)]
+        L.orc_grid_setup.argtypes = [_f32p, _f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(Grid)]
+        for fn in ("orc_l1_centers", "orc_l1_ray_origins", "orc_l2_centers", "orc_l2_ray_origins"):
+            getattr(L, fn).argtypes = [ctypes.POINTER(Grid), ctypes.c_int, _f32p]
+        L.orc_vertex_index.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
+        L.orc_vertex_index.restype = ctypes.c_int
+        L.orc_ray_column_bounds.argtypes = [_f32p, _f32p, _f32p, ctypes.c_double,
+                                            np.ctypeslib.ndpointer(dtype=np.float64),
+                                            np.ctypeslib.ndpointer(dtype=np.float64)]
+        L.orc_ray_column_bounds.restype = ctypes.c_int
+        L.orc_voxelize.argtypes = [_f32p, ctypes.c_int64, ctypes.POINTER(Grid), ctypes.c_int,
+                                   ctypes.c_int, ctypes.POINTER(_Result)]
+        L.orc_result_free.argtypes = [ctypes.POINTER(_Result)]
+        L.orc_save_raw.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(_Result), ctypes.c_int,
+                                   ctypes.c_char_p]
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference's own TriBoxOverlap / triangle_ray_intersection, or None
+    when oracle/_ref was not built (no reference tree on this machine)."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libgpvref.so")
+        if not os.path.exists(path):
+            return None
+        R = ctypes.CDLL(path)
+        R.ref_tri_box_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _f32p, _u8p]
+        R.ref_tri_ray_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _f32p, _u8p, _f32p]
+        _ref = R
+    return _ref
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def tri_box(c, h, tv, use_ref=False):
+    c, h, tv = _f32(c).reshape(-1, 3), _f32(h).reshape(-1, 3), _f32(tv).reshape(-1, 9)
+    out = np.zeros(len(c), np.uint8)
+    fn = ref().ref_tri_box_batch if use_ref else lib().orc_tri_box_batch
+    fn(len(c), c.ravel(), h.ravel(), tv.ravel(), out)
+    return out
+
+
+def tri_ray(tv, o, d, use_ref=False):
+    tv, o, d = _f32(tv).reshape(-1, 9), _f32(o).reshape(-1, 3), _f32(d).reshape(-1, 3)
+    hit = np.zeros(len(tv), np.uint8)
+    t = np.zeros(len(tv), np.float32)
+    fn = ref().ref_tri_ray_batch if use_ref else lib().orc_tri_ray_batch
+    fn(len(tv), tv.ravel(), o.ravel(), d.ravel(), hit, t)
+    return hit, t
+
+
+def tri_zray_split(tv, o):
+    tv, o = _f32(tv).reshape(-1, 9), _f32(o).reshape(-1, 3)
+    hit = np.zeros(len(tv), np.uint8)
+    lib().orc_tri_zray_split_batch(len(tv), tv.ravel(), o.ravel(), hit)
+    return hit
+
+
+def bbox(verts):
+    v = _f32(verts).reshape(-1, 3)
+    bmin = np.zeros(3, np.float32)
+    bmax = np.zeros(3, np.float32)
+    mms = ctypes.c_float()
+    lib().orc_bbox(v.ravel(), len(v), bmin, bmax, ctypes.byref(mms))
+    return bmin, bmax, np.float32(mms.value)
+
+
+def grid_setup(bmin, bmax, mms, voxel_count, voxel_count2):
+    g = Grid()
+    lib().orc_grid_setup(_f32(bmin), _f32(bmax), float(mms), int(voxel_count), int(voxel_count2),
+                         ctypes.byref(g))
+    return g
+
+
+def grid_for(tris, voxel_count, voxel_count2):
+    """bbox over the soup's vertices (every vertex is referenced) + grid."""
+    bmin, bmax, mms = bbox(np.asarray(tris, np.float32).reshape(-1, 3))
+    return grid_setup(bmin, bmax, mms, voxel_count, voxel_count2)
+
+
+def tables(g):
+    out = {}
+    for name, fn, per in (("c1", "orc_l1_centers", 1), ("r1", "orc_l1_ray_origins", 1),
+                          ("c2", "orc_l2_centers", max(g.n2, 1)),
+                          ("r2", "orc_l2_ray_origins", max(g.n2, 1))):
+        if per > 1 or name in ("c1", "r1") or g.n2 > 0:
+            arrs = []
+            for a in range(3):
+                buf = np.zeros(g.n[a] * per, np.float32)
+                getattr(lib(), fn)(ctypes.byref(g), a, buf)
+                arrs.append(buf)
+            out[name] = arrs
+    return out
+
+
+def column_bounds(tv, bmin, bmax, safety=1.0):
+    lo = np.zeros(2, np.float64)
+    hi = np.zeros(2, np.float64)
+    ok = lib().orc_ray_column_bounds(_f32(tv).ravel(), _f32(bmin), _f32(bmax), float(safety), lo, hi)
+    return bool(ok), lo, hi
+
+
+def voxelize(tris, g, mode=ORC_FAST, threads=0):
+    """Run the oracle pipeline; returns a dict of numpy arrays and counters."""
+    tris = _f32(tris).reshape(-1)
+    T = len(tris) // 9
+    r = _Result()
+    lib().orc_voxelize(tris, T, ctypes.byref(g), int(mode), int(threads), ctypes.byref(r))
+    n23 = g.n2 ** 3 if g.n2 > 0 else 0
+    try:
+        out = {
+            "l1": np.ctypeslib.as_array(r.l1, (r.n1,)).copy(),
+            "prefix": np.ctypeslib.as_array(r.prefix, (r.n1,)).copy(),
+            "bidx": np.ctypeslib.as_array(r.bidx, (r.nb,)).copy() if r.nb else np.zeros(0, np.int32),
+            "l2": (np.ctypeslib.as_array(r.l2, (r.nb * n23,)).copy()
+                   if (r.nb and n23) else np.zeros(0, np.uint8)),
+            "list_off": np.ctypeslib.as_array(r.list_off, (r.n1 + 1,)).copy(),
+            "list_tri": (np.ctypeslib.as_array(r.list_tri, (r.pairs,)).copy()
+                         if r.pairs else np.zeros(0, np.int32)),
+            "nb": r.nb, "pairs": r.pairs, "tests_l1": r.tests_l1, "tests_l2": r.tests_l2,
+            "l1_inside": r.l1_inside, "l1_boundary": r.l1_boundary,
+            "l2_inside": r.l2_inside, "l2_boundary": r.l2_boundary,
+            "max_tris_per_voxel": r.max_tris_per_voxel,
+            "t_stage": list(r.t_stage),
+        }
+    finally:
+        lib().orc_result_free(ctypes.byref(r))
+    return out
