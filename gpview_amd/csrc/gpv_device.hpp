@@ -36,6 +36,9 @@ __device__ __forceinline__ bool sat_separated(float pa, float pb, float rad)
     return (mn > rad) | (mx < -rad);
 }
 
+// kAxes = false skips the three box-axis predicates; only for callers that
+// have already established them with sat_axis_ok on the same center.
+template <bool kAxes = true>
 __device__ __forceinline__ bool tri_box_overlap(float cx, float cy, float cz,
                                                 float hx, float hy, float hz,
                                                 const float* __restrict__ tv)
@@ -45,7 +48,7 @@ __device__ __forceinline__ bool tri_box_overlap(float cx, float cy, float cz,
     const float v2x = tv[6] - cx, v2y = tv[7] - cy, v2z = tv[8] - cz;
 
     // box axes (FINDMINMAX, src/TriBoxIntersection.cpp:191-200)
-    {
+    if (kAxes) {
         float mn = v0x, mx = v0x;
         if (v1x < mn) mn = v1x;
         if (v1x > mx) mx = v1x;
@@ -53,7 +56,7 @@ __device__ __forceinline__ bool tri_box_overlap(float cx, float cy, float cz,
         if (v2x > mx) mx = v2x;
         if ((mn > hx) | (mx < -hx)) return false;
     }
-    {
+    if (kAxes) {
         float mn = v0y, mx = v0y;
         if (v1y < mn) mn = v1y;
         if (v1y > mx) mx = v1y;
@@ -61,7 +64,7 @@ __device__ __forceinline__ bool tri_box_overlap(float cx, float cy, float cz,
         if (v2y > mx) mx = v2y;
         if ((mn > hy) | (mx < -hy)) return false;
     }
-    {
+    if (kAxes) {
         float mn = v0z, mx = v0z;
         if (v1z < mn) mn = v1z;
         if (v1z > mx) mx = v1z;

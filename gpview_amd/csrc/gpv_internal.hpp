@@ -77,9 +77,8 @@ void launch_count_keys_hi(const uint64_t* key, uint64_t n,
 void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n,
                           const GridDev& g, const uint32_t* off, uint32_t* cnt,
                          float4* rec, hipStream_t s);
-void launch_scatter_cols(const uint64_t* key, const uint32_t* zmax, uint64_t n,
-                          const uint32_t* off, uint32_t* cnt, uint32_t* ct,
-                         uint32_t* cz, hipStream_t s);
+void launch_scatter_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n,
+                         const uint32_t* off, uint32_t* cnt, float4* rec, hipStream_t s);
 
 void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g, uint64_t* ins,
                       hipStream_t s);
@@ -96,9 +95,8 @@ void launch_pair_scatter(const uint64_t* pairs, const uint32_t* pb, uint64_t n,
 
 void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff,
                    const uint32_t* list, const GridDev& g, uint64_t* sat2, hipStream_t s);
-void launch_l2_parity(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* coff,
-                      const uint32_t* ct, const uint32_t* cz, const GridDev& g, uint64_t* par2,
-                      hipStream_t s);
+void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec, const GridDev& g,
+                      uint64_t* par2, hipStream_t s);
 
 void launch_count_states(const uint64_t* bnd, const uint64_t* ins, int64_t W,
                          const uint64_t* sat2, const uint64_t* par2, int64_t W2, Counters* ctr,

@@ -16,6 +16,7 @@
 // with integer OR / ballot only, so results are deterministic.
 #include "gpv_device.hpp"
 #include "gpv_internal.hpp"
+#include "gpv_slab.hpp"
 
 namespace gpv {
 
@@ -60,6 +61,24 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x)
 {
     const uint64_t s = wave_incl_scan<uint64_t>(x);
     return shfl64(s, 63);
+}
+
+// Block-wide sum folded into *dst with one atomic per block (blockDim <= 1024,
+// every thread of the block must call it).  Same-address atomics serialise in
+// L2, so kernels reduce per block before touching a shared counter.
+__device__ __forceinline__ void block_add_u64(uint64_t x, unsigned long long* dst)
+{
+    __shared__ uint64_t s_part[16];
+    x = wave_sum_u64(x);
+    const int wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (__lane_id() == 0) s_part[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) t += s_part[i];
+        if (t) atomicAdd(dst, (unsigned long long)t);
+    }
 }
 
 // Wave-aggregated append: one atomic per wave per call, lanes write at
@@ -205,9 +224,8 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
                                                    uint32_t* __restrict__ ray_items,
                                                    Counters* __restrict__ ctr)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t tests = 0, est = 0;
-    if (t < T) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x) {
         float tv[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) tv[c] = tris[t * 9 + c];
@@ -233,8 +251,8 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
         box1[t] = make_int4(dd[0], dd[1], dd[2], 0);
         const bool any = dd[0] > 0 && dd[1] > 0 && dd[2] > 0;
         sat_items[t] = any ? (uint32_t)(dd[0] * dd[1]) : 0u;
-        tests = any ? (uint64_t)dd[0] * dd[1] * dd[2] : 0;
-        est = any ? (uint64_t)dd[0] * dd[1] + (uint64_t)dd[1] * dd[2] + (uint64_t)dd[0] * dd[2] : 0;
+        tests += any ? (uint64_t)dd[0] * dd[1] * dd[2] : 0;
+        est += any ? (uint64_t)dd[0] * dd[1] + (uint64_t)dd[1] * dd[2] + (uint64_t)dd[0] * dd[2] : 0;
         uint32_t rit = 0;
         int4 rr = make_int4(0, 0, 0, 0);
         if (vis) {
@@ -254,12 +272,8 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
         ray_rng[t] = rr;
         ray_items[t] = rit;
     }
-    tests = wave_sum_u64(tests);
-    est = wave_sum_u64(est);
-    if (__lane_id() == 0) {
-        if (tests) atomicAdd(&ctr->tests_l1, (unsigned long long)tests);
-        if (est) atomicAdd(&ctr->pair_est, (unsigned long long)est);
-    }
+    block_add_u64(tests, &ctr->tests_l1);
+    block_add_u64(est, &ctr->pair_est);
 }
 
 void launch_tri_setup(const float* tris, int64_t T, const GridDev& g, int4* box0, int4* box1,
@@ -267,8 +281,8 @@ void launch_tri_setup(const float* tris, int64_t T, const GridDev& g, int4* box0
                       hipStream_t s)
 {
     if (T <= 0) return;
-    k_tri_setup<<<(unsigned)((T + 255) / 256), 256, 0, s>>>(tris, T, g, box0, box1, sat_items, ray_rng,
-                                                            ray_items, ctr);
+    k_tri_setup<<<grid_for(T, 256, 1024), 256, 0, s>>>(tris, T, g, box0, box1, sat_items, ray_rng, ray_items,
+                                                       ctr);
 }
 
 // ---------------------------------------------------------------------------
@@ -403,24 +417,110 @@ void launch_scan_u32_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t*
 // triangles).
 // ---------------------------------------------------------------------------
 constexpr int XB = 256;
-constexpr int XCAP = 512;
+constexpr int XCAP = 384;
 
-template <typename F>
+// 64-ary search by one wave: largest t in [lo, hi) with off[t] <= x, given
+// off[lo] <= x and (hi == n or off[hi] > x).  ~log64 rounds of dependent
+// loads instead of log2.
+__device__ __forceinline__ int64_t wave_find_segment(const uint64_t* __restrict__ off, int64_t lo, int64_t hi,
+                                                     uint64_t x)
+{
+    const int lane = __lane_id();
+    while (hi - lo > 1) {
+        const int64_t step = (hi - lo + 63) / 64;
+        const int64_t idx = lo + (int64_t)lane * step;
+        const bool le = idx < hi && off[idx] <= x;
+        const uint64_t m = __ballot(le);
+        const int L = 63 - __clzll((long long)m);
+        const int64_t nlo = lo + (int64_t)L * step;
+        hi = nlo + step < hi ? nlo + step : hi;
+        lo = nlo;
+    }
+    return lo;
+}
+
+// Block-local append buffer in LDS: wave-aggregated LDS atomics, one global
+// reservation per flush.  Entries that do not fit spill straight to the
+// global buffer with a wave-aggregated global atomic.
+struct LdsAppend {
+    uint64_t* key;
+    uint32_t* val;
+    uint32_t* count;
+    uint32_t cap;
+};
+
+__device__ __forceinline__ void lds_append(const LdsAppend& L, const Append64& G, bool pred, uint64_t key,
+                                           uint32_t val)
+{
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(L.count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    const uint32_t idx = base + (uint32_t)__popcll(m & lanes_below_mask());
+    if (pred && idx < L.cap) {
+        L.key[idx] = key;
+        if (L.val) L.val[idx] = val;
+    }
+    wave_append(G, pred && idx >= L.cap, key, val);
+}
+
+// block-synchronous: every thread of the block calls it
+__device__ __forceinline__ void lds_flush(const LdsAppend& L, const Append64& G, uint64_t* s_gbase)
+{
+    __syncthreads();
+    const uint32_t n = *L.count < L.cap ? *L.count : L.cap;
+    if (threadIdx.x == 0 && n) *s_gbase = atomicAdd(G.count, (unsigned long long)n);
+    __syncthreads();
+    if (n) {
+        const uint64_t gb = *s_gbase;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint64_t idx = gb + i;
+            if (idx < G.cap) {
+                G.key[idx] = L.key[i];
+                if (G.val) G.val[idx] = L.val[i];
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *L.count = 0;
+    __syncthreads();
+}
+
+struct NoAfter {
+    __device__ void operator()(bool) const {}
+};
+
+// Each block owns a contiguous range of items, walked XB at a time, so the
+// triangle searches of consecutive iterations start where the last ended.
+// after(last) runs block-synchronously after every iteration.
+template <typename F, typename A>
 __device__ __forceinline__ void expand_items(const float* __restrict__ tris, int64_t T,
-                                             const uint64_t* __restrict__ off, F&& body)
+                                             const uint64_t* __restrict__ off, F&& body, A&& after)
 {
     __shared__ uint64_t s_off[XCAP + 1];
     __shared__ float s_tv[XCAP * 9];
     __shared__ int64_t s_t[2];
     const uint64_t total = off[T];
     const int tid = threadIdx.x;
-    for (uint64_t base = (uint64_t)blockIdx.x * XB; base < total; base += (uint64_t)gridDim.x * XB) {
-        const uint64_t last = (base + XB < total ? base + XB : total) - 1;
-        if (tid == 0) s_t[0] = find_segment(off, T, base);
-        if (tid == 64) s_t[1] = find_segment(off, T, last);
+    const uint64_t per = ((total + gridDim.x - 1) / gridDim.x + XB - 1) / XB * XB;
+    const uint64_t b0 = (uint64_t)blockIdx.x * per;
+    const uint64_t b1 = b0 + per < total ? b0 + per : total;
+    int64_t lo_hint = 0;
+    for (uint64_t base = b0; base < b1; base += XB) {
+        const uint64_t last = (base + XB < b1 ? base + XB : b1) - 1;
+        if (tid < 64) {
+            const int64_t r = wave_find_segment(off, lo_hint, T, base);
+            if (tid == 0) s_t[0] = r;
+        } else if (tid < 128) {
+            const int64_t r = wave_find_segment(off, lo_hint, T, last);
+            if (tid == 64) s_t[1] = r;
+        }
         __syncthreads();
         const int64_t t0 = s_t[0];
         const int64_t nt = s_t[1] - t0 + 1;
+        lo_hint = s_t[1];
         const bool staged = nt <= XCAP;
         if (staged) {
             for (int k = tid; k <= nt; k += XB) s_off[k] = off[t0 + k];
@@ -452,6 +552,7 @@ __device__ __forceinline__ void expand_items(const float* __restrict__ tris, int
             body(t, l, tv);
         }
         __syncthreads();
+        after(base + XB >= b1);
     }
 }
 
@@ -466,6 +567,16 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
                                                    unsigned long long* __restrict__ bnd, Append64 pairs,
                                                    Append64 cols)
 {
+    constexpr uint32_t PB = 1536, CB = 768;
+    __shared__ uint64_t s_pk[PB];
+    __shared__ uint64_t s_ck[CB];
+    __shared__ uint32_t s_cv[CB];
+    __shared__ uint32_t s_n[2];
+    __shared__ uint64_t s_gbase;
+    if (threadIdx.x < 2) s_n[threadIdx.x] = 0;
+    __syncthreads();
+    const LdsAppend lp{s_pk, nullptr, &s_n[0], PB};
+    const LdsAppend lc{s_ck, s_cv, &s_n[1], CB};
     expand_items(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
         const int4 b0 = box0[t];
         const int4 b1 = box1[t];
@@ -482,10 +593,17 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
                 zmax = z;
                 atomicOr(&bnd[w], (unsigned long long)bit);
             }
-            wave_append(pairs, hit, ((w * 64 + (uint64_t)(p & 63)) << 32) | (uint64_t)t, 0);
+            lds_append(lp, pairs, hit, ((w * 64 + (uint64_t)(p & 63)) << 32) | (uint64_t)t, 0);
         }
         const bool emit = (zmax >= 0) && (b0.w != 0);
-        wave_append(cols, emit, ((uint64_t)((uint32_t)q * g.nx + p) << 32) | (uint64_t)t, (uint32_t)zmax);
+        lds_append(lc, cols, emit, ((uint64_t)((uint32_t)q * g.nx + p) << 32) | (uint64_t)t, (uint32_t)zmax);
+    }, [&](bool last) {
+        // flush when half full (block-uniform: counters read after a barrier)
+        __syncthreads();
+        const bool fp = last || s_n[0] >= PB / 2;
+        const bool fc = last || s_n[1] >= CB / 2;
+        if (fp) lds_flush(lp, pairs, &s_gbase);
+        if (fc) lds_flush(lc, cols, &s_gbase);
     });
 }
 
@@ -506,6 +624,13 @@ __global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tri
                                                    const uint64_t* __restrict__ off, GridDev g,
                                                    Append64 hits)
 {
+    constexpr uint32_t HB = 1536;
+    __shared__ uint64_t s_hk[HB];
+    __shared__ uint32_t s_n;
+    __shared__ uint64_t s_gbase;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const LdsAppend lh{s_hk, nullptr, &s_n, HB};
     expand_items(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
         const int4 r = rng[t];
         const int i = r.x + (int)(l % (uint64_t)r.z);
@@ -514,7 +639,10 @@ __global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tri
         bool ok = mt_setup(tv, m);
         float tx, ty;
         ok = ok && mt_column(m, tv[0], tv[1], g.r1x[i], g.r1y[j], tx, ty);
-        wave_append(hits, ok, ((uint64_t)((uint32_t)j * g.nx + i) << 32) | (uint64_t)t, 0);
+        lds_append(lh, hits, ok, ((uint64_t)((uint32_t)j * g.nx + i) << 32) | (uint64_t)t, 0);
+    }, [&](bool last) {
+        __syncthreads();
+        if (last || s_n >= HB / 2) lds_flush(lh, hits, &s_gbase);
     });
 }
 
@@ -574,26 +702,33 @@ void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n,
     k_scatter_hits<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(tris, key, n, g, off, cnt, rec);
 }
 
-__global__ void k_scatter_cols(const uint64_t* __restrict__ key, const uint32_t* __restrict__ zmax,
-                               uint64_t n, const uint32_t* __restrict__ off, uint32_t* __restrict__ cnt,
-                               uint32_t* __restrict__ ct, uint32_t* __restrict__ cz)
+// column entries -> CSR by column, each with the MT record its candidates need
+__global__ void k_scatter_cols(const float* __restrict__ tris, const uint64_t* __restrict__ key,
+                               const uint32_t* __restrict__ zmax, uint64_t n, const uint32_t* __restrict__ off,
+                               uint32_t* __restrict__ cnt, float4* __restrict__ rec)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t k = key[i];
         const uint32_t col = (uint32_t)(k >> 32);
+        const uint32_t t = (uint32_t)(k & 0xffffffffu);
         const uint32_t pos = off[col] + atomicSub(&cnt[col], 1u) - 1u;
-        ct[pos] = (uint32_t)(k & 0xffffffffu);
-        cz[pos] = zmax[i];
+        float tv[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) tv[c] = tris[(uint64_t)t * 9 + c];
+        MtTri m;
+        mt_setup(tv, m);   // entries exist only for triangles whose det passes
+        rec[3 * (uint64_t)pos + 0] = make_float4(m.e1x, m.e1y, m.e1z, m.e2x);
+        rec[3 * (uint64_t)pos + 1] = make_float4(m.e2y, m.e2z, m.inv_det, tv[0]);
+        rec[3 * (uint64_t)pos + 2] = make_float4(tv[1], tv[2], __int_as_float((int)zmax[i]), __uint_as_float(t));
     }
 }
 
-void launch_scatter_cols(const uint64_t* key, const uint32_t* zmax, uint64_t n,
-                          const uint32_t* off, uint32_t* cnt, uint32_t* ct, uint32_t* cz,
-                         hipStream_t s)
+void launch_scatter_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n,
+                         const uint32_t* off, uint32_t* cnt, float4* rec, hipStream_t s)
 {
     if (!n) return;
-    k_scatter_cols<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(key, zmax, n, off, cnt, ct, cz);
+    k_scatter_cols<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(tris, key, zmax, n, off, cnt, rec);
 }
 
 // ---------------------------------------------------------------------------
@@ -737,6 +872,32 @@ void launch_pair_scatter(const uint64_t* pairs, const uint32_t* pb, uint64_t n,
 // surviving sub-box over its lanes.  Bits accumulate in LDS.
 // ---------------------------------------------------------------------------
 constexpr int L2_WAVES = 4;
+constexpr int L2_QCAP = 128;   // ring of queued (triangle, sub-voxel) entries per wave
+
+// value of a float register in lane `l` (l uniform across the wave)
+__device__ __forceinline__ float lane_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Full float SAT for queued entries (one per lane); entry = p | q<<4 | r<<8 |
+// j<<12 | box<<18 where j is the staged triangle and box says the box-axis
+// predicates are already known to hold.
+__device__ __forceinline__ void l2_sat_run(uint32_t ent, bool valid, const float* stv, float cxl, float cyl,
+                                           float czl, int n2, const GridDev& g, unsigned long long* acc)
+{
+    const int p = ent & 15, q = (ent >> 4) & 15, r = (ent >> 8) & 15, j = (ent >> 12) & 63;
+    const bool box = (ent >> 18) & 1;
+    const float ccx = __shfl(cxl, p), ccy = __shfl(cyl, q), ccz = __shfl(czl, r);
+    const int s = (r * n2 + q) * n2 + p;
+    if (!valid || ((acc[s >> 6] >> (s & 63)) & 1ull)) return;
+    float tv[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) tv[c] = stv[j * 9 + c];
+    const bool hit = box ? tri_box_overlap<false>(ccx, ccy, ccz, g.h2x, g.h2y, g.h2z, tv)
+                         : tri_box_overlap<true>(ccx, ccy, ccz, g.h2x, g.h2y, g.h2z, tv);
+    if (hit) atomicOr(&acc[s >> 6], 1ull << (s & 63));
+}
 
 __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restrict__ tris,
                                                           const int32_t* __restrict__ bidx, int64_t B,
@@ -745,6 +906,10 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
                                                           uint64_t* __restrict__ sat2)
 {
     __shared__ unsigned long long s_acc[L2_WAVES][64];
+    __shared__ float s_tv[L2_WAVES][64 * 9];
+    __shared__ uint32_t s_mk[L2_WAVES][64 * 3];
+    __shared__ float s_sl[L2_WAVES][64 * 4];
+    __shared__ uint32_t s_q[L2_WAVES][L2_QCAP];
     const int wv = threadIdx.x >> 6;
     const int lane = __lane_id();
     const int64_t b = (int64_t)blockIdx.x * L2_WAVES + wv;
@@ -759,37 +924,112 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
     const int j1 = ij / g.nx;
     const int i1 = ij - j1 * g.nx;
     const int n2 = g.n2;
-    const float* cx = g.c2x + (int64_t)i1 * n2;
-    const float* cy = g.c2y + (int64_t)j1 * n2;
-    const float* cz = g.c2z + (int64_t)k1 * n2;
-    const float ax = lane < n2 ? cx[lane] : 0.0f;
-    const float ay = lane < n2 ? cy[lane] : 0.0f;
-    const float az = lane < n2 ? cz[lane] : 0.0f;
-    const uint32_t e1 = boff[b + 1];
-    for (uint32_t e = boff[b]; e < e1; ++e) {
-        const uint64_t t = list[e];
-        float tv[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) tv[c] = tris[t * 9 + c];
-        const uint64_t mx = __ballot(lane < n2 && sat_axis_ok(tv[0], tv[3], tv[6], ax, g.h2x));
-        const uint64_t my = __ballot(lane < n2 && sat_axis_ok(tv[1], tv[4], tv[7], ay, g.h2y));
-        const uint64_t mz = __ballot(lane < n2 && sat_axis_ok(tv[2], tv[5], tv[8], az, g.h2z));
-        if (!mx || !my || !mz) continue;
-        const int pa = __ffsll((unsigned long long)mx) - 1, pb = 63 - __clzll((long long)mx);
-        const int qa = __ffsll((unsigned long long)my) - 1, qb = 63 - __clzll((long long)my);
-        const int ra = __ffsll((unsigned long long)mz) - 1, rb = 63 - __clzll((long long)mz);
-        const int dp = pb - pa + 1, dq = qb - qa + 1, dr = rb - ra + 1;
-        const int V = dp * dq * dr;
-        for (int l = lane; l < V; l += 64) {
-            const int p = pa + l % dp;
-            const int rem = l / dp;
-            const int q = qa + rem % dq;
-            const int r = ra + rem / dq;
-            const int s = (r * n2 + q) * n2 + p;
-            if ((acc[s >> 6] >> (s & 63)) & 1ull) continue;   // already boundary
-            if (tri_box_overlap(cx[p], cy[q], cz[r], g.h2x, g.h2y, g.h2z, tv))
-                atomicOr(&acc[s >> 6], 1ull << (s & 63));
+    // the parent's n2 sub-centers per axis in registers, entry p in lane p
+    const float cxl = lane < n2 ? g.c2x[(int64_t)i1 * n2 + lane] : 0.0f;
+    const float cyl = lane < n2 ? g.c2y[(int64_t)j1 * n2 + lane] : 0.0f;
+    const float czl = lane < n2 ? g.c2z[(int64_t)k1 * n2 + lane] : 0.0f;
+    const double clo[3] = {lane_f(cxl, 0), lane_f(cyl, 0), lane_f(czl, 0)};
+    const double chi[3] = {lane_f(cxl, n2 - 1), lane_f(cyl, n2 - 1), lane_f(czl, n2 - 1)};
+    float* stv = s_tv[wv];
+    uint32_t* smk = s_mk[wv];
+    float* ssl = s_sl[wv];
+    uint32_t* sq = s_q[wv];
+    uint32_t qh = 0, qt = 0;   // ring head/tail (uniform)
+    const uint32_t e0 = boff[b], e1 = boff[b + 1];
+    for (uint32_t base = e0; base < e1; base += 64) {
+        // the queue refers to staged triangles by slot: drain it before restaging
+        while (qt != qh) {
+            const uint32_t n = qt - qh < 64 ? qt - qh : 64;
+            const bool valid = (uint32_t)lane < n;
+            const uint32_t ent = valid ? sq[(qh + lane) % L2_QCAP] : 0u;
+            l2_sat_run(ent, valid, stv, cxl, cyl, czl, n2, g, acc);
+            qh += n;
         }
+        __builtin_amdgcn_wave_barrier();
+        // lane-parallel staging: vertices, the passing sub-indices of the box-axis
+        // predicates, and a float plane slab outside of which the SAT fails
+        const uint32_t i = base + lane;
+        if (i < e1) {
+            const uint64_t t = list[i];
+            float tv[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                tv[c] = tris[t * 9 + c];
+                stv[lane * 9 + c] = tv[c];
+            }
+            uint32_t mx = 0, my = 0, mz = 0;
+            for (int p = 0; p < n2; ++p) {
+                mx |= (uint32_t)sat_axis_ok(tv[0], tv[3], tv[6], lane_f(cxl, p), g.h2x) << p;
+                my |= (uint32_t)sat_axis_ok(tv[1], tv[4], tv[7], lane_f(cyl, p), g.h2y) << p;
+                mz |= (uint32_t)sat_axis_ok(tv[2], tv[5], tv[8], lane_f(czl, p), g.h2z) << p;
+            }
+            smk[lane * 3 + 0] = mx;
+            smk[lane * 3 + 1] = my;
+            smk[lane * 3 + 2] = mz;
+            const PlaneSlab sl = plane_slab(tv, g.h2x, g.h2y, g.h2z, clo, chi);
+            // float evaluation of N.(c - t0): |error| <= 6U sum |N_q| max|c_q - t0_q|
+            double sa = 0.0;
+            sa += fabs(sl.nx) * fmax(fabs(clo[0] - tv[0]), fabs(chi[0] - tv[0]));
+            sa += fabs(sl.ny) * fmax(fabs(clo[1] - tv[1]), fabs(chi[1] - tv[1]));
+            sa += fabs(sl.nz) * fmax(fabs(clo[2] - tv[2]), fabs(chi[2] - tv[2]));
+            const double limd = sl.lim + 8.0 * 5.9604644775390625e-08 * sa;
+            float limf = (float)limd;
+            if ((double)limf < limd) limf = nextafterf(limf, INFINITY);
+            ssl[lane * 4 + 0] = (float)sl.nx;
+            ssl[lane * 4 + 1] = (float)sl.ny;
+            ssl[lane * 4 + 2] = (float)sl.nz;
+            ssl[lane * 4 + 3] = limf;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int nv = (int)(e1 - base < 64 ? e1 - base : 64);
+        for (int j = 0; j < nv; ++j) {
+            const uint32_t mx = smk[j * 3 + 0], my = smk[j * 3 + 1], mz = smk[j * 3 + 2];
+            if (!mx || !my || !mz) continue;
+            const int pa = __ffs(mx) - 1, pb = 31 - __clz(mx);
+            const int qa = __ffs(my) - 1, qb = 31 - __clz(my);
+            const int ra = __ffs(mz) - 1, rb = 31 - __clz(mz);
+            const int dp = pb - pa + 1, dq = qb - qa + 1, dr = rb - ra + 1;
+            // contiguous passing sets (centers are monotone in the index) mean the
+            // box-axis predicates hold on the whole sub-box
+            const uint32_t boxbit = (__popc(mx) == dp && __popc(my) == dq && __popc(mz) == dr) ? (1u << 18) : 0u;
+            const float nX = ssl[j * 4 + 0], nY = ssl[j * 4 + 1], nZ = ssl[j * 4 + 2], limf = ssl[j * 4 + 3];
+            const float t0x = stv[j * 9 + 0], t0y = stv[j * 9 + 1], t0z = stv[j * 9 + 2];
+            const int V = dp * dq * dr;
+            const float inv_dp = 1.0f / (float)dp, inv_dq = 1.0f / (float)dq;
+            for (int l0 = 0; l0 < V; l0 += 64) {
+                const int l = l0 + lane;
+                // exact small-integer division: (l + 0.5)/d is >= 1/32 from an integer
+                const int rem = (int)(((float)l + 0.5f) * inv_dp);
+                const int p = pa + l - rem * dp;
+                const int r3 = (int)(((float)rem + 0.5f) * inv_dq);
+                const int q = qa + rem - r3 * dq;
+                const int r = ra + r3;
+                const float ccx = __shfl(cxl, p & 63), ccy = __shfl(cyl, q & 63), ccz = __shfl(czl, r & 63);
+                const float d = nX * (ccx - t0x) + nY * (ccy - t0y) + nZ * (ccz - t0z);
+                const int s = (r * n2 + q) * n2 + p;
+                const bool keep = l < V && fabsf(d) <= limf && !((acc[(s & 4095) >> 6] >> (s & 63)) & 1ull);
+                const uint64_t m = __ballot(keep);
+                if (keep) {
+                    const uint32_t at = qt + (uint32_t)__popcll(m & lanes_below_mask());
+                    sq[at % L2_QCAP] = (uint32_t)p | ((uint32_t)q << 4) | ((uint32_t)r << 8) | ((uint32_t)j << 12) | boxbit;
+                }
+                qt += (uint32_t)__popcll(m);
+                __builtin_amdgcn_wave_barrier();
+                if (qt - qh >= 64) {   // a full wave of work
+                    const uint32_t ent = sq[(qh + lane) % L2_QCAP];
+                    l2_sat_run(ent, true, stv, cxl, cyl, czl, n2, g, acc);
+                    qh += 64;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+        }
+    }
+    while (qt != qh) {
+        const uint32_t n = qt - qh < 64 ? qt - qh : 64;
+        const bool valid = (uint32_t)lane < n;
+        const uint32_t ent = valid ? sq[(qh + lane) % L2_QCAP] : 0u;
+        l2_sat_run(ent, valid, stv, cxl, cyl, czl, n2, g, acc);
+        qh += n;
     }
     __builtin_amdgcn_wave_barrier();
     for (int i = lane; i < g.w2; i += 64) sat2[b * g.w2 + i] = acc[i];
@@ -806,22 +1046,53 @@ void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint
 // ---------------------------------------------------------------------------
 // Level2 ray parity (src/Object.cpp:1111-1207): one wave per boundary voxel
 // (i1, j1, k1).  Candidates are the column's (triangle, zmax) entries with
-// zmax >= k1 -- the deduplicated union of the bins at z' >= k1.  Lanes own
-// sub-columns (p, q): the det/u/v part runs once per sub-column, t per r.
+// zmax >= k1 -- the deduplicated union of the bins at z' >= k1 -- each with
+// a precomputed MT record (edges, 1/det, V1).  Per chunk of 64 candidates:
+// one coalesced load into LDS; the z-independent det/u/v test runs once per
+// (candidate, sub-column) and leaves a pass bit; t > 1e-6 then runs per
+// sub-voxel only for the passing pairs.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64 * L2_WAVES) void k_l2_parity(const float* __restrict__ tris,
-                                                             const int32_t* __restrict__ bidx, int64_t B,
-                                                             const uint32_t* __restrict__ coff,
-                                                             const uint32_t* __restrict__ ct,
-                                                             const uint32_t* __restrict__ cz, GridDev g,
-                                                             uint64_t* __restrict__ par2)
+struct ColRec {
+    float e1x, e1y, e1z, e2x, e2y, e2z, inv, v1x, v1y, v1z;
+    int zmax;
+};
+
+__device__ __forceinline__ ColRec load_colrec(const float4* __restrict__ rec, uint32_t i)
 {
-    __shared__ unsigned long long s_acc[L2_WAVES][64];
+    const float4 a = rec[3 * (uint64_t)i + 0];
+    const float4 b = rec[3 * (uint64_t)i + 1];
+    const float4 c = rec[3 * (uint64_t)i + 2];
+    ColRec r;
+    r.e1x = a.x; r.e1y = a.y; r.e1z = a.z; r.e2x = a.w;
+    r.e2y = b.x; r.e2z = b.y; r.inv = b.z; r.v1x = b.w;
+    r.v1y = c.x; r.v1z = c.y; r.zmax = __float_as_int(c.z);
+    return r;
+}
+
+__device__ __forceinline__ bool colrec_xy(const ColRec& r, float ox, float oy, float& tx, float& ty)
+{
+    MtTri m;
+    m.e1x = r.e1x; m.e1y = r.e1y; m.e1z = r.e1z;
+    m.e2x = r.e2x; m.e2y = r.e2y; m.e2z = r.e2z;
+    m.inv_det = r.inv;
+    return mt_column(m, r.v1x, r.v1y, ox, oy, tx, ty);
+}
+
+constexpr int L2P_WAVES = 4;
+
+__global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __restrict__ bidx, int64_t B,
+                                                              const uint32_t* __restrict__ coff,
+                                                              const float4* __restrict__ rec, GridDev g,
+                                                              uint64_t* __restrict__ par2)
+{
+    __shared__ unsigned long long s_acc[L2P_WAVES][64];
+    __shared__ ColRec s_rec[L2P_WAVES][64];
     const int wv = threadIdx.x >> 6;
     const int lane = __lane_id();
-    const int64_t b = (int64_t)blockIdx.x * L2_WAVES + wv;
+    const int64_t b = (int64_t)blockIdx.x * L2P_WAVES + wv;
     if (b >= B) return;
     unsigned long long* acc = s_acc[wv];
+    ColRec* cr = s_rec[wv];
     const int v = bidx[b];
     const int nxy = g.nx * g.ny;
     const int k1 = v / nxy;
@@ -834,29 +1105,57 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_parity(const float* __rest
     const float* rx = g.r2x + (int64_t)i1 * n2;
     const float* ry = g.r2y + (int64_t)j1 * n2;
     const float* rz = g.r2z + (int64_t)k1 * n2;
-    uint64_t bits = 0;   // bit (m*n2 + r) for sub-column lane + 64*m
-    const uint32_t col = (uint32_t)ij;
-    const uint32_t e1 = coff[col + 1];
-    for (uint32_t e = coff[col]; e < e1; ++e) {
-        if ((int)cz[e] < k1) continue;
-        const uint64_t t = ct[e];
-        float tv[9];
+    // this lane's sub-columns sc = lane + 64*mm: ray origins
+    float ox[4], oy[4];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) tv[c] = tris[t * 9 + c];
-        MtTri m;
-        if (!mt_setup(tv, m)) continue;
-        for (int mm = 0; mm < nm; ++mm) {
-            const int sc = lane + 64 * mm;
-            if (sc < nsc) {
-                const int p = sc % n2, q = sc / n2;
-                float tx, ty;
-                if (mt_column(m, tv[0], tv[1], rx[p], ry[q], tx, ty)) {
-                    for (int r = 0; r < n2; ++r)
-                        if (mt_t_hit(m.e1x, m.e1y, m.e1z, m.e2x, m.e2y, m.e2z, m.inv_det, tx, ty, tv[2], rz[r]))
-                            bits ^= 1ull << (mm * n2 + r);
+    for (int mm = 0; mm < 4; ++mm) {
+        const int sc = lane + 64 * mm;
+        ox[mm] = (mm < nm && sc < nsc) ? rx[sc % n2] : 0.0f;
+        oy[mm] = (mm < nm && sc < nsc) ? ry[sc / n2] : 0.0f;
+    }
+    float ozv[16];
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) ozv[rr] = rr < n2 ? rz[rr] : 0.0f;
+    uint64_t bits = 0;   // bit (mm*n2 + r) for sub-column lane + 64*mm
+    const uint32_t e0 = coff[ij], e1 = coff[ij + 1];
+    for (uint32_t base = e0; base < e1; base += 64) {
+        const uint32_t i = base + lane;
+        bool valid = false;
+        if (i < e1) {
+            const ColRec r = load_colrec(rec, i);
+            valid = r.zmax >= k1;
+            cr[lane] = r;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t vm = __ballot(valid);
+        // det/u/v per (candidate, sub-column): pass bits per sub-column slot
+        uint64_t pass[4] = {0, 0, 0, 0};
+        for (uint64_t m = vm; m; m &= m - 1) {
+            const int j = __ffsll((unsigned long long)m) - 1;
+            const ColRec r = cr[j];
+#pragma unroll
+            for (int mm = 0; mm < 4; ++mm) {
+                if (mm < nm && lane + 64 * mm < nsc) {
+                    float tx, ty;
+                    if (colrec_xy(r, ox[mm], oy[mm], tx, ty)) pass[mm] |= 1ull << j;
                 }
             }
         }
+        // t per sub-voxel for the passing pairs only
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) {
+            for (uint64_t m = pass[mm]; m; m &= m - 1) {
+                const int j = __ffsll((unsigned long long)m) - 1;
+                const ColRec r = cr[j];
+                const float tx = ox[mm] - r.v1x, ty = oy[mm] - r.v1y;
+#pragma unroll
+                for (int rr = 0; rr < 16; ++rr)
+                    if (rr < n2 &&
+                        mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z, ozv[rr]))
+                        bits ^= 1ull << (mm * n2 + rr);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
     }
     for (int i = lane; i < g.w2; i += 64) acc[i] = 0ull;
     __builtin_amdgcn_wave_barrier();
@@ -874,13 +1173,12 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_parity(const float* __rest
     for (int i = lane; i < g.w2; i += 64) par2[b * g.w2 + i] = acc[i];
 }
 
-void launch_l2_parity(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* coff,
-                      const uint32_t* ct, const uint32_t* cz, const GridDev& g, uint64_t* par2,
-                      hipStream_t s)
+void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec, const GridDev& g,
+                      uint64_t* par2, hipStream_t s)
 {
     if (B <= 0) return;
-    const int64_t blocks = (B + L2_WAVES - 1) / L2_WAVES;
-    k_l2_parity<<<(unsigned)blocks, 64 * L2_WAVES, 0, s>>>(tris, bidx, B, coff, ct, cz, g, par2);
+    const int64_t blocks = (B + L2P_WAVES - 1) / L2P_WAVES;
+    k_l2_parity<<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, g, par2);
 }
 
 // ---------------------------------------------------------------------------
@@ -897,14 +1195,9 @@ __global__ void k_count_states(const uint64_t* __restrict__ bnd, const uint64_t*
         b += (uint64_t)__popcll(par2[i] & ~sat2[i]);
         c += (uint64_t)__popcll(sat2[i]);
     }
-    a = wave_sum_u64(a);
-    b = wave_sum_u64(b);
-    c = wave_sum_u64(c);
-    if (__lane_id() == 0) {
-        if (a) atomicAdd(&ctr->l1_inside, (unsigned long long)a);
-        if (b) atomicAdd(&ctr->l2_inside, (unsigned long long)b);
-        if (c) atomicAdd(&ctr->l2_boundary, (unsigned long long)c);
-    }
+    block_add_u64(a, &ctr->l1_inside);
+    block_add_u64(b, &ctr->l2_inside);
+    block_add_u64(c, &ctr->l2_boundary);
 }
 
 void launch_count_states(const uint64_t* bnd, const uint64_t* ins, int64_t W, const uint64_t* sat2,
@@ -912,7 +1205,7 @@ void launch_count_states(const uint64_t* bnd, const uint64_t* ins, int64_t W, co
 {
     const int64_t n = W > W2 ? W : W2;
     if (n <= 0) return;
-    k_count_states<<<grid_for(n, 256, 4096), 256, 0, s>>>(bnd, ins, W, sat2, par2, W2, ctr);
+    k_count_states<<<grid_for(n, 256, 512), 256, 0, s>>>(bnd, ins, W, sat2, par2, W2, ctr);
 }
 
 // ---------------------------------------------------------------------------

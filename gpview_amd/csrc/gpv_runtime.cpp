@@ -114,7 +114,8 @@ struct gpv_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     Pool pool;
-    Counters* h_ctr = nullptr;   // pinned read-back slot
+    Counters* h_ctr = nullptr;   // pinned read-back slots
+    uint32_t* h_nb = nullptr;
     int num_cu = 256;
 };
 
@@ -151,16 +152,17 @@ struct gpv_grid {
     int32_t* bidx = nullptr;
     int64_t B = 0;
     uint32_t *pb = nullptr, *cntb = nullptr, *boff = nullptr, *list = nullptr;
-    uint32_t *ccnt = nullptr, *coff = nullptr, *ct = nullptr, *cz = nullptr;
+    uint32_t *ccnt = nullptr, *coff = nullptr;
+    float4* crec = nullptr;   // 3 float4 per column entry
     uint32_t *hcnt = nullptr, *hoff = nullptr;
     float4* hrec = nullptr;
     uint64_t *sat2 = nullptr, *par2 = nullptr;
     // progress
     bool setup_done = false, sat_done = false, inout_done = false, compact_done = false,
          l2p_done = false, l2s_done = false, finished = false;
-    // stream events: 0 setup, 1 L1 SAT, 2 L1 parity, 3 compaction+bins,
-    // 4 L2 parity, 5 L2 SAT start, 6 L2 SAT end, 7 finish
-    hipEvent_t ev[8] = {};
+    // stream events: 0 setup, 1 L1 SAT, 2 ray expansion, 3 parity fill,
+    // 4 compaction+bins, 5 L2 parity, 6 L2 SAT start, 7 L2 SAT end, 8 finish
+    hipEvent_t ev[9] = {};
     std::vector<void*> bufs;
 };
 
@@ -267,14 +269,6 @@ void build_tables(const gpv_grid_info& I, std::vector<float>& tab, size_t off[12
 
 int expand_blocks(const gpv_ctx* c) { return std::max(1, c->num_cu) * 8; }
 
-int sync_counters(gpv_grid* G)
-{
-    gpv_ctx* c = G->ctx;
-    HIP_TRY(hipMemcpyAsync(c->h_ctr, G->ctr, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return GPV_OK;
-}
-
 int ensure_scan_tmp(gpv_grid* G, int64_t n)
 {
     const size_t need = scan_tmp_elems(n);
@@ -356,11 +350,12 @@ int gpv_ctx_create(int device, unsigned flags, gpv_ctx** out)
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&c->h_ctr, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&c->h_ctr, sizeof(Counters) + 64, hipHostMallocDefault) != hipSuccess) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(GPV_ERR_HIP, "stream / pinned buffer creation failed");
     }
+    c->h_nb = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->h_ctr) + sizeof(Counters) + 32);
     *out = c;
     return GPV_OK;
 }
@@ -524,12 +519,16 @@ int gpv_grid_create(gpv_ctx* c, const float* tris, int64_t T, int on_device, con
     return GPV_OK;
 }
 
-int gpv_classify_tessellation(gpv_ctx* c, gpv_grid* G, int64_t* needed)
+// ---------------------------------------------------------------------------
+// stages, split into launch and completion halves so that the fused path
+// (gpv_voxelize) reads every data-dependent size in one host round trip
+// ---------------------------------------------------------------------------
+namespace {
+
+// Level1 SAT sweep + the boundary popcount scan (ranks need the final bits)
+int launch_sat_half(gpv_grid* G)
 {
-    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
-    if (needed) *needed = 0;
-    HIP_TRY(hipSetDevice(c->device));
-    TRY(stage_setup(G));
+    gpv_ctx* c = G->ctx;
     hipStream_t s = c->stream;
     HIP_TRY(hipEventRecord(G->ev[1], s));
     launch_fill_u64(G->bnd, 0, G->W, s);
@@ -537,13 +536,44 @@ int gpv_classify_tessellation(gpv_ctx* c, gpv_grid* G, int64_t* needed)
     Append64 pa{&G->ctr->pairs, G->pair_key, nullptr, G->pair_cap};
     Append64 co{&G->ctr->colents, G->col_key, G->col_z, G->col_cap};
     launch_sat_expand(G->tris, G->T, G->box0, G->box1, G->sat_off, G->g, G->bnd, pa, co, expand_blocks(c), s);
+    launch_popc_words(G->bnd, G->W, G->wcnt, s);
+    TRY(ensure_scan_tmp(G, G->W));
+    launch_scan_u32_u32(G->wcnt, G->wprefix, G->W, G->scan32, nullptr, s);
     HIP_TRY(hipGetLastError());
-    TRY(sync_counters(G));
+    return GPV_OK;
+}
+
+int launch_ray_half(gpv_grid* G)
+{
+    gpv_ctx* c = G->ctx;
+    hipStream_t s = c->stream;
+    HIP_TRY(hipEventRecord(G->ev[2], s));
+    launch_fill_u64(reinterpret_cast<uint64_t*>(&G->ctr->hits), 0, 1, s);
+    Append64 hi{&G->ctr->hits, G->hit_key, nullptr, G->hit_cap};
+    launch_ray_expand(G->tris, G->T, G->rng, G->ray_off, G->g, hi, expand_blocks(c), s);
+    HIP_TRY(hipGetLastError());
+    return GPV_OK;
+}
+
+// one round trip: the counters and, when asked, the boundary count B
+int read_counts(gpv_grid* G, bool with_b)
+{
+    gpv_ctx* c = G->ctx;
+    HIP_TRY(hipMemcpyAsync(c->h_ctr, G->ctr, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    if (with_b) HIP_TRY(hipMemcpyAsync(c->h_nb, G->wprefix + G->W, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPV_OK;
+}
+
+// after read_counts: accept the sweep or regrow the bin buffers to the exact
+// count (the reference's retry, src/Object.cpp:3198-3214)
+int complete_sat(gpv_grid* G, int64_t* needed)
+{
+    gpv_ctx* c = G->ctx;
     const uint64_t np = c->h_ctr->pairs, nc = c->h_ctr->colents;
     G->info.tests_l1 = (int64_t)c->h_ctr->tests_l1;
+    *needed = 0;
     if (np > G->pair_cap || nc > G->col_cap) {
-        // the reference's retry (src/Object.cpp:3198-3214): grow to the exact
-        // count just measured and let the caller run the sweep again
         const uint64_t need = std::max(np, nc);
         release(G, G->pair_key);
         release(G, G->col_key);
@@ -552,39 +582,38 @@ int gpv_classify_tessellation(gpv_ctx* c, gpv_grid* G, int64_t* needed)
         G->col_cap = G->pair_cap;
         TRY(make_appends(G));
         G->info.regrows++;
-        if (needed) *needed = (int64_t)need;
+        *needed = (int64_t)need;
         return GPV_OK;
     }
     G->n_pairs = np;
     G->n_cols = nc;
     G->info.pairs = (int64_t)np;
+    G->B = *c->h_nb;
+    G->info.nb = G->B;
+    G->info.l1_boundary = G->B;
     G->sat_done = true;
     return GPV_OK;
 }
 
-int gpv_classify_inout(gpv_ctx* c, gpv_grid* G)
+// after read_counts: false (and a regrown buffer) when the hits overflowed
+int complete_ray(gpv_grid* G, bool* ok)
 {
-    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
-    HIP_TRY(hipSetDevice(c->device));
-    TRY(stage_setup(G));
-    hipStream_t s = c->stream;
-    HIP_TRY(hipEventRecord(G->ev[2], s));
-    for (int attempt = 0; attempt < 3; ++attempt) {
-        launch_fill_u64(reinterpret_cast<uint64_t*>(&G->ctr->hits), 0, 1, s);
-        Append64 hi{&G->ctr->hits, G->hit_key, nullptr, G->hit_cap};
-        launch_ray_expand(G->tris, G->T, G->rng, G->ray_off, G->g, hi, expand_blocks(c), s);
-        HIP_TRY(hipGetLastError());
-        TRY(sync_counters(G));
-        const uint64_t nh = c->h_ctr->hits;
-        if (nh <= G->hit_cap) {
-            G->n_hits = nh;
-            break;
-        }
-        release(G, G->hit_key);
-        G->hit_cap = nh + nh / 8 + 1024;
-        TRY(alloc(G, &G->hit_key, G->hit_cap));
-        if (attempt == 2) return fail(GPV_ERR_CAPACITY, "ray hit capacity did not converge");
+    const uint64_t nh = G->ctx->h_ctr->hits;
+    *ok = nh <= G->hit_cap;
+    if (*ok) {
+        G->n_hits = nh;
+        return GPV_OK;
     }
+    release(G, G->hit_key);
+    G->hit_cap = nh + nh / 8 + 1024;
+    return alloc(G, &G->hit_key, G->hit_cap);
+}
+
+// hits grouped by column, then the parity of every Level1 voxel
+int launch_parity(gpv_grid* G)
+{
+    hipStream_t s = G->ctx->stream;
+    HIP_TRY(hipEventRecord(G->ev[3], s));
     const int64_t ncol = (int64_t)G->g.nx * G->g.ny;
     release(G, G->hcnt);
     release(G, G->hoff);
@@ -603,26 +632,14 @@ int gpv_classify_inout(gpv_ctx* c, gpv_grid* G)
     return GPV_OK;
 }
 
-int gpv_compact_boundary(gpv_ctx* c, gpv_grid* G)
+// boundary index list, bins per boundary voxel, column entries with records
+int launch_compact(gpv_grid* G)
 {
-    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
-    if (!G->sat_done) return fail(GPV_ERR_STATE, "gpv_classify_tessellation has not completed");
-    HIP_TRY(hipSetDevice(c->device));
-    hipStream_t s = c->stream;
-    HIP_TRY(hipEventRecord(G->ev[3], s));
-    launch_popc_words(G->bnd, G->W, G->wcnt, s);
-    TRY(ensure_scan_tmp(G, G->W));
-    launch_scan_u32_u32(G->wcnt, G->wprefix, G->W, G->scan32, nullptr, s);
-    uint32_t B32 = 0;
-    HIP_TRY(hipMemcpyAsync(&B32, G->wprefix + G->W, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    G->B = B32;
-    G->info.nb = G->B;
-    G->info.l1_boundary = G->B;
+    hipStream_t s = G->ctx->stream;
+    HIP_TRY(hipEventRecord(G->ev[4], s));
     release(G, G->bidx);
     TRY(alloc(G, &G->bidx, (size_t)std::max<int64_t>(G->B, 1)));
     launch_boundary_index(G->bnd, G->wprefix, G->g, G->bidx, s);
-    // bins per boundary voxel: rank -> count -> scan -> scatter
     release(G, G->pb);
     release(G, G->cntb);
     release(G, G->boff);
@@ -637,21 +654,18 @@ int gpv_compact_boundary(gpv_ctx* c, gpv_grid* G)
     TRY(ensure_scan_tmp(G, G->B));
     launch_scan_u32_u32(G->cntb, G->boff, G->B, G->scan32, &G->ctr->max_per_voxel, s);
     launch_pair_scatter(G->pair_key, G->pb, G->n_pairs, G->boff, G->cntb, G->list, s);
-    // (column, triangle, zmax) entries grouped by column
     const int64_t ncol = (int64_t)G->g.nx * G->g.ny;
     release(G, G->ccnt);
     release(G, G->coff);
-    release(G, G->ct);
-    release(G, G->cz);
+    release(G, G->crec);
     TRY(alloc(G, &G->ccnt, (size_t)ncol));
     TRY(alloc(G, &G->coff, (size_t)ncol + 1));
-    TRY(alloc(G, &G->ct, (size_t)std::max<uint64_t>(G->n_cols, 1)));
-    TRY(alloc(G, &G->cz, (size_t)std::max<uint64_t>(G->n_cols, 1)));
+    TRY(alloc(G, &G->crec, (size_t)std::max<uint64_t>(G->n_cols, 1) * 3));
     launch_fill_u32(G->ccnt, 0, ncol, s);
     launch_count_keys_hi(G->col_key, G->n_cols, G->ccnt, s);
     TRY(ensure_scan_tmp(G, ncol));
     launch_scan_u32_u32(G->ccnt, G->coff, ncol, G->scan32, nullptr, s);
-    launch_scatter_cols(G->col_key, G->col_z, G->n_cols, G->coff, G->ccnt, G->ct, G->cz, s);
+    launch_scatter_cols(G->tris, G->col_key, G->col_z, G->n_cols, G->coff, G->ccnt, G->crec, s);
     HIP_TRY(hipGetLastError());
     if (G->g.n2 > 0) {
         release(G, G->sat2);
@@ -664,14 +678,52 @@ int gpv_compact_boundary(gpv_ctx* c, gpv_grid* G)
     return GPV_OK;
 }
 
+}  // namespace
+
+int gpv_classify_tessellation(gpv_ctx* c, gpv_grid* G, int64_t* needed)
+{
+    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
+    int64_t nd = 0;
+    HIP_TRY(hipSetDevice(c->device));
+    TRY(stage_setup(G));
+    TRY(launch_sat_half(G));
+    TRY(read_counts(G, true));
+    TRY(complete_sat(G, &nd));
+    if (needed) *needed = nd;
+    return GPV_OK;
+}
+
+int gpv_classify_inout(gpv_ctx* c, gpv_grid* G)
+{
+    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
+    HIP_TRY(hipSetDevice(c->device));
+    TRY(stage_setup(G));
+    bool ok = false;
+    for (int attempt = 0; attempt < 2 && !ok; ++attempt) {
+        TRY(launch_ray_half(G));
+        TRY(read_counts(G, false));
+        TRY(complete_ray(G, &ok));
+    }
+    if (!ok) return fail(GPV_ERR_CAPACITY, "ray hit capacity did not converge");
+    return launch_parity(G);
+}
+
+int gpv_compact_boundary(gpv_ctx* c, gpv_grid* G)
+{
+    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
+    if (!G->sat_done) return fail(GPV_ERR_STATE, "gpv_classify_tessellation has not completed");
+    HIP_TRY(hipSetDevice(c->device));
+    return launch_compact(G);
+}
+
 int gpv_classify_inout_level2(gpv_ctx* c, gpv_grid* G)
 {
     if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
     if (!G->compact_done) return fail(GPV_ERR_STATE, "gpv_compact_boundary has not completed");
     if (G->g.n2 <= 0) return GPV_OK;
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipEventRecord(G->ev[4], c->stream));
-    launch_l2_parity(G->tris, G->bidx, G->B, G->coff, G->ct, G->cz, G->g, G->par2, c->stream);
+    HIP_TRY(hipEventRecord(G->ev[5], c->stream));
+    launch_l2_parity(G->bidx, G->B, G->coff, G->crec, G->g, G->par2, c->stream);
     HIP_TRY(hipGetLastError());
     G->l2p_done = true;
     return GPV_OK;
@@ -683,10 +735,10 @@ int gpv_classify_tessellation_level2(gpv_ctx* c, gpv_grid* G)
     if (!G->compact_done) return fail(GPV_ERR_STATE, "gpv_compact_boundary has not completed");
     if (G->g.n2 <= 0) return GPV_OK;
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipEventRecord(G->ev[5], c->stream));
+    HIP_TRY(hipEventRecord(G->ev[6], c->stream));
     launch_l2_sat(G->tris, G->bidx, G->B, G->boff, G->list, G->g, G->sat2, c->stream);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(G->ev[6], c->stream));
+    HIP_TRY(hipEventRecord(G->ev[7], c->stream));
     G->l2s_done = true;
     return GPV_OK;
 }
@@ -701,8 +753,8 @@ int gpv_grid_finish(gpv_ctx* c, gpv_grid* G)
     launch_fill_u64(reinterpret_cast<uint64_t*>(&G->ctr->l1_inside), 0, 3, s);
     const int64_t W2 = G->g.n2 > 0 ? G->B * G->g.w2 : 0;
     launch_count_states(G->bnd, G->ins, G->W, G->sat2, G->par2, W2, G->ctr, s);
-    HIP_TRY(hipEventRecord(G->ev[7], s));
-    TRY(sync_counters(G));
+    HIP_TRY(hipEventRecord(G->ev[8], s));
+    TRY(read_counts(G, false));
     const Counters& h = *c->h_ctr;
     G->info.l1_inside = (int64_t)h.l1_inside;
     G->info.l1_boundary = G->B;
@@ -710,7 +762,6 @@ int gpv_grid_finish(gpv_ctx* c, gpv_grid* G)
     G->info.l2_boundary = (int64_t)h.l2_boundary;
     G->info.max_tris_per_voxel = (int32_t)h.max_per_voxel;
     G->info.tests_l2 = G->g.n2 > 0 ? (int64_t)G->n_pairs * G->g.n2 * G->g.n2 * G->g.n2 : 0;
-    // stage times: setup, L1 SAT, L1 parity, compaction+bins, L2 parity, L2 SAT
     auto el = [&](int a, int b) -> float {
         float v = 0.0f;
         if (hipEventElapsedTime(&v, G->ev[a], G->ev[b]) != hipSuccess) {
@@ -719,17 +770,19 @@ int gpv_grid_finish(gpv_ctx* c, gpv_grid* G)
         }
         return v;
     };
+    // setup, L1 SAT (+ boundary scan), L1 parity (ray expansion, the size
+    // read-back and the parity fill), compaction + bins, L2 parity, L2 SAT
     G->info.ms_stage[0] = el(0, 1);
     G->info.ms_stage[1] = el(1, 2);
-    G->info.ms_stage[2] = el(2, 3);
+    G->info.ms_stage[2] = el(2, 4);
     if (G->g.n2 > 0) {
-        G->info.ms_stage[3] = el(3, 4);
-        G->info.ms_stage[4] = el(4, 5);
-        G->info.ms_stage[5] = el(5, 6);
+        G->info.ms_stage[3] = el(4, 5);
+        G->info.ms_stage[4] = el(5, 6);
+        G->info.ms_stage[5] = el(6, 7);
     } else {
-        G->info.ms_stage[3] = el(3, 7);
+        G->info.ms_stage[3] = el(4, 8);
     }
-    G->info.ms_total = el(0, 7);
+    G->info.ms_total = el(0, 8);
     G->finished = true;
     return GPV_OK;
 }
@@ -739,18 +792,33 @@ int gpv_voxelize(gpv_ctx* c, const float* tris, int64_t T, int on_device, const 
 {
     gpv_grid* G = nullptr;
     TRY(gpv_grid_create(c, tris, T, on_device, bmin, bmax, mms, params, &G));
-    int r = GPV_OK;
-    int64_t needed = 0;
-    for (int attempt = 0; attempt < 3; ++attempt) {
-        if ((r = gpv_classify_tessellation(c, G, &needed)) != GPV_OK) break;
-        if (needed == 0) break;
-        if (attempt == 2) r = fail(GPV_ERR_CAPACITY, "bin capacity did not converge");
-    }
-    if (r == GPV_OK) r = gpv_classify_inout(c, G);
-    if (r == GPV_OK) r = gpv_compact_boundary(c, G);
-    if (r == GPV_OK && G->g.n2 > 0) r = gpv_classify_inout_level2(c, G);
-    if (r == GPV_OK && G->g.n2 > 0) r = gpv_classify_tessellation_level2(c, G);
-    if (r == GPV_OK) r = gpv_grid_finish(c, G);
+    auto run = [&]() -> int {
+        TRY(stage_setup(G));
+        // SAT sweep, ray expansion and boundary scan back to back, one read-back
+        TRY(launch_sat_half(G));
+        TRY(launch_ray_half(G));
+        TRY(read_counts(G, true));
+        int64_t needed = 0;
+        bool ray_ok = false;
+        TRY(complete_sat(G, &needed));
+        TRY(complete_ray(G, &ray_ok));
+        for (int attempt = 0; attempt < 2 && (needed > 0 || !ray_ok); ++attempt) {
+            if (needed > 0) TRY(launch_sat_half(G));
+            if (!ray_ok) TRY(launch_ray_half(G));
+            TRY(read_counts(G, true));
+            if (needed > 0) TRY(complete_sat(G, &needed));
+            if (!ray_ok) TRY(complete_ray(G, &ray_ok));
+        }
+        if (needed > 0 || !ray_ok) return fail(GPV_ERR_CAPACITY, "buffer capacity did not converge");
+        TRY(launch_parity(G));
+        TRY(launch_compact(G));
+        if (G->g.n2 > 0) {
+            TRY(gpv_classify_inout_level2(c, G));
+            TRY(gpv_classify_tessellation_level2(c, G));
+        }
+        return gpv_grid_finish(c, G);
+    };
+    const int r = run();
     if (r != GPV_OK) {
         std::string keep = g_err;
         gpv_grid_release(c, G);

@@ -1,0 +1,91 @@
+// gpv_slab.hpp -- conservative plane-slab filter for the SAT's plane predicate.
+//
+// The SAT (src/TriBoxIntersection.cpp:135-210) is a conjunction; its plane
+// predicate (planeBoxOverlap, :64-86) is evaluated on float values relative
+// to the box center: v_i = fl(t_i - c), e0 = fl(v1 - v0), e1 = fl(v2 - v1),
+// n = fl(e0 x e1), d1 = fl(n . vmin), d2 = fl(n . vmax) with vmin/vmax chosen
+// by the sign of n.  In exact arithmetic, with N = (t1 - t0) x (t2 - t1) and
+// s = N . (c - t0), R = sum h_q |N_q|: d1 = s - R and d2 = s + R.  Rounding
+// moves d1, d2 by at most M (bounded below per term), so the float predicate
+// rejects for certain when |s| > R + M.  Boxes outside that slab can be
+// skipped without evaluating the SAT; every other box still runs the full
+// float SAT, so results are unchanged.
+//
+// M, per axis q, with A_iq bounding |t_iq - c_q| over the candidate centers:
+//   D0_q = U (A0 + A1 + |E0|)      |e0_q - E0_q|      (E0 = t1 - t0)
+//   D1_q = U (A1 + A2 + |E1|)      |e1_q - E1_q|      (E1 = t2 - t1)
+//   Nerr = cross(D, F) terms + 2U (F0 F1 terms)        |n_q - N_q|
+//   M = 3U sum |n| V + sum Nerr V + sum |N| Verr + sum 2 h Nerr
+// with V_q = h_q + A0_q and Verr_q = U (2 A0_q + h_q); every term carries a
+// 1% cushion and the whole M a factor kSlabSafety.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace gpv {
+
+constexpr double kSlabSafety = 4.0;
+
+struct PlaneSlab {
+    double nx, ny, nz;   // exact normal of the float triangle
+    double t0x, t0y, t0z;
+    double lim;          // keep a box iff |N . (c - t0)| <= lim
+};
+
+// clo/chi: smallest and largest candidate center per axis; h: half extents.
+__host__ __device__ inline PlaneSlab plane_slab(const float* tv, float hx, float hy, float hz,
+                                                const double clo[3], const double chi[3])
+{
+    const double U = 5.9604644775390625e-08 * 1.01;
+    const double h[3] = {hx, hy, hz};
+    double A[3][3], E0[3], E1[3];
+    for (int q = 0; q < 3; ++q) {
+        for (int i = 0; i < 3; ++i) {
+            const double t = tv[3 * i + q];
+            A[i][q] = fmax(fabs(t - clo[q]), fabs(t - chi[q]));
+        }
+        E0[q] = (double)tv[3 + q] - (double)tv[q];
+        E1[q] = (double)tv[6 + q] - (double)tv[3 + q];
+    }
+    double D0[3], D1[3], F0[3], F1[3];
+    for (int q = 0; q < 3; ++q) {
+        D0[q] = U * (A[0][q] + A[1][q] + fabs(E0[q]));
+        D1[q] = U * (A[1][q] + A[2][q] + fabs(E1[q]));
+        F0[q] = fabs(E0[q]) + D0[q];
+        F1[q] = fabs(E1[q]) + D1[q];
+    }
+    PlaneSlab s;
+    s.nx = E0[1] * E1[2] - E0[2] * E1[1];
+    s.ny = E0[2] * E1[0] - E0[0] * E1[2];
+    s.nz = E0[0] * E1[1] - E0[1] * E1[0];
+    s.t0x = tv[0];
+    s.t0y = tv[1];
+    s.t0z = tv[2];
+    const double N[3] = {s.nx, s.ny, s.nz};
+    // component q of the cross product uses axes a = q+1, b = q+2
+    double R = 0.0, M = 0.0;
+    for (int q = 0; q < 3; ++q) {
+        const int a = (q + 1) % 3, b = (q + 2) % 3;
+        const double ce = D0[a] * F1[b] + fabs(E0[a]) * D1[b] + D0[b] * F1[a] + fabs(E0[b]) * D1[a];
+        const double rc = 2.0 * U * (F0[a] * F1[b] + F0[b] * F1[a]);
+        const double nerr = (ce + rc) * 1.01 + 1e-12 * (fabs(E0[a] * E1[b]) + fabs(E0[b] * E1[a]));
+        const double nb = fabs(N[q]) + nerr;
+        const double V = h[q] + A[0][q] * 1.01;
+        const double verr = U * (2.0 * A[0][q] + h[q]);
+        R += h[q] * fabs(N[q]);
+        M += 3.0 * U * nb * V + nerr * V + fabs(N[q]) * verr + 2.0 * h[q] * nerr;
+    }
+    // evaluation slop of s itself (double products and sums of these sizes)
+    double cs = 0.0;
+    for (int q = 0; q < 3; ++q) cs += fabs(N[q]) * (fmax(fabs(clo[q]), fabs(chi[q])) + fabs((double)tv[q]));
+    s.lim = R + kSlabSafety * M + 1e-12 * cs;
+    return s;
+}
+
+__host__ __device__ inline double slab_dist(const PlaneSlab& s, double cx, double cy, double cz)
+{
+    return s.nx * (cx - s.t0x) + s.ny * (cy - s.t0y) + s.nz * (cz - s.t0z);
+}
+
+}  // namespace gpv
