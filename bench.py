@@ -10,7 +10,10 @@ p50/p99 step latency, per-stage device times, the roofline of the dominant
 kernel (HIP events on the library's stream) and the CPU oracle timed on this
 host on the same mesh.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+Also measured (key "batched"): the mixed-size stream of 256 meshes, one
+gpv_voxelize per mesh against one gpv_voxelize_batch for all of them.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--batch-meshes N]
 """
 import argparse
 import json
@@ -32,6 +35,79 @@ STAGES = ["setup", "l1_sat", "l1_parity", "compaction_bins", "l2_parity", "l2_sa
 def make_workload(gv):
     v, f = gv.torus_knot(p=3, q=7, segments=2048, ring=256, tube=0.3, jitter=0.01, extent=0.9, seed=2)
     return v, f, gv.flatten(v, f)
+
+
+def bench_batched(gv, ctx, n_meshes, steps, warmup, check):
+    """The mixed-size stream: n_meshes meshes (triangles log-uniform in
+    [1k, 200k], Level1 {64,128,256}, Level2 {4,8}), device-resident soups.
+    Plain = one gpv_voxelize per mesh; batched = one gpv_voxelize_batch over
+    all of them.  Both timed over whole passes, grids released each pass."""
+    from gpview_amd import workloads
+    meshes = workloads.mixed_stream(gv, n_meshes)
+    ptrs = [ctx.upload(m["soup"]) for m in meshes]
+    T = sum(len(m["soup"]) for m in meshes)
+
+    def plain_pass():
+        for m, p in zip(meshes, ptrs):
+            ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], m["voxel_count"], m["voxel_count2"], True,
+                         device_ptr=p).release()
+
+    def batch_pass():
+        gs = ctx.voxelize_batch(meshes, device_ptrs=ptrs)
+        info = gs[0].info()
+        for g in gs:
+            g.release()
+        return info
+
+    for _ in range(warmup):
+        plain_pass()
+        batch_pass()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plain_pass()
+    t_plain = (time.perf_counter() - t0) / steps
+    lat = []
+    binfo = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        binfo.append(batch_pass())
+        lat.append(time.perf_counter() - s0)
+    t_batch = (time.perf_counter() - t0) / steps
+
+    # totals and the per-mesh bit-exact check (outside the timed passes)
+    gs = ctx.voxelize_batch(meshes, device_ptrs=ptrs)
+    tests = 0
+    voxels = 0
+    exact = True if check else None
+    for m, p, g in zip(meshes, ptrs, gs):
+        i = g.info()
+        tests += i["tests_l1"] + i["tests_l2"]
+        voxels += i["n1"] + i["nb"] * i["n2"] ** 3
+        if check:
+            a = g.download()
+            ga = ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], m["voxel_count"], m["voxel_count2"],
+                              True, device_ptr=p)
+            b = ga.download()
+            ga.release()
+            exact = exact and all(np.array_equal(a[k], b[k]) for k in ("l1", "prefix", "bidx", "l2"))
+        g.release()
+    for p in ptrs:
+        ctx.free(p)
+    stage_ms = {s: float(np.median([i["ms_stage"][k] for i in binfo])) for k, s in enumerate(STAGES)}
+    return {
+        "workload": f"mixed-size stream, {n_meshes} meshes, {T} triangles (log-uniform 1k-200k; "
+                    "icospheres, tori, drilled boxes; L1 {64,128,256}, L2 {4,8})",
+        "meshes": n_meshes, "triangles": T,
+        "plain_meshes_per_s": n_meshes / t_plain,
+        "batched_meshes_per_s": n_meshes / t_batch,
+        "speedup": t_plain / t_batch,
+        "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3,
+        "batched_tri_box_tests_per_s": tests / t_batch,
+        "batched_voxels_per_s": voxels / t_batch,
+        "batched_stage_ms": stage_ms,
+        "bit_exact_vs_plain": exact,
+    }
 
 
 def kernel_bytes(stage, info, T):
@@ -56,6 +132,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")
+    ap.add_argument("--batch-meshes", type=int, default=256, help="meshes of the mixed-stream batch (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-check", action="store_true", help="skip the batched-vs-plain bit-exact check")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (meshes shard "
@@ -91,6 +170,9 @@ def main():
     got = g.download()
     g.release()
     ctx.free(dsoup)
+    batched = None
+    if args.batch_meshes > 0:
+        batched = bench_batched(gv, ctx, args.batch_meshes, args.batch_steps, 1, not args.no_check)
     ctx.close()
 
     meshes_per_s = args.steps / wall
@@ -154,6 +236,7 @@ def main():
         "roofline": rl,
         "roofline_valu": valu,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -52,6 +52,26 @@ class GridInfo(ctypes.Structure):
         return d
 
 
+class MeshDesc(ctypes.Structure):
+    """gpv_mesh_desc (include/gpv.h)."""
+    _fields_ = [("tris", ctypes.c_void_p), ("ntris", ctypes.c_int64), ("bmin", ctypes.c_float * 3),
+                ("bmax", ctypes.c_float * 3), ("max_model_size", ctypes.c_float), ("params", Params)]
+
+
+class ServerOpts(ctypes.Structure):
+    """gpv_server_opts (include/gpv_serve.h)."""
+    _fields_ = [("device", ctypes.c_int), ("lanes", ctypes.c_int), ("hbm_budget", ctypes.c_int64),
+                ("max_batch_meshes", ctypes.c_int), ("max_batch_tris", ctypes.c_int64)]
+
+
+class RequestStats(ctypes.Structure):
+    """gpv_request_stats (include/gpv_serve.h)."""
+    _fields_ = [("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("predicted_bytes", ctypes.c_int64), ("resident_bytes", ctypes.c_int64),
+                ("batch_id", ctypes.c_int32), ("batch_size", ctypes.c_int32), ("lane", ctypes.c_int32),
+                ("status", ctypes.c_int32)]
+
+
 class Mesh(ctypes.Structure):
     """gpv_mesh (include/gpv_mesh.h)."""
     _fields_ = [("nverts", ctypes.c_int64), ("ntris", ctypes.c_int64),
@@ -67,7 +87,7 @@ _lib = None
 EXPORTED = [
     "gpv_ctx_create", "gpv_ctx_destroy", "gpv_last_error", "gpv_status_string",
     "gpv_device_upload", "gpv_device_free",
-    "gpv_mesh_bbox", "gpv_grid_derive", "gpv_voxelize", "gpv_grid_create",
+    "gpv_mesh_bbox", "gpv_grid_derive", "gpv_voxelize", "gpv_voxelize_batch", "gpv_grid_create",
     "gpv_classify_tessellation", "gpv_classify_inout", "gpv_compact_boundary",
     "gpv_classify_inout_level2", "gpv_classify_tessellation_level2", "gpv_grid_finish",
     "gpv_grid_get_info", "gpv_grid_download", "gpv_grid_device_views", "gpv_save_raw",
@@ -75,6 +95,8 @@ EXPORTED = [
     "gpv_mesh_free", "gpv_mesh_flatten", "gpv_mesh_bounds", "gpv_mesh_icosphere",
     "gpv_mesh_torus_knot", "gpv_mesh_torus", "gpv_mesh_drilled_box", "gpv_mesh_mixed",
     "gpv_mesh_stress", "gpv_mesh_read_obj", "gpv_mesh_read_off",
+    "gpv_predict_footprint", "gpv_server_create", "gpv_server_destroy", "gpv_server_submit",
+    "gpv_server_wait", "gpv_server_release", "gpv_server_counters",
 ]
 
 
@@ -99,6 +121,14 @@ def lib():
     L.gpv_grid_derive.argtypes = [_f32p, _f32p, c_float, P(Params), P(GridInfo)]
     for fn in ("gpv_voxelize", "gpv_grid_create"):
         getattr(L, fn).argtypes = [_vp, _vp, c_i64, c_int, _f32p, _f32p, c_float, P(Params), P(_vp)]
+    L.gpv_voxelize_batch.argtypes = [_vp, P(MeshDesc), c_int, c_int, P(_vp)]
+    L.gpv_predict_footprint.argtypes = [P(MeshDesc), P(c_i64)]
+    L.gpv_server_create.argtypes = [P(ServerOpts), P(_vp)]
+    L.gpv_server_destroy.argtypes = [_vp]
+    L.gpv_server_submit.argtypes = [_vp, P(MeshDesc), c_int, P(ctypes.c_uint64)]
+    L.gpv_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.c_double, P(_vp), P(_vp), P(RequestStats)]
+    L.gpv_server_release.argtypes = [_vp, ctypes.c_uint64]
+    L.gpv_server_counters.argtypes = [_vp, P(c_i64), P(c_i64), P(c_i64), P(c_i64)]
     L.gpv_classify_tessellation.argtypes = [_vp, _vp, P(c_i64)]
     for fn in ("gpv_classify_inout", "gpv_compact_boundary", "gpv_classify_inout_level2",
                "gpv_classify_tessellation_level2", "gpv_grid_finish", "gpv_grid_release"):
@@ -116,7 +146,7 @@ def lib():
     L.gpv_mesh_torus.argtypes = [c_int, c_int, c_float, c_float, ctypes.c_uint64, P(Mesh)]
     L.gpv_mesh_drilled_box.argtypes = [c_i64, c_float, ctypes.c_uint64, P(Mesh)]
     L.gpv_mesh_mixed.argtypes = [c_int, c_i64, ctypes.c_uint64, P(Mesh)]
-    L.gpv_mesh_stress.argtypes = [P(Mesh), c_int, c_int, c_float, c_float, ctypes.c_uint64]
+    L.gpv_mesh_stress.argtypes = [P(Mesh), c_int, c_int, c_float, c_float, c_float, ctypes.c_uint64]
     L.gpv_mesh_read_obj.argtypes = [ctypes.c_char_p, P(Mesh)]
     L.gpv_mesh_read_off.argtypes = [ctypes.c_char_p, P(Mesh)]
     _lib = L
@@ -169,8 +199,9 @@ def mixed(kind, target_tris, seed):
     return _gen("gpv_mesh_mixed", int(kind), int(target_tris), seed)
 
 
-def stress(verts, faces, voxel_count, voxel_count2, p_snap=0.3, f_sliver=0.1, seed=5):
-    """Lattice snapping + slivers applied to a copy of (verts, faces)."""
+def stress(verts, faces, voxel_count, voxel_count2, p_snap=0.3, f_sliver=0.1, f_coplanar=0.0, seed=5):
+    """Lattice snapping, face-coplanar triangles and slivers applied to a
+    copy of (verts, faces)."""
     L = lib()
     v = np.ascontiguousarray(verts, np.float32)
     f = np.ascontiguousarray(faces, np.int32)
@@ -187,7 +218,7 @@ def stress(verts, faces, voxel_count, voxel_count2, p_snap=0.3, f_sliver=0.1, se
     m.verts = ctypes.cast(pv, ctypes.POINTER(ctypes.c_float))
     m.faces = ctypes.cast(pf, ctypes.POINTER(ctypes.c_int32))
     m.bbox_from_faces = 0
-    st = L.gpv_mesh_stress(ctypes.byref(m), voxel_count, voxel_count2, p_snap, f_sliver, seed)
+    st = L.gpv_mesh_stress(ctypes.byref(m), voxel_count, voxel_count2, p_snap, f_sliver, f_coplanar, seed)
     if st != GPV_OK:
         L.gpv_mesh_free(ctypes.byref(m))
         _check(st, "gpv_mesh_stress")
@@ -273,6 +304,24 @@ class Context:
                                   ctypes.byref(p), ctypes.byref(g)), "gpv_voxelize")
         return Grid(self, g)
 
+    def voxelize_batch(self, meshes, device_ptrs=None):
+        """gpv_voxelize_batch.  meshes: sequence of dicts with soup, bmin, bmax,
+        mms, voxel_count and optionally voxel_count2, level2,
+        initial_capacity.  device_ptrs: per-mesh device soups (then the
+        soups are only used for their triangle counts).  Returns one Grid per
+        mesh."""
+        n = len(meshes)
+        descs = (MeshDesc * n)()
+        keep = []
+        for i, m in enumerate(meshes):
+            d, soup = mesh_desc(m, device_ptrs[i] if device_ptrs is not None else None)
+            descs[i] = d
+            keep.append(soup)
+        out = (_vp * n)()
+        _check(lib().gpv_voxelize_batch(self.h, descs, n, 1 if device_ptrs is not None else 0, out),
+               "gpv_voxelize_batch")
+        return [Grid(self, _vp(out[i])) for i in range(n)]
+
     def create(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0):
         """gpv_grid_create: the staged form (call the stage methods on the Grid)."""
         soup = np.ascontiguousarray(soup, np.float32).reshape(-1, 9)
@@ -284,12 +333,90 @@ class Context:
         return Grid(self, g)
 
 
+def mesh_desc(m, device_ptr=None):
+    """gpv_mesh_desc of a mesh dict (soup, bmin, bmax, mms, voxel_count, ...);
+    returns (desc, soup) -- keep the soup alive while the desc is in use."""
+    soup = np.ascontiguousarray(m["soup"], np.float32).reshape(-1, 9)
+    d = MeshDesc()
+    d.tris = device_ptr if device_ptr is not None else soup.ctypes.data
+    d.ntris = len(soup)
+    d.bmin[:] = [float(x) for x in m["bmin"]]
+    d.bmax[:] = [float(x) for x in m["bmax"]]
+    d.max_model_size = float(m["mms"])
+    d.params = Params(int(m["voxel_count"]), int(m.get("voxel_count2", 4)), 1 if m.get("level2", True) else 0,
+                      int(m.get("initial_capacity", 0)))
+    return d, soup
+
+
+def predict_footprint(m):
+    d, _ = mesh_desc(m)
+    b = ctypes.c_int64()
+    _check(lib().gpv_predict_footprint(ctypes.byref(d), ctypes.byref(b)), "gpv_predict_footprint")
+    return b.value
+
+
+class _CtxRef:
+    """A context owned elsewhere (a server lane): usable, never destroyed."""
+
+    def __init__(self, h):
+        self.h = h
+
+
+class Server:
+    """gpv_server: continuous-batching request loop (include/gpv_serve.h)."""
+
+    def __init__(self, lanes=2, hbm_budget=0, max_batch_meshes=0, max_batch_tris=0, device=-1):
+        self.h = _vp()
+        o = ServerOpts(device, lanes, int(hbm_budget), max_batch_meshes, int(max_batch_tris))
+        _check(lib().gpv_server_create(ctypes.byref(o), ctypes.byref(self.h)), "gpv_server_create")
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            lib().gpv_server_destroy(self.h)
+            self.h = _vp()
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, m, device_ptr=None):
+        d, soup = mesh_desc(m, device_ptr)
+        rid = ctypes.c_uint64()
+        _check(lib().gpv_server_submit(self.h, ctypes.byref(d), 1 if device_ptr is not None else 0,
+                                       ctypes.byref(rid)), "gpv_server_submit")
+        self._keep[rid.value] = soup
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        """(Grid view, stats dict) of a finished request."""
+        g, c, st = _vp(), _vp(), RequestStats()
+        _check(lib().gpv_server_wait(self.h, rid, float(timeout), ctypes.byref(g), ctypes.byref(c),
+                                     ctypes.byref(st)), "gpv_server_wait")
+        stats = {k: getattr(st, k) for k, _ in RequestStats._fields_}
+        return Grid(_CtxRef(c), g, owned=False), stats
+
+    def release(self, rid):
+        _check(lib().gpv_server_release(self.h, rid), "gpv_server_release")
+        self._keep.pop(rid, None)
+
+    def counters(self):
+        a, b, c, d = (ctypes.c_int64() for _ in range(4))
+        _check(lib().gpv_server_counters(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                         ctypes.byref(d)), "gpv_server_counters")
+        return {"completed": a.value, "batches": b.value, "charged": c.value, "peak_charged": d.value}
+
+
 class Grid:
     """A device-resident voxelization (gpv_grid)."""
 
-    def __init__(self, ctx, h):
+    def __init__(self, ctx, h, owned=True):
         self.ctx = ctx
         self.h = h
+        self.owned = owned
 
     # staged entry points, in PerformVoxelization's order
     def classify_tessellation(self):
@@ -334,6 +461,6 @@ class Grid:
         _check(lib().gpv_save_raw(self.ctx.h, self.h, int(obj_id), os.fsencode(directory)), "gpv_save_raw")
 
     def release(self):
-        if self.h:
+        if self.h and self.owned:
             lib().gpv_grid_release(self.ctx.h, self.h)
-            self.h = _vp()
+        self.h = _vp()

@@ -23,6 +23,7 @@ SOURCES = [
     "gpv_runtime.cpp",
     "gpv_mesh.cpp",
     "gpv_object.cpp",
+    "gpv_serve.cpp",
 ]
 
 COMMON = [
@@ -46,17 +47,24 @@ def build(verbose=False):
     cc = hipcc()
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
+    cmds = []
     objs = []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(objdir, src + ".o")
-        cmd = [cc, "-x", "hip", "-c", path, "-o", obj] + COMMON
+        cmds.append([cc, "-x", "hip", "-c", path, "-o", obj, "-pthread"] + COMMON)
+        objs.append(obj)
+    from concurrent.futures import ThreadPoolExecutor
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd))
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1, 8)) as ex:
+        list(ex.map(run, cmds))
     tmp = LIB + ".tmp"
-    cmd = [cc, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC"]
+    cmd = [cc, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC", "-pthread"]
     if verbose:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)

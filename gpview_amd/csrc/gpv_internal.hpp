@@ -6,7 +6,7 @@
 
 namespace gpv {
 
-// Grid constants and per-axis center tables, passed to kernels by value.
+// Grid constants and per-axis center tables of one mesh.
 // Bit layout of every Level1 bitplane: word w = (z*ny + y)*nxw + x/64,
 // bit x%64 (rows padded to whole 64-bit words), so words are in the
 // reference's linear order z*ny*nx + y*nx + x and a boundary rank is a
@@ -24,6 +24,43 @@ struct GridDev {
     const float* r2x; const float* r2y; const float* r2z;   // [n*n2] Level2 ray origins
 };
 
+// Where one mesh of a batch lives in the batch-wide arrays.  A batch
+// concatenates the meshes' soups (tri0), Level1 bitplanes (word0), xy
+// columns (col0), boundary voxels (b0) and Level2 words (l2w0); entry nm is
+// the sentinel holding the totals.  b0 and l2w0 are filled on the device
+// once the boundary scan is known.
+struct MeshOff {
+    uint64_t word0;
+    uint64_t l2w0;
+    uint32_t col0;
+    uint32_t tri0;
+    uint32_t b0;
+    uint32_t pad;
+};
+
+// Per-mesh results accumulated on the device.
+struct MeshCtr {
+    unsigned long long tests_l1;      // sum of clipped index-AABB volumes
+    unsigned long long l1_inside;
+    unsigned long long l2_inside;
+    unsigned long long l2_boundary;
+    unsigned long long pairs;         // Level1 (voxel, triangle) bin entries
+    unsigned int max_per_voxel;
+    unsigned int pad;
+};
+
+// A batch as kernels see it.  With nm == 1 kernels run their single-mesh
+// instantiation: the grid comes from the by-value GridDev, every offset is
+// 0 and only ctr is used.
+struct Meshes {
+    const GridDev* grids;       // [nm]
+    MeshOff* off;               // [nm + 1]
+    const uint32_t* tri_mesh;   // [T]  mesh of each triangle (non-decreasing)
+    uint32_t* bmesh;            // [B]  mesh of each boundary voxel (non-decreasing)
+    MeshCtr* ctr;               // [nm]
+    int nm;
+};
+
 // Append-buffer descriptor: a device counter and a capacity.  Writers past
 // the capacity are dropped; the host sees count > cap and regrows.
 struct Append64 {
@@ -33,80 +70,78 @@ struct Append64 {
     uint64_t cap;
 };
 
-// Device-side counters of one voxelization (one cache line each avoided:
-// small, read back once per mesh).
+// Device-side counters of one pipeline run (batch-wide).
 struct Counters {
     unsigned long long pairs;       // Level1 SAT (voxel, triangle) pairs
     unsigned long long colents;     // (column, triangle, zmax) entries
     unsigned long long hits;        // (column, triangle) ray hits, Level1
-    unsigned long long tests_l1;    // sum of clipped index-AABB volumes
-    unsigned long long pair_est;    // sum of index-AABB face areas (capacity hint)
-    unsigned long long l1_inside;
-    unsigned long long l2_inside;
-    unsigned long long l2_boundary;
-    unsigned int max_per_voxel;
-    unsigned int pad;
+    unsigned long long pad;
 };
 
+// sets the calling thread's gpv_last_error text and returns code
+int set_error(int code, const char* msg);
+
 // ---- launch wrappers (gpv_kernels.hip) ------------------------------------
+// Launchers that take the batch view M and a grid g0 run the single-mesh
+// instantiation (grid g0) when M.nm == 1, the multi-mesh one otherwise.
 void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
 
-void launch_tri_setup(const float* tris, int64_t T, const GridDev& g, int4* box0, int4* box1,
-                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, Counters* ctr,
-                      hipStream_t s);
+void launch_mesh_ids(const MeshOff* off, int nm, int64_t T, uint32_t* tri_mesh, hipStream_t s);
+void launch_tri_setup(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
+                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, hipStream_t s);
 
 // exclusive scan of n u32 into out[0..n] (out[n] = total); tmp sized by
 // scan_tmp_elems(n).  If max_out != nullptr the max input is atomically
 // folded into it.
 size_t scan_tmp_elems(int64_t n);
-void launch_scan_u32_u64(const uint32_t* in, uint64_t* out, int64_t n, uint64_t* tmp,
+void launch_scan_u32_u64(const uint32_t* in, uint64_t* out, int64_t n, uint64_t* tmp, hipStream_t s);
+void launch_scan_u32_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, unsigned int* max_out,
                          hipStream_t s);
-void launch_scan_u32_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
-                         unsigned int* max_out, hipStream_t s);
 
-void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1,
-                       const uint64_t* off, const GridDev& g, uint64_t* bnd, Append64 pairs,
-                       Append64 cols, int blocks, hipStream_t s);
-void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off,
-                       const GridDev& g, Append64 hits, int blocks, hipStream_t s);
+void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const uint64_t* off,
+                       const GridDev& g0, const Meshes& M, uint64_t* bnd, Append64 pairs, Append64 cols, int blocks,
+                       hipStream_t s);
+void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off, const GridDev& g0,
+                       const Meshes& M, Append64 hits, int blocks, hipStream_t s);
 
-// counting sort helpers
-void launch_count_keys_hi(const uint64_t* key, uint64_t n,
-                          uint32_t* cnt, hipStream_t s);
-void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n,
-                          const GridDev& g, const uint32_t* off, uint32_t* cnt,
-                         float4* rec, hipStream_t s);
+// counting sort helpers (keys hold a batch-wide column index in the high word)
+void launch_count_keys_hi(const uint64_t* key, uint64_t n, uint32_t* cnt, hipStream_t s);
+void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n, const GridDev& g0, const Meshes& M,
+                         const uint32_t* off, uint32_t* cnt, float4* rec, hipStream_t s);
 void launch_scatter_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n,
                          const uint32_t* off, uint32_t* cnt, float4* rec, hipStream_t s);
 
-void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g, uint64_t* ins,
-                      hipStream_t s);
+void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0, const Meshes& M, int64_t W,
+                      uint64_t* ins, hipStream_t s);
 
 void launch_popc_words(const uint64_t* bits, int64_t W, uint32_t* cnt, hipStream_t s);
-void launch_boundary_index(const uint64_t* bnd, const uint32_t* wprefix, const GridDev& g,
-                           int32_t* bidx, hipStream_t s);
-void launch_pair_rank(const uint64_t* pairs, uint64_t n,
-                      const uint64_t* bnd, const uint32_t* wprefix, uint32_t* pb, uint32_t* cntb,
-                      hipStream_t s);
-void launch_pair_scatter(const uint64_t* pairs, const uint32_t* pb, uint64_t n,
-                          const uint32_t* boff, uint32_t* cntb, uint32_t* list,
-                         hipStream_t s);
+// b0 and l2w0 of every mesh (and the totals in entry nm) from the word prefix
+void launch_mesh_bounds(const uint32_t* wprefix, const Meshes& M, hipStream_t s);
+void launch_boundary_index(const uint64_t* bnd, const uint32_t* wprefix, const GridDev& g0, const Meshes& M,
+                           int64_t W, int32_t* bidx, hipStream_t s);
+// pairs: key = batch-wide boundary bit index, val = triangle
+void launch_pair_rank(const uint64_t* bits, uint64_t n, const uint64_t* bnd, const uint32_t* wprefix, uint32_t* pb,
+                      uint32_t* cntb, hipStream_t s);
+void launch_pair_scatter(const uint32_t* tri, const uint32_t* pb, uint64_t n, const uint32_t* boff, uint32_t* cntb,
+                         uint32_t* list, hipStream_t s);
+// per-mesh bin totals and largest bin (multi-mesh runs)
+void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStream_t s);
 
-void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff,
-                   const uint32_t* list, const GridDev& g, uint64_t* sat2, hipStream_t s);
-void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec, const GridDev& g,
-                      uint64_t* par2, hipStream_t s);
+void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
+                   const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s);
+void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec, const GridDev& g0,
+                      const Meshes& M, uint64_t* par2, hipStream_t s);
 
-void launch_count_states(const uint64_t* bnd, const uint64_t* ins, int64_t W,
-                         const uint64_t* sat2, const uint64_t* par2, int64_t W2, Counters* ctr,
-                         hipStream_t s);
+void launch_count_states(const uint64_t* bnd, const uint64_t* ins, int64_t W, const uint64_t* sat2,
+                         const uint64_t* par2, int64_t W2, const Meshes& M, hipStream_t s);
 
+// egress of one mesh (pointers already offset to that mesh's words)
 void launch_export_l1(const uint64_t* bnd, const uint64_t* ins, const GridDev& g, uint8_t unit, uint8_t* out,
                       hipStream_t s);
-void launch_export_prefix(const uint64_t* bnd, const uint32_t* wprefix, const GridDev& g,
+void launch_export_prefix(const uint64_t* bnd, const uint32_t* wprefix, uint32_t base, const GridDev& g,
                           int32_t* out, hipStream_t s);
-void launch_export_l2(const uint64_t* sat2, const uint64_t* par2, int64_t B, const GridDev& g,
-                      uint8_t unit, uint8_t* out, hipStream_t s);
+void launch_export_l2(const uint64_t* sat2, const uint64_t* par2, int64_t B, const GridDev& g, uint8_t unit,
+                      uint8_t* out, hipStream_t s);
 
 }  // namespace gpv

@@ -113,6 +113,31 @@ __device__ __forceinline__ int64_t find_segment(const uint64_t* __restrict__ off
 }
 
 // ---------------------------------------------------------------------------
+// multi-mesh lookups.  Every kernel that depends on the grid is a template on
+// kM: kM = false is the single-mesh form (grid passed by value, no offsets),
+// kM = true finds the mesh of its work item in the batch descriptors.
+// ---------------------------------------------------------------------------
+template <bool kM>
+__device__ __forceinline__ const GridDev& grid_of(const GridDev& g0, const Meshes& M, int m)
+{
+    if constexpr (kM) return M.grids[m];
+    else return g0;
+}
+
+// mesh owning batch-wide Level1 word w: largest m with off[m].word0 <= w
+__device__ __forceinline__ int mesh_of_word(const MeshOff* __restrict__ off, int nm, uint64_t w)
+{
+    int lo = 0, hi = nm;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid].word0 <= w) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ---------------------------------------------------------------------------
 // fills (used instead of memset so that the same code can sit in a graph)
 // ---------------------------------------------------------------------------
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n)
@@ -217,15 +242,41 @@ __device__ bool ray_column_bounds(const float* tv, const MtTri& m, const GridDev
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tris, int64_t T, GridDev g,
+// mesh of every triangle of the concatenated soup
+__global__ void k_mesh_ids(const MeshOff* __restrict__ off, int nm, int64_t T, uint32_t* __restrict__ tri_mesh)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nm;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((int64_t)off[mid].tri0 <= t) lo = mid; else hi = mid;
+        }
+        tri_mesh[t] = (uint32_t)lo;
+    }
+}
+
+void launch_mesh_ids(const MeshOff* off, int nm, int64_t T, uint32_t* tri_mesh, hipStream_t s)
+{
+    if (T <= 0) return;
+    k_mesh_ids<<<grid_for(T, 256, 8192), 256, 0, s>>>(off, nm, T, tri_mesh);
+}
+
+// Single-mesh form: grid-stride over all triangles, one block reduction into
+// the mesh's test counter.  Multi-mesh form: one triangle per thread; blocks
+// inside one mesh (tri_mesh is non-decreasing) reduce into that mesh's
+// counter, the few straddling blocks add per lane.
+template <bool kM>
+__global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tris, int64_t T, GridDev g0, Meshes M,
                                                    int4* __restrict__ box0, int4* __restrict__ box1,
                                                    uint32_t* __restrict__ sat_items,
                                                    int4* __restrict__ ray_rng,
-                                                   uint32_t* __restrict__ ray_items,
-                                                   Counters* __restrict__ ctr)
+                                                   uint32_t* __restrict__ ray_items)
 {
-    uint64_t tests = 0, est = 0;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t tests = 0;
+    const int64_t first = (int64_t)blockIdx.x * blockDim.x;
+    for (int64_t t = first + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x) {
+        const int mesh = kM ? (int)M.tri_mesh[t] : 0;
+        const GridDev& g = grid_of<kM>(g0, M, mesh);
         float tv[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) tv[c] = tris[t * 9 + c];
@@ -252,7 +303,6 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
         const bool any = dd[0] > 0 && dd[1] > 0 && dd[2] > 0;
         sat_items[t] = any ? (uint32_t)(dd[0] * dd[1]) : 0u;
         tests += any ? (uint64_t)dd[0] * dd[1] * dd[2] : 0;
-        est += any ? (uint64_t)dd[0] * dd[1] + (uint64_t)dd[1] * dd[2] + (uint64_t)dd[0] * dd[2] : 0;
         uint32_t rit = 0;
         int4 rr = make_int4(0, 0, 0, 0);
         if (vis) {
@@ -272,17 +322,29 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
         ray_rng[t] = rr;
         ray_items[t] = rit;
     }
-    block_add_u64(tests, &ctr->tests_l1);
-    block_add_u64(est, &ctr->pair_est);
+    if constexpr (!kM) {
+        block_add_u64(tests, &M.ctr[0].tests_l1);
+    } else {
+        const int64_t last = (first + (int64_t)blockDim.x < T ? first + (int64_t)blockDim.x : T) - 1;
+        const uint32_t ma = M.tri_mesh[first], mb = M.tri_mesh[last];
+        if (ma == mb) {
+            block_add_u64(tests, &M.ctr[ma].tests_l1);
+        } else if (tests) {
+            atomicAdd(&M.ctr[M.tri_mesh[first + threadIdx.x]].tests_l1, (unsigned long long)tests);
+        }
+    }
 }
 
-void launch_tri_setup(const float* tris, int64_t T, const GridDev& g, int4* box0, int4* box1,
-                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, Counters* ctr,
-                      hipStream_t s)
+void launch_tri_setup(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
+                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, hipStream_t s)
 {
     if (T <= 0) return;
-    k_tri_setup<<<grid_for(T, 256, 1024), 256, 0, s>>>(tris, T, g, box0, box1, sat_items, ray_rng, ray_items,
-                                                       ctr);
+    if (M.nm > 1)
+        k_tri_setup<true><<<grid_for(T, 256, 1 << 30), 256, 0, s>>>(tris, T, g0, M, box0, box1, sat_items, ray_rng,
+                                                                     ray_items);
+    else
+        k_tri_setup<false><<<grid_for(T, 256, 1024), 256, 0, s>>>(tris, T, g0, M, box0, box1, sat_items, ray_rng,
+                                                                   ray_items);
 }
 
 // ---------------------------------------------------------------------------
@@ -558,45 +620,56 @@ __device__ __forceinline__ void expand_items(const float* __restrict__ tris, int
 
 // Level1 SAT: one item = (triangle, xy column of its index-AABB); the lane
 // walks the AABB's z range.  Hits set the boundary bit, append the
-// (voxel, triangle) bin pair, and the lane's highest hit z becomes the
-// (column, triangle, zmax) entry used by the Level2 parity candidates.
+// (voxel, triangle) bin pair -- key = batch-wide bit index, value = triangle --
+// and the lane's highest hit z becomes the (column, triangle, zmax) entry used
+// by the Level2 parity candidates.
+template <bool kM>
 __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tris, int64_t T,
                                                    const int4* __restrict__ box0,
                                                    const int4* __restrict__ box1,
-                                                   const uint64_t* __restrict__ off, GridDev g,
+                                                   const uint64_t* __restrict__ off, GridDev g0, Meshes M,
                                                    unsigned long long* __restrict__ bnd, Append64 pairs,
                                                    Append64 cols)
 {
     constexpr uint32_t PB = 1536, CB = 768;
     __shared__ uint64_t s_pk[PB];
+    __shared__ uint32_t s_pv[PB];
     __shared__ uint64_t s_ck[CB];
     __shared__ uint32_t s_cv[CB];
     __shared__ uint32_t s_n[2];
     __shared__ uint64_t s_gbase;
     if (threadIdx.x < 2) s_n[threadIdx.x] = 0;
     __syncthreads();
-    const LdsAppend lp{s_pk, nullptr, &s_n[0], PB};
+    const LdsAppend lp{s_pk, s_pv, &s_n[0], PB};
     const LdsAppend lc{s_ck, s_cv, &s_n[1], CB};
     expand_items(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
+        const int mesh = kM ? (int)M.tri_mesh[t] : 0;
+        const GridDev& g = grid_of<kM>(g0, M, mesh);
+        const uint64_t word0 = kM ? M.off[mesh].word0 : 0;
+        const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
         const int4 b0 = box0[t];
         const int4 b1 = box1[t];
         const int p = b0.x + (int)(l % (uint64_t)b1.x);
         const int q = b0.y + (int)(l / (uint64_t)b1.x);
         const float cx = g.c1x[p], cy = g.c1y[q];
-        const uint64_t rowword = (uint64_t)q * g.nxw + (uint64_t)(p >> 6);
+        const float hx = g.h1x, hy = g.h1y, hz = g.h1z;
+        const uint64_t plane = (uint64_t)g.ny * g.nxw;
+        const uint64_t rowword = word0 + (uint64_t)q * g.nxw + (uint64_t)(p >> 6);
         const uint64_t bit = 1ull << (p & 63);
+        const float* c1z = g.c1z;
         int zmax = -1;
         for (int z = b0.z; z < b0.z + b1.z; ++z) {
-            const bool hit = tri_box_overlap(cx, cy, g.c1z[z], g.h1x, g.h1y, g.h1z, tv);
-            const uint64_t w = (uint64_t)z * g.ny * g.nxw + rowword;
+            const bool hit = tri_box_overlap(cx, cy, c1z[z], hx, hy, hz, tv);
+            const uint64_t w = (uint64_t)z * plane + rowword;
             if (hit) {
                 zmax = z;
                 atomicOr(&bnd[w], (unsigned long long)bit);
             }
-            lds_append(lp, pairs, hit, ((w * 64 + (uint64_t)(p & 63)) << 32) | (uint64_t)t, 0);
+            lds_append(lp, pairs, hit, w * 64 + (uint64_t)(p & 63), (uint32_t)t);
         }
         const bool emit = (zmax >= 0) && (b0.w != 0);
-        lds_append(lc, cols, emit, ((uint64_t)((uint32_t)q * g.nx + p) << 32) | (uint64_t)t, (uint32_t)zmax);
+        lds_append(lc, cols, emit, ((uint64_t)(col0 + (uint32_t)q * g.nx + p) << 32) | (uint64_t)t,
+                   (uint32_t)zmax);
     }, [&](bool last) {
         // flush when half full (block-uniform: counters read after a barrier)
         __syncthreads();
@@ -607,21 +680,25 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
     });
 }
 
-void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1,
-                       const uint64_t* off, const GridDev& g, uint64_t* bnd, Append64 pairs,
-                       Append64 cols, int blocks, hipStream_t s)
+void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const uint64_t* off,
+                       const GridDev& g0, const Meshes& M, uint64_t* bnd, Append64 pairs, Append64 cols, int blocks,
+                       hipStream_t s)
 {
     if (T <= 0) return;
-    k_sat_expand<<<blocks, XB, 0, s>>>(tris, T, box0, box1, off, g,
-                                       reinterpret_cast<unsigned long long*>(bnd), pairs, cols);
+    unsigned long long* b = reinterpret_cast<unsigned long long*>(bnd);
+    if (M.nm > 1)
+        k_sat_expand<true><<<blocks, XB, 0, s>>>(tris, T, box0, box1, off, g0, M, b, pairs, cols);
+    else
+        k_sat_expand<false><<<blocks, XB, 0, s>>>(tris, T, box0, box1, off, g0, M, b, pairs, cols);
 }
 
 // Level1 ray parity, first half: one item = (triangle, column inside its
 // conservative bound); the exact det/u/v test keeps the (column, triangle)
 // pairs whose ray the reference's MT accepts at some height.
+template <bool kM>
 __global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tris, int64_t T,
                                                    const int4* __restrict__ rng,
-                                                   const uint64_t* __restrict__ off, GridDev g,
+                                                   const uint64_t* __restrict__ off, GridDev g0, Meshes M,
                                                    Append64 hits)
 {
     constexpr uint32_t HB = 1536;
@@ -632,6 +709,9 @@ __global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tri
     __syncthreads();
     const LdsAppend lh{s_hk, nullptr, &s_n, HB};
     expand_items(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
+        const int mesh = kM ? (int)M.tri_mesh[t] : 0;
+        const GridDev& g = grid_of<kM>(g0, M, mesh);
+        const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
         const int4 r = rng[t];
         const int i = r.x + (int)(l % (uint64_t)r.z);
         const int j = r.y + (int)(l / (uint64_t)r.z);
@@ -639,18 +719,21 @@ __global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tri
         bool ok = mt_setup(tv, m);
         float tx, ty;
         ok = ok && mt_column(m, tv[0], tv[1], g.r1x[i], g.r1y[j], tx, ty);
-        lds_append(lh, hits, ok, ((uint64_t)((uint32_t)j * g.nx + i) << 32) | (uint64_t)t, 0);
+        lds_append(lh, hits, ok, ((uint64_t)(col0 + (uint32_t)j * g.nx + i) << 32) | (uint64_t)t, 0);
     }, [&](bool last) {
         __syncthreads();
         if (last || s_n >= HB / 2) lds_flush(lh, hits, &s_gbase);
     });
 }
 
-void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off,
-                       const GridDev& g, Append64 hits, int blocks, hipStream_t s)
+void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off, const GridDev& g0,
+                       const Meshes& M, Append64 hits, int blocks, hipStream_t s)
 {
     if (T <= 0) return;
-    k_ray_expand<<<blocks, XB, 0, s>>>(tris, T, ray_rng, off, g, hits);
+    if (M.nm > 1)
+        k_ray_expand<true><<<blocks, XB, 0, s>>>(tris, T, ray_rng, off, g0, M, hits);
+    else
+        k_ray_expand<false><<<blocks, XB, 0, s>>>(tris, T, ray_rng, off, g0, M, hits);
 }
 
 // ---------------------------------------------------------------------------
@@ -670,16 +753,20 @@ void launch_count_keys_hi(const uint64_t* key, uint64_t n,
     k_count_keys_hi<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(key, n, cnt);
 }
 
+template <bool kM>
 __global__ void k_scatter_hits(const float* __restrict__ tris, const uint64_t* __restrict__ key, uint64_t n,
-                               GridDev g, const uint32_t* __restrict__ off, uint32_t* __restrict__ cnt,
+                               GridDev g0, Meshes M, const uint32_t* __restrict__ off, uint32_t* __restrict__ cnt,
                                float4* __restrict__ rec)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t k = key[i];
-        const uint32_t col = (uint32_t)(k >> 32);
+        const uint32_t gcol = (uint32_t)(k >> 32);
         const uint32_t t = (uint32_t)(k & 0xffffffffu);
-        const uint32_t pos = off[col] + atomicSub(&cnt[col], 1u) - 1u;
+        const uint32_t pos = off[gcol] + atomicSub(&cnt[gcol], 1u) - 1u;
+        const int mesh = kM ? (int)M.tri_mesh[t] : 0;
+        const GridDev& g = grid_of<kM>(g0, M, mesh);
+        const uint32_t col = gcol - (kM ? M.off[mesh].col0 : 0u);
         float tv[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) tv[c] = tris[(uint64_t)t * 9 + c];
@@ -694,12 +781,15 @@ __global__ void k_scatter_hits(const float* __restrict__ tris, const uint64_t* _
     }
 }
 
-void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n,
-                          const GridDev& g, const uint32_t* off, uint32_t* cnt,
-                         float4* rec, hipStream_t s)
+void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n, const GridDev& g0, const Meshes& M,
+                         const uint32_t* off, uint32_t* cnt, float4* rec, hipStream_t s)
 {
     if (!n) return;
-    k_scatter_hits<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(tris, key, n, g, off, cnt, rec);
+    const int blocks = grid_for((int64_t)n, 256, 16384);
+    if (M.nm > 1)
+        k_scatter_hits<true><<<blocks, 256, 0, s>>>(tris, key, n, g0, M, off, cnt, rec);
+    else
+        k_scatter_hits<false><<<blocks, 256, 0, s>>>(tris, key, n, g0, M, off, cnt, rec);
 }
 
 // column entries -> CSR by column, each with the MT record its candidates need
@@ -735,22 +825,27 @@ void launch_scatter_cols(const float* tris, const uint64_t* key, const uint32_t*
 // Level1 parity: one wave per inside-bitplane word (64 x-adjacent voxels of
 // one row); each lane XORs t > 1e-6 over its column's accepted triangles.
 // ---------------------------------------------------------------------------
+template <bool kM>
 __global__ __launch_bounds__(256) void k_l1_parity(const float4* __restrict__ rec,
-                                                   const uint32_t* __restrict__ hoff, GridDev g,
+                                                   const uint32_t* __restrict__ hoff, GridDev g0, Meshes M,
                                                    uint64_t* __restrict__ ins, int64_t W)
 {
     const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (w >= W) return;
     const int lane = __lane_id();
-    const int xw = (int)(w % g.nxw);
-    const int64_t row = w / g.nxw;
+    const int mesh = kM ? uniform_i(mesh_of_word(M.off, M.nm, (uint64_t)w)) : 0;
+    const GridDev& g = grid_of<kM>(g0, M, mesh);
+    const int64_t wl = w - (int64_t)(kM ? M.off[mesh].word0 : 0);
+    const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
+    const int xw = (int)(wl % g.nxw);
+    const int64_t row = wl / g.nxw;
     const int j = (int)(row % g.ny);
     const int k = (int)(row / g.ny);
     const int x = xw * 64 + lane;
     int par = 0;
     if (x < g.nx) {
         const float oz = g.r1z[k];
-        const uint32_t c = (uint32_t)j * g.nx + x;
+        const uint32_t c = col0 + (uint32_t)j * g.nx + x;
         const uint32_t h1 = hoff[c + 1];
         for (uint32_t h = hoff[c]; h < h1; ++h) {
             const float4 a = rec[3 * (uint64_t)h + 0];
@@ -763,13 +858,15 @@ __global__ __launch_bounds__(256) void k_l1_parity(const float4* __restrict__ re
     if (lane == 0) ins[w] = m;
 }
 
-void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g, uint64_t* ins,
-                      hipStream_t s)
+void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0, const Meshes& M, int64_t W,
+                      uint64_t* ins, hipStream_t s)
 {
-    const int64_t W = (int64_t)g.nz * g.ny * g.nxw;
     if (W <= 0) return;
     const int64_t blocks = (W * 64 + 255) / 256;
-    k_l1_parity<<<(unsigned)blocks, 256, 0, s>>>(rec, hoff, g, ins, W);
+    if (M.nm > 1)
+        k_l1_parity<true><<<(unsigned)blocks, 256, 0, s>>>(rec, hoff, g0, M, ins, W);
+    else
+        k_l1_parity<false><<<(unsigned)blocks, 256, 0, s>>>(rec, hoff, g0, M, ins, W);
 }
 
 // ---------------------------------------------------------------------------
@@ -789,30 +886,59 @@ void launch_popc_words(const uint64_t* bits, int64_t W, uint32_t* cnt, hipStream
     k_popc_words<<<grid_for(W, 256, 16384), 256, 0, s>>>(bits, W, cnt);
 }
 
+// b0 (first boundary rank) and l2w0 (first Level2 word) of every mesh, from
+// the batch-wide word prefix; one thread walks the nm meshes in order.
+__global__ void k_mesh_bounds(const uint32_t* __restrict__ wprefix, Meshes M)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint64_t l2 = 0;
+    for (int m = 0; m <= M.nm; ++m) {
+        const uint32_t b0 = wprefix[M.off[m].word0];
+        M.off[m].b0 = b0;
+        M.off[m].l2w0 = l2;
+        if (m < M.nm) l2 += (uint64_t)(wprefix[M.off[m + 1].word0] - b0) * (uint64_t)M.grids[m].w2;
+    }
+}
+
+void launch_mesh_bounds(const uint32_t* wprefix, const Meshes& M, hipStream_t s)
+{
+    k_mesh_bounds<<<1, 64, 0, s>>>(wprefix, M);
+}
+
+// ascending linear indices (within the mesh) of the boundary voxels, and in a
+// batch the mesh of each
+template <bool kM>
 __global__ void k_boundary_index(const uint64_t* __restrict__ bnd, const uint32_t* __restrict__ wprefix,
-                                 GridDev g, int64_t W, int32_t* __restrict__ bidx)
+                                 GridDev g0, Meshes M, int64_t W, int32_t* __restrict__ bidx)
 {
     for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < W;
          w += (int64_t)gridDim.x * blockDim.x) {
         uint64_t bits = bnd[w];
         if (!bits) continue;
         uint32_t pos = wprefix[w];
-        const int xw = (int)(w % g.nxw);
-        const int64_t row = w / g.nxw;
+        const int mesh = kM ? mesh_of_word(M.off, M.nm, (uint64_t)w) : 0;
+        const GridDev& g = grid_of<kM>(g0, M, mesh);
+        const int64_t wl = w - (int64_t)(kM ? M.off[mesh].word0 : 0);
+        const int xw = (int)(wl % g.nxw);
+        const int64_t row = wl / g.nxw;
         while (bits) {
             const int bit = __ffsll((unsigned long long)bits) - 1;
             bits &= bits - 1;
+            if constexpr (kM) M.bmesh[pos] = (uint32_t)mesh;
             bidx[pos++] = (int32_t)(row * g.nx + xw * 64 + bit);
         }
     }
 }
 
-void launch_boundary_index(const uint64_t* bnd, const uint32_t* wprefix, const GridDev& g,
-                           int32_t* bidx, hipStream_t s)
+void launch_boundary_index(const uint64_t* bnd, const uint32_t* wprefix, const GridDev& g0, const Meshes& M,
+                           int64_t W, int32_t* bidx, hipStream_t s)
 {
-    const int64_t W = (int64_t)g.nz * g.ny * g.nxw;
     if (W <= 0) return;
-    k_boundary_index<<<grid_for(W, 256, 16384), 256, 0, s>>>(bnd, wprefix, g, W, bidx);
+    const int blocks = grid_for(W, 256, 16384);
+    if (M.nm > 1)
+        k_boundary_index<true><<<blocks, 256, 0, s>>>(bnd, wprefix, g0, M, W, bidx);
+    else
+        k_boundary_index<false><<<blocks, 256, 0, s>>>(bnd, wprefix, g0, M, W, bidx);
 }
 
 __device__ __forceinline__ uint32_t rank_of(const uint64_t* __restrict__ bnd,
@@ -824,27 +950,26 @@ __device__ __forceinline__ uint32_t rank_of(const uint64_t* __restrict__ bnd,
     return wprefix[w] + (uint32_t)__popcll(bnd[w] & below);
 }
 
-__global__ void k_pair_rank(const uint64_t* __restrict__ pairs, uint64_t n, const uint64_t* __restrict__ bnd,
+__global__ void k_pair_rank(const uint64_t* __restrict__ bits, uint64_t n, const uint64_t* __restrict__ bnd,
                             const uint32_t* __restrict__ wprefix, uint32_t* __restrict__ pb,
                             uint32_t* __restrict__ cntb)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t b = rank_of(bnd, wprefix, pairs[i] >> 32);
+        const uint32_t b = rank_of(bnd, wprefix, bits[i]);
         pb[i] = b;
         atomicAdd(&cntb[b], 1u);
     }
 }
 
-void launch_pair_rank(const uint64_t* pairs, uint64_t n,
-                      const uint64_t* bnd, const uint32_t* wprefix, uint32_t* pb, uint32_t* cntb,
-                      hipStream_t s)
+void launch_pair_rank(const uint64_t* bits, uint64_t n, const uint64_t* bnd, const uint32_t* wprefix, uint32_t* pb,
+                      uint32_t* cntb, hipStream_t s)
 {
     if (!n) return;
-    k_pair_rank<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(pairs, n, bnd, wprefix, pb, cntb);
+    k_pair_rank<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(bits, n, bnd, wprefix, pb, cntb);
 }
 
-__global__ void k_pair_scatter(const uint64_t* __restrict__ pairs, const uint32_t* __restrict__ pb, uint64_t n,
+__global__ void k_pair_scatter(const uint32_t* __restrict__ tri, const uint32_t* __restrict__ pb, uint64_t n,
                                const uint32_t* __restrict__ boff, uint32_t* __restrict__ cntb,
                                uint32_t* __restrict__ list)
 {
@@ -852,16 +977,51 @@ __global__ void k_pair_scatter(const uint64_t* __restrict__ pairs, const uint32_
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t b = pb[i];
         const uint32_t pos = boff[b] + atomicSub(&cntb[b], 1u) - 1u;
-        list[pos] = (uint32_t)(pairs[i] & 0xffffffffu);
+        list[pos] = tri[i];
     }
 }
 
-void launch_pair_scatter(const uint64_t* pairs, const uint32_t* pb, uint64_t n,
-                          const uint32_t* boff, uint32_t* cntb, uint32_t* list,
-                         hipStream_t s)
+void launch_pair_scatter(const uint32_t* tri, const uint32_t* pb, uint64_t n, const uint32_t* boff, uint32_t* cntb,
+                         uint32_t* list, hipStream_t s)
 {
     if (!n) return;
-    k_pair_scatter<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(pairs, pb, n, boff, cntb, list);
+    k_pair_scatter<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(tri, pb, n, boff, cntb, list);
+}
+
+// Per-mesh bin totals (pairs) and largest bin of a batch.  bmesh is
+// non-decreasing, so blocks inside one mesh reduce before one atomic.
+__global__ __launch_bounds__(256) void k_mesh_bins(const uint32_t* __restrict__ boff, int64_t B, Meshes M)
+{
+    __shared__ uint32_t s_max[4];
+    const int64_t first = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t b = first + threadIdx.x;
+    const uint32_t c = b < B ? boff[b + 1] - boff[b] : 0u;
+    const int64_t last = (first + (int64_t)blockDim.x < B ? first + (int64_t)blockDim.x : B) - 1;
+    const uint32_t ma = M.bmesh[first], mb = M.bmesh[last];
+    if (ma == mb) {
+        uint32_t mx = c;
+        for (unsigned d = 32; d >= 1; d >>= 1) {
+            const uint32_t o = __shfl_xor(mx, d);
+            mx = o > mx ? o : mx;
+        }
+        if (__lane_id() == 0) s_max[threadIdx.x >> 6] = mx;
+        block_add_u64(c, &M.ctr[ma].pairs);   // synchronises the block
+        if (threadIdx.x == 0) {
+            uint32_t v = 0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) v = s_max[i] > v ? s_max[i] : v;
+            if (v) atomicMax(&M.ctr[ma].max_per_voxel, v);
+        }
+    } else if (b < B && c) {
+        const uint32_t mm = M.bmesh[b];
+        atomicAdd(&M.ctr[mm].pairs, (unsigned long long)c);
+        atomicMax(&M.ctr[mm].max_per_voxel, c);
+    }
+}
+
+void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStream_t s)
+{
+    if (B <= 0) return;
+    k_mesh_bins<<<grid_for(B, 256, 1 << 30), 256, 0, s>>>(boff, B, M);
 }
 
 // ---------------------------------------------------------------------------
@@ -884,7 +1044,8 @@ __device__ __forceinline__ float lane_f(float v, int l)
 // j<<12 | box<<18 where j is the staged triangle and box says the box-axis
 // predicates are already known to hold.
 __device__ __forceinline__ void l2_sat_run(uint32_t ent, bool valid, const float* stv, float cxl, float cyl,
-                                           float czl, int n2, const GridDev& g, unsigned long long* acc)
+                                           float czl, int n2, float h2x, float h2y, float h2z,
+                                           unsigned long long* acc)
 {
     const int p = ent & 15, q = (ent >> 4) & 15, r = (ent >> 8) & 15, j = (ent >> 12) & 63;
     const bool box = (ent >> 18) & 1;
@@ -894,16 +1055,17 @@ __device__ __forceinline__ void l2_sat_run(uint32_t ent, bool valid, const float
     float tv[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) tv[c] = stv[j * 9 + c];
-    const bool hit = box ? tri_box_overlap<false>(ccx, ccy, ccz, g.h2x, g.h2y, g.h2z, tv)
-                         : tri_box_overlap<true>(ccx, ccy, ccz, g.h2x, g.h2y, g.h2z, tv);
+    const bool hit = box ? tri_box_overlap<false>(ccx, ccy, ccz, h2x, h2y, h2z, tv)
+                         : tri_box_overlap<true>(ccx, ccy, ccz, h2x, h2y, h2z, tv);
     if (hit) atomicOr(&acc[s >> 6], 1ull << (s & 63));
 }
 
+template <bool kM>
 __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restrict__ tris,
                                                           const int32_t* __restrict__ bidx, int64_t B,
                                                           const uint32_t* __restrict__ boff,
-                                                          const uint32_t* __restrict__ list, GridDev g,
-                                                          uint64_t* __restrict__ sat2)
+                                                          const uint32_t* __restrict__ list, GridDev g0,
+                                                          Meshes M, uint64_t* __restrict__ sat2)
 {
     __shared__ unsigned long long s_acc[L2_WAVES][64];
     __shared__ float s_tv[L2_WAVES][64 * 9];
@@ -914,16 +1076,21 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
     const int lane = __lane_id();
     const int64_t b = (int64_t)blockIdx.x * L2_WAVES + wv;
     if (b >= B) return;
+    const int mesh = kM ? uniform_i((int)M.bmesh[b]) : 0;
+    const GridDev& g = grid_of<kM>(g0, M, mesh);
+    const int n2 = g.n2, w2 = g.w2, gnx = g.nx, gny = g.ny;
+    const float h2x = g.h2x, h2y = g.h2y, h2z = g.h2z;
+    if (kM && n2 == 0) return;
+    uint64_t* out = sat2 + (kM ? M.off[mesh].l2w0 + (uint64_t)(b - M.off[mesh].b0) * w2 : (uint64_t)b * w2);
     unsigned long long* acc = s_acc[wv];
-    for (int i = lane; i < g.w2; i += 64) acc[i] = 0ull;
+    for (int i = lane; i < w2; i += 64) acc[i] = 0ull;
     __builtin_amdgcn_wave_barrier();
     const int v = bidx[b];
-    const int nxy = g.nx * g.ny;
+    const int nxy = gnx * gny;
     const int k1 = v / nxy;
     const int ij = v - k1 * nxy;
-    const int j1 = ij / g.nx;
-    const int i1 = ij - j1 * g.nx;
-    const int n2 = g.n2;
+    const int j1 = ij / gnx;
+    const int i1 = ij - j1 * gnx;
     // the parent's n2 sub-centers per axis in registers, entry p in lane p
     const float cxl = lane < n2 ? g.c2x[(int64_t)i1 * n2 + lane] : 0.0f;
     const float cyl = lane < n2 ? g.c2y[(int64_t)j1 * n2 + lane] : 0.0f;
@@ -942,7 +1109,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
             const uint32_t n = qt - qh < 64 ? qt - qh : 64;
             const bool valid = (uint32_t)lane < n;
             const uint32_t ent = valid ? sq[(qh + lane) % L2_QCAP] : 0u;
-            l2_sat_run(ent, valid, stv, cxl, cyl, czl, n2, g, acc);
+            l2_sat_run(ent, valid, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
             qh += n;
         }
         __builtin_amdgcn_wave_barrier();
@@ -959,14 +1126,14 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
             }
             uint32_t mx = 0, my = 0, mz = 0;
             for (int p = 0; p < n2; ++p) {
-                mx |= (uint32_t)sat_axis_ok(tv[0], tv[3], tv[6], lane_f(cxl, p), g.h2x) << p;
-                my |= (uint32_t)sat_axis_ok(tv[1], tv[4], tv[7], lane_f(cyl, p), g.h2y) << p;
-                mz |= (uint32_t)sat_axis_ok(tv[2], tv[5], tv[8], lane_f(czl, p), g.h2z) << p;
+                mx |= (uint32_t)sat_axis_ok(tv[0], tv[3], tv[6], lane_f(cxl, p), h2x) << p;
+                my |= (uint32_t)sat_axis_ok(tv[1], tv[4], tv[7], lane_f(cyl, p), h2y) << p;
+                mz |= (uint32_t)sat_axis_ok(tv[2], tv[5], tv[8], lane_f(czl, p), h2z) << p;
             }
             smk[lane * 3 + 0] = mx;
             smk[lane * 3 + 1] = my;
             smk[lane * 3 + 2] = mz;
-            const PlaneSlab sl = plane_slab(tv, g.h2x, g.h2y, g.h2z, clo, chi);
+            const PlaneSlab sl = plane_slab(tv, h2x, h2y, h2z, clo, chi);
             // float evaluation of N.(c - t0): |error| <= 6U sum |N_q| max|c_q - t0_q|
             double sa = 0.0;
             sa += fabs(sl.nx) * fmax(fabs(clo[0] - tv[0]), fabs(chi[0] - tv[0]));
@@ -1017,7 +1184,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
                 __builtin_amdgcn_wave_barrier();
                 if (qt - qh >= 64) {   // a full wave of work
                     const uint32_t ent = sq[(qh + lane) % L2_QCAP];
-                    l2_sat_run(ent, true, stv, cxl, cyl, czl, n2, g, acc);
+                    l2_sat_run(ent, true, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
                     qh += 64;
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -1028,19 +1195,22 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
         const uint32_t n = qt - qh < 64 ? qt - qh : 64;
         const bool valid = (uint32_t)lane < n;
         const uint32_t ent = valid ? sq[(qh + lane) % L2_QCAP] : 0u;
-        l2_sat_run(ent, valid, stv, cxl, cyl, czl, n2, g, acc);
+        l2_sat_run(ent, valid, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
         qh += n;
     }
     __builtin_amdgcn_wave_barrier();
-    for (int i = lane; i < g.w2; i += 64) sat2[b * g.w2 + i] = acc[i];
+    for (int i = lane; i < w2; i += 64) out[i] = acc[i];
 }
 
-void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff,
-                   const uint32_t* list, const GridDev& g, uint64_t* sat2, hipStream_t s)
+void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
+                   const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s)
 {
     if (B <= 0) return;
     const int64_t blocks = (B + L2_WAVES - 1) / L2_WAVES;
-    k_l2_sat<<<(unsigned)blocks, 64 * L2_WAVES, 0, s>>>(tris, bidx, B, boff, list, g, sat2);
+    if (M.nm > 1)
+        k_l2_sat<true><<<(unsigned)blocks, 64 * L2_WAVES, 0, s>>>(tris, bidx, B, boff, list, g0, M, sat2);
+    else
+        k_l2_sat<false><<<(unsigned)blocks, 64 * L2_WAVES, 0, s>>>(tris, bidx, B, boff, list, g0, M, sat2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1080,10 +1250,11 @@ __device__ __forceinline__ bool colrec_xy(const ColRec& r, float ox, float oy, f
 
 constexpr int L2P_WAVES = 4;
 
+template <bool kM>
 __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __restrict__ bidx, int64_t B,
                                                               const uint32_t* __restrict__ coff,
-                                                              const float4* __restrict__ rec, GridDev g,
-                                                              uint64_t* __restrict__ par2)
+                                                              const float4* __restrict__ rec, GridDev g0,
+                                                              Meshes M, uint64_t* __restrict__ par2)
 {
     __shared__ unsigned long long s_acc[L2P_WAVES][64];
     __shared__ ColRec s_rec[L2P_WAVES][64];
@@ -1091,15 +1262,20 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
     const int lane = __lane_id();
     const int64_t b = (int64_t)blockIdx.x * L2P_WAVES + wv;
     if (b >= B) return;
+    const int mesh = kM ? uniform_i((int)M.bmesh[b]) : 0;
+    const GridDev& g = grid_of<kM>(g0, M, mesh);
+    const int n2 = g.n2, w2 = g.w2, gnx = g.nx, gny = g.ny;
+    if (kM && n2 == 0) return;
+    uint64_t* out = par2 + (kM ? M.off[mesh].l2w0 + (uint64_t)(b - M.off[mesh].b0) * w2 : (uint64_t)b * w2);
+    const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
     unsigned long long* acc = s_acc[wv];
     ColRec* cr = s_rec[wv];
     const int v = bidx[b];
-    const int nxy = g.nx * g.ny;
+    const int nxy = gnx * gny;
     const int k1 = v / nxy;
     const int ij = v - k1 * nxy;
-    const int j1 = ij / g.nx;
-    const int i1 = ij - j1 * g.nx;
-    const int n2 = g.n2;
+    const int j1 = ij / gnx;
+    const int i1 = ij - j1 * gnx;
     const int nsc = n2 * n2;
     const int nm = (nsc + 63) >> 6;
     const float* rx = g.r2x + (int64_t)i1 * n2;
@@ -1117,7 +1293,7 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) ozv[rr] = rr < n2 ? rz[rr] : 0.0f;
     uint64_t bits = 0;   // bit (mm*n2 + r) for sub-column lane + 64*mm
-    const uint32_t e0 = coff[ij], e1 = coff[ij + 1];
+    const uint32_t e0 = coff[col0 + ij], e1 = coff[col0 + ij + 1];
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t i = base + lane;
         bool valid = false;
@@ -1157,7 +1333,7 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
         }
         __builtin_amdgcn_wave_barrier();
     }
-    for (int i = lane; i < g.w2; i += 64) acc[i] = 0ull;
+    for (int i = lane; i < w2; i += 64) acc[i] = 0ull;
     __builtin_amdgcn_wave_barrier();
     for (int mm = 0; mm < nm; ++mm) {
         const int sc = lane + 64 * mm;
@@ -1170,42 +1346,58 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
         }
     }
     __builtin_amdgcn_wave_barrier();
-    for (int i = lane; i < g.w2; i += 64) par2[b * g.w2 + i] = acc[i];
+    for (int i = lane; i < w2; i += 64) out[i] = acc[i];
 }
 
-void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec, const GridDev& g,
-                      uint64_t* par2, hipStream_t s)
+void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec, const GridDev& g0,
+                      const Meshes& M, uint64_t* par2, hipStream_t s)
 {
     if (B <= 0) return;
     const int64_t blocks = (B + L2P_WAVES - 1) / L2P_WAVES;
-    k_l2_parity<<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, g, par2);
+    if (M.nm > 1)
+        k_l2_parity<true><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, g0, M, par2);
+    else
+        k_l2_parity<false><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, g0, M, par2);
 }
 
 // ---------------------------------------------------------------------------
 // counts (src/Object.cpp:3353-3378)
 // ---------------------------------------------------------------------------
+// Single-mesh form: a grid-stride pass over both planes.  Multi-mesh form:
+// blockIdx.y is the mesh, blockIdx.x strides over that mesh's words.
+template <bool kM>
 __global__ void k_count_states(const uint64_t* __restrict__ bnd, const uint64_t* __restrict__ ins, int64_t W,
                                const uint64_t* __restrict__ sat2, const uint64_t* __restrict__ par2,
-                               int64_t W2, Counters* __restrict__ ctr)
+                               int64_t W2, Meshes M)
 {
+    const int mesh = kM ? (int)blockIdx.y : 0;
+    const uint64_t w_lo = kM ? M.off[mesh].word0 : 0, w_hi = kM ? M.off[mesh + 1].word0 : (uint64_t)W;
+    const uint64_t l_lo = kM ? M.off[mesh].l2w0 : 0, l_hi = kM ? M.off[mesh + 1].l2w0 : (uint64_t)W2;
     uint64_t a = 0, b = 0, c = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < W; i += (int64_t)gridDim.x * blockDim.x)
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = w_lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < w_hi; i += stride)
         a += (uint64_t)__popcll(ins[i] & ~bnd[i]);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < W2; i += (int64_t)gridDim.x * blockDim.x) {
+    for (uint64_t i = l_lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < l_hi; i += stride) {
         b += (uint64_t)__popcll(par2[i] & ~sat2[i]);
         c += (uint64_t)__popcll(sat2[i]);
     }
-    block_add_u64(a, &ctr->l1_inside);
-    block_add_u64(b, &ctr->l2_inside);
-    block_add_u64(c, &ctr->l2_boundary);
+    block_add_u64(a, &M.ctr[mesh].l1_inside);
+    block_add_u64(b, &M.ctr[mesh].l2_inside);
+    block_add_u64(c, &M.ctr[mesh].l2_boundary);
 }
 
 void launch_count_states(const uint64_t* bnd, const uint64_t* ins, int64_t W, const uint64_t* sat2,
-                         const uint64_t* par2, int64_t W2, Counters* ctr, hipStream_t s)
+                         const uint64_t* par2, int64_t W2, const Meshes& M, hipStream_t s)
 {
     const int64_t n = W > W2 ? W : W2;
     if (n <= 0) return;
-    k_count_states<<<grid_for(n, 256, 512), 256, 0, s>>>(bnd, ins, W, sat2, par2, W2, ctr);
+    if (M.nm > 1) {
+        const int64_t per = (n + M.nm - 1) / M.nm;
+        const dim3 grid((unsigned)grid_for(per, 256, 64), (unsigned)M.nm);
+        k_count_states<true><<<grid, 256, 0, s>>>(bnd, ins, W, sat2, par2, W2, M);
+    } else {
+        k_count_states<false><<<grid_for(n, 256, 512), 256, 0, s>>>(bnd, ins, W, sat2, par2, W2, M);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1233,23 +1425,23 @@ void launch_export_l1(const uint64_t* bnd, const uint64_t* ins, const GridDev& g
     k_export_l1<<<grid_for(n1, 256, 65536), 256, 0, s>>>(bnd, ins, g, n1, unit, out);
 }
 
-__global__ void k_export_prefix(const uint64_t* __restrict__ bnd, const uint32_t* __restrict__ wprefix, GridDev g,
-                                int64_t n1, int32_t* __restrict__ out)
+__global__ void k_export_prefix(const uint64_t* __restrict__ bnd, const uint32_t* __restrict__ wprefix,
+                                uint32_t base, GridDev g, int64_t n1, int32_t* __restrict__ out)
 {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n1; v += (int64_t)gridDim.x * blockDim.x) {
         const int x = (int)(v % g.nx);
         const int64_t row = v / g.nx;
         const uint64_t bitidx = (uint64_t)(row * g.nxw + (x >> 6)) * 64 + (uint64_t)(x & 63);
-        out[v] = (int32_t)rank_of(bnd, wprefix, bitidx);
+        out[v] = (int32_t)(rank_of(bnd, wprefix, bitidx) - base);
     }
 }
 
-void launch_export_prefix(const uint64_t* bnd, const uint32_t* wprefix, const GridDev& g, int32_t* out,
-                          hipStream_t s)
+void launch_export_prefix(const uint64_t* bnd, const uint32_t* wprefix, uint32_t base, const GridDev& g,
+                          int32_t* out, hipStream_t s)
 {
     const int64_t n1 = (int64_t)g.nx * g.ny * g.nz;
     if (n1 <= 0) return;
-    k_export_prefix<<<grid_for(n1, 256, 65536), 256, 0, s>>>(bnd, wprefix, g, n1, out);
+    k_export_prefix<<<grid_for(n1, 256, 65536), 256, 0, s>>>(bnd, wprefix, base, g, n1, out);
 }
 
 __global__ void k_export_l2(const uint64_t* __restrict__ sat2, const uint64_t* __restrict__ par2, int64_t B,

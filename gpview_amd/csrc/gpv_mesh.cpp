@@ -459,7 +459,8 @@ int gpv_mesh_mixed(int kind, int64_t target, uint64_t seed, gpv_mesh* out)
     return GPV_ERR_INVALID;
 }
 
-int gpv_mesh_stress(gpv_mesh* m, int voxel_count, int voxel_count2, float p_snap, float f_sliver, uint64_t seed)
+int gpv_mesh_stress(gpv_mesh* m, int voxel_count, int voxel_count2, float p_snap, float f_sliver, float f_coplanar,
+                    uint64_t seed)
 {
     if (!m || m->nverts <= 0 || voxel_count < 1 || voxel_count2 < 1) return GPV_ERR_INVALID;
     Rng R(seed);
@@ -489,6 +490,28 @@ int gpv_mesh_stress(gpv_mesh* m, int voxel_count, int voxel_count2, float p_snap
             const double k = std::nearbyint(((double)p[q] - bmin[q]) / step);
             const float snapped = (float)((double)bmin[q] + k * step);
             if (snapped > lo[q] && snapped < hi[q]) p[q] = snapped;
+        }
+    }
+    // face-coplanar triangles: the three vertices onto one Level1 lattice plane
+    if (f_coplanar > 0 && m->ntris > 0) {
+        const int64_t want = (int64_t)std::llround(f_coplanar * (double)m->ntris);
+        for (int64_t s = 0, tries = 0; s < want && tries < 20 * want + 100; tries++) {
+            const int64_t t = R.randint(0, m->ntris - 1);
+            const int q = (int)R.randint(0, 2);
+            float* p[3];
+            bool extreme = false;
+            for (int k = 0; k < 3; k++) {
+                p[k] = m->verts + 3 * (int64_t)m->faces[3 * t + k];
+                for (int a = 0; a < 3; a++) extreme |= (p[k][a] == lo[a] || p[k][a] == hi[a]);
+            }
+            if (extreme) continue;
+            const double c = ((double)p[0][q] + p[1][q] + p[2][q]) / 3.0;
+            const double step = (double)info.g[q];
+            const double k = std::nearbyint((c - bmin[q]) / step);
+            const float plane = (float)((double)bmin[q] + k * step);
+            if (!(plane > lo[q] && plane < hi[q])) continue;
+            for (int v = 0; v < 3; v++) p[v][q] = plane;
+            s++;
         }
     }
     // slivers: split both triangles of an edge at 1e-4 of its length

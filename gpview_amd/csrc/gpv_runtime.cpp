@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -35,6 +36,18 @@ int fail(int code, const char* fmt, ...)
     g_err = buf;
     return code;
 }
+
+}  // namespace
+
+namespace gpv {
+int set_error(int code, const char* msg)
+{
+    g_err = msg ? msg : "";
+    return code;
+}
+}  // namespace gpv
+
+namespace {
 
 #define HIP_TRY(expr)                                                                    \
     do {                                                                                 \
@@ -111,23 +124,40 @@ private:
 }  // namespace
 
 struct gpv_ctx {
+    // every entry point that touches the context holds this, so a context
+    // may be shared by threads (calls on it are serialised)
+    std::recursive_mutex mu;
     int device = 0;
     hipStream_t stream = nullptr;
     Pool pool;
-    Counters* h_ctr = nullptr;   // pinned read-back slots
-    uint32_t* h_nb = nullptr;
+    char* h_buf = nullptr;   // pinned read-back staging
+    size_t h_cap = 0;
     int num_cu = 256;
 };
 
-struct gpv_grid {
+namespace gpv {
+
+// One pipeline run over nm meshes; nm = 1 is a plain voxelization.  Every
+// stage is one launch (or one short launch sequence) over all meshes.  The
+// per-mesh grids handed to callers (struct gpv_grid) are views into a job;
+// the job's device memory returns to the pool with the last view.
+struct Job {
     gpv_ctx* ctx = nullptr;
-    gpv_grid_info info{};
-    GridDev g{};
-    gpv_params params{};
+    int refs = 0;
+    int nm = 1;
+    std::vector<gpv_grid_info> info;   // [nm]
+    std::vector<GridDev> g;            // [nm], device table pointers
+    std::vector<MeshOff> off;          // [nm + 1]
+    std::vector<float> h_tab;
     int64_t T = 0;
     const float* tris = nullptr;
-    bool own_tris = false;
+    float* own_soup = nullptr;
     float* tables = nullptr;
+    GridDev* d_grids = nullptr;
+    MeshOff* d_off = nullptr;
+    uint32_t* tri_mesh = nullptr;
+    uint32_t* bmesh = nullptr;
+    MeshCtr* mctr = nullptr;
     // per triangle
     int4 *box0 = nullptr, *box1 = nullptr, *rng = nullptr;
     uint32_t *sat_items = nullptr, *ray_items = nullptr;
@@ -135,11 +165,12 @@ struct gpv_grid {
     uint32_t* scan32 = nullptr;
     size_t scan_cap = 0;
     Counters* ctr = nullptr;
-    // Level1 bitplanes
-    int64_t W = 0;
+    // Level1 bitplanes (all meshes) and xy columns
+    int64_t W = 0, ncol = 0;
     uint64_t *bnd = nullptr, *ins = nullptr;
     // append buffers
     uint64_t* pair_key = nullptr;
+    uint32_t* pair_tri = nullptr;
     uint64_t pair_cap = 0;
     uint64_t* col_key = nullptr;
     uint32_t* col_z = nullptr;
@@ -147,16 +178,18 @@ struct gpv_grid {
     uint64_t* hit_key = nullptr;
     uint64_t hit_cap = 0;
     uint64_t n_pairs = 0, n_cols = 0, n_hits = 0;
+    int regrows = 0;
     // compaction and bins
     uint32_t *wcnt = nullptr, *wprefix = nullptr;
     int32_t* bidx = nullptr;
-    int64_t B = 0;
+    int64_t B = 0, W2 = 0;
     uint32_t *pb = nullptr, *cntb = nullptr, *boff = nullptr, *list = nullptr;
     uint32_t *ccnt = nullptr, *coff = nullptr;
     float4* crec = nullptr;   // 3 float4 per column entry
     uint32_t *hcnt = nullptr, *hoff = nullptr;
     float4* hrec = nullptr;
     uint64_t *sat2 = nullptr, *par2 = nullptr;
+    int n2max = 0;
     // progress
     bool setup_done = false, sat_done = false, inout_done = false, compact_done = false,
          l2p_done = false, l2s_done = false, finished = false;
@@ -164,37 +197,48 @@ struct gpv_grid {
     // 4 compaction+bins, 5 L2 parity, 6 L2 SAT start, 7 L2 SAT end, 8 finish
     hipEvent_t ev[9] = {};
     std::vector<void*> bufs;
+
+    Meshes view() const { return Meshes{d_grids, d_off, tri_mesh, bmesh, mctr, nm}; }
+    int64_t nb(int m) const { return (int64_t)off[m + 1].b0 - (int64_t)off[m].b0; }
+};
+
+}  // namespace gpv
+
+struct gpv_grid {
+    gpv_ctx* ctx = nullptr;
+    gpv::Job* job = nullptr;
+    int m = 0;
 };
 
 namespace {
 
 template <typename T>
-int alloc(gpv_grid* G, T** p, size_t count)
+int alloc(Job* J, T** p, size_t count)
 {
     void* q = nullptr;
-    hipError_t e = G->ctx->pool.get(std::max<size_t>(count, 1) * sizeof(T), &q);
+    hipError_t e = J->ctx->pool.get(std::max<size_t>(count, 1) * sizeof(T), &q);
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? GPV_ERR_OOM : GPV_ERR_HIP, "device allocation of %zu bytes: %s",
                     count * sizeof(T), hipGetErrorString(e));
     *p = static_cast<T*>(q);
-    G->bufs.push_back(q);
+    J->bufs.push_back(q);
     return GPV_OK;
 }
 
 template <typename T>
-void release(gpv_grid* G, T*& p)
+void release(Job* J, T*& p)
 {
     if (!p) return;
     void* q = (void*)p;
-    G->ctx->pool.put(q);
-    auto it = std::find(G->bufs.begin(), G->bufs.end(), q);
-    if (it != G->bufs.end()) G->bufs.erase(it);
+    J->ctx->pool.put(q);
+    auto it = std::find(J->bufs.begin(), J->bufs.end(), q);
+    if (it != J->bufs.end()) J->bufs.erase(it);
     p = nullptr;
 }
 
-#define TRY(expr)                 \
-    do {                          \
-        int r_ = (expr);          \
+#define TRY(expr)                    \
+    do {                             \
+        int r_ = (expr);             \
         if (r_ != GPV_OK) return r_; \
     } while (0)
 
@@ -269,44 +313,530 @@ void build_tables(const gpv_grid_info& I, std::vector<float>& tab, size_t off[12
 
 int expand_blocks(const gpv_ctx* c) { return std::max(1, c->num_cu) * 8; }
 
-int ensure_scan_tmp(gpv_grid* G, int64_t n)
+int ensure_hbuf(gpv_ctx* c, size_t bytes)
+{
+    if (bytes <= c->h_cap) return GPV_OK;
+    if (c->h_buf) (void)hipHostFree(c->h_buf);
+    c->h_buf = nullptr;
+    c->h_cap = 0;
+    bytes = std::max<size_t>(bytes, 4096);
+    HIP_TRY(hipHostMalloc((void**)&c->h_buf, bytes, hipHostMallocDefault));
+    c->h_cap = bytes;
+    return GPV_OK;
+}
+
+// pinned staging layout: Counters | MeshOff[nm + 1] | MeshCtr[nm]
+constexpr size_t kHOff = 64;
+size_t hctr_at(int nm) { return (kHOff + sizeof(MeshOff) * (size_t)(nm + 1) + 63) / 64 * 64; }
+
+int ensure_scan_tmp(Job* J, int64_t n)
 {
     const size_t need = scan_tmp_elems(n);
-    if (need <= G->scan_cap) return GPV_OK;
-    release(G, G->scan64);
-    G->scan_cap = need;
-    TRY(alloc(G, &G->scan64, need));
-    G->scan32 = reinterpret_cast<uint32_t*>(G->scan64);   // u32 scans use the same storage
+    if (need <= J->scan_cap) return GPV_OK;
+    release(J, J->scan64);
+    J->scan_cap = need;
+    TRY(alloc(J, &J->scan64, need));
+    J->scan32 = reinterpret_cast<uint32_t*>(J->scan64);   // u32 scans use the same storage
+    return GPV_OK;
+}
+
+void job_free(Job* J)
+{
+    gpv_ctx* c = J->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);   // pooled blocks may be reused right away
+    for (void* p : J->bufs) c->pool.put(p);
+    for (auto& e : J->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete J;
+}
+
+// GridDev of one mesh from its derived grid (device table pointers set later)
+GridDev grid_dev(const gpv_grid_info& info)
+{
+    GridDev g{};
+    g.nx = info.n[0]; g.ny = info.n[1]; g.nz = info.n[2];
+    g.nxw = (g.nx + 63) / 64;
+    g.n2 = info.n2;
+    g.w2 = info.n2 > 0 ? (info.n2 * info.n2 * info.n2 + 63) / 64 : 0;
+    g.bminx = info.bmin[0]; g.bminy = info.bmin[1]; g.bminz = info.bmin[2];
+    g.bmaxx = info.bmax[0]; g.bmaxy = info.bmax[1]; g.bmaxz = info.bmax[2];
+    g.h1x = (float)((double)info.g[0] / 2.0); g.h1y = (float)((double)info.g[1] / 2.0);
+    g.h1z = (float)((double)info.g[2] / 2.0);
+    g.h2x = (float)((double)info.g2[0] / 2.0); g.h2y = (float)((double)info.g2[1] / 2.0);
+    g.h2z = (float)((double)info.g2[2] / 2.0);
+    return g;
+}
+
+// Grids, offsets, tables, the concatenated soup and every size-independent
+// buffer of a job.  nm == 1 with a device soup uses the caller's pointer.
+int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** out)
+{
+    *out = nullptr;
+    if (nm < 1) return fail(GPV_ERR_INVALID, "a batch needs at least one mesh");
+    std::vector<gpv_grid_info> info((size_t)nm);
+    int64_t T = 0, W = 0, ncol = 0, vox = 0;
+    uint64_t pair_cap = 0, hit_cap = 0;
+    for (int m = 0; m < nm; ++m) {
+        if (d[m].ntris < 0 || (d[m].ntris > 0 && !d[m].tris))
+            return fail(GPV_ERR_INVALID, "mesh %d: bad triangle soup", m);
+        TRY(derive_grid(d[m].bmin, d[m].bmax, d[m].max_model_size, &d[m].params, &info[m]));
+        const int64_t nx = info[m].n[0], ny = info[m].n[1], nz = info[m].n[2];
+        if (nx * ny >= (int64_t(1) << 31)) return fail(GPV_ERR_INVALID, "mesh %d: too many columns", m);
+        info[m].n1 = nx * ny * nz;
+        T += d[m].ntris;
+        W += nz * ny * ((nx + 63) / 64);
+        ncol += nx * ny;
+        vox += nz * ny * ((nx + 63) / 64) * 64;
+        const int64_t nmax = std::max(nx, std::max(ny, nz));
+        // bin-pair capacity: the caller's, or an estimate from the triangle
+        // count and the surface voxels a resolution can hold
+        const int64_t pc = d[m].params.initial_capacity > 0
+                               ? d[m].params.initial_capacity
+                               : std::max<int64_t>(int64_t(1) << 16, 16 * d[m].ntris + 8 * nmax * nmax);
+        pair_cap += (uint64_t)pc;
+        hit_cap += (uint64_t)(8 * d[m].ntris + 4 * nx * ny);
+    }
+    if (T >= (int64_t(1) << 31)) return fail(GPV_ERR_INVALID, "at most 2^31-1 triangles per batch");
+    if (ncol >= (int64_t(1) << 32) - 1) return fail(GPV_ERR_INVALID, "at most 2^32-2 xy columns per batch");
+    if (vox >= (int64_t(1) << 32)) return fail(GPV_ERR_INVALID, "at most 2^32-1 Level1 voxels per batch");
+    if (T > 0 && on_device && nm == 1 && !d[0].tris) return fail(GPV_ERR_INVALID, "NULL soup");
+    HIP_TRY(hipSetDevice(c->device));
+    TRY(ensure_hbuf(c, hctr_at(nm) + sizeof(MeshCtr) * (size_t)nm + 64));
+
+    Job* J = new Job();
+    J->ctx = c;
+    J->nm = nm;
+    J->info = std::move(info);
+    J->T = T;
+    J->W = W;
+    J->ncol = ncol;
+    auto bail = [&](int r) {
+        job_free(J);
+        return r;
+    };
+    for (auto& e : J->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(GPV_ERR_HIP, "event creation failed"));
+
+    // per-mesh grids, offsets and the concatenated center tables
+    J->g.resize((size_t)nm);
+    J->off.assign((size_t)nm + 1, MeshOff{});
+    std::vector<size_t> tab_at((size_t)nm * 12);
+    uint64_t word0 = 0;
+    uint32_t col0 = 0, tri0 = 0;
+    for (int m = 0; m < nm; ++m) {
+        GridDev& g = J->g[m];
+        g = grid_dev(J->info[m]);
+        J->n2max = std::max(J->n2max, g.n2);
+        J->off[m].word0 = word0;
+        J->off[m].col0 = col0;
+        J->off[m].tri0 = tri0;
+        word0 += (uint64_t)g.nz * g.ny * g.nxw;
+        col0 += (uint32_t)(g.nx * g.ny);
+        tri0 += (uint32_t)d[m].ntris;
+        std::vector<float> tab;
+        size_t o[12];
+        build_tables(J->info[m], tab, o);
+        for (int k = 0; k < 12; ++k) tab_at[(size_t)m * 12 + k] = J->h_tab.size() + o[k];
+        J->h_tab.insert(J->h_tab.end(), tab.begin(), tab.end());
+    }
+    J->off[nm].word0 = word0;
+    J->off[nm].col0 = col0;
+    J->off[nm].tri0 = tri0;
+    int r;
+    if ((r = alloc(J, &J->tables, J->h_tab.size())) != GPV_OK) return bail(r);
+    if (hipMemcpyAsync(J->tables, J->h_tab.data(), J->h_tab.size() * sizeof(float), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess)
+        return bail(fail(GPV_ERR_HIP, "table upload failed"));
+    for (int m = 0; m < nm; ++m) {
+        const float* tb = J->tables;
+        const size_t* o = &tab_at[(size_t)m * 12];
+        GridDev& g = J->g[m];
+        g.c1x = tb + o[0]; g.c1y = tb + o[1]; g.c1z = tb + o[2];
+        g.r1x = tb + o[3]; g.r1y = tb + o[4]; g.r1z = tb + o[5];
+        g.c2x = tb + o[6]; g.c2y = tb + o[7]; g.c2z = tb + o[8];
+        g.r2x = tb + o[9]; g.r2y = tb + o[10]; g.r2z = tb + o[11];
+    }
+
+    // the soup
+    if (nm == 1 && on_device) {
+        J->tris = d[0].tris;
+    } else if (T > 0) {
+        float* s = nullptr;
+        if ((r = alloc(J, &s, (size_t)T * 9)) != GPV_OK) return bail(r);
+        const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        for (int m = 0; m < nm; ++m)
+            if (d[m].ntris > 0 &&
+                hipMemcpyAsync(s + (size_t)J->off[m].tri0 * 9, d[m].tris, (size_t)d[m].ntris * 9 * sizeof(float),
+                               kind, c->stream) != hipSuccess)
+                return bail(fail(GPV_ERR_HIP, "soup copy of mesh %d failed", m));
+        J->tris = s;
+        J->own_soup = s;
+    }
+
+    // batch descriptors
+    if ((r = alloc(J, &J->mctr, (size_t)nm)) != GPV_OK) return bail(r);
+    if (nm > 1) {
+        if ((r = alloc(J, &J->d_grids, (size_t)nm)) || (r = alloc(J, &J->d_off, (size_t)nm + 1)) ||
+            (r = alloc(J, &J->tri_mesh, (size_t)std::max<int64_t>(T, 1))))
+            return bail(r);
+        if (hipMemcpyAsync(J->d_grids, J->g.data(), sizeof(GridDev) * (size_t)nm, hipMemcpyHostToDevice,
+                           c->stream) != hipSuccess ||
+            hipMemcpyAsync(J->d_off, J->off.data(), sizeof(MeshOff) * ((size_t)nm + 1), hipMemcpyHostToDevice,
+                           c->stream) != hipSuccess)
+            return bail(fail(GPV_ERR_HIP, "batch descriptor upload failed"));
+    }
+
+    const size_t Tn = (size_t)std::max<int64_t>(T, 1);
+    if ((r = alloc(J, &J->box0, Tn)) || (r = alloc(J, &J->box1, Tn)) || (r = alloc(J, &J->rng, Tn)) ||
+        (r = alloc(J, &J->sat_items, Tn)) || (r = alloc(J, &J->ray_items, Tn)) ||
+        (r = alloc(J, &J->sat_off, Tn + 1)) || (r = alloc(J, &J->ray_off, Tn + 1)) ||
+        (r = alloc(J, &J->ctr, 1)) || (r = alloc(J, &J->bnd, (size_t)W)) || (r = alloc(J, &J->ins, (size_t)W)) ||
+        (r = alloc(J, &J->wcnt, (size_t)W)) || (r = alloc(J, &J->wprefix, (size_t)W + 1)))
+        return bail(r);
+    J->pair_cap = pair_cap;
+    J->col_cap = J->pair_cap;
+    J->hit_cap = std::max<uint64_t>(hit_cap, 1u << 16);
+    if ((r = alloc(J, &J->pair_key, J->pair_cap)) || (r = alloc(J, &J->pair_tri, J->pair_cap)) ||
+        (r = alloc(J, &J->col_key, J->col_cap)) || (r = alloc(J, &J->col_z, J->col_cap)) ||
+        (r = alloc(J, &J->hit_key, J->hit_cap)))
+        return bail(r);
+    *out = J;
     return GPV_OK;
 }
 
 // per-triangle setup shared by the SAT and the ray paths
-int stage_setup(gpv_grid* G)
+int stage_setup(Job* J)
 {
-    if (G->setup_done) return GPV_OK;
-    gpv_ctx* c = G->ctx;
+    if (J->setup_done) return GPV_OK;
+    gpv_ctx* c = J->ctx;
     hipStream_t s = c->stream;
-    HIP_TRY(hipEventRecord(G->ev[0], s));
-    launch_fill_u64(reinterpret_cast<uint64_t*>(G->ctr), 0, sizeof(Counters) / 8, s);
-    launch_tri_setup(G->tris, G->T, G->g, G->box0, G->box1, G->sat_items, G->rng, G->ray_items, G->ctr, s);
-    TRY(ensure_scan_tmp(G, G->T));
-    launch_scan_u32_u64(G->sat_items, G->sat_off, G->T, G->scan64, s);
-    launch_scan_u32_u64(G->ray_items, G->ray_off, G->T, G->scan64, s);
+    HIP_TRY(hipEventRecord(J->ev[0], s));
+    launch_fill_u64(reinterpret_cast<uint64_t*>(J->mctr), 0, (int64_t)(sizeof(MeshCtr) / 8) * J->nm, s);
+    if (J->nm > 1) launch_mesh_ids(J->d_off, J->nm, J->T, J->tri_mesh, s);
+    launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items, s);
+    TRY(ensure_scan_tmp(J, J->T));
+    launch_scan_u32_u64(J->sat_items, J->sat_off, J->T, J->scan64, s);
+    launch_scan_u32_u64(J->ray_items, J->ray_off, J->T, J->scan64, s);
     HIP_TRY(hipGetLastError());
-    G->setup_done = true;
+    J->setup_done = true;
     return GPV_OK;
 }
 
-int make_appends(gpv_grid* G)
+// Level1 SAT sweep + the boundary popcount scan (ranks need the final bits)
+int launch_sat_half(Job* J)
 {
-    TRY(alloc(G, &G->pair_key, G->pair_cap));
-    TRY(alloc(G, &G->col_key, G->col_cap));
-    TRY(alloc(G, &G->col_z, G->col_cap));
+    gpv_ctx* c = J->ctx;
+    hipStream_t s = c->stream;
+    HIP_TRY(hipEventRecord(J->ev[1], s));
+    launch_fill_u64(J->bnd, 0, J->W, s);
+    launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->pairs), 0, 2, s);   // pairs, colents
+    Append64 pa{&J->ctr->pairs, J->pair_key, J->pair_tri, J->pair_cap};
+    Append64 co{&J->ctr->colents, J->col_key, J->col_z, J->col_cap};
+    launch_sat_expand(J->tris, J->T, J->box0, J->box1, J->sat_off, J->g[0], J->view(), J->bnd, pa, co,
+                      expand_blocks(c), s);
+    launch_popc_words(J->bnd, J->W, J->wcnt, s);
+    TRY(ensure_scan_tmp(J, J->W));
+    launch_scan_u32_u32(J->wcnt, J->wprefix, J->W, J->scan32, nullptr, s);
+    if (J->nm > 1) launch_mesh_bounds(J->wprefix, J->view(), s);
+    HIP_TRY(hipGetLastError());
+    return GPV_OK;
+}
+
+int launch_ray_half(Job* J)
+{
+    gpv_ctx* c = J->ctx;
+    hipStream_t s = c->stream;
+    HIP_TRY(hipEventRecord(J->ev[2], s));
+    launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->hits), 0, 1, s);
+    Append64 hi{&J->ctr->hits, J->hit_key, nullptr, J->hit_cap};
+    launch_ray_expand(J->tris, J->T, J->rng, J->ray_off, J->g[0], J->view(), hi, expand_blocks(c), s);
+    HIP_TRY(hipGetLastError());
+    return GPV_OK;
+}
+
+// one round trip: the counters and, when asked, the boundary offsets
+int read_counts(Job* J, bool with_b)
+{
+    gpv_ctx* c = J->ctx;
+    hipStream_t s = c->stream;
+    HIP_TRY(hipMemcpyAsync(c->h_buf, J->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    if (with_b) {
+        if (J->nm > 1)
+            HIP_TRY(hipMemcpyAsync(c->h_buf + kHOff, J->d_off, sizeof(MeshOff) * ((size_t)J->nm + 1),
+                                   hipMemcpyDeviceToHost, s));
+        else
+            HIP_TRY(hipMemcpyAsync(c->h_buf + kHOff, J->wprefix + J->W, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return GPV_OK;
+}
+
+const Counters& hctr(const Job* J) { return *reinterpret_cast<const Counters*>(J->ctx->h_buf); }
+
+// after read_counts: accept the sweep or regrow the bin buffers to the exact
+// count (the reference's retry, src/Object.cpp:3198-3214)
+int complete_sat(Job* J, int64_t* needed)
+{
+    gpv_ctx* c = J->ctx;
+    const uint64_t np = hctr(J).pairs, nc = hctr(J).colents;
+    *needed = 0;
+    if (np > J->pair_cap || nc > J->col_cap) {
+        const uint64_t need = std::max(np, nc);
+        release(J, J->pair_key);
+        release(J, J->pair_tri);
+        release(J, J->col_key);
+        release(J, J->col_z);
+        J->pair_cap = need + need / 8 + 1024;
+        J->col_cap = J->pair_cap;
+        TRY(alloc(J, &J->pair_key, J->pair_cap));
+        TRY(alloc(J, &J->pair_tri, J->pair_cap));
+        TRY(alloc(J, &J->col_key, J->col_cap));
+        TRY(alloc(J, &J->col_z, J->col_cap));
+        J->regrows++;
+        *needed = (int64_t)need;
+        return GPV_OK;
+    }
+    if (np >= (uint64_t(1) << 32) || nc >= (uint64_t(1) << 32))
+        return fail(GPV_ERR_CAPACITY, "%llu bin entries exceed the 2^32-1 a run can index; split the batch",
+                    (unsigned long long)std::max(np, nc));
+    J->n_pairs = np;
+    J->n_cols = nc;
+    if (J->nm > 1) {
+        memcpy(J->off.data(), c->h_buf + kHOff, sizeof(MeshOff) * ((size_t)J->nm + 1));
+    } else {
+        const uint32_t B = *reinterpret_cast<const uint32_t*>(c->h_buf + kHOff);
+        J->off[0].b0 = 0;
+        J->off[0].l2w0 = 0;
+        J->off[1].b0 = B;
+        J->off[1].l2w0 = (uint64_t)B * (uint64_t)J->g[0].w2;
+    }
+    J->B = J->off[J->nm].b0;
+    J->W2 = (int64_t)J->off[J->nm].l2w0;
+    J->sat_done = true;
+    return GPV_OK;
+}
+
+// after read_counts: false (and a regrown buffer) when the hits overflowed
+int complete_ray(Job* J, bool* ok)
+{
+    const uint64_t nh = hctr(J).hits;
+    *ok = nh <= J->hit_cap && nh < (uint64_t(1) << 32);
+    if (*ok) {
+        J->n_hits = nh;
+        return GPV_OK;
+    }
+    if (nh >= (uint64_t(1) << 32)) return fail(GPV_ERR_CAPACITY, "more than 2^32-1 ray hits; split the batch");
+    release(J, J->hit_key);
+    J->hit_cap = nh + nh / 8 + 1024;
+    return alloc(J, &J->hit_key, J->hit_cap);
+}
+
+// hits grouped by column, then the parity of every Level1 voxel
+int launch_parity(Job* J)
+{
+    hipStream_t s = J->ctx->stream;
+    HIP_TRY(hipEventRecord(J->ev[3], s));
+    release(J, J->hcnt);
+    release(J, J->hoff);
+    release(J, J->hrec);
+    TRY(alloc(J, &J->hcnt, (size_t)J->ncol));
+    TRY(alloc(J, &J->hoff, (size_t)J->ncol + 1));
+    TRY(alloc(J, &J->hrec, (size_t)std::max<uint64_t>(J->n_hits, 1) * 3));
+    launch_fill_u32(J->hcnt, 0, J->ncol, s);
+    launch_count_keys_hi(J->hit_key, J->n_hits, J->hcnt, s);
+    TRY(ensure_scan_tmp(J, J->ncol));
+    launch_scan_u32_u32(J->hcnt, J->hoff, J->ncol, J->scan32, nullptr, s);
+    launch_scatter_hits(J->tris, J->hit_key, J->n_hits, J->g[0], J->view(), J->hoff, J->hcnt, J->hrec, s);
+    launch_l1_parity(J->hrec, J->hoff, J->g[0], J->view(), J->W, J->ins, s);
+    HIP_TRY(hipGetLastError());
+    J->inout_done = true;
+    return GPV_OK;
+}
+
+// boundary index list, bins per boundary voxel, column entries with records
+int launch_compact(Job* J)
+{
+    hipStream_t s = J->ctx->stream;
+    HIP_TRY(hipEventRecord(J->ev[4], s));
+    const size_t Bn = (size_t)std::max<int64_t>(J->B, 1);
+    release(J, J->bidx);
+    release(J, J->bmesh);
+    TRY(alloc(J, &J->bidx, Bn));
+    if (J->nm > 1) TRY(alloc(J, &J->bmesh, Bn));
+    launch_boundary_index(J->bnd, J->wprefix, J->g[0], J->view(), J->W, J->bidx, s);
+    release(J, J->pb);
+    release(J, J->cntb);
+    release(J, J->boff);
+    release(J, J->list);
+    TRY(alloc(J, &J->pb, (size_t)std::max<uint64_t>(J->n_pairs, 1)));
+    TRY(alloc(J, &J->cntb, Bn));
+    TRY(alloc(J, &J->boff, (size_t)J->B + 1));
+    TRY(alloc(J, &J->list, (size_t)std::max<uint64_t>(J->n_pairs, 1)));
+    launch_fill_u32(J->cntb, 0, J->B, s);
+    launch_pair_rank(J->pair_key, J->n_pairs, J->bnd, J->wprefix, J->pb, J->cntb, s);
+    TRY(ensure_scan_tmp(J, J->B));
+    launch_scan_u32_u32(J->cntb, J->boff, J->B, J->scan32, J->nm > 1 ? nullptr : &J->mctr[0].max_per_voxel, s);
+    launch_pair_scatter(J->pair_tri, J->pb, J->n_pairs, J->boff, J->cntb, J->list, s);
+    if (J->nm > 1) launch_mesh_bins(J->boff, J->B, J->view(), s);
+    release(J, J->ccnt);
+    release(J, J->coff);
+    release(J, J->crec);
+    TRY(alloc(J, &J->ccnt, (size_t)J->ncol));
+    TRY(alloc(J, &J->coff, (size_t)J->ncol + 1));
+    TRY(alloc(J, &J->crec, (size_t)std::max<uint64_t>(J->n_cols, 1) * 3));
+    launch_fill_u32(J->ccnt, 0, J->ncol, s);
+    launch_count_keys_hi(J->col_key, J->n_cols, J->ccnt, s);
+    TRY(ensure_scan_tmp(J, J->ncol));
+    launch_scan_u32_u32(J->ccnt, J->coff, J->ncol, J->scan32, nullptr, s);
+    launch_scatter_cols(J->tris, J->col_key, J->col_z, J->n_cols, J->coff, J->ccnt, J->crec, s);
+    HIP_TRY(hipGetLastError());
+    if (J->n2max > 0) {
+        release(J, J->sat2);
+        release(J, J->par2);
+        TRY(alloc(J, &J->sat2, (size_t)std::max<int64_t>(J->W2, 1)));
+        TRY(alloc(J, &J->par2, (size_t)std::max<int64_t>(J->W2, 1)));
+    }
+    J->compact_done = true;
+    return GPV_OK;
+}
+
+int run_l2_parity(Job* J)
+{
+    if (J->n2max <= 0) return GPV_OK;
+    hipStream_t s = J->ctx->stream;
+    HIP_TRY(hipEventRecord(J->ev[5], s));
+    launch_l2_parity(J->bidx, J->B, J->coff, J->crec, J->g[0], J->view(), J->par2, s);
+    HIP_TRY(hipGetLastError());
+    J->l2p_done = true;
+    return GPV_OK;
+}
+
+int run_l2_sat(Job* J)
+{
+    if (J->n2max <= 0) return GPV_OK;
+    hipStream_t s = J->ctx->stream;
+    HIP_TRY(hipEventRecord(J->ev[6], s));
+    launch_l2_sat(J->tris, J->bidx, J->B, J->boff, J->list, J->g[0], J->view(), J->sat2, s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(J->ev[7], s));
+    J->l2s_done = true;
+    return GPV_OK;
+}
+
+// counts, per-mesh info, and the release of everything but the results
+int job_finish(Job* J)
+{
+    gpv_ctx* c = J->ctx;
+    hipStream_t s = c->stream;
+    launch_count_states(J->bnd, J->ins, J->W, J->sat2, J->par2, J->W2, J->view(), s);
+    HIP_TRY(hipEventRecord(J->ev[8], s));
+    const size_t at = hctr_at(J->nm);
+    HIP_TRY(hipMemcpyAsync(c->h_buf + at, J->mctr, sizeof(MeshCtr) * (size_t)J->nm, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const MeshCtr* mc = reinterpret_cast<const MeshCtr*>(c->h_buf + at);
+    auto el = [&](int a, int b) -> float {
+        float v = 0.0f;
+        if (hipEventElapsedTime(&v, J->ev[a], J->ev[b]) != hipSuccess) {
+            (void)hipGetLastError();
+            v = 0.0f;
+        }
+        return v;
+    };
+    // setup, L1 SAT (+ boundary scan), L1 parity (ray expansion, the size
+    // read-back and the parity fill), compaction + bins, L2 parity, L2 SAT;
+    // for a batch these are the batch's stage times
+    float ms[6] = {el(0, 1), el(1, 2), el(2, 4), 0.0f, 0.0f, 0.0f};
+    if (J->n2max > 0) {
+        ms[3] = el(4, 5);
+        ms[4] = el(5, 6);
+        ms[5] = el(6, 7);
+    } else {
+        ms[3] = el(4, 8);
+    }
+    const float total = el(0, 8);
+    for (int m = 0; m < J->nm; ++m) {
+        gpv_grid_info& I = J->info[m];
+        const int64_t n23 = (int64_t)I.n2 * I.n2 * I.n2;
+        I.nb = J->nb(m);
+        I.l1_boundary = I.nb;
+        I.l1_inside = (int64_t)mc[m].l1_inside;
+        I.l2_inside = (int64_t)mc[m].l2_inside;
+        I.l2_boundary = (int64_t)mc[m].l2_boundary;
+        I.tests_l1 = (int64_t)mc[m].tests_l1;
+        I.pairs = J->nm > 1 ? (int64_t)mc[m].pairs : (int64_t)J->n_pairs;
+        I.tests_l2 = I.n2 > 0 ? I.pairs * n23 : 0;
+        I.max_tris_per_voxel = (int32_t)mc[m].max_per_voxel;
+        I.regrows = J->regrows;
+        for (int k = 0; k < 6; ++k) I.ms_stage[k] = ms[k];
+        I.ms_total = total;
+    }
+    // only the result planes stay resident
+    release(J, J->box0); release(J, J->box1); release(J, J->rng);
+    release(J, J->sat_items); release(J, J->ray_items); release(J, J->sat_off); release(J, J->ray_off);
+    release(J, J->scan64); J->scan32 = nullptr; J->scan_cap = 0;
+    release(J, J->pair_key); release(J, J->pair_tri); release(J, J->col_key); release(J, J->col_z);
+    release(J, J->hit_key); release(J, J->wcnt); release(J, J->pb); release(J, J->cntb); release(J, J->boff);
+    release(J, J->list); release(J, J->ccnt); release(J, J->coff); release(J, J->crec); release(J, J->hcnt);
+    release(J, J->hoff); release(J, J->hrec); release(J, J->tri_mesh); release(J, J->bmesh);
+    release(J, J->d_grids); release(J, J->d_off); release(J, J->own_soup); release(J, J->mctr);
+    J->tris = nullptr;
+    J->finished = true;
+    return GPV_OK;
+}
+
+// the fused pipeline: every stage back to back, one size read-back
+int job_run(Job* J)
+{
+    TRY(stage_setup(J));
+    TRY(launch_sat_half(J));
+    TRY(launch_ray_half(J));
+    TRY(read_counts(J, true));
+    int64_t needed = 0;
+    bool ray_ok = false;
+    TRY(complete_sat(J, &needed));
+    TRY(complete_ray(J, &ray_ok));
+    for (int attempt = 0; attempt < 2 && (needed > 0 || !ray_ok); ++attempt) {
+        if (needed > 0) TRY(launch_sat_half(J));
+        if (!ray_ok) TRY(launch_ray_half(J));
+        TRY(read_counts(J, true));
+        if (needed > 0) TRY(complete_sat(J, &needed));
+        if (!ray_ok) TRY(complete_ray(J, &ray_ok));
+    }
+    if (needed > 0 || !ray_ok) return fail(GPV_ERR_CAPACITY, "buffer capacity did not converge");
+    TRY(launch_parity(J));
+    TRY(launch_compact(J));
+    TRY(run_l2_parity(J));
+    TRY(run_l2_sat(J));
+    return job_finish(J);
+}
+
+gpv_grid* make_view(Job* J, int m)
+{
+    gpv_grid* G = new gpv_grid();
+    G->ctx = J->ctx;
+    G->job = J;
+    G->m = m;
+    J->refs++;
+    return G;
+}
+
+int check_grid(gpv_ctx* c, const gpv_grid* G)
+{
+    if (!c || !G || G->ctx != c || !G->job) return fail(GPV_ERR_INVALID, "bad context/grid");
+    return GPV_OK;
+}
+
+// staged entry points drive single-mesh jobs only
+int check_staged(gpv_ctx* c, const gpv_grid* G)
+{
+    TRY(check_grid(c, G));
+    if (G->job->nm != 1) return fail(GPV_ERR_STATE, "stage entry points take grids from gpv_grid_create");
+    if (G->job->finished) return fail(GPV_ERR_STATE, "grid is already finished");
     return GPV_OK;
 }
 
 }  // namespace
-
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -349,13 +879,15 @@ int gpv_ctx_create(int device, unsigned flags, gpv_ctx** out)
     gpv_ctx* c = new gpv_ctx();
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&c->h_ctr, sizeof(Counters) + 64, hipHostMallocDefault) != hipSuccess) {
-        if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
-        return fail(GPV_ERR_HIP, "stream / pinned buffer creation failed");
+        return fail(GPV_ERR_HIP, "stream creation failed");
     }
-    c->h_nb = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->h_ctr) + sizeof(Counters) + 32);
+    if (ensure_hbuf(c, 4096) != GPV_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(GPV_ERR_HIP, "pinned staging buffer allocation failed");
+    }
     *out = c;
     return GPV_OK;
 }
@@ -363,17 +895,22 @@ int gpv_ctx_create(int device, unsigned flags, gpv_ctx** out)
 int gpv_ctx_destroy(gpv_ctx* c)
 {
     if (!c) return GPV_OK;
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    c->pool.trim();
-    if (c->h_ctr) (void)hipHostFree(c->h_ctr);
-    (void)hipStreamDestroy(c->stream);
+    {
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        c->pool.trim();
+        if (c->h_buf) (void)hipHostFree(c->h_buf);
+        (void)hipStreamDestroy(c->stream);
+    }
     delete c;
     return GPV_OK;
 }
 
 int gpv_device_upload(gpv_ctx* c, const void* host, int64_t bytes, void** dev)
 {
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
     if (!c || !dev || bytes < 0 || (bytes > 0 && !host)) return fail(GPV_ERR_INVALID, "bad upload arguments");
     HIP_TRY(hipSetDevice(c->device));
     void* p = nullptr;
@@ -393,6 +930,8 @@ int gpv_device_upload(gpv_ctx* c, const void* host, int64_t bytes, void** dev)
 
 int gpv_device_free(gpv_ctx* c, void* dev)
 {
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
     if (!c) return fail(GPV_ERR_INVALID, "NULL context");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -438,387 +977,111 @@ int gpv_grid_derive(const float bmin[3], const float bmax[3], float mms, const g
 int gpv_grid_create(gpv_ctx* c, const float* tris, int64_t T, int on_device, const float bmin[3],
                     const float bmax[3], float mms, const gpv_params* params, gpv_grid** out)
 {
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
     if (!c || !out || !bmin || !bmax || !params) return fail(GPV_ERR_INVALID, "NULL argument");
     if (T < 0 || (T > 0 && !tris)) return fail(GPV_ERR_INVALID, "bad triangle soup");
     if (T >= (int64_t(1) << 31)) return fail(GPV_ERR_INVALID, "at most 2^31-1 triangles");
     *out = nullptr;
-    gpv_grid_info info;
-    TRY(derive_grid(bmin, bmax, mms, params, &info));
-    HIP_TRY(hipSetDevice(c->device));
-    gpv_grid* G = new gpv_grid();
-    G->ctx = c;
-    G->info = info;
-    G->params = *params;
-    G->T = T;
-    auto bail = [&](int r) {
-        for (void* p : G->bufs) c->pool.put(p);
-        for (auto& e : G->ev) if (e) (void)hipEventDestroy(e);
-        delete G;
-        return r;
-    };
-    for (auto& e : G->ev)
-        if (hipEventCreate(&e) != hipSuccess) return bail(fail(GPV_ERR_HIP, "event creation failed"));
-
-    const int nx = info.n[0], ny = info.n[1], nz = info.n[2];
-    const int nxw = (nx + 63) / 64;
-    const int n2 = info.n2;
-    GridDev& g = G->g;
-    g.nx = nx; g.ny = ny; g.nz = nz; g.nxw = nxw;
-    g.n2 = n2;
-    g.w2 = n2 > 0 ? (n2 * n2 * n2 + 63) / 64 : 0;
-    g.bminx = info.bmin[0]; g.bminy = info.bmin[1]; g.bminz = info.bmin[2];
-    g.bmaxx = info.bmax[0]; g.bmaxy = info.bmax[1]; g.bmaxz = info.bmax[2];
-    g.h1x = (float)((double)info.g[0] / 2.0); g.h1y = (float)((double)info.g[1] / 2.0);
-    g.h1z = (float)((double)info.g[2] / 2.0);
-    g.h2x = (float)((double)info.g2[0] / 2.0); g.h2y = (float)((double)info.g2[1] / 2.0);
-    g.h2z = (float)((double)info.g2[2] / 2.0);
-    if ((int64_t)nx * ny >= (int64_t(1) << 31)) return bail(fail(GPV_ERR_INVALID, "too many columns"));
-
-    std::vector<float> tab;
-    size_t off[12];
-    build_tables(info, tab, off);
-    int r;
-    if ((r = alloc(G, &G->tables, tab.size())) != GPV_OK) return bail(r);
-    if (hipMemcpyAsync(G->tables, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream) !=
-        hipSuccess)
-        return bail(fail(GPV_ERR_HIP, "table upload failed"));
-    const float* tb = G->tables;
-    g.c1x = tb + off[0]; g.c1y = tb + off[1]; g.c1z = tb + off[2];
-    g.r1x = tb + off[3]; g.r1y = tb + off[4]; g.r1z = tb + off[5];
-    g.c2x = tb + off[6]; g.c2y = tb + off[7]; g.c2z = tb + off[8];
-    g.r2x = tb + off[9]; g.r2y = tb + off[10]; g.r2z = tb + off[11];
-
-    if (on_device) {
-        G->tris = tris;
-    } else if (T > 0) {
-        float* d = nullptr;
-        if ((r = alloc(G, &d, (size_t)T * 9)) != GPV_OK) return bail(r);
-        if (hipMemcpyAsync(d, tris, (size_t)T * 9 * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
-            return bail(fail(GPV_ERR_HIP, "soup upload failed"));
-        G->tris = d;
-        G->own_tris = true;
+    gpv_mesh_desc d{};
+    d.tris = tris;
+    d.ntris = T;
+    for (int q = 0; q < 3; ++q) {
+        d.bmin[q] = bmin[q];
+        d.bmax[q] = bmax[q];
     }
-    const size_t Tn = (size_t)std::max<int64_t>(T, 1);
-    G->W = (int64_t)nz * ny * nxw;
-    if ((r = alloc(G, &G->box0, Tn)) || (r = alloc(G, &G->box1, Tn)) || (r = alloc(G, &G->rng, Tn)) ||
-        (r = alloc(G, &G->sat_items, Tn)) || (r = alloc(G, &G->ray_items, Tn)) ||
-        (r = alloc(G, &G->sat_off, Tn + 1)) || (r = alloc(G, &G->ray_off, Tn + 1)) ||
-        (r = alloc(G, &G->ctr, 1)) || (r = alloc(G, &G->bnd, (size_t)G->W)) ||
-        (r = alloc(G, &G->ins, (size_t)G->W)) || (r = alloc(G, &G->wcnt, (size_t)G->W)) ||
-        (r = alloc(G, &G->wprefix, (size_t)G->W + 1)))
-        return bail(r);
-    const int64_t cap0 = params->initial_capacity > 0 ? params->initial_capacity
-                                                      : std::max<int64_t>(int64_t(1) << 16, 16 * T);
-    G->pair_cap = (uint64_t)cap0;
-    G->col_cap = (uint64_t)cap0;
-    G->hit_cap = (uint64_t)std::max<int64_t>(int64_t(1) << 16, 8 * T);
-    if ((r = make_appends(G)) != GPV_OK) return bail(r);
-    if ((r = alloc(G, &G->hit_key, G->hit_cap)) != GPV_OK) return bail(r);
-    G->info.n1 = (int64_t)nx * ny * nz;
-    *out = G;
+    d.max_model_size = mms;
+    d.params = *params;
+    Job* J = nullptr;
+    TRY(job_create(c, &d, 1, on_device, &J));
+    *out = make_view(J, 0);
     return GPV_OK;
 }
-
-// ---------------------------------------------------------------------------
-// stages, split into launch and completion halves so that the fused path
-// (gpv_voxelize) reads every data-dependent size in one host round trip
-// ---------------------------------------------------------------------------
-namespace {
-
-// Level1 SAT sweep + the boundary popcount scan (ranks need the final bits)
-int launch_sat_half(gpv_grid* G)
-{
-    gpv_ctx* c = G->ctx;
-    hipStream_t s = c->stream;
-    HIP_TRY(hipEventRecord(G->ev[1], s));
-    launch_fill_u64(G->bnd, 0, G->W, s);
-    launch_fill_u64(reinterpret_cast<uint64_t*>(&G->ctr->pairs), 0, 2, s);  // pairs, colents
-    Append64 pa{&G->ctr->pairs, G->pair_key, nullptr, G->pair_cap};
-    Append64 co{&G->ctr->colents, G->col_key, G->col_z, G->col_cap};
-    launch_sat_expand(G->tris, G->T, G->box0, G->box1, G->sat_off, G->g, G->bnd, pa, co, expand_blocks(c), s);
-    launch_popc_words(G->bnd, G->W, G->wcnt, s);
-    TRY(ensure_scan_tmp(G, G->W));
-    launch_scan_u32_u32(G->wcnt, G->wprefix, G->W, G->scan32, nullptr, s);
-    HIP_TRY(hipGetLastError());
-    return GPV_OK;
-}
-
-int launch_ray_half(gpv_grid* G)
-{
-    gpv_ctx* c = G->ctx;
-    hipStream_t s = c->stream;
-    HIP_TRY(hipEventRecord(G->ev[2], s));
-    launch_fill_u64(reinterpret_cast<uint64_t*>(&G->ctr->hits), 0, 1, s);
-    Append64 hi{&G->ctr->hits, G->hit_key, nullptr, G->hit_cap};
-    launch_ray_expand(G->tris, G->T, G->rng, G->ray_off, G->g, hi, expand_blocks(c), s);
-    HIP_TRY(hipGetLastError());
-    return GPV_OK;
-}
-
-// one round trip: the counters and, when asked, the boundary count B
-int read_counts(gpv_grid* G, bool with_b)
-{
-    gpv_ctx* c = G->ctx;
-    HIP_TRY(hipMemcpyAsync(c->h_ctr, G->ctr, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-    if (with_b) HIP_TRY(hipMemcpyAsync(c->h_nb, G->wprefix + G->W, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return GPV_OK;
-}
-
-// after read_counts: accept the sweep or regrow the bin buffers to the exact
-// count (the reference's retry, src/Object.cpp:3198-3214)
-int complete_sat(gpv_grid* G, int64_t* needed)
-{
-    gpv_ctx* c = G->ctx;
-    const uint64_t np = c->h_ctr->pairs, nc = c->h_ctr->colents;
-    G->info.tests_l1 = (int64_t)c->h_ctr->tests_l1;
-    *needed = 0;
-    if (np > G->pair_cap || nc > G->col_cap) {
-        const uint64_t need = std::max(np, nc);
-        release(G, G->pair_key);
-        release(G, G->col_key);
-        release(G, G->col_z);
-        G->pair_cap = need + need / 8 + 1024;
-        G->col_cap = G->pair_cap;
-        TRY(make_appends(G));
-        G->info.regrows++;
-        *needed = (int64_t)need;
-        return GPV_OK;
-    }
-    G->n_pairs = np;
-    G->n_cols = nc;
-    G->info.pairs = (int64_t)np;
-    G->B = *c->h_nb;
-    G->info.nb = G->B;
-    G->info.l1_boundary = G->B;
-    G->sat_done = true;
-    return GPV_OK;
-}
-
-// after read_counts: false (and a regrown buffer) when the hits overflowed
-int complete_ray(gpv_grid* G, bool* ok)
-{
-    const uint64_t nh = G->ctx->h_ctr->hits;
-    *ok = nh <= G->hit_cap;
-    if (*ok) {
-        G->n_hits = nh;
-        return GPV_OK;
-    }
-    release(G, G->hit_key);
-    G->hit_cap = nh + nh / 8 + 1024;
-    return alloc(G, &G->hit_key, G->hit_cap);
-}
-
-// hits grouped by column, then the parity of every Level1 voxel
-int launch_parity(gpv_grid* G)
-{
-    hipStream_t s = G->ctx->stream;
-    HIP_TRY(hipEventRecord(G->ev[3], s));
-    const int64_t ncol = (int64_t)G->g.nx * G->g.ny;
-    release(G, G->hcnt);
-    release(G, G->hoff);
-    release(G, G->hrec);
-    TRY(alloc(G, &G->hcnt, (size_t)ncol));
-    TRY(alloc(G, &G->hoff, (size_t)ncol + 1));
-    TRY(alloc(G, &G->hrec, (size_t)std::max<uint64_t>(G->n_hits, 1) * 3));
-    launch_fill_u32(G->hcnt, 0, ncol, s);
-    launch_count_keys_hi(G->hit_key, G->n_hits, G->hcnt, s);
-    TRY(ensure_scan_tmp(G, ncol));
-    launch_scan_u32_u32(G->hcnt, G->hoff, ncol, G->scan32, nullptr, s);
-    launch_scatter_hits(G->tris, G->hit_key, G->n_hits, G->g, G->hoff, G->hcnt, G->hrec, s);
-    launch_l1_parity(G->hrec, G->hoff, G->g, G->ins, s);
-    HIP_TRY(hipGetLastError());
-    G->inout_done = true;
-    return GPV_OK;
-}
-
-// boundary index list, bins per boundary voxel, column entries with records
-int launch_compact(gpv_grid* G)
-{
-    hipStream_t s = G->ctx->stream;
-    HIP_TRY(hipEventRecord(G->ev[4], s));
-    release(G, G->bidx);
-    TRY(alloc(G, &G->bidx, (size_t)std::max<int64_t>(G->B, 1)));
-    launch_boundary_index(G->bnd, G->wprefix, G->g, G->bidx, s);
-    release(G, G->pb);
-    release(G, G->cntb);
-    release(G, G->boff);
-    release(G, G->list);
-    TRY(alloc(G, &G->pb, (size_t)std::max<uint64_t>(G->n_pairs, 1)));
-    TRY(alloc(G, &G->cntb, (size_t)std::max<int64_t>(G->B, 1)));
-    TRY(alloc(G, &G->boff, (size_t)G->B + 1));
-    TRY(alloc(G, &G->list, (size_t)std::max<uint64_t>(G->n_pairs, 1)));
-    launch_fill_u32(G->cntb, 0, G->B, s);
-    launch_fill_u32(&G->ctr->max_per_voxel, 0, 1, s);
-    launch_pair_rank(G->pair_key, G->n_pairs, G->bnd, G->wprefix, G->pb, G->cntb, s);
-    TRY(ensure_scan_tmp(G, G->B));
-    launch_scan_u32_u32(G->cntb, G->boff, G->B, G->scan32, &G->ctr->max_per_voxel, s);
-    launch_pair_scatter(G->pair_key, G->pb, G->n_pairs, G->boff, G->cntb, G->list, s);
-    const int64_t ncol = (int64_t)G->g.nx * G->g.ny;
-    release(G, G->ccnt);
-    release(G, G->coff);
-    release(G, G->crec);
-    TRY(alloc(G, &G->ccnt, (size_t)ncol));
-    TRY(alloc(G, &G->coff, (size_t)ncol + 1));
-    TRY(alloc(G, &G->crec, (size_t)std::max<uint64_t>(G->n_cols, 1) * 3));
-    launch_fill_u32(G->ccnt, 0, ncol, s);
-    launch_count_keys_hi(G->col_key, G->n_cols, G->ccnt, s);
-    TRY(ensure_scan_tmp(G, ncol));
-    launch_scan_u32_u32(G->ccnt, G->coff, ncol, G->scan32, nullptr, s);
-    launch_scatter_cols(G->tris, G->col_key, G->col_z, G->n_cols, G->coff, G->ccnt, G->crec, s);
-    HIP_TRY(hipGetLastError());
-    if (G->g.n2 > 0) {
-        release(G, G->sat2);
-        release(G, G->par2);
-        const size_t w2n = (size_t)std::max<int64_t>(G->B * G->g.w2, 1);
-        TRY(alloc(G, &G->sat2, w2n));
-        TRY(alloc(G, &G->par2, w2n));
-    }
-    G->compact_done = true;
-    return GPV_OK;
-}
-
-}  // namespace
 
 int gpv_classify_tessellation(gpv_ctx* c, gpv_grid* G, int64_t* needed)
 {
-    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    TRY(check_staged(c, G));
+    Job* J = G->job;
     int64_t nd = 0;
     HIP_TRY(hipSetDevice(c->device));
-    TRY(stage_setup(G));
-    TRY(launch_sat_half(G));
-    TRY(read_counts(G, true));
-    TRY(complete_sat(G, &nd));
+    TRY(stage_setup(J));
+    TRY(launch_sat_half(J));
+    TRY(read_counts(J, true));
+    TRY(complete_sat(J, &nd));
     if (needed) *needed = nd;
     return GPV_OK;
 }
 
 int gpv_classify_inout(gpv_ctx* c, gpv_grid* G)
 {
-    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    TRY(check_staged(c, G));
+    Job* J = G->job;
     HIP_TRY(hipSetDevice(c->device));
-    TRY(stage_setup(G));
+    TRY(stage_setup(J));
     bool ok = false;
     for (int attempt = 0; attempt < 2 && !ok; ++attempt) {
-        TRY(launch_ray_half(G));
-        TRY(read_counts(G, false));
-        TRY(complete_ray(G, &ok));
+        TRY(launch_ray_half(J));
+        TRY(read_counts(J, false));
+        TRY(complete_ray(J, &ok));
     }
     if (!ok) return fail(GPV_ERR_CAPACITY, "ray hit capacity did not converge");
-    return launch_parity(G);
+    return launch_parity(J);
 }
 
 int gpv_compact_boundary(gpv_ctx* c, gpv_grid* G)
 {
-    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
-    if (!G->sat_done) return fail(GPV_ERR_STATE, "gpv_classify_tessellation has not completed");
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    TRY(check_staged(c, G));
+    if (!G->job->sat_done) return fail(GPV_ERR_STATE, "gpv_classify_tessellation has not completed");
     HIP_TRY(hipSetDevice(c->device));
-    return launch_compact(G);
+    return launch_compact(G->job);
 }
 
 int gpv_classify_inout_level2(gpv_ctx* c, gpv_grid* G)
 {
-    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
-    if (!G->compact_done) return fail(GPV_ERR_STATE, "gpv_compact_boundary has not completed");
-    if (G->g.n2 <= 0) return GPV_OK;
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    TRY(check_staged(c, G));
+    if (!G->job->compact_done) return fail(GPV_ERR_STATE, "gpv_compact_boundary has not completed");
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipEventRecord(G->ev[5], c->stream));
-    launch_l2_parity(G->bidx, G->B, G->coff, G->crec, G->g, G->par2, c->stream);
-    HIP_TRY(hipGetLastError());
-    G->l2p_done = true;
-    return GPV_OK;
+    return run_l2_parity(G->job);
 }
 
 int gpv_classify_tessellation_level2(gpv_ctx* c, gpv_grid* G)
 {
-    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
-    if (!G->compact_done) return fail(GPV_ERR_STATE, "gpv_compact_boundary has not completed");
-    if (G->g.n2 <= 0) return GPV_OK;
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    TRY(check_staged(c, G));
+    if (!G->job->compact_done) return fail(GPV_ERR_STATE, "gpv_compact_boundary has not completed");
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipEventRecord(G->ev[6], c->stream));
-    launch_l2_sat(G->tris, G->bidx, G->B, G->boff, G->list, G->g, G->sat2, c->stream);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(G->ev[7], c->stream));
-    G->l2s_done = true;
-    return GPV_OK;
+    return run_l2_sat(G->job);
 }
 
 int gpv_grid_finish(gpv_ctx* c, gpv_grid* G)
 {
-    if (!c || !G || G->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
-    if (!G->compact_done || !G->inout_done) return fail(GPV_ERR_STATE, "Level1 stages have not completed");
-    if (G->g.n2 > 0 && (!G->l2p_done || !G->l2s_done)) return fail(GPV_ERR_STATE, "Level2 stages have not completed");
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    TRY(check_staged(c, G));
+    Job* J = G->job;
+    if (!J->compact_done || !J->inout_done) return fail(GPV_ERR_STATE, "Level1 stages have not completed");
+    if (J->n2max > 0 && (!J->l2p_done || !J->l2s_done)) return fail(GPV_ERR_STATE, "Level2 stages have not completed");
     HIP_TRY(hipSetDevice(c->device));
-    hipStream_t s = c->stream;
-    launch_fill_u64(reinterpret_cast<uint64_t*>(&G->ctr->l1_inside), 0, 3, s);
-    const int64_t W2 = G->g.n2 > 0 ? G->B * G->g.w2 : 0;
-    launch_count_states(G->bnd, G->ins, G->W, G->sat2, G->par2, W2, G->ctr, s);
-    HIP_TRY(hipEventRecord(G->ev[8], s));
-    TRY(read_counts(G, false));
-    const Counters& h = *c->h_ctr;
-    G->info.l1_inside = (int64_t)h.l1_inside;
-    G->info.l1_boundary = G->B;
-    G->info.l2_inside = (int64_t)h.l2_inside;
-    G->info.l2_boundary = (int64_t)h.l2_boundary;
-    G->info.max_tris_per_voxel = (int32_t)h.max_per_voxel;
-    G->info.tests_l2 = G->g.n2 > 0 ? (int64_t)G->n_pairs * G->g.n2 * G->g.n2 * G->g.n2 : 0;
-    auto el = [&](int a, int b) -> float {
-        float v = 0.0f;
-        if (hipEventElapsedTime(&v, G->ev[a], G->ev[b]) != hipSuccess) {
-            (void)hipGetLastError();
-            v = 0.0f;
-        }
-        return v;
-    };
-    // setup, L1 SAT (+ boundary scan), L1 parity (ray expansion, the size
-    // read-back and the parity fill), compaction + bins, L2 parity, L2 SAT
-    G->info.ms_stage[0] = el(0, 1);
-    G->info.ms_stage[1] = el(1, 2);
-    G->info.ms_stage[2] = el(2, 4);
-    if (G->g.n2 > 0) {
-        G->info.ms_stage[3] = el(4, 5);
-        G->info.ms_stage[4] = el(5, 6);
-        G->info.ms_stage[5] = el(6, 7);
-    } else {
-        G->info.ms_stage[3] = el(4, 8);
-    }
-    G->info.ms_total = el(0, 8);
-    G->finished = true;
-    return GPV_OK;
+    return job_finish(J);
 }
 
 int gpv_voxelize(gpv_ctx* c, const float* tris, int64_t T, int on_device, const float bmin[3],
                  const float bmax[3], float mms, const gpv_params* params, gpv_grid** out)
 {
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
     gpv_grid* G = nullptr;
     TRY(gpv_grid_create(c, tris, T, on_device, bmin, bmax, mms, params, &G));
-    auto run = [&]() -> int {
-        TRY(stage_setup(G));
-        // SAT sweep, ray expansion and boundary scan back to back, one read-back
-        TRY(launch_sat_half(G));
-        TRY(launch_ray_half(G));
-        TRY(read_counts(G, true));
-        int64_t needed = 0;
-        bool ray_ok = false;
-        TRY(complete_sat(G, &needed));
-        TRY(complete_ray(G, &ray_ok));
-        for (int attempt = 0; attempt < 2 && (needed > 0 || !ray_ok); ++attempt) {
-            if (needed > 0) TRY(launch_sat_half(G));
-            if (!ray_ok) TRY(launch_ray_half(G));
-            TRY(read_counts(G, true));
-            if (needed > 0) TRY(complete_sat(G, &needed));
-            if (!ray_ok) TRY(complete_ray(G, &ray_ok));
-        }
-        if (needed > 0 || !ray_ok) return fail(GPV_ERR_CAPACITY, "buffer capacity did not converge");
-        TRY(launch_parity(G));
-        TRY(launch_compact(G));
-        if (G->g.n2 > 0) {
-            TRY(gpv_classify_inout_level2(c, G));
-            TRY(gpv_classify_tessellation_level2(c, G));
-        }
-        return gpv_grid_finish(c, G);
-    };
-    const int r = run();
+    const int r = job_run(G->job);
     if (r != GPV_OK) {
         std::string keep = g_err;
         gpv_grid_release(c, G);
@@ -829,62 +1092,89 @@ int gpv_voxelize(gpv_ctx* c, const float* tris, int64_t T, int on_device, const 
     return GPV_OK;
 }
 
+int gpv_voxelize_batch(gpv_ctx* c, const gpv_mesh_desc* meshes, int nmeshes, int on_device, gpv_grid** out)
+{
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    if (!c || !meshes || !out || nmeshes < 1) return fail(GPV_ERR_INVALID, "bad batch arguments");
+    for (int m = 0; m < nmeshes; ++m) out[m] = nullptr;
+    Job* J = nullptr;
+    TRY(job_create(c, meshes, nmeshes, on_device, &J));
+    const int r = job_run(J);
+    if (r != GPV_OK) {
+        std::string keep = g_err;
+        job_free(J);
+        g_err = keep;
+        return r;
+    }
+    for (int m = 0; m < nmeshes; ++m) out[m] = make_view(J, m);
+    return GPV_OK;
+}
+
 int gpv_grid_get_info(const gpv_grid* G, gpv_grid_info* info)
 {
-    if (!G || !info) return fail(GPV_ERR_INVALID, "NULL argument");
-    *info = G->info;
+    if (!G || !G->job || !info) return fail(GPV_ERR_INVALID, "NULL argument");
+    *info = G->job->info[G->m];
     return GPV_OK;
 }
 
 int gpv_grid_device_views(const gpv_grid* G, const uint64_t** l1b, const uint64_t** l1i, const uint64_t** l2b,
                           const uint64_t** l2i, const int32_t** bidx)
 {
-    if (!G) return fail(GPV_ERR_INVALID, "NULL grid");
-    if (l1b) *l1b = G->bnd;
-    if (l1i) *l1i = G->ins;
-    if (l2b) *l2b = G->sat2;
-    if (l2i) *l2i = G->par2;
-    if (bidx) *bidx = G->bidx;
+    if (!G || !G->job) return fail(GPV_ERR_INVALID, "NULL grid");
+    const Job* J = G->job;
+    const MeshOff& o = J->off[G->m];
+    if (l1b) *l1b = J->bnd ? J->bnd + o.word0 : nullptr;
+    if (l1i) *l1i = J->ins ? J->ins + o.word0 : nullptr;
+    if (l2b) *l2b = J->sat2 && J->g[G->m].n2 > 0 ? J->sat2 + o.l2w0 : nullptr;
+    if (l2i) *l2i = J->par2 && J->g[G->m].n2 > 0 ? J->par2 + o.l2w0 : nullptr;
+    if (bidx) *bidx = J->bidx ? J->bidx + o.b0 : nullptr;
     return GPV_OK;
 }
 
-static int download_impl(gpv_ctx* c, const gpv_grid* Gc, uint8_t unit, uint8_t* l1, int32_t* prefix,
+static int download_impl(gpv_ctx* c, const gpv_grid* G, uint8_t unit, uint8_t* l1, int32_t* prefix,
                          int32_t* bidx, uint8_t* l2)
 {
-    if (!c || !Gc || Gc->ctx != c) return fail(GPV_ERR_INVALID, "bad context/grid");
-    if (!Gc->finished) return fail(GPV_ERR_STATE, "grid is not finished");
-    gpv_grid* G = const_cast<gpv_grid*>(Gc);
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    TRY(check_grid(c, G));
+    Job* J = G->job;
+    if (!J->finished) return fail(GPV_ERR_STATE, "grid is not finished");
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
-    const int64_t n1 = G->info.n1;
-    const int64_t n23 = (int64_t)G->g.n2 * G->g.n2 * G->g.n2;
+    const int m = G->m;
+    const GridDev& g = J->g[m];
+    const MeshOff& o = J->off[m];
+    const int64_t n1 = J->info[m].n1;
+    const int64_t nb = J->nb(m);
+    const int64_t n23 = (int64_t)g.n2 * g.n2 * g.n2;
     if (l1) {
         uint8_t* d = nullptr;
-        TRY(alloc(G, &d, (size_t)n1));
-        launch_export_l1(G->bnd, G->ins, G->g, unit, d, s);
+        TRY(alloc(J, &d, (size_t)n1));
+        launch_export_l1(J->bnd + o.word0, J->ins + o.word0, g, unit, d, s);
         HIP_TRY(hipMemcpyAsync(l1, d, (size_t)n1, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        release(G, d);
+        release(J, d);
     }
     if (prefix) {
         int32_t* d = nullptr;
-        TRY(alloc(G, &d, (size_t)n1));
-        launch_export_prefix(G->bnd, G->wprefix, G->g, d, s);
+        TRY(alloc(J, &d, (size_t)n1));
+        launch_export_prefix(J->bnd + o.word0, J->wprefix + o.word0, o.b0, g, d, s);
         HIP_TRY(hipMemcpyAsync(prefix, d, (size_t)n1 * 4, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        release(G, d);
+        release(J, d);
     }
-    if (bidx && G->B > 0) {
-        HIP_TRY(hipMemcpyAsync(bidx, G->bidx, (size_t)G->B * 4, hipMemcpyDeviceToHost, s));
+    if (bidx && nb > 0) {
+        HIP_TRY(hipMemcpyAsync(bidx, J->bidx + o.b0, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
     }
-    if (l2 && G->g.n2 > 0 && G->B > 0) {
+    if (l2 && g.n2 > 0 && nb > 0) {
         uint8_t* d = nullptr;
-        TRY(alloc(G, &d, (size_t)(G->B * n23)));
-        launch_export_l2(G->sat2, G->par2, G->B, G->g, unit, d, s);
-        HIP_TRY(hipMemcpyAsync(l2, d, (size_t)(G->B * n23), hipMemcpyDeviceToHost, s));
+        TRY(alloc(J, &d, (size_t)(nb * n23)));
+        launch_export_l2(J->sat2 + o.l2w0, J->par2 + o.l2w0, nb, g, unit, d, s);
+        HIP_TRY(hipMemcpyAsync(l2, d, (size_t)(nb * n23), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        release(G, d);
+        release(J, d);
     }
     return GPV_OK;
 }
@@ -899,13 +1189,15 @@ int gpv_grid_download(gpv_ctx* c, const gpv_grid* G, uint8_t* l1, int32_t* prefi
 // config files; the config uses the ostream default of 6 significant digits).
 int gpv_save_raw(gpv_ctx* c, const gpv_grid* G, int obj_id, const char* dir)
 {
-    if (!c || !G) return fail(GPV_ERR_INVALID, "NULL argument");
-    const gpv_grid_info& I = G->info;
-    const bool l2 = G->g.n2 > 0;
-    const int64_t n23 = (int64_t)G->g.n2 * G->g.n2 * G->g.n2;
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    TRY(check_grid(c, G));
+    const gpv_grid_info& I = G->job->info[G->m];
+    const bool l2 = I.n2 > 0;
+    const int64_t n23 = (int64_t)I.n2 * I.n2 * I.n2;
     std::vector<uint8_t> l1v((size_t)I.n1);
     std::vector<int32_t> pre(l2 ? (size_t)I.n1 : 0);
-    std::vector<uint8_t> l2v(l2 ? (size_t)(G->B * n23) : 0);
+    std::vector<uint8_t> l2v(l2 ? (size_t)(I.nb * n23) : 0);
     // uint8(state * 127.0f), src/Object.cpp:3031-3034, 3046-3050
     TRY(download_impl(c, G, 127, l1v.data(), l2 ? pre.data() : nullptr, nullptr, l2 ? l2v.data() : nullptr));
     const std::string base = std::string(dir && *dir ? dir : ".") + "/Obj" + std::to_string(obj_id);
@@ -940,14 +1232,13 @@ int gpv_save_raw(gpv_ctx* c, const gpv_grid* G, int obj_id, const char* dir)
 
 int gpv_grid_release(gpv_ctx* c, gpv_grid* G)
 {
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
     if (!G) return GPV_OK;
     if (!c || G->ctx != c) return fail(GPV_ERR_INVALID, "grid belongs to another context");
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);   // pooled blocks may be reused right away
-    for (void* p : G->bufs) c->pool.put(p);
-    for (auto& e : G->ev)
-        if (e) (void)hipEventDestroy(e);
+    Job* J = G->job;
     delete G;
+    if (J && --J->refs == 0) job_free(J);
     return GPV_OK;
 }
 

@@ -110,6 +110,30 @@ int gpv_voxelize(gpv_ctx *ctx, const float *tris, int64_t ntris, int tris_on_dev
                  const float bmin[3], const float bmax[3], float max_model_size,
                  const gpv_params *params, gpv_grid **out);
 
+/* ---- batches ---------------------------------------------------------------- */
+/* One mesh of a batch: the arguments gpv_voxelize takes for it. */
+typedef struct gpv_mesh_desc {
+    const float *tris;        /* [ntris*9] soup, host or device (tris_on_device) */
+    int64_t ntris;
+    float bmin[3], bmax[3];   /* from gpv_mesh_bbox */
+    float max_model_size;
+    gpv_params params;        /* per-mesh Level1 / Level2 resolution */
+} gpv_mesh_desc;
+
+/* Voxelizes nmeshes meshes of any sizes and resolutions as one run: every
+ * stage of the path is one launch over all meshes (the reference voxelizes
+ * one Object per PerformVoxelization call, src/GPView.cpp:1378-1380), with
+ * one size read-back for the whole batch.  out[m] receives mesh m's grid;
+ * each grid holds exactly what gpv_voxelize of that mesh alone would hold
+ * and works with every result call below (get_info, download, save_raw,
+ * device_views).  The soups are copied into the batch, so the inputs may be
+ * reused on return.  Per-mesh ms_stage/ms_total report the batch's stage
+ * times.  The batch's device memory returns to the context's pool when the
+ * last of its grids is released.  Limits per batch: 2^31-1 triangles,
+ * 2^32-1 Level1 voxels (rows padded to 64), 2^32-1 bin entries. */
+int gpv_voxelize_batch(gpv_ctx *ctx, const gpv_mesh_desc *meshes, int nmeshes, int tris_on_device,
+                       gpv_grid **out);
+
 /* ---- stage entry points (same order PerformVoxelization calls them) ------- */
 /* Allocates the grid, uploads the soup and the center tables: the set-up of
  * PerformVoxelization (src/Object.cpp:3094-3193). */

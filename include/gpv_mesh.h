@@ -66,10 +66,15 @@ int gpv_mesh_mixed(int kind, int64_t target_tris, uint64_t seed, gpv_mesh *out);
  *  - vertices snapped onto the Level1/Level2 lattice planes of the grid the
  *    mesh's own bbox derives (voxel_count, voxel_count2) with probability
  *    p_snap (bbox-defining vertices are left alone so the grid is unchanged);
+ *  - a fraction f_coplanar of triangles made coplanar with a Level1 voxel
+ *    face: their three vertices moved onto the lattice plane nearest the
+ *    centroid along a random axis (shared vertices move, so the mesh stays
+ *    watertight; later moves may undo earlier ones, so the fraction is
+ *    approximate);
  *  - a fraction f_sliver of triangles split into slivers (aspect > 1e3) by
  *    inserting a vertex at 1e-4 of an edge, both neighbours split (watertight). */
 int gpv_mesh_stress(gpv_mesh *m, int voxel_count, int voxel_count2, float p_snap, float f_sliver,
-                    uint64_t seed);
+                    float f_coplanar, uint64_t seed);
 
 /* Readers with the reference's semantics: Object::ReadObject
  * (src/Object.cpp:395-584; bbox over every `v` line) and

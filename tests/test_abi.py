@@ -20,7 +20,7 @@ def _declared(header):
 
 def test_exports_every_declared_symbol(gv):
     L = gv.lib()
-    declared = _declared("gpv.h") + _declared("gpv_mesh.h")
+    declared = _declared("gpv.h") + _declared("gpv_mesh.h") + _declared("gpv_serve.h")
     assert len(declared) >= 25
     for name in declared:
         assert hasattr(L, name), f"{name} declared but not exported"
@@ -140,6 +140,25 @@ def test_stress_keeps_watertight_and_bbox(gv):
     assert np.array_equal(gv.mesh_bbox(v)[0], gv.mesh_bbox(vs)[0])
 
 
+def test_stress_coplanar_fraction(gv):
+    """f_coplanar = 0.05: about 5% of triangles end with all three vertices on
+    one Level1 lattice plane (bmin + k*g along some axis), watertight, bbox kept."""
+    v, f = gv.icosphere(4)
+    vs, fs = gv.stress(v, f, 64, 4, p_snap=0.0, f_sliver=0.0, f_coplanar=0.05, seed=9)
+    assert _watertight(fs) and np.array_equal(gv.mesh_bbox(v)[0], gv.mesh_bbox(vs)[0])
+    bmin, bmax, mms = gv.mesh_bbox(vs)
+    info = gv.grid_derive(bmin, bmax, mms, 64, 4).as_dict()
+    t = vs[fs]                                    # [T, 3, 3]
+    on = np.zeros(len(fs), bool)
+    for q in range(3):
+        same = (t[:, 0, q] == t[:, 1, q]) & (t[:, 1, q] == t[:, 2, q])
+        k = np.rint((t[:, 0, q].astype(np.float64) - bmin[q]) / np.float64(info["g"][q]))
+        plane = (bmin[q] + k * np.float64(info["g"][q])).astype(np.float32)
+        on |= same & (plane == t[:, 0, q])
+    frac = on.mean()
+    assert 0.03 < frac < 0.08, frac
+
+
 def test_invalid_arguments(gv):
     with pytest.raises(gv.GpvError):
         gv.grid_derive(np.float32([0, 0, 0]), np.float32([1, 1, 1]), 1.0, 8, 17)
@@ -167,3 +186,24 @@ def test_codegen_exactness():
             ok = ("v_div_scale_f32" in win or "v_div_fmas_f32" in win or "v_rcp_iflag_f32" in win
                   or ("v_rcp_f32" in win and ("v_cvt_u32_f32" in win or "v_mul_hi" in win)))
             assert ok, f"unexplained {x}"
+
+
+def test_predict_footprint_monotone(gv):
+    """Admission estimate: grows with triangles and with either resolution."""
+    from gpview_amd import workloads
+    m = workloads.plain_small(gv)
+    base = gv.predict_footprint(m)
+    assert base > 0
+    bigger = dict(m, soup=np.concatenate([m["soup"]] * 4))
+    assert gv.predict_footprint(bigger) > base
+    assert gv.predict_footprint(dict(m, voxel_count=128)) > base
+    assert gv.predict_footprint(dict(m, voxel_count2=8)) > base
+    with pytest.raises(gv.GpvError):
+        gv.predict_footprint(dict(m, voxel_count2=17))
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-device error path")
+def test_server_without_device_fails_cleanly(gv):
+    with pytest.raises(gv.GpvError) as e:
+        gv.Server(lanes=2)
+    assert e.value.status == gv.GPV_ERR_NO_DEVICE

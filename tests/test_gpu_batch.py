@@ -1,0 +1,180 @@
+"""Batched voxelization (gpv_voxelize_batch) through the C ABI.
+
+Every mesh of a batch must hold exactly what it holds when voxelized alone:
+bit-exact against the oracle at sizes the oracle finishes quickly, and
+against the single-mesh HIP path (itself pinned to the oracle by
+test_gpu_parity.py) at the batch sizes of the mixed-stream workload.  The
+reference voxelizes one Object per PerformVoxelization call
+(src/GPView.cpp:1378-1380), so "alone" is the reference's own unit.
+"""
+import numpy as np
+import pytest
+
+from conftest import compare_results, say
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("l1_inside", "l1_boundary", "l2_inside", "l2_boundary", "tests_l1", "tests_l2", "pairs",
+        "max_tris_per_voxel", "nb", "n1", "n", "n2")
+
+
+def mesh_entry(gv, verts, faces, vc, vc2, level2=True, initial_capacity=0):
+    soup = gv.flatten(verts, faces)
+    bmin, bmax, mms = gv.mesh_bbox(verts)
+    return {"soup": soup, "bmin": bmin, "bmax": bmax, "mms": mms, "voxel_count": vc, "voxel_count2": vc2,
+            "level2": level2, "initial_capacity": initial_capacity}
+
+
+def small_mix(gv, seed=100):
+    """Eleven meshes: the three generator kinds at mixed sizes and
+    resolutions, Level2 off for one, a single triangle and an empty soup."""
+    out = []
+    specs = [(0, 2000, 32, 4), (1, 3000, 16, 2), (2, 4000, 32, 8), (0, 800, 48, 4), (1, 6000, 24, 4),
+             (2, 2500, 20, 2), (0, 1500, 32, 8), (1, 1200, 40, 4)]
+    for i, (kind, target, vc, vc2) in enumerate(specs):
+        v, f = gv.mixed(kind, target, seed + i)
+        out.append(mesh_entry(gv, v, f, vc, vc2))
+    v, f = gv.mixed(2, 3000, seed + 50)
+    out.append(mesh_entry(gv, v, f, 32, 4, level2=False))
+    tri_v = np.array([[0.1, 0.2, 0.3], [0.9, 0.3, 0.5], [0.4, 0.8, 0.7]], np.float32)
+    out.append(mesh_entry(gv, tri_v, np.array([[0, 1, 2]], np.int32), 16, 4))
+    e = mesh_entry(gv, tri_v, np.array([[0, 1, 2]], np.int32), 16, 4)
+    e["soup"] = np.zeros((0, 9), np.float32)
+    out.append(e)
+    return out
+
+
+def oracle_of(orc, m):
+    g = orc.grid_setup(m["bmin"], m["bmax"], m["mms"], m["voxel_count"],
+                       m["voxel_count2"] if m["level2"] else 0)
+    return orc.voxelize(m["soup"], g, orc.ORC_FAST, 0), g.n2
+
+
+def alone(ctx, m):
+    grid = ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], m["voxel_count"], m["voxel_count2"],
+                        m["level2"], m.get("initial_capacity", 0))
+    try:
+        return grid.download(), grid.info()
+    finally:
+        grid.release()
+
+
+def batch_results(ctx, meshes, **kw):
+    grids = ctx.voxelize_batch(meshes, **kw)
+    out = []
+    try:
+        for g in grids:
+            out.append((g.download(), g.info()))
+    finally:
+        for g in grids:
+            g.release()
+    return out
+
+
+def check_info(name, info, ref):
+    for k in KEYS:
+        if k in ref:
+            assert info[k] == ref[k], f"{name}: {k} {info[k]} != {ref[k]}"
+
+
+def test_batch_small_mix_vs_oracle(gv, orc, ctx):
+    meshes = small_mix(gv)
+    say(f"batch small mix: {len(meshes)} meshes, T={sum(len(m['soup']) for m in meshes)}")
+    res = batch_results(ctx, meshes)
+    for i, (m, (got, info)) in enumerate(zip(meshes, res)):
+        want, n2 = oracle_of(orc, m)
+        compare_results(f"mesh {i}", got, want, n2)
+        check_info(f"mesh {i}", info, want)
+
+
+def test_batch_order_independent(gv, ctx):
+    """Each mesh's grid is the same in any batch position (alone vs in the mix)."""
+    meshes = small_mix(gv, seed=300)
+    say("batch order: forward")
+    fwd = batch_results(ctx, meshes)
+    perm = [5, 0, 10, 3, 9, 1, 7, 2, 8, 4, 6]
+    say("batch order: permuted")
+    rev = batch_results(ctx, [meshes[i] for i in perm])
+    for j, i in enumerate(perm):
+        compare_results(f"mesh {i} at {j}", rev[j][0], fwd[i][0], fwd[i][1]["n2"])
+        check_info(f"mesh {i} at {j}", rev[j][1], fwd[i][1])
+
+
+def test_batch_of_one_equals_plain(gv, ctx):
+    v, f = gv.mixed(0, 20000, 7)
+    m = mesh_entry(gv, v, f, 64, 8)
+    say("batch of one")
+    (got, info), = batch_results(ctx, [m])
+    want, winfo = alone(ctx, m)
+    compare_results("batch-of-one", got, want, 8)
+    check_info("batch-of-one", info, winfo)
+
+
+def test_batch_capacity_regrow(gv, orc, ctx):
+    meshes = small_mix(gv, seed=500)[:6]
+    for m in meshes:
+        m["initial_capacity"] = 100
+    say("batch regrow")
+    res = batch_results(ctx, meshes)
+    assert res[0][1]["regrows"] >= 1
+    for i, (m, (got, info)) in enumerate(zip(meshes, res)):
+        want, n2 = oracle_of(orc, m)
+        compare_results(f"regrow mesh {i}", got, want, n2)
+
+
+def test_batch_device_soups(gv, ctx):
+    """Device-resident inputs (tris_on_device = 1) give the same grids."""
+    meshes = small_mix(gv, seed=700)[:5]
+    ptrs = [ctx.upload(m["soup"]) if len(m["soup"]) else None for m in meshes]
+    try:
+        say("batch device soups")
+        dev = batch_results(ctx, meshes, device_ptrs=ptrs)
+    finally:
+        for p in ptrs:
+            if p:
+                ctx.free(p)
+    host = batch_results(ctx, meshes)
+    for i in range(len(meshes)):
+        compare_results(f"device mesh {i}", dev[i][0], host[i][0], host[i][1]["n2"])
+
+
+def test_batch_mixed_stream_vs_plain(gv, ctx):
+    """A slice of the mixed-size stream (triangle counts log-uniform in
+    [1k, 200k], Level1 {64,128,256}, Level2 {4,8}) against the single-mesh
+    path, mesh by mesh."""
+    rng = np.random.default_rng(2026)
+    meshes = []
+    for i in range(12):
+        kind = int(rng.integers(0, 3))
+        target = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
+        vc = int(rng.choice([64, 128, 256]))
+        vc2 = int(rng.choice([4, 8]))
+        v, f = gv.mixed(kind, target, 9000 + i)
+        meshes.append(mesh_entry(gv, v, f, vc, vc2))
+    say(f"mixed stream slice: T={sum(len(m['soup']) for m in meshes)}")
+    res = batch_results(ctx, meshes)
+    for i, m in enumerate(meshes):
+        want, winfo = alone(ctx, m)
+        compare_results(f"stream mesh {i}", res[i][0], want, winfo["n2"])
+        check_info(f"stream mesh {i}", res[i][1], winfo)
+
+
+def test_batch_views_and_save(gv, ctx, tmp_path):
+    """save_raw of a batch member writes the same files as the mesh alone."""
+    meshes = small_mix(gv, seed=900)[:3]
+    grids = ctx.voxelize_batch(meshes)
+    try:
+        (tmp_path / "b").mkdir()
+        grids[1].save_raw(5, str(tmp_path / "b"))
+    finally:
+        for g in grids:
+            g.release()
+    m = meshes[1]
+    g = ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], m["voxel_count"], m["voxel_count2"], True)
+    try:
+        (tmp_path / "a").mkdir()
+        g.save_raw(5, str(tmp_path / "a"))
+    finally:
+        g.release()
+    for name in ("VoxelConfig.txt", "Level1InOut.raw", "Level1BoundaryPrefixSum.raw", "Level2InOut.raw"):
+        assert (tmp_path / "a" / f"Obj5{name}").read_bytes() == (tmp_path / "b" / f"Obj5{name}").read_bytes()

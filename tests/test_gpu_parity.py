@@ -68,15 +68,16 @@ def test_mixed_kinds(gv, orc, ctx, kind, target, vc, vc2, seed):
 
 
 def test_stress_snapped_slivers(gv, orc, ctx):
-    """Lattice-snapped vertices (p=0.3) and 10% slivers on a drilled box."""
+    """Lattice-snapped vertices (p=0.3), 5% face-coplanar triangles and 10%
+    slivers on a drilled box."""
     v, f = gv.mixed(2, 20000, 21)
-    vs, fs = gv.stress(v, f, 64, 8, p_snap=0.3, f_sliver=0.1, seed=22)
+    vs, fs = gv.stress(v, f, 64, 8, p_snap=0.3, f_sliver=0.1, f_coplanar=0.05, seed=22)
     run_case(gv, orc, ctx, "stress-box", vs, fs, 64, 8)
 
 
 def test_stress_icosphere_exact(gv, orc, ctx):
     v, f = gv.icosphere(3, 0.4, (0.5, 0.5, 0.5), 0.02, seed=3)
-    vs, fs = gv.stress(v, f, 32, 4, p_snap=0.5, f_sliver=0.2, seed=23)
+    vs, fs = gv.stress(v, f, 32, 4, p_snap=0.5, f_sliver=0.2, f_coplanar=0.1, seed=23)
     run_case(gv, orc, ctx, "stress-ico-exact", vs, fs, 32, 4, mode=orc.ORC_EXACT)
 
 
