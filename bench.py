@@ -11,7 +11,9 @@ kernel (HIP events on the library's stream) and the CPU oracle timed on this
 host on the same mesh.
 
 Also measured (key "batched"): the mixed-size stream of 256 meshes, one
-gpv_voxelize per mesh against one gpv_voxelize_batch for all of them.
+gpv_voxelize per mesh against one gpv_voxelize_batch for all of them; and
+(key "served") a Poisson request stream through the serving loop, with
+p50/p99 latency at the stated arrival rate.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--batch-meshes N]
 """
@@ -79,11 +81,14 @@ def bench_batched(gv, ctx, n_meshes, steps, warmup, check):
     gs = ctx.voxelize_batch(meshes, device_ptrs=ptrs)
     tests = 0
     voxels = 0
+    tot = {"tests_l1": 0, "tests_l2": 0, "pairs": 0, "nb": 0, "n1": 0}
     exact = True if check else None
     for m, p, g in zip(meshes, ptrs, gs):
         i = g.info()
         tests += i["tests_l1"] + i["tests_l2"]
         voxels += i["n1"] + i["nb"] * i["n2"] ** 3
+        for k in tot:
+            tot[k] += i[k]
         if check:
             a = g.download()
             ga = ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], m["voxel_count"], m["voxel_count2"],
@@ -106,7 +111,74 @@ def bench_batched(gv, ctx, n_meshes, steps, warmup, check):
         "batched_tri_box_tests_per_s": tests / t_batch,
         "batched_voxels_per_s": voxels / t_batch,
         "batched_stage_ms": stage_ms,
+        "totals": tot,
         "bit_exact_vs_plain": exact,
+    }
+
+
+def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48):
+    """The served stream: Poisson arrivals at `rate` requests/s, each drawing
+    a mesh from a library made by the mixed-stream generators (triangles
+    log-uniform in [1k, 2M]) and a resolution (Level1 {64,128,256,512},
+    Level2 {4,8,16}).  Library soups are device-resident; results stay
+    resident until the client releases them (right after it has waited)."""
+    import threading
+    from gpview_amd import workloads
+    lib_meshes = workloads.mixed_stream(gv, n_library, seed=seed, tri_range=(1000, 2000000))
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, n_library, n_requests)
+    l1 = rng.choice([64, 128, 256, 512], n_requests, p=[0.3, 0.3, 0.3, 0.1])
+    l2 = rng.choice([4, 8, 16], n_requests, p=[0.45, 0.45, 0.1])
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    srv = gv.Server(lanes=lanes, hbm_budget=200e9)
+    ctx = gv.Context(0)
+    ptrs = [ctx.upload(m["soup"]) for m in lib_meshes]
+    ids = [None] * n_requests
+    ready = threading.Semaphore(0)
+    stats = [None] * n_requests
+    tris = np.zeros(n_requests, np.int64)
+
+    def submitter():
+        t_next = time.perf_counter()
+        for i in range(n_requests):
+            t_next += gaps[i]
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            m = dict(lib_meshes[pick[i]], voxel_count=int(l1[i]), voxel_count2=int(l2[i]))
+            tris[i] = len(m["soup"])
+            ids[i] = srv.submit(m, device_ptr=ptrs[pick[i]])
+            ready.release()
+
+    th = threading.Thread(target=submitter)
+    t0 = time.perf_counter()
+    th.start()
+    for i in range(n_requests):
+        ready.acquire()
+        _, st = srv.wait(ids[i], timeout=600)
+        stats[i] = st
+        srv.release(ids[i])
+    th.join()
+    wall = time.perf_counter() - t0
+    c = srv.counters()
+    srv.close()
+    for p in ptrs:
+        ctx.free(p)
+    ctx.close()
+    lat = np.array([s["t_done"] - s["t_submit"] for s in stats])
+    span = max(s["t_done"] for s in stats) - min(s["t_submit"] for s in stats)
+    sizes = np.array([s["batch_size"] for s in stats])
+    pred = np.array([s["predicted_bytes"] for s in stats], np.float64)
+    res = np.array([s["resident_bytes"] for s in stats], np.float64)
+    return {
+        "workload": f"served stream: {n_requests} Poisson requests at {rate:g}/s over a {n_library}-mesh "
+                    "device-resident library (1k-2M triangles), L1 {64,128,256,512}, L2 {4,8,16}",
+        "requests": n_requests, "rate_per_s": rate, "lanes": lanes, "hbm_budget_bytes": 200e9,
+        "meshes_per_s": n_requests / span, "wall_s": wall, "triangles": int(tris.sum()),
+        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_batch": float(sizes.mean()), "batches": c["batches"],
+        "peak_charged_bytes": c["peak_charged"],
+        "predicted_over_resident": float(np.median(pred / np.maximum(res, 1))),
     }
 
 
@@ -135,6 +207,9 @@ def main():
     ap.add_argument("--batch-meshes", type=int, default=256, help="meshes of the mixed-stream batch (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-check", action="store_true", help="skip the batched-vs-plain bit-exact check")
+    ap.add_argument("--serve-requests", type=int, default=1000, help="served-stream requests (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="served-stream arrivals per second")
+    ap.add_argument("--serve-lanes", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (meshes shard "
@@ -174,6 +249,9 @@ def main():
     if args.batch_meshes > 0:
         batched = bench_batched(gv, ctx, args.batch_meshes, args.batch_steps, 1, not args.no_check)
     ctx.close()
+    served = None
+    if args.serve_requests > 0:
+        served = bench_serve(gv, args.serve_requests, args.serve_rate, args.serve_lanes)
 
     meshes_per_s = args.steps / wall
     tests = info["tests_l1"] + info["tests_l2"]
@@ -237,6 +315,7 @@ def main():
         "roofline_valu": valu,
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

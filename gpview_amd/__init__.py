@@ -28,7 +28,7 @@ class GpvError(RuntimeError):
 class Params(ctypes.Structure):
     """gpv_params (include/gpv.h)."""
     _fields_ = [("voxel_count", ctypes.c_int), ("voxel_count2", ctypes.c_int), ("level2", ctypes.c_int),
-                ("initial_capacity", ctypes.c_int64)]
+                ("normals", ctypes.c_int), ("initial_capacity", ctypes.c_int64)]
 
 
 class GridInfo(ctypes.Structure):
@@ -42,6 +42,7 @@ class GridInfo(ctypes.Structure):
         ("tests_l1", ctypes.c_int64), ("tests_l2", ctypes.c_int64), ("pairs", ctypes.c_int64),
         ("max_tris_per_voxel", ctypes.c_int32), ("regrows", ctypes.c_int32),
         ("ms_stage", ctypes.c_float * 6), ("ms_total", ctypes.c_float),
+        ("has_normals", ctypes.c_int32), ("ms_normals", ctypes.c_float),
     ]
 
     def as_dict(self):
@@ -90,7 +91,7 @@ EXPORTED = [
     "gpv_mesh_bbox", "gpv_grid_derive", "gpv_voxelize", "gpv_voxelize_batch", "gpv_grid_create",
     "gpv_classify_tessellation", "gpv_classify_inout", "gpv_compact_boundary",
     "gpv_classify_inout_level2", "gpv_classify_tessellation_level2", "gpv_grid_finish",
-    "gpv_grid_get_info", "gpv_grid_download", "gpv_grid_device_views", "gpv_save_raw",
+    "gpv_grid_get_info", "gpv_grid_download", "gpv_grid_download_normals", "gpv_grid_device_views", "gpv_save_raw",
     "gpv_grid_release",
     "gpv_mesh_free", "gpv_mesh_flatten", "gpv_mesh_bounds", "gpv_mesh_icosphere",
     "gpv_mesh_torus_knot", "gpv_mesh_torus", "gpv_mesh_drilled_box", "gpv_mesh_mixed",
@@ -135,6 +136,7 @@ def lib():
         getattr(L, fn).argtypes = [_vp, _vp]
     L.gpv_grid_get_info.argtypes = [_vp, P(GridInfo)]
     L.gpv_grid_download.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+    L.gpv_grid_download_normals.argtypes = [_vp, _vp, _vp, _vp]
     L.gpv_grid_device_views.argtypes = [_vp, P(_vp), P(_vp), P(_vp), P(_vp), P(_vp)]
     L.gpv_save_raw.argtypes = [_vp, _vp, c_int, ctypes.c_char_p]
     L.gpv_mesh_free.argtypes = [P(Mesh)]
@@ -255,7 +257,7 @@ def mesh_bbox(verts):
 
 def grid_derive(bmin, bmax, mms, voxel_count, voxel_count2, level2=True):
     info = GridInfo()
-    p = Params(voxel_count, voxel_count2, 1 if level2 else 0, 0)
+    p = Params(voxel_count=voxel_count, voxel_count2=voxel_count2, level2=1 if level2 else 0)
     _check(lib().gpv_grid_derive(np.asarray(bmin, np.float32), np.asarray(bmax, np.float32), float(mms),
                                  ctypes.byref(p), ctypes.byref(info)), "gpv_grid_derive")
     return info
@@ -293,10 +295,11 @@ class Context:
         _check(lib().gpv_device_free(self.h, ptr), "gpv_device_free")
 
     def voxelize(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0,
-                 device_ptr=None):
+                 device_ptr=None, normals=False):
         """Object::PerformVoxelization on a flat soup; returns a Grid."""
         soup = np.ascontiguousarray(soup, np.float32).reshape(-1, 9)
-        p = Params(int(voxel_count), int(voxel_count2), 1 if level2 else 0, int(initial_capacity))
+        p = Params(voxel_count=int(voxel_count), voxel_count2=int(voxel_count2), level2=1 if level2 else 0,
+                   normals=1 if normals else 0, initial_capacity=int(initial_capacity))
         g = _vp()
         ptr = device_ptr if device_ptr is not None else soup.ctypes.data
         _check(lib().gpv_voxelize(self.h, ptr, len(soup), 1 if device_ptr is not None else 0,
@@ -322,10 +325,12 @@ class Context:
                "gpv_voxelize_batch")
         return [Grid(self, _vp(out[i])) for i in range(n)]
 
-    def create(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0):
+    def create(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0,
+               normals=False):
         """gpv_grid_create: the staged form (call the stage methods on the Grid)."""
         soup = np.ascontiguousarray(soup, np.float32).reshape(-1, 9)
-        p = Params(int(voxel_count), int(voxel_count2), 1 if level2 else 0, int(initial_capacity))
+        p = Params(voxel_count=int(voxel_count), voxel_count2=int(voxel_count2), level2=1 if level2 else 0,
+                   normals=1 if normals else 0, initial_capacity=int(initial_capacity))
         g = _vp()
         _check(lib().gpv_grid_create(self.h, soup.ctypes.data, len(soup), 0, np.asarray(bmin, np.float32),
                                      np.asarray(bmax, np.float32), float(mms), ctypes.byref(p),
@@ -343,8 +348,9 @@ def mesh_desc(m, device_ptr=None):
     d.bmin[:] = [float(x) for x in m["bmin"]]
     d.bmax[:] = [float(x) for x in m["bmax"]]
     d.max_model_size = float(m["mms"])
-    d.params = Params(int(m["voxel_count"]), int(m.get("voxel_count2", 4)), 1 if m.get("level2", True) else 0,
-                      int(m.get("initial_capacity", 0)))
+    d.params = Params(voxel_count=int(m["voxel_count"]), voxel_count2=int(m.get("voxel_count2", 4)),
+                      level2=1 if m.get("level2", True) else 0, normals=1 if m.get("normals", False) else 0,
+                      initial_capacity=int(m.get("initial_capacity", 0)))
     return d, soup
 
 
@@ -456,6 +462,16 @@ class Grid:
                                        bidx.ctypes.data if nb else None,
                                        l2.ctypes.data if (nb and n2) else None), "gpv_grid_download")
         return {"l1": l1, "prefix": prefix, "bidx": bidx[:nb], "l2": l2[:nb * n2 ** 3]}
+
+    def download_normals(self):
+        """(l1n [nb,3], l2n [nb*n2^3,3]) float32 of a grid made with normals."""
+        i = self.info()
+        nb, n2 = i["nb"], i["n2"]
+        l1n = np.zeros((max(nb, 1), 3), np.float32)
+        l2n = np.zeros((max(nb * n2 ** 3, 1), 3), np.float32)
+        _check(lib().gpv_grid_download_normals(self.ctx.h, self.h, l1n.ctypes.data,
+                                               l2n.ctypes.data if n2 else None), "gpv_grid_download_normals")
+        return l1n[:nb], l2n[:nb * n2 ** 3]
 
     def save_raw(self, obj_id, directory):
         _check(lib().gpv_save_raw(self.ctx.h, self.h, int(obj_id), os.fsencode(directory)), "gpv_save_raw")

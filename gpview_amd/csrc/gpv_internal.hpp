@@ -15,6 +15,7 @@ namespace gpv {
 struct GridDev {
     int nx, ny, nz, nxw;
     int n2, w2;
+    int normals, pad;      // normals: this mesh asks for Level1/Level2 normals
     float bminx, bminy, bminz, bmaxx, bmaxy, bmaxz;
     float h1x, h1y, h1z;   // Level1 half extents
     float h2x, h2y, h2z;   // Level2 half extents
@@ -32,6 +33,7 @@ struct GridDev {
 struct MeshOff {
     uint64_t word0;
     uint64_t l2w0;
+    uint64_t l2n0;   // first Level2 normal (one per Level2 voxel of meshes with normals)
     uint32_t col0;
     uint32_t tri0;
     uint32_t b0;
@@ -132,6 +134,19 @@ void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint
                    const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s);
 void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec, const GridDev& g0,
                       const Meshes& M, uint64_t* par2, hipStream_t s);
+
+// normals (src/Object.cpp:3219-3253 Level1, 2372-2421 + 2705-2720 Level2)
+void launch_face_normals(const float* tris, int64_t T, float* fn, hipStream_t s);
+void launch_sort_bins(const uint32_t* list, const uint32_t* boff, int64_t B, const GridDev& g0, const Meshes& M,
+                      uint32_t* slist, hipStream_t s);
+void launch_l1_normals(const uint32_t* slist, const uint32_t* boff, int64_t B, const float* fn, const GridDev& g0,
+                       const Meshes& M, float* l1n, hipStream_t s);
+void launch_l2_normals(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff,
+                       const uint32_t* slist, const float* fn, const GridDev& g0, const Meshes& M, float* l2n,
+                       hipStream_t s);
+void launch_export_l1_normals(const uint64_t* bnd, const uint32_t* wprefix, uint32_t base, const float* l1n,
+                              const GridDev& g, uint8_t* out, hipStream_t s);
+void launch_export_l2_normals(const float* l2n, int64_t n, uint8_t* out, hipStream_t s);
 
 void launch_count_states(const uint64_t* bnd, const uint64_t* ins, int64_t W, const uint64_t* sat2,
                          const uint64_t* par2, int64_t W2, const Meshes& M, hipStream_t s);

@@ -137,6 +137,26 @@ __device__ __forceinline__ int mesh_of_word(const MeshOff* __restrict__ off, int
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// The same lookup by a whole wave for a wave-uniform w: 64-ary search, two
+// rounds of loads for a few hundred meshes instead of log2(nm) dependent ones.
+// Every lane of the wave must call it.
+__device__ __forceinline__ int wave_mesh_of_word(const MeshOff* __restrict__ off, int nm, uint64_t w)
+{
+    const int lane = __lane_id();
+    int lo = 0, hi = nm;   // off[lo].word0 <= w; the answer lies in [lo, hi)
+    while (hi - lo > 1) {
+        const int step = (hi - lo + 63) / 64;
+        const int idx = lo + lane * step;
+        const bool le = idx < hi && off[idx].word0 <= w;
+        const uint64_t m = __ballot(le);
+        const int L = 63 - __clzll((long long)m);
+        const int nlo = lo + L * step;
+        hi = nlo + step < hi ? nlo + step : hi;
+        lo = nlo;
+    }
+    return uniform_i(lo);
+}
+
 // ---------------------------------------------------------------------------
 // fills (used instead of memset so that the same code can sit in a graph)
 // ---------------------------------------------------------------------------
@@ -833,7 +853,7 @@ __global__ __launch_bounds__(256) void k_l1_parity(const float4* __restrict__ re
     const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (w >= W) return;
     const int lane = __lane_id();
-    const int mesh = kM ? uniform_i(mesh_of_word(M.off, M.nm, (uint64_t)w)) : 0;
+    const int mesh = kM ? wave_mesh_of_word(M.off, M.nm, (uint64_t)w) : 0;
     const GridDev& g = grid_of<kM>(g0, M, mesh);
     const int64_t wl = w - (int64_t)(kM ? M.off[mesh].word0 : 0);
     const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
@@ -886,17 +906,24 @@ void launch_popc_words(const uint64_t* bits, int64_t W, uint32_t* cnt, hipStream
     k_popc_words<<<grid_for(W, 256, 16384), 256, 0, s>>>(bits, W, cnt);
 }
 
-// b0 (first boundary rank) and l2w0 (first Level2 word) of every mesh, from
-// the batch-wide word prefix; one thread walks the nm meshes in order.
+// b0 (first boundary rank), l2w0 (first Level2 word) and l2n0 (first Level2
+// normal) of every mesh, from the batch-wide word prefix; one thread walks the
+// nm meshes in order.
 __global__ void k_mesh_bounds(const uint32_t* __restrict__ wprefix, Meshes M)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    uint64_t l2 = 0;
+    uint64_t l2 = 0, l2n = 0;
     for (int m = 0; m <= M.nm; ++m) {
         const uint32_t b0 = wprefix[M.off[m].word0];
         M.off[m].b0 = b0;
         M.off[m].l2w0 = l2;
-        if (m < M.nm) l2 += (uint64_t)(wprefix[M.off[m + 1].word0] - b0) * (uint64_t)M.grids[m].w2;
+        M.off[m].l2n0 = l2n;
+        if (m < M.nm) {
+            const uint64_t nb = wprefix[M.off[m + 1].word0] - b0;
+            const GridDev& g = M.grids[m];
+            l2 += nb * (uint64_t)g.w2;
+            if (g.normals) l2n += nb * (uint64_t)g.n2 * g.n2 * g.n2;
+        }
     }
 }
 
@@ -1358,6 +1385,294 @@ void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, cons
         k_l2_parity<true><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, g0, M, par2);
     else
         k_l2_parity<false><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, g0, M, par2);
+}
+
+// ---------------------------------------------------------------------------
+// normals.  Face normal: side1 = v1 - v0, side2 = v2 - v0, their cross
+// product, VectorNormalize (src/Object.cpp:3233-3240; the load-time
+// faceNormal of :521-526 is the same float expression on the same points).
+// VectorNormalize (includes/FloatVector.h:332-341) divides by the float
+// magnitude of the left-to-right sum of squares when it is nonzero; sqrtf and
+// the division are correctly rounded here.  The reference sums in bin order,
+// which its CPU sweep makes ascending in triangle id, so bins are sorted
+// before any sum.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z)
+{
+    const float m = sqrtf(x * x + y * y + z * z);
+    if (m != 0.0f) {
+        x = x / m;
+        y = y / m;
+        z = z / m;
+    }
+}
+
+__global__ void k_face_normals(const float* __restrict__ tris, int64_t T, float* __restrict__ fn)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x) {
+        const float* tv = tris + 9 * t;
+        const float a0 = tv[3] - tv[0], a1 = tv[4] - tv[1], a2 = tv[5] - tv[2];
+        const float b0 = tv[6] - tv[0], b1 = tv[7] - tv[1], b2 = tv[8] - tv[2];
+        float x = a1 * b2 - b1 * a2;
+        float y = a2 * b0 - b2 * a0;
+        float z = a0 * b1 - b0 * a1;
+        normalize3(x, y, z);
+        fn[3 * t + 0] = x;
+        fn[3 * t + 1] = y;
+        fn[3 * t + 2] = z;
+    }
+}
+
+void launch_face_normals(const float* tris, int64_t T, float* fn, hipStream_t s)
+{
+    if (T <= 0) return;
+    k_face_normals<<<grid_for(T, 256, 16384), 256, 0, s>>>(tris, T, fn);
+}
+
+// true when boundary voxel b's mesh asks for normals (wave-uniform)
+template <bool kM>
+__device__ __forceinline__ bool wants_normals(const Meshes& M, int64_t b)
+{
+    if constexpr (kM) return M.grids[uniform_i((int)M.bmesh[b])].normals != 0;
+    else return true;
+}
+
+// One wave per boundary voxel: the bin's triangle ids (distinct) ranked by
+// counting smaller ids, written in ascending order to slist.
+template <bool kM>
+__global__ __launch_bounds__(256) void k_sort_bins(const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ boff, int64_t B, Meshes M,
+                                                   uint32_t* __restrict__ slist)
+{
+    const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (b >= B || !wants_normals<kM>(M, b)) return;
+    const int lane = __lane_id();
+    const uint32_t e0 = boff[b], n = boff[b + 1] - e0;
+    for (uint32_t bi = 0; bi < n; bi += 64) {
+        const uint32_t ti = bi + lane < n ? list[e0 + bi + lane] : 0xffffffffu;
+        uint32_t rank = 0;
+        for (uint32_t bj = 0; bj < n; bj += 64) {
+            const uint32_t tj = bj + lane < n ? list[e0 + bj + lane] : 0xffffffffu;
+            const uint32_t m = n - bj < 64 ? n - bj : 64;
+            for (uint32_t k = 0; k < m; ++k)
+                rank += (uint32_t)((uint32_t)__builtin_amdgcn_readlane((int)tj, (int)k) < ti);
+        }
+        if (bi + lane < n) slist[e0 + rank] = ti;
+    }
+}
+
+void launch_sort_bins(const uint32_t* list, const uint32_t* boff, int64_t B, const GridDev& g0, const Meshes& M,
+                      uint32_t* slist, hipStream_t s)
+{
+    (void)g0;
+    if (B <= 0) return;
+    const unsigned blocks = (unsigned)((B * 64 + 255) / 256);
+    if (M.nm > 1) k_sort_bins<true><<<blocks, 256, 0, s>>>(list, boff, B, M, slist);
+    else k_sort_bins<false><<<blocks, 256, 0, s>>>(list, boff, B, M, slist);
+}
+
+// Level1 normal of a boundary voxel (src/Object.cpp:3219-3253): the face
+// normals of its sorted bin summed in order, divided by the count, normalized.
+template <bool kM>
+__global__ __launch_bounds__(256) void k_l1_normals(const uint32_t* __restrict__ slist,
+                                                    const uint32_t* __restrict__ boff, int64_t B,
+                                                    const float* __restrict__ fn, Meshes M,
+                                                    float* __restrict__ l1n)
+{
+    const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (b >= B || !wants_normals<kM>(M, b)) return;
+    const int lane = __lane_id();
+    const uint32_t e0 = boff[b], n = boff[b + 1] - e0;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (uint32_t bi = 0; bi < n; bi += 64) {
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        if (bi + lane < n) {
+            const uint64_t t = slist[e0 + bi + lane];
+            fx = fn[3 * t];
+            fy = fn[3 * t + 1];
+            fz = fn[3 * t + 2];
+        }
+        const uint32_t m = n - bi < 64 ? n - bi : 64;
+        for (uint32_t k = 0; k < m; ++k) {
+            sx = sx + lane_f(fx, (int)k);
+            sy = sy + lane_f(fy, (int)k);
+            sz = sz + lane_f(fz, (int)k);
+        }
+    }
+    const float cnt = (float)n;
+    float x = sx / cnt, y = sy / cnt, z = sz / cnt;
+    normalize3(x, y, z);
+    if (lane == 0) {
+        l1n[3 * b + 0] = x;
+        l1n[3 * b + 1] = y;
+        l1n[3 * b + 2] = z;
+    }
+}
+
+void launch_l1_normals(const uint32_t* slist, const uint32_t* boff, int64_t B, const float* fn, const GridDev& g0,
+                       const Meshes& M, float* l1n, hipStream_t s)
+{
+    (void)g0;
+    if (B <= 0) return;
+    const unsigned blocks = (unsigned)((B * 64 + 255) / 256);
+    if (M.nm > 1) k_l1_normals<true><<<blocks, 256, 0, s>>>(slist, boff, B, fn, M, l1n);
+    else k_l1_normals<false><<<blocks, 256, 0, s>>>(slist, boff, B, fn, M, l1n);
+}
+
+// Level2 normals (AddPointToVoxel src/Object.cpp:2705-2720 under the loop nest
+// of :2372-2400, averaged at :2404-2421): per Level2 voxel, the face normals
+// of the bin triangles whose SAT hits it, in bin order, over their count,
+// normalized; (0, 0, 0) where none hits.  One wave per boundary voxel, each
+// lane owning L2N_G Level2 voxels per pass; the box-axis predicates are
+// evaluated once per (triangle, sub-index) by ballot, as in k_l2_sat.
+constexpr int L2N_G = 8;
+
+template <bool kM>
+__global__ __launch_bounds__(256) void k_l2_normals(const float* __restrict__ tris,
+                                                    const int32_t* __restrict__ bidx, int64_t B,
+                                                    const uint32_t* __restrict__ boff,
+                                                    const uint32_t* __restrict__ slist,
+                                                    const float* __restrict__ fn, GridDev g0, Meshes M,
+                                                    float* __restrict__ l2n)
+{
+    const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (b >= B) return;
+    const int mesh = kM ? uniform_i((int)M.bmesh[b]) : 0;
+    const GridDev& g = grid_of<kM>(g0, M, mesh);
+    const int n2 = g.n2, gnx = g.nx, gny = g.ny;
+    if (n2 == 0 || (kM && !g.normals)) return;
+    const float h2x = g.h2x, h2y = g.h2y, h2z = g.h2z;
+    const int S = n2 * n2 * n2;
+    float* out = l2n + 3 * (kM ? M.off[mesh].l2n0 + (uint64_t)(b - M.off[mesh].b0) * S : (uint64_t)b * S);
+    const int v = bidx[b];
+    const int nxy = gnx * gny;
+    const int k1 = v / nxy;
+    const int ij = v - k1 * nxy;
+    const int j1 = ij / gnx;
+    const int i1 = ij - j1 * gnx;
+    const float* c2x = g.c2x + (int64_t)i1 * n2;
+    const float* c2y = g.c2y + (int64_t)j1 * n2;
+    const float* c2z = g.c2z + (int64_t)k1 * n2;
+    const int lane = __lane_id();
+    const float cxl = lane < n2 ? c2x[lane] : 0.0f;
+    const float cyl = lane < n2 ? c2y[lane] : 0.0f;
+    const float czl = lane < n2 ? c2z[lane] : 0.0f;
+    const uint32_t e0 = boff[b], n = boff[b + 1] - e0;
+    for (int s0 = 0; s0 < S; s0 += 64 * L2N_G) {
+        float ax[L2N_G], ay[L2N_G], az[L2N_G], ac[L2N_G];
+        float ccx[L2N_G], ccy[L2N_G], ccz[L2N_G];
+        int sp[L2N_G], sq[L2N_G], sr[L2N_G];
+        // centers of this lane's Level2 voxels, gathered while every lane is active
+#pragma unroll
+        for (int k = 0; k < L2N_G; ++k) {
+            const int sv = s0 + k * 64 + lane;
+            ax[k] = ay[k] = az[k] = ac[k] = 0.0f;
+            sp[k] = sv % n2;
+            sq[k] = (sv / n2) % n2;
+            sr[k] = (sv / (n2 * n2)) % n2;
+            ccx[k] = __shfl(cxl, sp[k]);
+            ccy[k] = __shfl(cyl, sq[k]);
+            ccz[k] = __shfl(czl, sr[k]);
+        }
+        for (uint32_t e = 0; e < n; ++e) {
+            const uint64_t t = slist[e0 + e];
+            float tv[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) tv[c] = tris[t * 9 + c];
+            const uint64_t mx = __ballot(lane < n2 && sat_axis_ok(tv[0], tv[3], tv[6], cxl, h2x));
+            const uint64_t my = __ballot(lane < n2 && sat_axis_ok(tv[1], tv[4], tv[7], cyl, h2y));
+            const uint64_t mz = __ballot(lane < n2 && sat_axis_ok(tv[2], tv[5], tv[8], czl, h2z));
+            if (!mx || !my || !mz) continue;
+            const float fx = fn[3 * t], fy = fn[3 * t + 1], fz = fn[3 * t + 2];
+#pragma unroll
+            for (int k = 0; k < L2N_G; ++k) {
+                const int sv = s0 + k * 64 + lane;
+                if (sv >= S || !((mx >> sp[k]) & (my >> sq[k]) & (mz >> sr[k]) & 1ull)) continue;
+                if (tri_box_overlap<false>(ccx[k], ccy[k], ccz[k], h2x, h2y, h2z, tv)) {
+                    ax[k] = ax[k] + fx;
+                    ay[k] = ay[k] + fy;
+                    az[k] = az[k] + fz;
+                    ac[k] = ac[k] + 1.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < L2N_G; ++k) {
+            const int sv = s0 + k * 64 + lane;
+            if (sv >= S) continue;
+            float x = 0.0f, y = 0.0f, z = 0.0f;
+            if (ac[k] > 0.0f) {
+                x = ax[k] / ac[k];
+                y = ay[k] / ac[k];
+                z = az[k] / ac[k];
+                normalize3(x, y, z);
+            }
+            out[3 * (int64_t)sv + 0] = x;
+            out[3 * (int64_t)sv + 1] = y;
+            out[3 * (int64_t)sv + 2] = z;
+        }
+    }
+}
+
+void launch_l2_normals(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff,
+                       const uint32_t* slist, const float* fn, const GridDev& g0, const Meshes& M, float* l2n,
+                       hipStream_t s)
+{
+    if (B <= 0) return;
+    const unsigned blocks = (unsigned)((B * 64 + 255) / 256);
+    if (M.nm > 1) k_l2_normals<true><<<blocks, 256, 0, s>>>(tris, bidx, B, boff, slist, fn, g0, M, l2n);
+    else k_l2_normals<false><<<blocks, 256, 0, s>>>(tris, bidx, B, boff, slist, fn, g0, M, l2n);
+}
+
+// .raw normal byte: outputDType(n * normScale + normBias), normScale =
+// 256.0/3.0 and normBias = 127.0 as floats (src/Object.cpp:2940-2942, 3032-3034)
+__device__ __forceinline__ uint8_t normal_byte(float v)
+{
+    const float y = v * 85.33333587646484375f;
+    return (uint8_t)(int)(y + 127.0f);
+}
+
+// Level1Normal.raw: 3 bytes per Level1 voxel, [z][y][x]; only boundary voxels
+// carry a normal, the rest are 0 -> 127
+__global__ void k_export_l1_normals(const uint64_t* __restrict__ bnd, const uint32_t* __restrict__ wprefix,
+                                    uint32_t base, const float* __restrict__ l1n, GridDev g, int64_t n1,
+                                    uint8_t* __restrict__ out)
+{
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n1; v += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(v % g.nx);
+        const int64_t row = v / g.nx;
+        const int64_t w = row * g.nxw + (x >> 6);
+        float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+        if ((bnd[w] >> (x & 63)) & 1ull) {
+            const uint64_t r = rank_of(bnd, wprefix, (uint64_t)w * 64 + (uint64_t)(x & 63)) - base;
+            nx = l1n[3 * r];
+            ny = l1n[3 * r + 1];
+            nz = l1n[3 * r + 2];
+        }
+        out[3 * v + 0] = normal_byte(nx);
+        out[3 * v + 1] = normal_byte(ny);
+        out[3 * v + 2] = normal_byte(nz);
+    }
+}
+
+void launch_export_l1_normals(const uint64_t* bnd, const uint32_t* wprefix, uint32_t base, const float* l1n,
+                              const GridDev& g, uint8_t* out, hipStream_t s)
+{
+    const int64_t n1 = (int64_t)g.nx * g.ny * g.nz;
+    if (n1 <= 0) return;
+    k_export_l1_normals<<<grid_for(n1, 256, 65536), 256, 0, s>>>(bnd, wprefix, base, l1n, g, n1, out);
+}
+
+__global__ void k_export_l2_normals(const float* __restrict__ l2n, int64_t n, uint8_t* __restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = normal_byte(l2n[i]);
+}
+
+void launch_export_l2_normals(const float* l2n, int64_t n, uint8_t* out, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_export_l2_normals<<<grid_for(n, 256, 65536), 256, 0, s>>>(l2n, n, out);
 }
 
 // ---------------------------------------------------------------------------

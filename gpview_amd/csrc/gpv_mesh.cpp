@@ -467,7 +467,7 @@ int gpv_mesh_stress(gpv_mesh* m, int voxel_count, int voxel_count2, float p_snap
     float bmin[3], bmax[3], mms;
     int r = gpv_mesh_bbox(m->verts, m->nverts, bmin, bmax, &mms);
     if (r != GPV_OK) return r;
-    gpv_params prm{voxel_count, voxel_count2, 1, 0};
+    gpv_params prm{voxel_count, voxel_count2, 1, 0, 0};
     gpv_grid_info info;
     if ((r = gpv_grid_derive(bmin, bmax, mms, &prm, &info)) != GPV_OK) return r;
     float lo[3], hi[3];

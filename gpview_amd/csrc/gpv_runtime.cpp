@@ -190,12 +190,20 @@ struct Job {
     float4* hrec = nullptr;
     uint64_t *sat2 = nullptr, *par2 = nullptr;
     int n2max = 0;
+    // normals (meshes with params.normals): face normals, sorted bins, and the
+    // resident results per boundary voxel / per Level2 voxel
+    bool any_normals = false;
+    float* fnorm = nullptr;
+    uint32_t* slist = nullptr;
+    float *l1n = nullptr, *l2n = nullptr;
+    int64_t NL2 = 0;
     // progress
     bool setup_done = false, sat_done = false, inout_done = false, compact_done = false,
          l2p_done = false, l2s_done = false, finished = false;
     // stream events: 0 setup, 1 L1 SAT, 2 ray expansion, 3 parity fill,
-    // 4 compaction+bins, 5 L2 parity, 6 L2 SAT start, 7 L2 SAT end, 8 finish
-    hipEvent_t ev[9] = {};
+    // 4 compaction+bins, 5 L2 parity, 6 L2 SAT start, 7 L2 SAT end, 8 finish,
+    // 9 normals start, 10 normals end
+    hipEvent_t ev[11] = {};
     std::vector<void*> bufs;
 
     Meshes view() const { return Meshes{d_grids, d_off, tri_mesh, bmesh, mctr, nm}; }
@@ -427,6 +435,9 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
     for (int m = 0; m < nm; ++m) {
         GridDev& g = J->g[m];
         g = grid_dev(J->info[m]);
+        g.normals = d[m].params.normals ? 1 : 0;
+        J->info[m].has_normals = g.normals;
+        J->any_normals = J->any_normals || g.normals;
         J->n2max = std::max(J->n2max, g.n2);
         J->off[m].word0 = word0;
         J->off[m].col0 = col0;
@@ -610,9 +621,12 @@ int complete_sat(Job* J, int64_t* needed)
         J->off[0].l2w0 = 0;
         J->off[1].b0 = B;
         J->off[1].l2w0 = (uint64_t)B * (uint64_t)J->g[0].w2;
+        J->off[0].l2n0 = 0;
+        J->off[1].l2n0 = J->g[0].normals ? (uint64_t)B * J->g[0].n2 * J->g[0].n2 * J->g[0].n2 : 0;
     }
     J->B = J->off[J->nm].b0;
     J->W2 = (int64_t)J->off[J->nm].l2w0;
+    J->NL2 = (int64_t)J->off[J->nm].l2n0;
     J->sat_done = true;
     return GPV_OK;
 }
@@ -724,6 +738,30 @@ int run_l2_sat(Job* J)
     return GPV_OK;
 }
 
+// Level1 and Level2 normals of the meshes that ask for them
+int run_normals(Job* J)
+{
+    if (!J->any_normals) return GPV_OK;
+    hipStream_t s = J->ctx->stream;
+    HIP_TRY(hipEventRecord(J->ev[9], s));
+    release(J, J->fnorm);
+    release(J, J->slist);
+    release(J, J->l1n);
+    release(J, J->l2n);
+    TRY(alloc(J, &J->fnorm, (size_t)std::max<int64_t>(J->T, 1) * 3));
+    TRY(alloc(J, &J->slist, (size_t)std::max<uint64_t>(J->n_pairs, 1)));
+    TRY(alloc(J, &J->l1n, (size_t)std::max<int64_t>(J->B, 1) * 3));
+    TRY(alloc(J, &J->l2n, (size_t)std::max<int64_t>(J->NL2, 1) * 3));
+    launch_face_normals(J->tris, J->T, J->fnorm, s);
+    launch_sort_bins(J->list, J->boff, J->B, J->g[0], J->view(), J->slist, s);
+    launch_l1_normals(J->slist, J->boff, J->B, J->fnorm, J->g[0], J->view(), J->l1n, s);
+    if (J->NL2 > 0)
+        launch_l2_normals(J->tris, J->bidx, J->B, J->boff, J->slist, J->fnorm, J->g[0], J->view(), J->l2n, s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(J->ev[10], s));
+    return GPV_OK;
+}
+
 // counts, per-mesh info, and the release of everything but the results
 int job_finish(Job* J)
 {
@@ -755,6 +793,7 @@ int job_finish(Job* J)
         ms[3] = el(4, 8);
     }
     const float total = el(0, 8);
+    const float ms_nrm = J->any_normals ? el(9, 10) : 0.0f;
     for (int m = 0; m < J->nm; ++m) {
         gpv_grid_info& I = J->info[m];
         const int64_t n23 = (int64_t)I.n2 * I.n2 * I.n2;
@@ -770,6 +809,7 @@ int job_finish(Job* J)
         I.regrows = J->regrows;
         for (int k = 0; k < 6; ++k) I.ms_stage[k] = ms[k];
         I.ms_total = total;
+        I.ms_normals = ms_nrm;
     }
     // only the result planes stay resident
     release(J, J->box0); release(J, J->box1); release(J, J->rng);
@@ -780,6 +820,7 @@ int job_finish(Job* J)
     release(J, J->list); release(J, J->ccnt); release(J, J->coff); release(J, J->crec); release(J, J->hcnt);
     release(J, J->hoff); release(J, J->hrec); release(J, J->tri_mesh); release(J, J->bmesh);
     release(J, J->d_grids); release(J, J->d_off); release(J, J->own_soup); release(J, J->mctr);
+    release(J, J->fnorm); release(J, J->slist);
     J->tris = nullptr;
     J->finished = true;
     return GPV_OK;
@@ -808,6 +849,7 @@ int job_run(Job* J)
     TRY(launch_compact(J));
     TRY(run_l2_parity(J));
     TRY(run_l2_sat(J));
+    TRY(run_normals(J));
     return job_finish(J);
 }
 
@@ -1071,6 +1113,7 @@ int gpv_grid_finish(gpv_ctx* c, gpv_grid* G)
     if (!J->compact_done || !J->inout_done) return fail(GPV_ERR_STATE, "Level1 stages have not completed");
     if (J->n2max > 0 && (!J->l2p_done || !J->l2s_done)) return fail(GPV_ERR_STATE, "Level2 stages have not completed");
     HIP_TRY(hipSetDevice(c->device));
+    TRY(run_normals(J));
     return job_finish(J);
 }
 
@@ -1185,6 +1228,57 @@ int gpv_grid_download(gpv_ctx* c, const gpv_grid* G, uint8_t* l1, int32_t* prefi
     return download_impl(c, G, 1, l1, prefix, bidx, l2);
 }
 
+int gpv_grid_download_normals(gpv_ctx* c, const gpv_grid* G, float* l1n, float* l2n)
+{
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    TRY(check_grid(c, G));
+    Job* J = G->job;
+    if (!J->finished) return fail(GPV_ERR_STATE, "grid is not finished");
+    if (!J->g[G->m].normals) return fail(GPV_ERR_STATE, "grid was made without params.normals");
+    HIP_TRY(hipSetDevice(c->device));
+    const MeshOff& o = J->off[G->m];
+    const int64_t nb = J->nb(G->m);
+    const int n2 = J->g[G->m].n2;
+    const int64_t n23 = (int64_t)n2 * n2 * n2;
+    if (l1n && nb > 0)
+        HIP_TRY(hipMemcpyAsync(l1n, J->l1n + 3 * (size_t)o.b0, (size_t)nb * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                               c->stream));
+    if (l2n && nb > 0 && n23 > 0)
+        HIP_TRY(hipMemcpyAsync(l2n, J->l2n + 3 * (size_t)o.l2n0, (size_t)(nb * n23) * 3 * sizeof(float),
+                               hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPV_OK;
+}
+
+// quantised normal files of SaveVoxelization (src/Object.cpp:3024-3050)
+static int normal_bytes(gpv_ctx* c, const gpv_grid* G, std::vector<uint8_t>& l1b, std::vector<uint8_t>& l2b)
+{
+    Job* J = G->job;
+    hipStream_t s = c->stream;
+    const int m = G->m;
+    const GridDev& g = J->g[m];
+    const MeshOff& o = J->off[m];
+    const int64_t n1 = J->info[m].n1, nb = J->nb(m);
+    const int64_t n23 = (int64_t)g.n2 * g.n2 * g.n2;
+    l1b.assign((size_t)n1 * 3, 0);
+    uint8_t* d = nullptr;
+    TRY(alloc(J, &d, (size_t)n1 * 3));
+    launch_export_l1_normals(J->bnd + o.word0, J->wprefix + o.word0, o.b0, J->l1n + 3 * (size_t)o.b0, g, d, s);
+    HIP_TRY(hipMemcpyAsync(l1b.data(), d, (size_t)n1 * 3, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    release(J, d);
+    l2b.assign(g.n2 > 0 ? (size_t)(nb * n23 * 3) : 0, 0);
+    if (!l2b.empty()) {
+        TRY(alloc(J, &d, l2b.size()));
+        launch_export_l2_normals(J->l2n + 3 * (size_t)o.l2n0, (int64_t)l2b.size(), d, s);
+        HIP_TRY(hipMemcpyAsync(l2b.data(), d, l2b.size(), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        release(J, d);
+    }
+    return GPV_OK;
+}
+
 // Object::SaveVoxelization, src/Object.cpp:2934-3075 (in/out, prefix and
 // config files; the config uses the ostream default of 6 significant digits).
 int gpv_save_raw(gpv_ctx* c, const gpv_grid* G, int obj_id, const char* dir)
@@ -1226,6 +1320,12 @@ int gpv_save_raw(gpv_ctx* c, const gpv_grid* G, int obj_id, const char* dir)
     if (l2) {
         TRY(put("Level1BoundaryPrefixSum.raw", pre.data(), pre.size() * 4));
         TRY(put("Level2InOut.raw", l2v.data(), l2v.size()));
+    }
+    if (G->job->g[G->m].normals) {
+        std::vector<uint8_t> n1b, n2b;
+        TRY(normal_bytes(c, G, n1b, n2b));
+        TRY(put("Level1Normal.raw", n1b.data(), n1b.size()));
+        if (l2) TRY(put("Level2Normal.raw", n2b.data(), n2b.size()));
     }
     return GPV_OK;
 }

@@ -13,8 +13,9 @@
  * boundary compaction (:3270-3279), Level2 ray parity (:1111-1207), Level2
  * SAT (:2344-2424), .raw output (:2934-3075).
  *
- * Threading: one context is used by one host thread at a time.  Calls on a
- * context are ordered on its stream; separate contexts run concurrently.
+ * Threading: calls on one context are serialised by the library (a context
+ * may be shared by threads) and ordered on its stream; separate contexts run
+ * concurrently on their own streams.
  */
 #ifndef GPV_H
 #define GPV_H
@@ -43,6 +44,9 @@ typedef struct gpv_params {
     int voxel_count;          /* GLParameters::voxelCount  (Level1 nominal divisions) */
     int voxel_count2;         /* GLParameters::voxelCount2 (Level2 divisions per axis) */
     int level2;               /* GLParameters::level2Voxels */
+    int normals;              /* also compute the Level1/Level2 normals that
+                                 PerformVoxelization stores (src/Object.cpp:3219-3253,
+                                 2404-2421) and SaveVoxelization writes */
     int64_t initial_capacity; /* bin-pair capacity; <= 0 picks a default.  The
                                  analogue of PerformVoxelization's bufferSize
                                  (src/Object.cpp:3088-3089); overflow regrows and
@@ -68,6 +72,8 @@ typedef struct gpv_grid_info {
     float ms_stage[6];        /* device time per stage: setup, L1 SAT, L1 parity,
                                  compaction+bins, L2 parity, L2 SAT */
     float ms_total;           /* device time, first to last stage */
+    int32_t has_normals;      /* normals were computed (params.normals) */
+    float ms_normals;         /* device time of the normals stage */
 } gpv_grid_info;
 
 typedef struct gpv_ctx gpv_ctx;     /* device, stream, memory pool */
@@ -169,6 +175,15 @@ int gpv_grid_get_info(const gpv_grid *grid, gpv_grid_info *info);
  *   l2_state[nb*n2^3]   0/1/2 per Level2 voxel, [b][r][q][p] */
 int gpv_grid_download(gpv_ctx *ctx, const gpv_grid *grid, uint8_t *l1_state, int32_t *prefix,
                       int32_t *bidx, uint8_t *l2_state);
+/* Normals of a grid made with params.normals (GPV_ERR_STATE otherwise), as
+ * the reference stores them before quantising (any may be NULL):
+ *   l1_normals[nb*3]       per boundary voxel in bidx order: the average of the
+ *                          face normals of its Level1 bin, normalized
+ *                          (src/Object.cpp:3219-3253; other voxels have none)
+ *   l2_normals[nb*n2^3*3]  per Level2 voxel, [b][r][q][p]: the average of the
+ *                          face normals of the bin triangles that hit it,
+ *                          normalized, or 0 (src/Object.cpp:2404-2421, 2705-2720) */
+int gpv_grid_download_normals(gpv_ctx *ctx, const gpv_grid *grid, float *l1_normals, float *l2_normals);
 /* Raw device views (valid until release), for zero-copy consumers.
  * Level1 bitplanes: word (z*ny + y)*ceil(nx/64) + x/64, bit x%64 (rows padded
  * to whole 64-bit words).  Level2 bitplanes: ceil(n2^3/64) words per boundary
@@ -180,7 +195,9 @@ int gpv_grid_device_views(const gpv_grid *grid, const uint64_t **l1_boundary_bit
 /* Object::SaveVoxelization (src/Object.cpp:2934-3075): writes
  * Obj<obj_id>VoxelConfig.txt, Obj<obj_id>Level1InOut.raw and, with Level2,
  * Obj<obj_id>Level1BoundaryPrefixSum.raw and Obj<obj_id>Level2InOut.raw
- * under dir (NULL = current directory). */
+ * under dir (NULL = current directory); for a grid made with params.normals
+ * also Obj<obj_id>Level1Normal.raw and (with Level2) Obj<obj_id>Level2Normal.raw,
+ * uint8(n * 256/3 + 127) per component. */
 int gpv_save_raw(gpv_ctx *ctx, const gpv_grid *grid, int obj_id, const char *dir);
 int gpv_grid_release(gpv_ctx *ctx, gpv_grid *grid);
 

@@ -844,6 +844,170 @@ int orc_voxelize(const float *tris, int64_t T, const orc_grid *g, int mode, int 
     return 0;
 }
 
+/* ---------------------------------------------------------------------------
+ * normals (the part of the path that feeds Level1Normal.raw / Level2Normal.raw)
+ * ------------------------------------------------------------------------- */
+
+/* VectorNormalize, includes/FloatVector.h:332-341: float magnitude of the
+ * left-to-right float sum of squares; components divided when nonzero. */
+void orc_normalize(float v[3])
+{
+    float m = (float)sqrt((double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+    if (m != 0) {
+        v[0] /= m;
+        v[1] /= m;
+        v[2] /= m;
+    }
+}
+
+/* Face normal of the flat soup triangle, src/Object.cpp:3233-3240 (and the
+ * load-time faceNormal of :521-526, the same float expressions on the same
+ * points): side1 = v1 - v0, side2 = v2 - v0, VectorCrossProduct
+ * (includes/FloatVector.h:293-296), VectorNormalize. */
+void orc_face_normal(const float tv[9], float n[3])
+{
+    float a[3], b[3];
+    for (int q = 0; q < 3; q++) {
+        a[q] = tv[3 + q] - tv[q];
+        b[q] = tv[6 + q] - tv[q];
+    }
+    n[0] = a[1] * b[2] - b[1] * a[2];
+    n[1] = a[2] * b[0] - b[2] * a[0];
+    n[2] = a[0] * b[1] - b[0] * a[1];
+    orc_normalize(n);
+}
+
+void orc_face_normal_batch(int64_t n, const float *tv, float *out)
+{
+    for (int64_t i = 0; i < n; i++) orc_face_normal(tv + 9 * i, out + 3 * i);
+}
+
+void orc_normalize_batch(int64_t n, const float *v, float *out)
+{
+    for (int64_t i = 0; i < n; i++) {
+        out[3 * i] = v[3 * i];
+        out[3 * i + 1] = v[3 * i + 1];
+        out[3 * i + 2] = v[3 * i + 2];
+        orc_normalize(out + 3 * i);
+    }
+}
+
+/* Level1SAT list average, src/Object.cpp:3219-3253: face normals summed in bin
+ * order (ascending triangle id: ClassifyTessellation walks the triangles in
+ * order and AddPointToVoxel appends, :2256-2342, 2705-2748), divided by the
+ * (float) count, normalized. */
+static void average_normal(const float *fn, int64_t n, float out[3])
+{
+    float s[3] = {0, 0, 0};
+    for (int64_t e = 0; e < n; e++) {
+        s[0] += fn[3 * e];
+        s[1] += fn[3 * e + 1];
+        s[2] += fn[3 * e + 2];
+    }
+    const float cnt = (float)n;
+    out[0] = s[0] / cnt;
+    out[1] = s[1] / cnt;
+    out[2] = s[2] / cnt;
+    orc_normalize(out);
+}
+
+void orc_average_normals(int64_t nl, const int64_t *off, const float *fn, float *out)
+{
+    for (int64_t l = 0; l < nl; l++) average_normal(fn + 3 * off[l], off[l + 1] - off[l], out + 3 * l);
+}
+
+static void l1_normal(const float *tris, const int32_t *list, int64_t n, float out[3])
+{
+    float *fn = (float *)xcalloc((size_t)(n > 0 ? n : 1) * 3, sizeof(float));
+    for (int64_t e = 0; e < n; e++) orc_face_normal(tris + 9 * (int64_t)list[e], fn + 3 * e);
+    average_normal(fn, n, out);
+    free(fn);
+}
+
+/* Level1 normals per boundary voxel (l1n[nb*3]) and Level2 normals per Level2
+ * voxel (l2n[nb*n2^3*3]; NULL to skip).  Level2: every bin triangle whose SAT
+ * hits the Level2 voxel adds its face normal and 1 to a count
+ * (AddPointToVoxel, src/Object.cpp:2705-2720) in bin order (the loop nest of
+ * :2372-2400); the sum is then divided by the count and normalized
+ * (:2404-2421).  Voxels with no hit keep (0, 0, 0). */
+int orc_normals(const float *tris, const orc_grid *g, const orc_result *r, int threads, float *l1n,
+                float *l2n)
+{
+    const int nt = threads > 0 ? threads : omp_get_max_threads();
+    const int64_t nx = g->n[0], ny = g->n[1];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+    for (int64_t b = 0; b < r->nb; b++) {
+        const int64_t v = r->bidx[b];
+        l1_normal(tris, r->list_tri + r->list_off[v], r->list_off[v + 1] - r->list_off[v], l1n + 3 * b);
+    }
+    if (!l2n || g->n2 <= 0) return 0;
+    const int n2 = g->n2;
+    const int64_t n23 = (int64_t)n2 * n2 * n2;
+    float *sx = (float *)xcalloc((size_t)g->n[0] * n2, sizeof(float));
+    float *sy = (float *)xcalloc((size_t)g->n[1] * n2, sizeof(float));
+    float *sz = (float *)xcalloc((size_t)g->n[2] * n2, sizeof(float));
+    orc_l2_centers(g, 0, sx);
+    orc_l2_centers(g, 1, sy);
+    orc_l2_centers(g, 2, sz);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+    for (int64_t b = 0; b < r->nb; b++) {
+        const int64_t l1 = r->bidx[b];
+        const int64_t k1 = l1 / (nx * ny), ij = l1 - k1 * nx * ny, j1 = ij / nx, i1 = ij % nx;
+        float *acc = (float *)xcalloc((size_t)n23 * 4, sizeof(float));
+        for (int64_t e = r->list_off[l1]; e < r->list_off[l1 + 1]; e++) {
+            const float *tv = tris + 9 * (int64_t)r->list_tri[e];
+            float fn[3];
+            orc_face_normal(tv, fn);
+            for (int p = 0; p < n2; p++)
+                for (int q = 0; q < n2; q++)
+                    for (int s = 0; s < n2; s++) {
+                        const float ctr[3] = {sx[i1 * n2 + p], sy[j1 * n2 + q], sz[k1 * n2 + s]};
+                        if (orc_tri_box_overlap(ctr, g->ext2, tv)) {
+                            float *a = acc + 4 * (((int64_t)s * n2 + q) * n2 + p);
+                            a[0] += fn[0];
+                            a[1] += fn[1];
+                            a[2] += fn[2];
+                            a[3] += 1;
+                        }
+                    }
+        }
+        for (int64_t i = 0; i < n23; i++) {
+            float o[3] = {0, 0, 0};
+            const float cnt = acc[4 * i + 3];
+            if (cnt > 0) {
+                o[0] = acc[4 * i + 0] / cnt;
+                o[1] = acc[4 * i + 1] / cnt;
+                o[2] = acc[4 * i + 2] / cnt;
+                orc_normalize(o);
+            }
+            float *d = l2n + 3 * (b * n23 + i);
+            d[0] = o[0];
+            d[1] = o[1];
+            d[2] = o[2];
+        }
+        free(acc);
+    }
+    free(sx);
+    free(sy);
+    free(sz);
+    return 0;
+}
+
+/* .raw normal byte, src/Object.cpp:2940-2942 and 3032-3034, 3047-3049:
+ * outputDType(n * normScale + normBias) with normScale = 256.0/3.0 and
+ * normBias = 127.0 held in floats. */
+uint8_t orc_quantize_normal(float v)
+{
+    const float scale = (float)(256.0 / 3.0), bias = 127.0f;
+    const float y = v * scale;
+    return (uint8_t)(y + bias);
+}
+
+void orc_quantize_normals(int64_t n, const float *v, uint8_t *out)
+{
+    for (int64_t i = 0; i < n; i++) out[i] = orc_quantize_normal(v[i]);
+}
+
 void orc_result_free(orc_result *r)
 {
     free(r->l1);

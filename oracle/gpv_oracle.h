@@ -16,6 +16,7 @@
  *   Level2 ray parity     src/Object.cpp:1111-1207
  *   Level2 SAT            src/Object.cpp:2344-2424
  *   counts                src/Object.cpp:3353-3378
+ *   normals               src/Object.cpp:3219-3253 (Level1), 2372-2421 + 2705-2720 (Level2)
  *   .raw writer           src/Object.cpp:2934-3075
  *
  * Built with gcc -O2 -ffp-contract=off (x86-64 SSE2 float semantics, no
@@ -102,6 +103,20 @@ int orc_ray_column_bounds(const float tv[9], const float bmin[3], const float bm
 int orc_voxelize(const float *tris, int64_t ntris, const orc_grid *g, int mode,
                  int threads, orc_result *out);
 void orc_result_free(orc_result *r);
+
+/* Normals: VectorNormalize, the flat-soup face normal, Level1 normals per
+ * boundary voxel [nb*3] and Level2 normals per Level2 voxel [nb*n2^3*3]
+ * (l2n may be NULL), and the .raw normal byte. */
+void orc_normalize(float v[3]);
+void orc_face_normal(const float tv[9], float n[3]);
+void orc_face_normal_batch(int64_t n, const float *tv, float *out);
+void orc_normalize_batch(int64_t n, const float *v, float *out);
+/* Level1 average of nl lists of face normals fn (CSR off[nl+1]), in order. */
+void orc_average_normals(int64_t nl, const int64_t *off, const float *fn, float *out);
+int orc_normals(const float *tris, const orc_grid *g, const orc_result *r, int threads, float *l1n,
+                float *l2n);
+uint8_t orc_quantize_normal(float v);
+void orc_quantize_normals(int64_t n, const float *v, uint8_t *out);
 
 /* Writes Obj<obj_id>{VoxelConfig.txt,Level1InOut.raw,Level1BoundaryPrefixSum.raw,
  * Level2InOut.raw} under dir.  Returns 0 on success. */

@@ -88,6 +88,12 @@ def lib():
         L.orc_voxelize.argtypes = [_f32p, ctypes.c_int64, ctypes.POINTER(Grid), ctypes.c_int,
                                    ctypes.c_int, ctypes.POINTER(_Result)]
         L.orc_result_free.argtypes = [ctypes.POINTER(_Result)]
+        L.orc_face_normal_batch.argtypes = [ctypes.c_int64, _f32p, _f32p]
+        L.orc_normalize_batch.argtypes = [ctypes.c_int64, _f32p, _f32p]
+        L.orc_average_normals.argtypes = [ctypes.c_int64, np.ctypeslib.ndpointer(np.int64), _f32p, _f32p]
+        L.orc_normals.argtypes = [_f32p, ctypes.POINTER(Grid), ctypes.POINTER(_Result), ctypes.c_int,
+                                  _f32p, ctypes.c_void_p]
+        L.orc_quantize_normals.argtypes = [ctypes.c_int64, _f32p, _u8p]
         L.orc_save_raw.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(_Result), ctypes.c_int,
                                    ctypes.c_char_p]
         _lib = L
@@ -105,6 +111,9 @@ def ref():
         R = ctypes.CDLL(path)
         R.ref_tri_box_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _f32p, _u8p]
         R.ref_tri_ray_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _f32p, _u8p, _f32p]
+        R.ref_face_normal_batch.argtypes = [ctypes.c_int64, _f32p, _f32p]
+        R.ref_normalize_batch.argtypes = [ctypes.c_int64, _f32p, _f32p]
+        R.ref_average_normals.argtypes = [ctypes.c_int64, np.ctypeslib.ndpointer(np.int64), _f32p, _f32p]
         _ref = R
     return _ref
 
@@ -181,8 +190,39 @@ def column_bounds(tv, bmin, bmax, safety=1.0):
     return bool(ok), lo, hi
 
 
-def voxelize(tris, g, mode=ORC_FAST, threads=0):
-    """Run the oracle pipeline; returns a dict of numpy arrays and counters."""
+def face_normals(tv, use_ref=False):
+    tv = _f32(tv).reshape(-1, 9)
+    out = np.zeros((len(tv), 3), np.float32)
+    (ref().ref_face_normal_batch if use_ref else lib().orc_face_normal_batch)(len(tv), tv.ravel(), out.ravel())
+    return out
+
+
+def normalize(v, use_ref=False):
+    v = _f32(v).reshape(-1, 3)
+    out = np.zeros_like(v)
+    (ref().ref_normalize_batch if use_ref else lib().orc_normalize_batch)(len(v), v.ravel(), out.ravel())
+    return out
+
+
+def average_normals(off, fn, use_ref=False):
+    off = np.ascontiguousarray(off, np.int64)
+    fn = _f32(fn).reshape(-1, 3)
+    out = np.zeros((len(off) - 1, 3), np.float32)
+    (ref().ref_average_normals if use_ref else lib().orc_average_normals)(len(off) - 1, off, fn.ravel(),
+                                                                          out.ravel())
+    return out
+
+
+def quantize_normals(v):
+    v = _f32(v).ravel()
+    out = np.zeros(len(v), np.uint8)
+    lib().orc_quantize_normals(len(v), v, out)
+    return out
+
+
+def voxelize(tris, g, mode=ORC_FAST, threads=0, normals=False):
+    """Run the oracle pipeline; returns a dict of numpy arrays and counters
+    (with normals=True also l1n [nb,3] and l2n [nb*n2^3,3])."""
     tris = _f32(tris).reshape(-1)
     T = len(tris) // 9
     r = _Result()
@@ -204,6 +244,13 @@ def voxelize(tris, g, mode=ORC_FAST, threads=0):
             "max_tris_per_voxel": r.max_tris_per_voxel,
             "t_stage": list(r.t_stage),
         }
+        if normals:
+            l1n = np.zeros((max(r.nb, 1), 3), np.float32)
+            l2n = np.zeros((max(r.nb * n23, 1), 3), np.float32)
+            lib().orc_normals(tris, ctypes.byref(g), ctypes.byref(r), int(threads), l1n.ravel(),
+                              l2n.ctypes.data if n23 else None)
+            out["l1n"] = l1n[:r.nb]
+            out["l2n"] = l2n[:r.nb * n23]
     finally:
         lib().orc_result_free(ctypes.byref(r))
     return out

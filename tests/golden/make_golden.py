@@ -9,7 +9,8 @@ cases with the adversarial families that bit-exactness is fragile on.
 Run in a tree where /root/reference is mounted:
     make -C oracle && python tests/golden/make_golden.py
 Outputs (committed): tests/golden/sat_cases.npz, tests/golden/mt_cases.npz,
-tests/golden/manifest.json.
+tests/golden/normal_cases.npz (from includes/FloatVector.h through
+oracle/ref_vec_shim.cpp), tests/golden/manifest.json.
 """
 import hashlib
 import json
@@ -160,6 +161,28 @@ def mt_cases(rng):
     return cases
 
 
+def normal_cases(rng, sat):
+    """Face normals of every SAT-family triangle (degenerate ones included:
+    zero cross products stay zero), VectorNormalize on vectors from 1e-30 to
+    1e30 plus zeros and denormals, and Level1 averages over lists of 1..80
+    unit normals (the bins' sum / count / normalize)."""
+    tv = np.concatenate([v[2] for v in sat.values()])
+    n = 20000
+    v = _f(rng.normal(0, 1, (n, 3)) * 10 ** rng.uniform(-30, 30, (n, 1)))
+    v[:50] = 0
+    v[50:100] = _f(rng.normal(0, 1, (50, 3)) * 1e-40)
+    v[100:150, 1:] = 0
+    sizes = rng.integers(1, 81, 1000)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    fn = orc.face_normals(_f(rng.normal(0, 1, (int(off[-1]), 9))), use_ref=True)
+    avg = np.zeros((len(sizes), 3), F32)
+    orc.ref().ref_average_normals(len(sizes), off, fn.ravel(), avg.ravel())
+    # face_n follows the triangles of sat_cases.npz, families in file order
+    return {"face_n": orc.face_normals(tv, use_ref=True),
+            "norm_v": v, "norm_out": orc.normalize(v, use_ref=True),
+            "avg_off": off, "avg_fn": fn, "avg_out": avg}
+
+
 def main():
     R = orc.ref()
     if R is None:
@@ -178,6 +201,8 @@ def main():
         out2[f"{k}_tv"], out2[f"{k}_o"], out2[f"{k}_d"] = tv, o, d
         out2[f"{k}_hit"], out2[f"{k}_t"] = hit, t
     np.savez_compressed(os.path.join(HERE, "mt_cases.npz"), **out2)
+    nc = normal_cases(np.random.default_rng(20261203), sat)
+    np.savez_compressed(os.path.join(HERE, "normal_cases.npz"), **nc)
     src = "/root/reference/src"
     man = {
         "generator": "tests/golden/make_golden.py",
@@ -186,6 +211,10 @@ def main():
         "compiled_with": "g++ -O2 -ffp-contract=off -fno-fast-math (oracle/Makefile)",
         "sha256": {f: hashlib.sha256(open(os.path.join(src, f), "rb").read()).hexdigest()
                    for f in ("TriBoxIntersection.cpp", "TriRayIntersection.cpp")},
+        "normals_from": "includes/FloatVector.h (Float3, VectorCrossProduct, VectorNormalize) via "
+                        "oracle/ref_vec_shim.cpp; normal_cases.npz uses seed 20261203",
+        "sha256_includes": {"FloatVector.h": hashlib.sha256(
+            open("/root/reference/includes/FloatVector.h", "rb").read()).hexdigest()},
         "sat_families": {k: int(len(v[0])) for k, v in sat.items()},
         "sat_hit_rate": {k: float(out[f"{k}_res"].mean()) for k in sat},
         "mt_families": {k: int(len(v[0])) for k, v in mt.items()},

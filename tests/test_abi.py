@@ -170,9 +170,10 @@ def test_invalid_arguments(gv):
                     reason="needs hipcc")
 def test_codegen_exactness():
     """Our own gfx950 ISA: f32 denormals kept, f32 division is the
-    correctly rounded v_div_scale/v_div_fmas/v_div_fixup expansion, no packed
-    FP32, and every f32 FMA-family instruction belongs to a division
-    expansion (no contracted multiply-add)."""
+    correctly rounded v_div_scale/v_div_fmas/v_div_fixup expansion, f32 sqrt
+    the correctly rounded v_sqrt_f32 + neighbour-residual expansion, no packed
+    FP32, and every f32 FMA-family instruction belongs to one of those
+    expansions (no contracted multiply-add)."""
     from gpview_amd import build
     s = build.disassemble().splitlines()
     assert not any(re.match(r"\s+v_pk_(add|mul|fma)_f32", l) for l in s)
@@ -184,8 +185,15 @@ def test_codegen_exactness():
         if fma.match(x):
             win = " ".join(ins[max(0, i - 12):i + 12])
             ok = ("v_div_scale_f32" in win or "v_div_fmas_f32" in win or "v_rcp_iflag_f32" in win
-                  or ("v_rcp_f32" in win and ("v_cvt_u32_f32" in win or "v_mul_hi" in win)))
+                  or ("v_rcp_f32" in win and ("v_cvt_u32_f32" in win or "v_mul_hi" in win))
+                  or "v_sqrt_f32" in " ".join(ins[max(0, i - 4):i + 1]))
             assert ok, f"unexplained {x}"
+    # every hardware sqrt (1 ulp) is followed by the neighbour residual test
+    for i, x in enumerate(ins):
+        if x.startswith("v_sqrt_f32"):
+            win = ins[i + 1:i + 12]
+            assert sum(1 for y in win if y.startswith("v_fma_f32")) >= 2, "bare v_sqrt_f32"
+            assert any(y.startswith("v_add_u32") and ", -1," in y for y in win), "bare v_sqrt_f32"
 
 
 def test_predict_footprint_monotone(gv):
