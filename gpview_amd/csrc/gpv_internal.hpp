@@ -91,7 +91,8 @@ void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
 
 void launch_mesh_ids(const MeshOff* off, int nm, int64_t T, uint32_t* tri_mesh, hipStream_t s);
 void launch_tri_setup(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
-                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, hipStream_t s);
+                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, double4* slab2,
+                      hipStream_t s);
 
 // exclusive scan of n u32 into out[0..n] (out[n] = total); tmp sized by
 // scan_tmp_elems(n).  If max_out != nullptr the max input is atomically
@@ -101,9 +102,9 @@ void launch_scan_u32_u64(const uint32_t* in, uint64_t* out, int64_t n, uint64_t*
 void launch_scan_u32_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, unsigned int* max_out,
                          hipStream_t s);
 
-void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const uint64_t* off,
-                       const GridDev& g0, const Meshes& M, uint64_t* bnd, Append64 pairs, Append64 cols, int blocks,
-                       hipStream_t s);
+void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const double4* slab1,
+                       const uint64_t* off, const GridDev& g0, const Meshes& M, uint64_t* bnd, Append64 pairs,
+                       Append64 cols, int blocks, hipStream_t s);
 void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off, const GridDev& g0,
                        const Meshes& M, Append64 hits, int blocks, hipStream_t s);
 
@@ -126,12 +127,14 @@ void launch_boundary_index(const uint64_t* bnd, const uint32_t* wprefix, const G
 void launch_pair_rank(const uint64_t* bits, uint64_t n, const uint64_t* bnd, const uint32_t* wprefix, uint32_t* pb,
                       uint32_t* cntb, hipStream_t s);
 void launch_pair_scatter(const uint32_t* tri, const uint32_t* pb, uint64_t n, const uint32_t* boff, uint32_t* cntb,
-                         uint32_t* list, hipStream_t s);
+                         uint32_t* list, const float* tris, const int32_t* bidx, const GridDev& g0, const Meshes& M,
+                         uint64_t* pmask, hipStream_t s);
 // per-mesh bin totals and largest bin (multi-mesh runs)
 void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStream_t s);
 
 void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
-                   const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s);
+                   const uint64_t* pmask, const double4* slab2, const GridDev& g0, const Meshes& M, uint64_t* sat2,
+                   hipStream_t s);
 void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec, const GridDev& g0,
                       const Meshes& M, uint64_t* par2, hipStream_t s);
 

@@ -285,12 +285,19 @@ void launch_mesh_ids(const MeshOff* off, int nm, int64_t T, uint32_t* tri_mesh, 
 // the mesh's test counter.  Multi-mesh form: one triangle per thread; blocks
 // inside one mesh (tri_mesh is non-decreasing) reduce into that mesh's
 // counter, the few straddling blocks add per lane.
+// Triangles whose index AABB spans at least kZSlabMin Level1 layers also get a
+// plane slab (gpv_slab.hpp) over the centers of that AABB: the Level1 sweep
+// then visits, per column, only the layers the slab can keep.
+constexpr int kZSlabMin = 3;
+
 template <bool kM>
 __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tris, int64_t T, GridDev g0, Meshes M,
                                                    int4* __restrict__ box0, int4* __restrict__ box1,
                                                    uint32_t* __restrict__ sat_items,
                                                    int4* __restrict__ ray_rng,
-                                                   uint32_t* __restrict__ ray_items)
+                                                   uint32_t* __restrict__ ray_items,
+                                                   double4* __restrict__ slab1,
+                                                   double4* __restrict__ slab2)
 {
     uint64_t tests = 0;
     const int64_t first = (int64_t)blockIdx.x * blockDim.x;
@@ -323,6 +330,22 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
         const bool any = dd[0] > 0 && dd[1] > 0 && dd[2] > 0;
         sat_items[t] = any ? (uint32_t)(dd[0] * dd[1]) : 0u;
         tests += any ? (uint64_t)dd[0] * dd[1] * dd[2] : 0;
+        if (any && dd[2] >= kZSlabMin) {
+            const double clo[3] = {g.c1x[lo[0]], g.c1y[lo[1]], g.c1z[lo[2]]};
+            const double chi[3] = {g.c1x[lo[0] + dd[0] - 1], g.c1y[lo[1] + dd[1] - 1], g.c1z[lo[2] + dd[2] - 1]};
+            const PlaneSlab sl = plane_slab(tv, g.h1x, g.h1y, g.h1z, clo, chi);
+            slab1[t] = make_double4(sl.nx, sl.ny, sl.nz, sl.lim);
+        }
+        if (any && g.n2 > 0) {
+            // over every Level2 center of the index AABB's Level1 voxels (the
+            // tables are monotone, so the extreme entries bound them)
+            const int n2 = g.n2;
+            const double clo[3] = {g.c2x[(int64_t)lo[0] * n2], g.c2y[(int64_t)lo[1] * n2], g.c2z[(int64_t)lo[2] * n2]};
+            const double chi[3] = {g.c2x[(int64_t)(lo[0] + dd[0]) * n2 - 1], g.c2y[(int64_t)(lo[1] + dd[1]) * n2 - 1],
+                                   g.c2z[(int64_t)(lo[2] + dd[2]) * n2 - 1]};
+            const PlaneSlab sl = plane_slab(tv, g.h2x, g.h2y, g.h2z, clo, chi);
+            slab2[t] = make_double4(sl.nx, sl.ny, sl.nz, sl.lim);
+        }
         uint32_t rit = 0;
         int4 rr = make_int4(0, 0, 0, 0);
         if (vis) {
@@ -356,15 +379,16 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
 }
 
 void launch_tri_setup(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
-                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, hipStream_t s)
+                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, double4* slab2,
+                      hipStream_t s)
 {
     if (T <= 0) return;
     if (M.nm > 1)
         k_tri_setup<true><<<grid_for(T, 256, 1 << 30), 256, 0, s>>>(tris, T, g0, M, box0, box1, sat_items, ray_rng,
-                                                                     ray_items);
+                                                                     ray_items, slab1, slab2);
     else
         k_tri_setup<false><<<grid_for(T, 256, 1024), 256, 0, s>>>(tris, T, g0, M, box0, box1, sat_items, ray_rng,
-                                                                   ray_items);
+                                                                   ray_items, slab1, slab2);
 }
 
 // ---------------------------------------------------------------------------
@@ -647,6 +671,7 @@ template <bool kM>
 __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tris, int64_t T,
                                                    const int4* __restrict__ box0,
                                                    const int4* __restrict__ box1,
+                                                   const double4* __restrict__ slab1,
                                                    const uint64_t* __restrict__ off, GridDev g0, Meshes M,
                                                    unsigned long long* __restrict__ bnd, Append64 pairs,
                                                    Append64 cols)
@@ -677,8 +702,13 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
         const uint64_t rowword = word0 + (uint64_t)q * g.nxw + (uint64_t)(p >> 6);
         const uint64_t bit = 1ull << (p & 63);
         const float* c1z = g.c1z;
+        int za = b0.z, zb = b0.z + b1.z;
+        if (b1.z >= kZSlabMin) {
+            const double4 sl = slab1[t];
+            slab_layers(sl.x, sl.y, sl.z, sl.w, cx, cy, tv, c1z, b0.z, b0.z + b1.z, za, zb);
+        }
         int zmax = -1;
-        for (int z = b0.z; z < b0.z + b1.z; ++z) {
+        for (int z = za; z < zb; ++z) {
             const bool hit = tri_box_overlap(cx, cy, c1z[z], hx, hy, hz, tv);
             const uint64_t w = (uint64_t)z * plane + rowword;
             if (hit) {
@@ -700,16 +730,16 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
     });
 }
 
-void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const uint64_t* off,
-                       const GridDev& g0, const Meshes& M, uint64_t* bnd, Append64 pairs, Append64 cols, int blocks,
-                       hipStream_t s)
+void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const double4* slab1,
+                       const uint64_t* off, const GridDev& g0, const Meshes& M, uint64_t* bnd, Append64 pairs,
+                       Append64 cols, int blocks, hipStream_t s)
 {
     if (T <= 0) return;
     unsigned long long* b = reinterpret_cast<unsigned long long*>(bnd);
     if (M.nm > 1)
-        k_sat_expand<true><<<blocks, XB, 0, s>>>(tris, T, box0, box1, off, g0, M, b, pairs, cols);
+        k_sat_expand<true><<<blocks, XB, 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols);
     else
-        k_sat_expand<false><<<blocks, XB, 0, s>>>(tris, T, box0, box1, off, g0, M, b, pairs, cols);
+        k_sat_expand<false><<<blocks, XB, 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols);
 }
 
 // Level1 ray parity, first half: one item = (triangle, column inside its
@@ -996,23 +1026,59 @@ void launch_pair_rank(const uint64_t* bits, uint64_t n, const uint64_t* bnd, con
     k_pair_rank<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(bits, n, bnd, wprefix, pb, cntb);
 }
 
+// Bins by boundary voxel, and for each bin entry the Level2 box-axis masks:
+// bit p of mask x says the SAT's x box-axis predicate holds for the triangle
+// and the Level2 sub-index p of that voxel (the same float test the full SAT
+// runs), likewise y, z; packed x | y << 16 | z << 32.
+template <bool kM>
 __global__ void k_pair_scatter(const uint32_t* __restrict__ tri, const uint32_t* __restrict__ pb, uint64_t n,
                                const uint32_t* __restrict__ boff, uint32_t* __restrict__ cntb,
-                               uint32_t* __restrict__ list)
+                               uint32_t* __restrict__ list, const float* __restrict__ tris,
+                               const int32_t* __restrict__ bidx, GridDev g0, Meshes M,
+                               uint64_t* __restrict__ pmask)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t b = pb[i];
+        const uint32_t t = tri[i];
         const uint32_t pos = boff[b] + atomicSub(&cntb[b], 1u) - 1u;
-        list[pos] = tri[i];
+        list[pos] = t;
+        const int mesh = kM ? (int)M.bmesh[b] : 0;
+        const GridDev& g = grid_of<kM>(g0, M, mesh);
+        const int n2 = g.n2;
+        if (!pmask || n2 == 0) continue;
+        const int v = bidx[b];
+        const int nxy = g.nx * g.ny;
+        const int k1 = v / nxy;
+        const int ij = v - k1 * nxy;
+        const int j1 = ij / g.nx;
+        const int i1 = ij - j1 * g.nx;
+        const float* tv = tris + 9 * (uint64_t)t;
+        const float a0 = tv[0], a1 = tv[1], a2 = tv[2], a3 = tv[3], a4 = tv[4], a5 = tv[5], a6 = tv[6],
+                    a7 = tv[7], a8 = tv[8];
+        const float* cx = g.c2x + (int64_t)i1 * n2;
+        const float* cy = g.c2y + (int64_t)j1 * n2;
+        const float* cz = g.c2z + (int64_t)k1 * n2;
+        uint64_t mx = 0, my = 0, mz = 0;
+        for (int p = 0; p < n2; ++p) {
+            mx |= (uint64_t)sat_axis_ok(a0, a3, a6, cx[p], g.h2x) << p;
+            my |= (uint64_t)sat_axis_ok(a1, a4, a7, cy[p], g.h2y) << p;
+            mz |= (uint64_t)sat_axis_ok(a2, a5, a8, cz[p], g.h2z) << p;
+        }
+        pmask[pos] = mx | (my << 16) | (mz << 32);
     }
 }
 
 void launch_pair_scatter(const uint32_t* tri, const uint32_t* pb, uint64_t n, const uint32_t* boff, uint32_t* cntb,
-                         uint32_t* list, hipStream_t s)
+                         uint32_t* list, const float* tris, const int32_t* bidx, const GridDev& g0, const Meshes& M,
+                         uint64_t* pmask, hipStream_t s)
 {
     if (!n) return;
-    k_pair_scatter<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(tri, pb, n, boff, cntb, list);
+    const int blocks = grid_for((int64_t)n, 256, 16384);
+    if (M.nm > 1)
+        k_pair_scatter<true><<<blocks, 256, 0, s>>>(tri, pb, n, boff, cntb, list, tris, bidx, g0, M, pmask);
+    else
+        k_pair_scatter<false><<<blocks, 256, 0, s>>>(tri, pb, n, boff, cntb, list, tris, bidx, g0, M, pmask);
 }
 
 // Per-mesh bin totals (pairs) and largest bin of a batch.  bmesh is
@@ -1052,11 +1118,14 @@ void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStrea
 }
 
 // ---------------------------------------------------------------------------
-// Level2 SAT (src/Object.cpp:2344-2424): one wave per boundary voxel.  For
-// each triangle of the parent's bin the wave first evaluates the box-axis
-// predicate of the SAT on every sub-index of each axis (the same float test
-// the full SAT runs, so sub-voxels failing it are misses), then spreads the
-// surviving sub-box over its lanes.  Bits accumulate in LDS.
+// Level2 SAT (src/Object.cpp:2344-2424): one wave per boundary voxel.  Each
+// bin entry carries its box-axis masks (the SAT's own box-axis predicate per
+// sub-index, computed with the bins) and its triangle a plane slab
+// (gpv_slab.hpp) valid over every Level2 center of its index AABB.  For each
+// triangle the lanes take the sub-columns (p, q) its x/y masks allow and keep
+// the sub-voxels r whose |N.(c - t0)| the slab admits (an exact double bound:
+// the SAT rejects every other one); those queue for the full float SAT in
+// 64-lane batches.  Sub-voxels already set are skipped (the result is an OR).
 // ---------------------------------------------------------------------------
 constexpr int L2_WAVES = 4;
 constexpr int L2_QCAP = 128;   // ring of queued (triangle, sub-voxel) entries per wave
@@ -1065,6 +1134,21 @@ constexpr int L2_QCAP = 128;   // ring of queued (triangle, sub-voxel) entries p
 __device__ __forceinline__ float lane_f(float v, int l)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ double lane_d(double v, int l)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int l)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Full float SAT for queued entries (one per lane); entry = p | q<<4 | r<<8 |
@@ -1091,13 +1175,13 @@ template <bool kM>
 __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restrict__ tris,
                                                           const int32_t* __restrict__ bidx, int64_t B,
                                                           const uint32_t* __restrict__ boff,
-                                                          const uint32_t* __restrict__ list, GridDev g0,
+                                                          const uint32_t* __restrict__ list,
+                                                          const uint64_t* __restrict__ pmask,
+                                                          const double4* __restrict__ slab2, GridDev g0,
                                                           Meshes M, uint64_t* __restrict__ sat2)
 {
     __shared__ unsigned long long s_acc[L2_WAVES][64];
     __shared__ float s_tv[L2_WAVES][64 * 9];
-    __shared__ uint32_t s_mk[L2_WAVES][64 * 3];
-    __shared__ float s_sl[L2_WAVES][64 * 4];
     __shared__ uint32_t s_q[L2_WAVES][L2_QCAP];
     const int wv = threadIdx.x >> 6;
     const int lane = __lane_id();
@@ -1122,11 +1206,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
     const float cxl = lane < n2 ? g.c2x[(int64_t)i1 * n2 + lane] : 0.0f;
     const float cyl = lane < n2 ? g.c2y[(int64_t)j1 * n2 + lane] : 0.0f;
     const float czl = lane < n2 ? g.c2z[(int64_t)k1 * n2 + lane] : 0.0f;
-    const double clo[3] = {lane_f(cxl, 0), lane_f(cyl, 0), lane_f(czl, 0)};
-    const double chi[3] = {lane_f(cxl, n2 - 1), lane_f(cyl, n2 - 1), lane_f(czl, n2 - 1)};
     float* stv = s_tv[wv];
-    uint32_t* smk = s_mk[wv];
-    float* ssl = s_sl[wv];
     uint32_t* sq = s_q[wv];
     uint32_t qh = 0, qt = 0;   // ring head/tail (uniform)
     const uint32_t e0 = boff[b], e1 = boff[b + 1];
@@ -1140,44 +1220,23 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
             qh += n;
         }
         __builtin_amdgcn_wave_barrier();
-        // lane-parallel staging: vertices, the passing sub-indices of the box-axis
-        // predicates, and a float plane slab outside of which the SAT fails
+        // staging: vertices to LDS; masks and slab stay in the lane's registers
         const uint32_t i = base + lane;
+        uint64_t mk = 0;
+        double4 sl = make_double4(0.0, 0.0, 0.0, 0.0);
         if (i < e1) {
             const uint64_t t = list[i];
-            float tv[9];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) {
-                tv[c] = tris[t * 9 + c];
-                stv[lane * 9 + c] = tv[c];
-            }
-            uint32_t mx = 0, my = 0, mz = 0;
-            for (int p = 0; p < n2; ++p) {
-                mx |= (uint32_t)sat_axis_ok(tv[0], tv[3], tv[6], lane_f(cxl, p), h2x) << p;
-                my |= (uint32_t)sat_axis_ok(tv[1], tv[4], tv[7], lane_f(cyl, p), h2y) << p;
-                mz |= (uint32_t)sat_axis_ok(tv[2], tv[5], tv[8], lane_f(czl, p), h2z) << p;
-            }
-            smk[lane * 3 + 0] = mx;
-            smk[lane * 3 + 1] = my;
-            smk[lane * 3 + 2] = mz;
-            const PlaneSlab sl = plane_slab(tv, h2x, h2y, h2z, clo, chi);
-            // float evaluation of N.(c - t0): |error| <= 6U sum |N_q| max|c_q - t0_q|
-            double sa = 0.0;
-            sa += fabs(sl.nx) * fmax(fabs(clo[0] - tv[0]), fabs(chi[0] - tv[0]));
-            sa += fabs(sl.ny) * fmax(fabs(clo[1] - tv[1]), fabs(chi[1] - tv[1]));
-            sa += fabs(sl.nz) * fmax(fabs(clo[2] - tv[2]), fabs(chi[2] - tv[2]));
-            const double limd = sl.lim + 8.0 * 5.9604644775390625e-08 * sa;
-            float limf = (float)limd;
-            if ((double)limf < limd) limf = nextafterf(limf, INFINITY);
-            ssl[lane * 4 + 0] = (float)sl.nx;
-            ssl[lane * 4 + 1] = (float)sl.ny;
-            ssl[lane * 4 + 2] = (float)sl.nz;
-            ssl[lane * 4 + 3] = limf;
+            for (int c = 0; c < 9; ++c) stv[lane * 9 + c] = tris[t * 9 + c];
+            mk = pmask[i];
+            sl = slab2[t];
         }
         __builtin_amdgcn_wave_barrier();
         const int nv = (int)(e1 - base < 64 ? e1 - base : 64);
         for (int j = 0; j < nv; ++j) {
-            const uint32_t mx = smk[j * 3 + 0], my = smk[j * 3 + 1], mz = smk[j * 3 + 2];
+            const uint64_t mkj = lane_u64(mk, j);
+            const uint32_t mx = (uint32_t)(mkj & 0xffffu), my = (uint32_t)((mkj >> 16) & 0xffffu),
+                           mz = (uint32_t)((mkj >> 32) & 0xffffu);
             if (!mx || !my || !mz) continue;
             const int pa = __ffs(mx) - 1, pb = 31 - __clz(mx);
             const int qa = __ffs(my) - 1, qb = 31 - __clz(my);
@@ -1186,34 +1245,46 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
             // contiguous passing sets (centers are monotone in the index) mean the
             // box-axis predicates hold on the whole sub-box
             const uint32_t boxbit = (__popc(mx) == dp && __popc(my) == dq && __popc(mz) == dr) ? (1u << 18) : 0u;
-            const float nX = ssl[j * 4 + 0], nY = ssl[j * 4 + 1], nZ = ssl[j * 4 + 2], limf = ssl[j * 4 + 3];
-            const float t0x = stv[j * 9 + 0], t0y = stv[j * 9 + 1], t0z = stv[j * 9 + 2];
-            const int V = dp * dq * dr;
-            const float inv_dp = 1.0f / (float)dp, inv_dq = 1.0f / (float)dq;
-            for (int l0 = 0; l0 < V; l0 += 64) {
+            const double Nx = lane_d(sl.x, j), Ny = lane_d(sl.y, j), Nz = lane_d(sl.z, j), lim = lane_d(sl.w, j);
+            const double t0x = stv[j * 9 + 0], t0y = stv[j * 9 + 1], t0z = stv[j * 9 + 2];
+            const int P = dp * dq;
+            const float inv_dp = 1.0f / (float)dp;
+            for (int l0 = 0; l0 < P; l0 += 64) {
                 const int l = l0 + lane;
-                // exact small-integer division: (l + 0.5)/d is >= 1/32 from an integer
+                // exact small-integer division: (l + 0.5)/dp is >= 1/32 from an integer
                 const int rem = (int)(((float)l + 0.5f) * inv_dp);
                 const int p = pa + l - rem * dp;
-                const int r3 = (int)(((float)rem + 0.5f) * inv_dq);
-                const int q = qa + rem - r3 * dq;
-                const int r = ra + r3;
-                const float ccx = __shfl(cxl, p & 63), ccy = __shfl(cyl, q & 63), ccz = __shfl(czl, r & 63);
-                const float d = nX * (ccx - t0x) + nY * (ccy - t0y) + nZ * (ccz - t0z);
-                const int s = (r * n2 + q) * n2 + p;
-                const bool keep = l < V && fabsf(d) <= limf && !((acc[(s & 4095) >> 6] >> (s & 63)) & 1ull);
-                const uint64_t m = __ballot(keep);
-                if (keep) {
-                    const uint32_t at = qt + (uint32_t)__popcll(m & lanes_below_mask());
-                    sq[at % L2_QCAP] = (uint32_t)p | ((uint32_t)q << 4) | ((uint32_t)r << 8) | ((uint32_t)j << 12) | boxbit;
+                const int q = qa + rem;
+                const float ccx = __shfl(cxl, p & 63), ccy = __shfl(cyl, q & 63);
+                uint32_t keep = 0;
+                if (l < P && ((mx >> p) & (my >> q) & 1u)) {
+                    const double A = Nx * ((double)ccx - t0x) + Ny * ((double)ccy - t0y);
+                    for (int r = ra; r <= rb; ++r) {
+                        const double sd = A + Nz * ((double)lane_f(czl, r) - t0z);
+                        keep |= (uint32_t)(fabs(sd) <= lim && ((mz >> r) & 1u)) << r;
+                    }
                 }
-                qt += (uint32_t)__popcll(m);
-                __builtin_amdgcn_wave_barrier();
-                if (qt - qh >= 64) {   // a full wave of work
-                    const uint32_t ent = sq[(qh + lane) % L2_QCAP];
-                    l2_sat_run(ent, true, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
-                    qh += 64;
+                // queue the kept sub-voxels not yet set, one r per lane per round
+                while (__ballot(keep != 0)) {
+                    const int r = keep ? __ffs(keep) - 1 : 0;
+                    const bool has = keep != 0;
+                    keep &= keep - 1;
+                    const int sv = (r * n2 + q) * n2 + p;
+                    const bool push = has && !((acc[(sv & 4095) >> 6] >> (sv & 63)) & 1ull);
+                    const uint64_t m = __ballot(push);
+                    if (push) {
+                        const uint32_t at = qt + (uint32_t)__popcll(m & lanes_below_mask());
+                        sq[at % L2_QCAP] = (uint32_t)p | ((uint32_t)q << 4) | ((uint32_t)r << 8) |
+                                           ((uint32_t)j << 12) | boxbit;
+                    }
+                    qt += (uint32_t)__popcll(m);
                     __builtin_amdgcn_wave_barrier();
+                    if (qt - qh >= 64) {   // a full wave of work
+                        const uint32_t ent = sq[(qh + lane) % L2_QCAP];
+                        l2_sat_run(ent, true, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
+                        qh += 64;
+                        __builtin_amdgcn_wave_barrier();
+                    }
                 }
             }
         }
@@ -1230,14 +1301,17 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
 }
 
 void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
-                   const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s)
+                   const uint64_t* pmask, const double4* slab2, const GridDev& g0, const Meshes& M, uint64_t* sat2,
+                   hipStream_t s)
 {
     if (B <= 0) return;
     const int64_t blocks = (B + L2_WAVES - 1) / L2_WAVES;
     if (M.nm > 1)
-        k_l2_sat<true><<<(unsigned)blocks, 64 * L2_WAVES, 0, s>>>(tris, bidx, B, boff, list, g0, M, sat2);
+        k_l2_sat<true><<<(unsigned)blocks, 64 * L2_WAVES, 0, s>>>(tris, bidx, B, boff, list, pmask, slab2, g0, M,
+                                                                  sat2);
     else
-        k_l2_sat<false><<<(unsigned)blocks, 64 * L2_WAVES, 0, s>>>(tris, bidx, B, boff, list, g0, M, sat2);
+        k_l2_sat<false><<<(unsigned)blocks, 64 * L2_WAVES, 0, s>>>(tris, bidx, B, boff, list, pmask, slab2, g0, M,
+                                                                   sat2);
 }
 
 // ---------------------------------------------------------------------------

@@ -160,6 +160,8 @@ struct Job {
     MeshCtr* mctr = nullptr;
     // per triangle
     int4 *box0 = nullptr, *box1 = nullptr, *rng = nullptr;
+    double4* slab1 = nullptr;   // plane slab of triangles spanning several layers
+    double4* slab2 = nullptr;   // plane slab over the Level2 centers of the index AABB
     uint32_t *sat_items = nullptr, *ray_items = nullptr;
     uint64_t *sat_off = nullptr, *ray_off = nullptr, *scan64 = nullptr;
     uint32_t* scan32 = nullptr;
@@ -184,6 +186,7 @@ struct Job {
     int32_t* bidx = nullptr;
     int64_t B = 0, W2 = 0;
     uint32_t *pb = nullptr, *cntb = nullptr, *boff = nullptr, *list = nullptr;
+    uint64_t* pmask = nullptr;   // Level2 box-axis masks per bin entry
     uint32_t *ccnt = nullptr, *coff = nullptr;
     float4* crec = nullptr;   // 3 float4 per column entry
     uint32_t *hcnt = nullptr, *hoff = nullptr;
@@ -500,6 +503,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
 
     const size_t Tn = (size_t)std::max<int64_t>(T, 1);
     if ((r = alloc(J, &J->box0, Tn)) || (r = alloc(J, &J->box1, Tn)) || (r = alloc(J, &J->rng, Tn)) ||
+        (r = alloc(J, &J->slab1, Tn)) || (r = alloc(J, &J->slab2, Tn)) ||
         (r = alloc(J, &J->sat_items, Tn)) || (r = alloc(J, &J->ray_items, Tn)) ||
         (r = alloc(J, &J->sat_off, Tn + 1)) || (r = alloc(J, &J->ray_off, Tn + 1)) ||
         (r = alloc(J, &J->ctr, 1)) || (r = alloc(J, &J->bnd, (size_t)W)) || (r = alloc(J, &J->ins, (size_t)W)) ||
@@ -525,7 +529,8 @@ int stage_setup(Job* J)
     HIP_TRY(hipEventRecord(J->ev[0], s));
     launch_fill_u64(reinterpret_cast<uint64_t*>(J->mctr), 0, (int64_t)(sizeof(MeshCtr) / 8) * J->nm, s);
     if (J->nm > 1) launch_mesh_ids(J->d_off, J->nm, J->T, J->tri_mesh, s);
-    launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items, s);
+    launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items,
+                     J->slab1, J->slab2, s);
     TRY(ensure_scan_tmp(J, J->T));
     launch_scan_u32_u64(J->sat_items, J->sat_off, J->T, J->scan64, s);
     launch_scan_u32_u64(J->ray_items, J->ray_off, J->T, J->scan64, s);
@@ -544,7 +549,7 @@ int launch_sat_half(Job* J)
     launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->pairs), 0, 2, s);   // pairs, colents
     Append64 pa{&J->ctr->pairs, J->pair_key, J->pair_tri, J->pair_cap};
     Append64 co{&J->ctr->colents, J->col_key, J->col_z, J->col_cap};
-    launch_sat_expand(J->tris, J->T, J->box0, J->box1, J->sat_off, J->g[0], J->view(), J->bnd, pa, co,
+    launch_sat_expand(J->tris, J->T, J->box0, J->box1, J->slab1, J->sat_off, J->g[0], J->view(), J->bnd, pa, co,
                       expand_blocks(c), s);
     launch_popc_words(J->bnd, J->W, J->wcnt, s);
     TRY(ensure_scan_tmp(J, J->W));
@@ -683,15 +688,18 @@ int launch_compact(Job* J)
     release(J, J->cntb);
     release(J, J->boff);
     release(J, J->list);
+    release(J, J->pmask);
     TRY(alloc(J, &J->pb, (size_t)std::max<uint64_t>(J->n_pairs, 1)));
     TRY(alloc(J, &J->cntb, Bn));
     TRY(alloc(J, &J->boff, (size_t)J->B + 1));
     TRY(alloc(J, &J->list, (size_t)std::max<uint64_t>(J->n_pairs, 1)));
+    if (J->n2max > 0) TRY(alloc(J, &J->pmask, (size_t)std::max<uint64_t>(J->n_pairs, 1)));
     launch_fill_u32(J->cntb, 0, J->B, s);
     launch_pair_rank(J->pair_key, J->n_pairs, J->bnd, J->wprefix, J->pb, J->cntb, s);
     TRY(ensure_scan_tmp(J, J->B));
     launch_scan_u32_u32(J->cntb, J->boff, J->B, J->scan32, J->nm > 1 ? nullptr : &J->mctr[0].max_per_voxel, s);
-    launch_pair_scatter(J->pair_tri, J->pb, J->n_pairs, J->boff, J->cntb, J->list, s);
+    launch_pair_scatter(J->pair_tri, J->pb, J->n_pairs, J->boff, J->cntb, J->list, J->tris, J->bidx, J->g[0],
+                        J->view(), J->pmask, s);
     if (J->nm > 1) launch_mesh_bins(J->boff, J->B, J->view(), s);
     release(J, J->ccnt);
     release(J, J->coff);
@@ -731,7 +739,7 @@ int run_l2_sat(Job* J)
     if (J->n2max <= 0) return GPV_OK;
     hipStream_t s = J->ctx->stream;
     HIP_TRY(hipEventRecord(J->ev[6], s));
-    launch_l2_sat(J->tris, J->bidx, J->B, J->boff, J->list, J->g[0], J->view(), J->sat2, s);
+    launch_l2_sat(J->tris, J->bidx, J->B, J->boff, J->list, J->pmask, J->slab2, J->g[0], J->view(), J->sat2, s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(J->ev[7], s));
     J->l2s_done = true;
@@ -812,7 +820,8 @@ int job_finish(Job* J)
         I.ms_normals = ms_nrm;
     }
     // only the result planes stay resident
-    release(J, J->box0); release(J, J->box1); release(J, J->rng);
+    release(J, J->box0); release(J, J->box1); release(J, J->rng); release(J, J->slab1); release(J, J->slab2);
+    release(J, J->pmask);
     release(J, J->sat_items); release(J, J->ray_items); release(J, J->sat_off); release(J, J->ray_off);
     release(J, J->scan64); J->scan32 = nullptr; J->scan_cap = 0;
     release(J, J->pair_key); release(J, J->pair_tri); release(J, J->col_key); release(J, J->col_z);

@@ -88,4 +88,43 @@ __host__ __device__ inline double slab_dist(const PlaneSlab& s, double cx, doubl
     return s.nx * (cx - s.t0x) + s.ny * (cy - s.t0y) + s.nz * (cz - s.t0z);
 }
 
+// Layers [za, zb) of the column with center (cx, cy) that a slab (normal n,
+// bound lim, anchored at vertex tv[0..2]) can keep: |n.(c - t0)| <= lim is
+// linear in the center's z, so the kept centers form an interval, located on
+// the monotone center table c1z[z0, z1) itself and widened by one layer on
+// each side.  Every kept layer still runs the full SAT; layers outside are
+// rejected by it.
+__host__ __device__ inline void slab_layers(double nx, double ny, double nz, double lim, float cx, float cy,
+                                            const float* tv, const float* c1z, int z0, int z1, int& za, int& zb)
+{
+    za = z0;
+    zb = z1;
+    const double A = nx * ((double)cx - (double)tv[0]) + ny * ((double)cy - (double)tv[1]);
+    if (nz == 0.0) {
+        if (fabs(A) > lim) zb = za;
+        return;
+    }
+    double lo = (-lim - A) / nz, hi = (lim - A) / nz;
+    if (lo > hi) {
+        const double t = lo;
+        lo = hi;
+        hi = t;
+    }
+    lo += (double)tv[2];
+    hi += (double)tv[2];
+    if (!(lo <= hi)) return;   // NaN: keep every layer
+    int a = z0, b = z1;        // first layer with center >= lo
+    while (b > a) {
+        const int m = (a + b) >> 1;
+        if ((double)c1z[m] < lo) a = m + 1; else b = m;
+    }
+    int c = a, d = z1;         // first layer with center > hi
+    while (d > c) {
+        const int m = (c + d) >> 1;
+        if ((double)c1z[m] <= hi) c = m + 1; else d = m;
+    }
+    za = a - 1 > z0 ? a - 1 : z0;
+    zb = c + 1 < z1 ? c + 1 : z1;
+}
+
 }  // namespace gpv

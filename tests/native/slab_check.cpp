@@ -13,3 +13,27 @@ extern "C" void slab_keep_batch(int64_t n, const float* tv, const float* c, cons
         keep[i] = (uint8_t)(fabs(d) <= s.lim);
     }
 }
+
+// Level1 layer intervals: triangle i with index box lo[3i..], hi[3i..]
+// (inclusive) writes, for every column of its box in (q, p) order from
+// off[i], the kept layer interval [za, zb) of gpv::slab_layers.
+extern "C" void slab_layers_batch(int64_t n, const float* tv, const int32_t* lo, const int32_t* hi,
+                                  const float* c1x, const float* c1y, const float* c1z, const float* h,
+                                  const int64_t* off, int32_t* za, int32_t* zb)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t* l = lo + 3 * i;
+        const int32_t* u = hi + 3 * i;
+        const double clo[3] = {c1x[l[0]], c1y[l[1]], c1z[l[2]]};
+        const double chi[3] = {c1x[u[0]], c1y[u[1]], c1z[u[2]]};
+        const gpv::PlaneSlab s = gpv::plane_slab(tv + 9 * i, h[0], h[1], h[2], clo, chi);
+        int64_t k = off[i];
+        for (int q = l[1]; q <= u[1]; ++q)
+            for (int p = l[0]; p <= u[0]; ++p, ++k) {
+                int a, b;
+                gpv::slab_layers(s.nx, s.ny, s.nz, s.lim, c1x[p], c1y[q], tv + 9 * i, c1z, l[2], u[2] + 1, a, b);
+                za[k] = a;
+                zb[k] = b;
+            }
+    }
+}

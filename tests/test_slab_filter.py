@@ -87,3 +87,98 @@ def test_slab_prunes_far_boxes(slab):
     h = np.full((n, 3), 1 / 512, np.float32)
     c = (t[:, 0] + N * 5 / 256).astype(np.float32)
     assert slab(tv, c, h).sum() == 0
+
+
+def _index_box(tv, bmin, bmax, n):
+    """Index AABB of src/Object.cpp:2272-2318 in float32: int((v-min)/(max-min)*n),
+    pulled back at the max plane, clipped to the grid."""
+    v = tv.reshape(-1, 3, 3).astype(np.float32)
+    mn = np.asarray(bmin, np.float32)
+    mx = np.asarray(bmax, np.float32)
+    r = (v - mn) / (mx - mn)
+    s = np.clip(r * np.asarray(n, np.float32), -1, np.asarray(n, np.float32) + 1)
+    i = np.trunc(s).astype(np.int64)
+    i = np.where((i == np.asarray(n)) & (v == mx), i - 1, i)
+    lo = np.maximum(i.min(1), 0)
+    hi = np.minimum(i.max(1), np.asarray(n) - 1)
+    return lo.astype(np.int32), hi.astype(np.int32)
+
+
+def _big_triangles(rng, bmin, bmax, g, n_each):
+    ext = np.asarray(bmax, np.float64) - bmin
+    out = []
+    # random, spanning up to ~12 voxels each way
+    c = bmin + rng.uniform(0.2, 0.8, (n_each, 3)) * ext
+    out.append(c[:, None, :] + rng.uniform(-12, 12, (n_each, 3, 3)) * np.asarray(g))
+    # vertices on Level1 lattice planes / centers
+    k = rng.integers(8, 56, (n_each, 3, 3))
+    lat = bmin + k * np.asarray(g) + rng.choice([0.0, 0.5], (n_each, 3, 3)) * np.asarray(g)
+    out.append(lat)
+    # planes parallel to a grid plane (one shared coordinate), incl. exact layer planes
+    t = c[:, None, :] + rng.uniform(-12, 12, (n_each, 3, 3)) * np.asarray(g)
+    ax = rng.integers(0, 3, n_each)
+    val = bmin[ax] + rng.integers(8, 56, n_each) * np.asarray(g)[ax] * rng.choice([1.0, 1.0 + 1e-7], n_each)
+    t[np.arange(n_each), :, ax] = val[:, None]
+    out.append(t)
+    # long slivers
+    a = c + rng.uniform(-10, 10, (n_each, 3)) * np.asarray(g)
+    d = rng.normal(0, 1, (n_each, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    b = a + d * 20 * np.asarray(g).max()
+    w = rng.normal(0, 1, (n_each, 3)) * 1e-3 * np.asarray(g).max()
+    out.append(np.stack([a, b, (a + b) / 2 + w], 1))
+    tv = np.concatenate(out).astype(np.float32).reshape(-1, 9)
+    lo = np.asarray(bmin, np.float32)
+    hi = np.asarray(bmax, np.float32)
+    return np.clip(tv.reshape(-1, 3, 3), lo, hi).reshape(-1, 9)
+
+
+def test_level1_layers_never_drop_a_sat_hit(orc):
+    """gpv::slab_layers (the Level1 sweep's per-column layer interval for
+    triangles spanning several layers): every layer where the reference SAT
+    hits lies inside the interval; and the interval is useful (prunes most
+    of the index-AABB layers of large triangles)."""
+    so = os.path.join(NATIVE, "libslabcheck.so")
+    if not os.path.exists(so):
+        subprocess.run(["make", "-s", "-C", NATIVE], check=True)
+    L = ctypes.CDLL(so)
+    f32 = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+    L.slab_layers_batch.argtypes = [ctypes.c_int64, f32, i32, i32, f32, f32, f32, f32, i64, i32, i32]
+    use_ref = orc.ref() is not None
+    rng = np.random.default_rng(2024)
+    bmin = np.array([-0.0123, 0.01, 0.3], np.float32)
+    bmax = np.array([0.98, 1.02, 1.25], np.float32)
+    g = orc.grid_setup(bmin, bmax, float(np.max(bmax - bmin)), 64, 4)
+    n = np.array(list(g.n))
+    tabs = orc.tables(g)
+    c1 = [np.ascontiguousarray(x, np.float32) for x in tabs["c1"]]
+    h = np.ascontiguousarray(g.ext1, np.float32)
+    tv = _big_triangles(rng, bmin.astype(np.float64), bmax.astype(np.float64), np.array(list(g.g)), 300)
+    lo, hi = _index_box(tv, bmin, bmax, n)
+    keep = np.all(hi - lo + 1 >= 1, axis=1) & (hi[:, 2] - lo[:, 2] + 1 >= 3)
+    tv, lo, hi = tv[keep], np.ascontiguousarray(lo[keep]), np.ascontiguousarray(hi[keep])
+    cols = (hi[:, 0] - lo[:, 0] + 1) * (hi[:, 1] - lo[:, 1] + 1)
+    off = np.concatenate([[0], np.cumsum(cols)]).astype(np.int64)
+    za = np.zeros(off[-1], np.int32)
+    zb = np.zeros(off[-1], np.int32)
+    L.slab_layers_batch(len(tv), np.ascontiguousarray(tv), lo.ravel(), hi.ravel(), c1[0], c1[1], c1[2], h, off, za, zb)
+    kept = total = hits = 0
+    for i in range(len(tv)):
+        p = np.arange(lo[i, 0], hi[i, 0] + 1)
+        q = np.arange(lo[i, 1], hi[i, 1] + 1)
+        z = np.arange(lo[i, 2], hi[i, 2] + 1)
+        Q, P, Z = np.meshgrid(q, p, z, indexing="ij")          # column order (q, p), then z
+        ctr = np.stack([c1[0][P], c1[1][Q], c1[2][Z]], -1).reshape(-1, 3)
+        res = orc.tri_box(ctr, np.tile(h, (len(ctr), 1)), np.tile(tv[i], (len(ctr), 1)),
+                          use_ref=use_ref).reshape(len(q), len(p), len(z)).astype(bool)
+        a = za[off[i]:off[i + 1]].reshape(len(q), len(p), 1)
+        b = zb[off[i]:off[i + 1]].reshape(len(q), len(p), 1)
+        inside = (Z >= a) & (Z < b)
+        assert not np.any(res & ~inside), f"triangle {i}: SAT hit outside its layer interval"
+        hits += int(res.sum())
+        kept += int(inside.sum())
+        total += inside.size
+    assert hits > 50000
+    assert kept < 0.5 * total, f"pruning too weak: kept {kept} of {total}"
