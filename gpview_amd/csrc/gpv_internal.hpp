@@ -37,7 +37,7 @@ struct MeshOff {
     uint32_t col0;
     uint32_t tri0;
     uint32_t b0;
-    uint32_t pad;
+    uint32_t ptask0;   // first Level1 parity task (16 layers x 64 columns of one row)
 };
 
 // Per-mesh results accumulated on the device.
@@ -115,7 +115,10 @@ void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n, con
 void launch_scatter_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n,
                          const uint32_t* off, uint32_t* cnt, float4* rec, hipStream_t s);
 
-void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0, const Meshes& M, int64_t W,
+// Level1 parity tasks: L1P_Z consecutive layers of one 64-column row strip
+constexpr int L1P_Z = 16;
+inline int64_t l1_parity_tasks(const GridDev& g) { return (int64_t)((g.nz + L1P_Z - 1) / L1P_Z) * g.ny * g.nxw; }
+void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0, const Meshes& M, int64_t NT,
                       uint64_t* ins, hipStream_t s);
 
 void launch_popc_words(const uint64_t* bits, int64_t W, uint32_t* cnt, hipStream_t s);

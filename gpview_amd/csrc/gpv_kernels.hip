@@ -872,51 +872,78 @@ void launch_scatter_cols(const float* tris, const uint64_t* key, const uint32_t*
 }
 
 // ---------------------------------------------------------------------------
-// Level1 parity: one wave per inside-bitplane word (64 x-adjacent voxels of
-// one row); each lane XORs t > 1e-6 over its column's accepted triangles.
+// Level1 parity: one wave per task = L1P_Z consecutive layers of a 64-column
+// strip of one row; each lane loads its column's accepted (triangle, column)
+// records once and XORs t > 1e-6 over them at each of the task's layers.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_mesh_of_task(const MeshOff* __restrict__ off, int nm, uint64_t task)
+{
+    const int lane = __lane_id();
+    int lo = 0, hi = nm;   // off[lo].ptask0 <= task; the answer lies in [lo, hi)
+    while (hi - lo > 1) {
+        const int step = (hi - lo + 63) / 64;
+        const int idx = lo + lane * step;
+        const bool le = idx < hi && (uint64_t)off[idx].ptask0 <= task;
+        const uint64_t m = __ballot(le);
+        const int L = 63 - __clzll((long long)m);
+        const int nlo = lo + L * step;
+        hi = nlo + step < hi ? nlo + step : hi;
+        lo = nlo;
+    }
+    return uniform_i(lo);
+}
+
 template <bool kM>
 __global__ __launch_bounds__(256) void k_l1_parity(const float4* __restrict__ rec,
                                                    const uint32_t* __restrict__ hoff, GridDev g0, Meshes M,
-                                                   uint64_t* __restrict__ ins, int64_t W)
+                                                   uint64_t* __restrict__ ins, int64_t NT)
 {
-    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (w >= W) return;
+    const int64_t task = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (task >= NT) return;
     const int lane = __lane_id();
-    const int mesh = kM ? wave_mesh_of_word(M.off, M.nm, (uint64_t)w) : 0;
+    const int mesh = kM ? wave_mesh_of_task(M.off, M.nm, (uint64_t)task) : 0;
     const GridDev& g = grid_of<kM>(g0, M, mesh);
-    const int64_t wl = w - (int64_t)(kM ? M.off[mesh].word0 : 0);
+    const int64_t tl = task - (int64_t)(kM ? M.off[mesh].ptask0 : 0);
+    const uint64_t word0 = kM ? M.off[mesh].word0 : 0;
     const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
-    const int xw = (int)(wl % g.nxw);
-    const int64_t row = wl / g.nxw;
-    const int j = (int)(row % g.ny);
-    const int k = (int)(row / g.ny);
+    const int nxw = g.nxw, ny = g.ny, nx = g.nx, nz = g.nz;
+    const int xw = (int)(tl % nxw);
+    const int64_t rowt = tl / nxw;
+    const int j = (int)(rowt % ny);
+    const int z0 = (int)(rowt / ny) * L1P_Z;
+    const int nzc = nz - z0 < L1P_Z ? nz - z0 : L1P_Z;
+    float oz[L1P_Z];
+#pragma unroll
+    for (int k = 0; k < L1P_Z; ++k) oz[k] = k < nzc ? g.r1z[z0 + k] : 0.0f;
     const int x = xw * 64 + lane;
-    int par = 0;
-    if (x < g.nx) {
-        const float oz = g.r1z[k];
-        const uint32_t c = col0 + (uint32_t)j * g.nx + x;
+    uint32_t par = 0;   // bit k: parity at layer z0 + k
+    if (x < nx) {
+        const uint32_t c = col0 + (uint32_t)j * nx + x;
         const uint32_t h1 = hoff[c + 1];
         for (uint32_t h = hoff[c]; h < h1; ++h) {
             const float4 a = rec[3 * (uint64_t)h + 0];
             const float4 b = rec[3 * (uint64_t)h + 1];
             const float4 d = rec[3 * (uint64_t)h + 2];
-            par ^= (int)mt_t_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, d.x, d.y, oz);
+#pragma unroll
+            for (int k = 0; k < L1P_Z; ++k)
+                if (k < nzc) par ^= (uint32_t)mt_t_hit(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, d.x, d.y, oz[k]) << k;
         }
     }
-    const uint64_t m = __ballot(par);
-    if (lane == 0) ins[w] = m;
+    for (int k = 0; k < nzc; ++k) {
+        const uint64_t m = __ballot((par >> k) & 1u);
+        if (lane == 0) ins[word0 + ((uint64_t)(z0 + k) * ny + j) * nxw + xw] = m;
+    }
 }
 
-void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0, const Meshes& M, int64_t W,
+void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0, const Meshes& M, int64_t NT,
                       uint64_t* ins, hipStream_t s)
 {
-    if (W <= 0) return;
-    const int64_t blocks = (W * 64 + 255) / 256;
+    if (NT <= 0) return;
+    const int64_t blocks = (NT * 64 + 255) / 256;
     if (M.nm > 1)
-        k_l1_parity<true><<<(unsigned)blocks, 256, 0, s>>>(rec, hoff, g0, M, ins, W);
+        k_l1_parity<true><<<(unsigned)blocks, 256, 0, s>>>(rec, hoff, g0, M, ins, NT);
     else
-        k_l1_parity<false><<<(unsigned)blocks, 256, 0, s>>>(rec, hoff, g0, M, ins, W);
+        k_l1_parity<false><<<(unsigned)blocks, 256, 0, s>>>(rec, hoff, g0, M, ins, NT);
 }
 
 // ---------------------------------------------------------------------------
@@ -937,23 +964,34 @@ void launch_popc_words(const uint64_t* bits, int64_t W, uint32_t* cnt, hipStream
 }
 
 // b0 (first boundary rank), l2w0 (first Level2 word) and l2n0 (first Level2
-// normal) of every mesh, from the batch-wide word prefix; one thread walks the
-// nm meshes in order.
-__global__ void k_mesh_bounds(const uint32_t* __restrict__ wprefix, Meshes M)
+// normal) of every mesh, from the batch-wide word prefix.
+__global__ __launch_bounds__(64) void k_mesh_bounds(const uint32_t* __restrict__ wprefix, Meshes M)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    // one wave: 64 meshes per round, exclusive scans of the Level2 word and
+    // normal counts carried across rounds
+    const int lane = __lane_id();
     uint64_t l2 = 0, l2n = 0;
-    for (int m = 0; m <= M.nm; ++m) {
-        const uint32_t b0 = wprefix[M.off[m].word0];
-        M.off[m].b0 = b0;
-        M.off[m].l2w0 = l2;
-        M.off[m].l2n0 = l2n;
-        if (m < M.nm) {
-            const uint64_t nb = wprefix[M.off[m + 1].word0] - b0;
-            const GridDev& g = M.grids[m];
-            l2 += nb * (uint64_t)g.w2;
-            if (g.normals) l2n += nb * (uint64_t)g.n2 * g.n2 * g.n2;
+    for (int base = 0; base <= M.nm; base += 64) {
+        const int m = base + lane;
+        uint32_t b0 = 0;
+        uint64_t w = 0, nrm = 0;
+        if (m <= M.nm) {
+            b0 = wprefix[M.off[m].word0];
+            if (m < M.nm) {
+                const uint64_t nb = wprefix[M.off[m + 1].word0] - b0;
+                const GridDev& g = M.grids[m];
+                w = nb * (uint64_t)g.w2;
+                nrm = g.normals ? nb * (uint64_t)g.n2 * g.n2 * g.n2 : 0;
+            }
         }
+        const uint64_t wi = wave_incl_scan<uint64_t>(w), ni = wave_incl_scan<uint64_t>(nrm);
+        if (m <= M.nm) {
+            M.off[m].b0 = b0;
+            M.off[m].l2w0 = l2 + wi - w;
+            M.off[m].l2n0 = l2n + ni - nrm;
+        }
+        l2 += shfl64(wi, 63);
+        l2n += shfl64(ni, 63);
     }
 }
 
@@ -1390,9 +1428,8 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
         ox[mm] = (mm < nm && sc < nsc) ? rx[sc % n2] : 0.0f;
         oy[mm] = (mm < nm && sc < nsc) ? ry[sc / n2] : 0.0f;
     }
-    float ozv[16];
-#pragma unroll
-    for (int rr = 0; rr < 16; ++rr) ozv[rr] = rr < n2 ? rz[rr] : 0.0f;
+    // layer origins of the sub-voxels: entry r in lane r
+    const float ozl = lane < n2 ? rz[lane] : 0.0f;
     uint64_t bits = 0;   // bit (mm*n2 + r) for sub-column lane + 64*mm
     const uint32_t e0 = coff[col0 + ij], e1 = coff[col0 + ij + 1];
     for (uint32_t base = e0; base < e1; base += 64) {
@@ -1425,10 +1462,8 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
                 const int j = __ffsll((unsigned long long)m) - 1;
                 const ColRec r = cr[j];
                 const float tx = ox[mm] - r.v1x, ty = oy[mm] - r.v1y;
-#pragma unroll
-                for (int rr = 0; rr < 16; ++rr)
-                    if (rr < n2 &&
-                        mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z, ozv[rr]))
+                for (int rr = 0; rr < n2; ++rr)
+                    if (mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z, lane_f(ozl, rr)))
                         bits ^= 1ull << (mm * n2 + rr);
             }
         }

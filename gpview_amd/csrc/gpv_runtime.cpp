@@ -434,7 +434,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
     J->off.assign((size_t)nm + 1, MeshOff{});
     std::vector<size_t> tab_at((size_t)nm * 12);
     uint64_t word0 = 0;
-    uint32_t col0 = 0, tri0 = 0;
+    uint32_t col0 = 0, tri0 = 0, ptask0 = 0;
     for (int m = 0; m < nm; ++m) {
         GridDev& g = J->g[m];
         g = grid_dev(J->info[m]);
@@ -445,6 +445,8 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         J->off[m].word0 = word0;
         J->off[m].col0 = col0;
         J->off[m].tri0 = tri0;
+        J->off[m].ptask0 = ptask0;
+        ptask0 += (uint32_t)l1_parity_tasks(g);
         word0 += (uint64_t)g.nz * g.ny * g.nxw;
         col0 += (uint32_t)(g.nx * g.ny);
         tri0 += (uint32_t)d[m].ntris;
@@ -457,6 +459,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
     J->off[nm].word0 = word0;
     J->off[nm].col0 = col0;
     J->off[nm].tri0 = tri0;
+    J->off[nm].ptask0 = ptask0;
     int r;
     if ((r = alloc(J, &J->tables, J->h_tab.size())) != GPV_OK) return bail(r);
     if (hipMemcpyAsync(J->tables, J->h_tab.data(), J->h_tab.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -667,7 +670,7 @@ int launch_parity(Job* J)
     TRY(ensure_scan_tmp(J, J->ncol));
     launch_scan_u32_u32(J->hcnt, J->hoff, J->ncol, J->scan32, nullptr, s);
     launch_scatter_hits(J->tris, J->hit_key, J->n_hits, J->g[0], J->view(), J->hoff, J->hcnt, J->hrec, s);
-    launch_l1_parity(J->hrec, J->hoff, J->g[0], J->view(), J->W, J->ins, s);
+    launch_l1_parity(J->hrec, J->hoff, J->g[0], J->view(), (int64_t)J->off[J->nm].ptask0, J->ins, s);
     HIP_TRY(hipGetLastError());
     J->inout_done = true;
     return GPV_OK;
