@@ -91,7 +91,7 @@ void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
 
 void launch_mesh_ids(const MeshOff* off, int nm, int64_t T, uint32_t* tri_mesh, hipStream_t s);
 void launch_tri_setup(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
-                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, double4* slab2,
+                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, float4* slab2,
                       hipStream_t s);
 
 // exclusive scan of n u32 into out[0..n] (out[n] = total); tmp sized by
@@ -136,7 +136,7 @@ void launch_pair_scatter(const uint32_t* tri, const uint32_t* pb, uint64_t n, co
 void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStream_t s);
 
 void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
-                   const uint64_t* pmask, const double4* slab2, const GridDev& g0, const Meshes& M, uint64_t* sat2,
+                   const uint64_t* pmask, const float4* slab2, const GridDev& g0, const Meshes& M, uint64_t* sat2,
                    hipStream_t s);
 void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec, const GridDev& g0,
                       const Meshes& M, uint64_t* par2, hipStream_t s);

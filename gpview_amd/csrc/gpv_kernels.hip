@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
                                                    int4* __restrict__ ray_rng,
                                                    uint32_t* __restrict__ ray_items,
                                                    double4* __restrict__ slab1,
-                                                   double4* __restrict__ slab2)
+                                                   float4* __restrict__ slab2)
 {
     uint64_t tests = 0;
     const int64_t first = (int64_t)blockIdx.x * blockDim.x;
@@ -343,8 +343,8 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
             const double clo[3] = {g.c2x[(int64_t)lo[0] * n2], g.c2y[(int64_t)lo[1] * n2], g.c2z[(int64_t)lo[2] * n2]};
             const double chi[3] = {g.c2x[(int64_t)(lo[0] + dd[0]) * n2 - 1], g.c2y[(int64_t)(lo[1] + dd[1]) * n2 - 1],
                                    g.c2z[(int64_t)(lo[2] + dd[2]) * n2 - 1]};
-            const PlaneSlab sl = plane_slab(tv, g.h2x, g.h2y, g.h2z, clo, chi);
-            slab2[t] = make_double4(sl.nx, sl.ny, sl.nz, sl.lim);
+            const FloatSlab f = float_slab(plane_slab(tv, g.h2x, g.h2y, g.h2z, clo, chi), tv, clo, chi);
+            slab2[t] = make_float4(f.nx, f.ny, f.nz, f.lim);
         }
         uint32_t rit = 0;
         int4 rr = make_int4(0, 0, 0, 0);
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
 }
 
 void launch_tri_setup(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
-                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, double4* slab2,
+                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, float4* slab2,
                       hipStream_t s)
 {
     if (T <= 0) return;
@@ -1161,8 +1161,9 @@ void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStrea
 // sub-index, computed with the bins) and its triangle a plane slab
 // (gpv_slab.hpp) valid over every Level2 center of its index AABB.  For each
 // triangle the lanes take the sub-columns (p, q) its x/y masks allow and keep
-// the sub-voxels r whose |N.(c - t0)| the slab admits (an exact double bound:
-// the SAT rejects every other one); those queue for the full float SAT in
+// the sub-voxels r whose float |N.(c - t0)| the slab admits (the bound is
+// grown by that evaluation's error, so the SAT rejects every other one);
+// those queue for the full float SAT in
 // 64-lane batches.  Sub-voxels already set are skipped (the result is an OR).
 // ---------------------------------------------------------------------------
 constexpr int L2_WAVES = 4;
@@ -1215,7 +1216,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
                                                           const uint32_t* __restrict__ boff,
                                                           const uint32_t* __restrict__ list,
                                                           const uint64_t* __restrict__ pmask,
-                                                          const double4* __restrict__ slab2, GridDev g0,
+                                                          const float4* __restrict__ slab2, GridDev g0,
                                                           Meshes M, uint64_t* __restrict__ sat2)
 {
     __shared__ unsigned long long s_acc[L2_WAVES][64];
@@ -1261,7 +1262,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
         // staging: vertices to LDS; masks and slab stay in the lane's registers
         const uint32_t i = base + lane;
         uint64_t mk = 0;
-        double4 sl = make_double4(0.0, 0.0, 0.0, 0.0);
+        float4 sl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (i < e1) {
             const uint64_t t = list[i];
 #pragma unroll
@@ -1283,8 +1284,8 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
             // contiguous passing sets (centers are monotone in the index) mean the
             // box-axis predicates hold on the whole sub-box
             const uint32_t boxbit = (__popc(mx) == dp && __popc(my) == dq && __popc(mz) == dr) ? (1u << 18) : 0u;
-            const double Nx = lane_d(sl.x, j), Ny = lane_d(sl.y, j), Nz = lane_d(sl.z, j), lim = lane_d(sl.w, j);
-            const double t0x = stv[j * 9 + 0], t0y = stv[j * 9 + 1], t0z = stv[j * 9 + 2];
+            const float Nx = lane_f(sl.x, j), Ny = lane_f(sl.y, j), Nz = lane_f(sl.z, j), lim = lane_f(sl.w, j);
+            const float t0x = stv[j * 9 + 0], t0y = stv[j * 9 + 1], t0z = stv[j * 9 + 2];
             const int P = dp * dq;
             const float inv_dp = 1.0f / (float)dp;
             for (int l0 = 0; l0 < P; l0 += 64) {
@@ -1296,10 +1297,11 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
                 const float ccx = __shfl(cxl, p & 63), ccy = __shfl(cyl, q & 63);
                 uint32_t keep = 0;
                 if (l < P && ((mx >> p) & (my >> q) & 1u)) {
-                    const double A = Nx * ((double)ccx - t0x) + Ny * ((double)ccy - t0y);
+                    // float_slab_keeps, with the (x, y) part shared by the sub-column
+                    const float A = Nx * (ccx - t0x) + Ny * (ccy - t0y);
                     for (int r = ra; r <= rb; ++r) {
-                        const double sd = A + Nz * ((double)lane_f(czl, r) - t0z);
-                        keep |= (uint32_t)(fabs(sd) <= lim && ((mz >> r) & 1u)) << r;
+                        const float sd = A + Nz * (lane_f(czl, r) - t0z);
+                        keep |= (uint32_t)(fabsf(sd) <= lim && ((mz >> r) & 1u)) << r;
                     }
                 }
                 // queue the kept sub-voxels not yet set, one r per lane per round
@@ -1339,7 +1341,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
 }
 
 void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
-                   const uint64_t* pmask, const double4* slab2, const GridDev& g0, const Meshes& M, uint64_t* sat2,
+                   const uint64_t* pmask, const float4* slab2, const GridDev& g0, const Meshes& M, uint64_t* sat2,
                    hipStream_t s)
 {
     if (B <= 0) return;

@@ -161,7 +161,7 @@ struct Job {
     // per triangle
     int4 *box0 = nullptr, *box1 = nullptr, *rng = nullptr;
     double4* slab1 = nullptr;   // plane slab of triangles spanning several layers
-    double4* slab2 = nullptr;   // plane slab over the Level2 centers of the index AABB
+    float4* slab2 = nullptr;    // plane slab over the Level2 centers of the index AABB (float form)
     uint32_t *sat_items = nullptr, *ray_items = nullptr;
     uint64_t *sat_off = nullptr, *ray_off = nullptr, *scan64 = nullptr;
     uint32_t* scan32 = nullptr;

@@ -83,6 +83,36 @@ __host__ __device__ inline PlaneSlab plane_slab(const float* tv, float hx, float
     return s;
 }
 
+// Float form of a slab for centers within [clo, chi]: the normal rounded to
+// float and the bound grown by the float evaluation error of
+// fl(fl(fl(nx*(cx - t0x)) + fl(ny*(cy - t0y))) + fl(nz*(cz - t0z))), at most
+// 6U sum |N_q| max|c_q - t0_q| (1U for the rounded normal, 1U per subtraction,
+// product and sum), taken as 8U and rounded up to a float.
+struct FloatSlab {
+    float nx, ny, nz, lim;
+};
+
+__host__ __device__ inline FloatSlab float_slab(const PlaneSlab& sl, const float* tv, const double clo[3],
+                                                const double chi[3])
+{
+    double sa = 0.0;
+    sa += fabs(sl.nx) * fmax(fabs(clo[0] - tv[0]), fabs(chi[0] - tv[0]));
+    sa += fabs(sl.ny) * fmax(fabs(clo[1] - tv[1]), fabs(chi[1] - tv[1]));
+    sa += fabs(sl.nz) * fmax(fabs(clo[2] - tv[2]), fabs(chi[2] - tv[2]));
+    const double limd = sl.lim + 8.0 * 5.9604644775390625e-08 * sa;
+    float limf = (float)limd;
+    if ((double)limf < limd) limf = nextafterf(limf, INFINITY);
+    return FloatSlab{(float)sl.nx, (float)sl.ny, (float)sl.nz, limf};
+}
+
+// the float test every caller of a FloatSlab evaluates, in this order
+__host__ __device__ inline bool float_slab_keeps(const FloatSlab& f, const float* tv, float cx, float cy, float cz)
+{
+    const float a = f.nx * (cx - tv[0]) + f.ny * (cy - tv[1]);
+    const float d = a + f.nz * (cz - tv[2]);
+    return fabsf(d) <= f.lim;
+}
+
 __host__ __device__ inline double slab_dist(const PlaneSlab& s, double cx, double cy, double cz)
 {
     return s.nx * (cx - s.t0x) + s.ny * (cy - s.t0y) + s.nz * (cz - s.t0z);

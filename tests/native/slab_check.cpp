@@ -37,3 +37,17 @@ extern "C" void slab_layers_batch(int64_t n, const float* tv, const int32_t* lo,
             }
     }
 }
+
+// The Level2 float slab: slab and bound over the centers range [clo, chi] of
+// each case (per case 6 doubles), the float test at center c.
+extern "C" void float_slab_keep_batch(int64_t n, const float* tv, const float* c, const float* h,
+                                      const double* lohi, uint8_t* keep)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const double* r = lohi + 6 * i;
+        const double lo[3] = {r[0], r[1], r[2]}, hi[3] = {r[3], r[4], r[5]};
+        const gpv::PlaneSlab s = gpv::plane_slab(tv + 9 * i, h[3 * i], h[3 * i + 1], h[3 * i + 2], lo, hi);
+        const gpv::FloatSlab f = gpv::float_slab(s, tv + 9 * i, lo, hi);
+        keep[i] = (uint8_t)gpv::float_slab_keeps(f, tv + 9 * i, c[3 * i], c[3 * i + 1], c[3 * i + 2]);
+    }
+}

@@ -182,3 +182,45 @@ def test_level1_layers_never_drop_a_sat_hit(orc):
         total += inside.size
     assert hits > 50000
     assert kept < 0.5 * total, f"pruning too weak: kept {kept} of {total}"
+
+
+def test_float_slab_never_drops_a_sat_hit(orc):
+    """The Level2 float slab (gpv::float_slab over a range of centers, tested
+    with gpv::float_slab_keeps): borderline planes at single centers, and
+    large triangles whose slab spans a wide range of centers around the box."""
+    so = os.path.join(NATIVE, "libslabcheck.so")
+    if not os.path.exists(so):
+        subprocess.run(["make", "-s", "-C", NATIVE], check=True)
+    L = ctypes.CDLL(so)
+    f32 = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    f64 = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    L.float_slab_keep_batch.argtypes = [ctypes.c_int64, f32, f32, f32, f64, np.ctypeslib.ndpointer(np.uint8)]
+    use_ref = orc.ref() is not None
+    rng = np.random.default_rng(77)
+    hits = 0
+    for rnd in range(4):
+        tv, c, h = _plane_borderline(rng, 200000)
+        c = np.ascontiguousarray(c, np.float32)
+        if rnd % 2:
+            # a wide centers range around c (up to 40 boxes each way)
+            w = rng.uniform(0, 40, (len(c), 1)) * h.astype(np.float64)
+            lohi = np.concatenate([c - w * rng.random((len(c), 3)), c + w * rng.random((len(c), 3))], 1)
+        else:
+            lohi = np.concatenate([c, c], 1).astype(np.float64)
+        keep = np.zeros(len(c), np.uint8)
+        L.float_slab_keep_batch(len(c), np.ascontiguousarray(tv), c, np.ascontiguousarray(h, np.float32),
+                                np.ascontiguousarray(lohi, np.float64), keep)
+        res = orc.tri_box(c, h, tv, use_ref=use_ref).astype(bool)
+        assert not np.any(res & ~keep.astype(bool)), f"{np.count_nonzero(res & ~keep.astype(bool))} hits dropped"
+        hits += int(res.sum())
+    z = np.load(os.path.join(GOLDEN, "sat_cases.npz"), allow_pickle=False)
+    for fam in ("random", "face_snapped", "lattice", "sliver", "axis_plane", "degenerate"):
+        c = np.ascontiguousarray(z[f"{fam}_c"], np.float32)
+        h = np.ascontiguousarray(z[f"{fam}_h"], np.float32)
+        tv = np.ascontiguousarray(z[f"{fam}_tv"], np.float32)
+        lohi = np.ascontiguousarray(np.concatenate([c, c], 1), np.float64)
+        keep = np.zeros(len(c), np.uint8)
+        L.float_slab_keep_batch(len(c), tv, c, h, lohi, keep)
+        res = z[f"{fam}_res"].astype(bool)
+        assert not np.any(res & ~keep.astype(bool)), fam
+    assert hits > 50000
