@@ -38,13 +38,18 @@ def make_workload(gv):
     return v, f, gv.flatten(v, f)
 
 
-def bench_batched(gv, ctx, n_meshes, steps, warmup, check):
+def bench_batched(gv, ctx, n_meshes, steps, warmup, check, small=False):
     """The mixed-size stream: n_meshes meshes (triangles log-uniform in
     [1k, 200k], Level1 {64,128,256}, Level2 {4,8}), device-resident soups.
     Plain = one gpv_voxelize per mesh; batched = one gpv_voxelize_batch over
-    all of them.  Both timed over whole passes, grids released each pass."""
+    all of them.  Both timed over whole passes, grids released each pass.
+    small=True: the same generators at 1k-20k triangles, Level1 64, Level2 4
+    (meshes whose single-mesh run is dominated by per-call costs)."""
     from gpview_amd import workloads
-    meshes = workloads.mixed_stream(gv, n_meshes)
+    if small:
+        meshes = workloads.mixed_stream(gv, n_meshes, seed=7, tri_range=(1000, 20000), level1=(64,), level2=(4,))
+    else:
+        meshes = workloads.mixed_stream(gv, n_meshes)
     ptrs = [ctx.upload(m["soup"]) for m in meshes]
     T = sum(len(m["soup"]) for m in meshes)
 
@@ -99,9 +104,10 @@ def bench_batched(gv, ctx, n_meshes, steps, warmup, check):
     for p in ptrs:
         ctx.free(p)
     stage_ms = {s: float(np.median([i["ms_stage"][k] for i in binfo])) for k, s in enumerate(STAGES)}
+    desc = ("log-uniform 1k-20k; icospheres, tori, drilled boxes; L1 64, L2 4" if small else
+            "log-uniform 1k-200k; icospheres, tori, drilled boxes; L1 {64,128,256}, L2 {4,8}")
     return {
-        "workload": f"mixed-size stream, {n_meshes} meshes, {T} triangles (log-uniform 1k-200k; "
-                    "icospheres, tori, drilled boxes; L1 {64,128,256}, L2 {4,8})",
+        "workload": f"{'small-mesh' if small else 'mixed-size'} stream, {n_meshes} meshes, {T} triangles ({desc})",
         "meshes": n_meshes, "triangles": T,
         "plain_meshes_per_s": n_meshes / t_plain,
         "batched_meshes_per_s": n_meshes / t_batch,
@@ -242,6 +248,8 @@ def main():
     ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy that calibrates the traffic counters")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the served stream (--serve-requests at --serve-rate); one JSON line")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -252,6 +260,11 @@ def main():
         sys.exit(2)
 
     import gpview_amd as gv
+    if args.serve_only:
+        sv = bench_serve(gv, args.serve_requests, args.serve_rate, args.serve_lanes)
+        print(json.dumps({"metric": "served meshes/s; p50/p99 request latency", "value": sv["meshes_per_s"],
+                          "unit": "meshes/s", "higher_is_better": True, "n_gpus": 1, "served": sv}), flush=True)
+        return
     v, f, soup = make_workload(gv)
     T = len(soup)
     bmin, bmax, mms = gv.mesh_bbox(v)
@@ -280,9 +293,10 @@ def main():
     got = g.download()
     g.release()
     ctx.free(dsoup)
-    batched = None
+    batched = batched_small = None
     if args.batch_meshes > 0:
         batched = bench_batched(gv, ctx, args.batch_meshes, args.batch_steps, 1, not args.no_check)
+        batched_small = bench_batched(gv, ctx, args.batch_meshes, args.batch_steps, 1, not args.no_check, small=True)
     ctx.close()
     served = None
     if args.serve_requests > 0:
@@ -356,6 +370,7 @@ def main():
         "roofline_valu": valu,
         "cpu_baseline": cpu,
         "batched": batched,
+        "batched_small_meshes": batched_small,
         "served": served,
     }
     print(json.dumps(out), flush=True)

@@ -1286,6 +1286,8 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
             const uint32_t boxbit = (__popc(mx) == dp && __popc(my) == dq && __popc(mz) == dr) ? (1u << 18) : 0u;
             const float Nx = lane_f(sl.x, j), Ny = lane_f(sl.y, j), Nz = lane_f(sl.z, j), lim = lane_f(sl.w, j);
             const float t0x = stv[j * 9 + 0], t0y = stv[j * 9 + 1], t0z = stv[j * 9 + 2];
+            // the z term of float_slab_keeps for layer r, in lane r
+            const float zt = Nz * (czl - t0z);
             const int P = dp * dq;
             const float inv_dp = 1.0f / (float)dp;
             for (int l0 = 0; l0 < P; l0 += 64) {
@@ -1300,7 +1302,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
                     // float_slab_keeps, with the (x, y) part shared by the sub-column
                     const float A = Nx * (ccx - t0x) + Ny * (ccy - t0y);
                     for (int r = ra; r <= rb; ++r) {
-                        const float sd = A + Nz * (lane_f(czl, r) - t0z);
+                        const float sd = A + lane_f(zt, r);
                         keep |= (uint32_t)(fabsf(sd) <= lim && ((mz >> r) & 1u)) << r;
                     }
                 }
