@@ -121,8 +121,9 @@ def bench_batched(gv, ctx, n_meshes, steps, warmup, check, small=False):
     }
 
 
-def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48):
-    """The served stream: Poisson arrivals at `rate` requests/s, each drawing
+def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48, plain_sample=300):
+    """The served stream: Poisson arrivals at `rate` requests/s (0: all at
+    once, the saturated case), each drawing
     a mesh from a library made by the mixed-stream generators (triangles
     log-uniform in [1k, 2M]) and a resolution (Level1 {64,128,256,512},
     Level2 {4,8,16}).  Library soups are device-resident; results stay
@@ -134,7 +135,7 @@ def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48):
     pick = rng.integers(0, n_library, n_requests)
     l1 = rng.choice([64, 128, 256, 512], n_requests, p=[0.3, 0.3, 0.3, 0.1])
     l2 = rng.choice([4, 8, 16], n_requests, p=[0.45, 0.45, 0.1])
-    gaps = rng.exponential(1.0 / rate, n_requests)
+    gaps = rng.exponential(1.0 / rate, n_requests) if rate > 0 else np.zeros(n_requests)
     srv = gv.Server(lanes=lanes, hbm_budget=200e9)
     ctx = gv.Context(0)
     ptrs = [ctx.upload(m["soup"]) for m in lib_meshes]
@@ -155,6 +156,15 @@ def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48):
             ids[i] = srv.submit(m, device_ptr=ptrs[pick[i]])
             ready.release()
 
+    # the same request sequence through the single-mesh path, one call after
+    # another (a bounded prefix), for the served-vs-plain comparison
+    n_plain = min(n_requests, plain_sample)
+    t0 = time.perf_counter()
+    for i in range(n_plain):
+        m = lib_meshes[pick[i]]
+        ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], int(l1[i]), int(l2[i]), True,
+                     device_ptr=ptrs[pick[i]]).release()
+    plain_rate = n_plain / (time.perf_counter() - t0) if n_plain else None
     th = threading.Thread(target=submitter)
     t0 = time.perf_counter()
     th.start()
@@ -176,11 +186,13 @@ def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48):
     pred = np.array([s["predicted_bytes"] for s in stats], np.float64)
     res = np.array([s["resident_bytes"] for s in stats], np.float64)
     return {
-        "workload": f"served stream: {n_requests} Poisson requests at {rate:g}/s over a {n_library}-mesh "
+        "workload": f"served stream: {n_requests} " + (f"Poisson requests at {rate:g}/s" if rate > 0 else
+                                                          "requests submitted at once") + f" over a {n_library}-mesh "
                     "device-resident library (1k-2M triangles), L1 {64,128,256,512}, L2 {4,8,16}",
         "requests": n_requests, "rate_per_s": rate, "lanes": lanes, "hbm_budget_bytes": 200e9,
         "meshes_per_s": n_requests / span, "wall_s": wall, "triangles": int(tris.sum()),
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "plain_meshes_per_s": plain_rate, "plain_sample": n_plain,
         "mean_batch": float(sizes.mean()), "batches": c["batches"],
         "peak_charged_bytes": c["peak_charged"],
         "predicted_over_resident": float(np.median(pred / np.maximum(res, 1))),
