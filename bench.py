@@ -176,6 +176,7 @@ def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48, plain_sample=
     th.join()
     wall = time.perf_counter() - t0
     c = srv.counters()
+    pool = srv.pool_stats()
     srv.close()
     for p in ptrs:
         ctx.free(p)
@@ -195,6 +196,7 @@ def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48, plain_sample=
         "plain_meshes_per_s": plain_rate, "plain_sample": n_plain,
         "mean_batch": float(sizes.mean()), "batches": c["batches"],
         "peak_charged_bytes": c["peak_charged"],
+        "pool": {k: pool[k] for k in ("held_bytes", "allocations", "reuses", "evictions")},
         "predicted_over_resident": float(np.median(pred / np.maximum(res, 1))),
     }
 

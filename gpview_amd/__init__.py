@@ -53,6 +53,15 @@ class GridInfo(ctypes.Structure):
         return d
 
 
+class PoolStats(ctypes.Structure):
+    """gpv_pool_stats (include/gpv.h)."""
+    _fields_ = [("held_bytes", ctypes.c_int64), ("cached_bytes", ctypes.c_int64), ("cap_bytes", ctypes.c_int64),
+                ("allocations", ctypes.c_int64), ("reuses", ctypes.c_int64), ("evictions", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class MeshDesc(ctypes.Structure):
     """gpv_mesh_desc (include/gpv.h)."""
     _fields_ = [("tris", ctypes.c_void_p), ("ntris", ctypes.c_int64), ("bmin", ctypes.c_float * 3),
@@ -87,7 +96,7 @@ _lib = None
 # every symbol include/gpv.h and include/gpv_mesh.h declare
 EXPORTED = [
     "gpv_ctx_create", "gpv_ctx_destroy", "gpv_last_error", "gpv_status_string",
-    "gpv_device_upload", "gpv_device_free",
+    "gpv_device_upload", "gpv_device_free", "gpv_ctx_pool_stats",
     "gpv_mesh_bbox", "gpv_grid_derive", "gpv_voxelize", "gpv_voxelize_batch", "gpv_grid_create",
     "gpv_classify_tessellation", "gpv_classify_inout", "gpv_compact_boundary",
     "gpv_classify_inout_level2", "gpv_classify_tessellation_level2", "gpv_grid_finish",
@@ -97,7 +106,7 @@ EXPORTED = [
     "gpv_mesh_torus_knot", "gpv_mesh_torus", "gpv_mesh_drilled_box", "gpv_mesh_mixed",
     "gpv_mesh_stress", "gpv_mesh_read_obj", "gpv_mesh_read_off",
     "gpv_predict_footprint", "gpv_server_create", "gpv_server_destroy", "gpv_server_submit",
-    "gpv_server_wait", "gpv_server_release", "gpv_server_counters",
+    "gpv_server_wait", "gpv_server_release", "gpv_server_counters", "gpv_server_pool_stats",
 ]
 
 
@@ -118,6 +127,8 @@ def lib():
     L.gpv_ctx_destroy.argtypes = [_vp]
     L.gpv_device_upload.argtypes = [_vp, _vp, c_i64, P(_vp)]
     L.gpv_device_free.argtypes = [_vp, _vp]
+    L.gpv_ctx_pool_stats.argtypes = [_vp, P(PoolStats)]
+    L.gpv_server_pool_stats.argtypes = [_vp, P(PoolStats)]
     L.gpv_mesh_bbox.argtypes = [_f32p, c_i64, _f32p, _f32p, P(c_float)]
     L.gpv_grid_derive.argtypes = [_f32p, _f32p, c_float, P(Params), P(GridInfo)]
     for fn in ("gpv_voxelize", "gpv_grid_create"):
@@ -294,6 +305,12 @@ class Context:
     def free(self, ptr):
         _check(lib().gpv_device_free(self.h, ptr), "gpv_device_free")
 
+    def pool_stats(self):
+        """gpv_ctx_pool_stats as a dict."""
+        s = PoolStats()
+        _check(lib().gpv_ctx_pool_stats(self.h, ctypes.byref(s)), "gpv_ctx_pool_stats")
+        return s.as_dict()
+
     def voxelize(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0,
                  device_ptr=None, normals=False):
         """Object::PerformVoxelization on a flat soup; returns a Grid."""
@@ -414,6 +431,12 @@ class Server:
         _check(lib().gpv_server_counters(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
                                          ctypes.byref(d)), "gpv_server_counters")
         return {"completed": a.value, "batches": b.value, "charged": c.value, "peak_charged": d.value}
+
+    def pool_stats(self):
+        """gpv_server_pool_stats: the lanes' pools, summed."""
+        s = PoolStats()
+        _check(lib().gpv_server_pool_stats(self.h, ctypes.byref(s)), "gpv_server_pool_stats")
+        return s.as_dict()
 
 
 class Grid:

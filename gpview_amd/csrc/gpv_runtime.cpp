@@ -59,7 +59,12 @@ namespace {
 
 // Device memory pool: size classes (powers of two up to 256 MiB, 16 MiB
 // granules above), blocks cached until the context is destroyed, so the
-// steady state of repeated voxelizations allocates nothing.
+// steady state of repeated voxelizations allocates nothing.  A request takes
+// the smallest cached block of at least its class and at most twice it, so
+// batches of varying size reuse each other's blocks.  Cached (free) bytes are
+// held under a cap: a miss that would exceed it first frees the largest
+// cached blocks, and a failed allocation frees every cached block and tries
+// once more.
 class Pool {
 public:
     ~Pool() { trim(); }
@@ -74,16 +79,20 @@ public:
         const size_t gran = size_t(16) << 20;
         return (n + gran - 1) / gran * gran;
     }
+    void set_cap(size_t cap) { cap_ = cap; }
     hipError_t get(size_t n, void** out)
     {
         const size_t c = size_class(n);
-        auto it = free_.find(c);
-        if (it != free_.end()) {
+        auto it = free_.lower_bound(c);
+        if (it != free_.end() && it->first <= 2 * c) {
             *out = it->second;
+            live_[*out] = it->first;
+            cached_ -= it->first;
             free_.erase(it);
-            live_[*out] = c;
+            ++reuses_;
             return hipSuccess;
         }
+        while (!free_.empty() && cached_ + c > cap_) evict(std::prev(free_.end()));
         void* p = nullptr;
         hipError_t e = hipMalloc(&p, c);
         if (e != hipSuccess) {
@@ -94,6 +103,7 @@ public:
         }
         live_[p] = c;
         bytes_ += c;
+        ++allocs_;
         *out = p;
         return hipSuccess;
     }
@@ -103,22 +113,36 @@ public:
         auto it = live_.find(p);
         if (it == live_.end()) return;
         free_.emplace(it->second, p);
+        cached_ += it->second;
         live_.erase(it);
     }
     void trim()
     {
-        for (auto& kv : free_) {
-            (void)hipFree(kv.second);
-            bytes_ -= kv.first;
-        }
-        free_.clear();
+        while (!free_.empty()) evict(free_.begin());
     }
-    size_t bytes() const { return bytes_; }
+    void stats(gpv_pool_stats* s) const
+    {
+        s->held_bytes = (int64_t)bytes_;
+        s->cached_bytes = (int64_t)cached_;
+        s->cap_bytes = (int64_t)cap_;
+        s->allocations = allocs_;
+        s->reuses = reuses_;
+        s->evictions = evictions_;
+    }
 
 private:
+    void evict(std::multimap<size_t, void*>::iterator it)
+    {
+        (void)hipFree(it->second);
+        bytes_ -= it->first;
+        cached_ -= it->first;
+        ++evictions_;
+        free_.erase(it);
+    }
     std::multimap<size_t, void*> free_;
     std::unordered_map<void*, size_t> live_;
-    size_t bytes_ = 0;
+    size_t bytes_ = 0, cached_ = 0, cap_ = ~size_t(0);
+    int64_t allocs_ = 0, reuses_ = 0, evictions_ = 0;
 };
 
 }  // namespace
@@ -933,6 +957,7 @@ int gpv_ctx_create(int device, unsigned flags, gpv_ctx** out)
     gpv_ctx* c = new gpv_ctx();
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
+    c->pool.set_cap(prop.totalGlobalMem / 8);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(GPV_ERR_HIP, "stream creation failed");
@@ -979,6 +1004,14 @@ int gpv_device_upload(gpv_ctx* c, const void* host, int64_t bytes, void** dev)
         }
     }
     *dev = p;
+    return GPV_OK;
+}
+
+int gpv_ctx_pool_stats(gpv_ctx* c, gpv_pool_stats* out)
+{
+    if (!c || !out) return fail(GPV_ERR_INVALID, "NULL argument");
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->pool.stats(out);
     return GPV_OK;
 }
 

@@ -326,4 +326,23 @@ int gpv_server_counters(gpv_server* S, int64_t* completed, int64_t* batches, int
     return GPV_OK;
 }
 
+int gpv_server_pool_stats(gpv_server* S, gpv_pool_stats* out)
+{
+    if (!S || !out) return err(GPV_ERR_INVALID, "NULL argument");
+    gpv_pool_stats sum{};
+    for (gpv_ctx* c : S->ctx) {
+        gpv_pool_stats p{};
+        const int st = gpv_ctx_pool_stats(c, &p);
+        if (st != GPV_OK) return st;
+        sum.held_bytes += p.held_bytes;
+        sum.cached_bytes += p.cached_bytes;
+        sum.cap_bytes += p.cap_bytes;
+        sum.allocations += p.allocations;
+        sum.reuses += p.reuses;
+        sum.evictions += p.evictions;
+    }
+    *out = sum;
+    return GPV_OK;
+}
+
 }  // extern "C"

@@ -95,6 +95,22 @@ const char *gpv_status_string(int status);
 int gpv_device_upload(gpv_ctx *ctx, const void *host, int64_t bytes, void **dev);
 int gpv_device_free(gpv_ctx *ctx, void *dev);
 
+/* The context's device memory pool.  No counterpart in the reference, which
+ * cudaMallocs and frees around every call; serving loops read it to see that
+ * steady state allocates nothing.  A request takes the smallest cached block
+ * of one to two times its size class; before each new device allocation the
+ * largest cached blocks are freed until cached bytes plus the new block fit
+ * under cap_bytes (1/8 of the device's memory). */
+typedef struct gpv_pool_stats {
+    int64_t held_bytes;     /* device memory the pool owns (in use + cached) */
+    int64_t cached_bytes;   /* free blocks kept for reuse */
+    int64_t cap_bytes;
+    int64_t allocations;    /* hipMalloc calls so far */
+    int64_t reuses;         /* requests served from cached blocks */
+    int64_t evictions;      /* cached blocks freed (cap, out of memory, destroy) */
+} gpv_pool_stats;
+int gpv_ctx_pool_stats(gpv_ctx *ctx, gpv_pool_stats *out);
+
 /* ---- host-side geometry set-up ----------------------------------------- */
 /* bbox over vertices + 0.1% pad + maxModelSize: Object::ReadObject
  * (src/Object.cpp:441-450, 572-583).  verts: nverts*3 floats (host). */

@@ -71,6 +71,8 @@ int gpv_server_release(gpv_server *srv, uint64_t id);
 /* Totals: requests completed, batches run, bytes charged now, peak charge. */
 int gpv_server_counters(gpv_server *srv, int64_t *completed, int64_t *batches, int64_t *charged,
                         int64_t *peak_charged);
+/* The lanes' device memory pools, summed (gpv_ctx_pool_stats per lane). */
+int gpv_server_pool_stats(gpv_server *srv, gpv_pool_stats *out);
 
 #ifdef __cplusplus
 }

@@ -178,3 +178,54 @@ def test_batch_views_and_save(gv, ctx, tmp_path):
         g.release()
     for name in ("VoxelConfig.txt", "Level1InOut.raw", "Level1BoundaryPrefixSum.raw", "Level2InOut.raw"):
         assert (tmp_path / "a" / f"Obj5{name}").read_bytes() == (tmp_path / "b" / f"Obj5{name}").read_bytes()
+
+
+def test_batch_stress_vs_oracle(gv, orc, ctx):
+    """The exactness-stress generators (lattice-snapped vertices p = 0.3, ~5%
+    face-coplanar triangles, 10% slivers) at Level2 16, reduced in size so the
+    oracle checks every mesh: each batch member bit-exact."""
+    from gpview_amd import workloads
+    meshes = workloads.stress(gv, 4, seed=91, tri_range=(20000, 60000), vc=96, vc2=16)
+    say(f"stress batch: T={sum(len(m['soup']) for m in meshes)} L1 96 L2 16")
+    res = batch_results(ctx, meshes)
+    for i, (m, (got, info)) in enumerate(zip(meshes, res)):
+        want, n2 = oracle_of(orc, m)
+        compare_results(f"stress mesh {i}", got, want, n2)
+        check_info(f"stress mesh {i}", info, want)
+
+
+def test_pool_reuses_blocks_across_batch_sizes(gv):
+    """Batches of different sizes, repeated: the second pass over the same
+    sequence is served from cached blocks (a serving loop's steady state
+    allocates nothing), and results are unchanged by block reuse."""
+    from gpview_amd import workloads
+    ms = workloads.mixed_stream(gv, 12, seed=91, tri_range=(1000, 20000), level1=(32, 64), level2=(4, 8))
+    cuts = [(0, 12), (0, 5), (5, 12), (2, 4), (0, 12)]
+    with gv.Context(0) as c:
+        ptrs = [c.upload(m["soup"]) for m in ms]
+        first = {}
+
+        def one_pass():
+            for a, b in cuts:
+                gs = c.voxelize_batch(ms[a:b], device_ptrs=ptrs[a:b])
+                for i, g in zip(range(a, b), gs):
+                    d = g.download()
+                    got = np.concatenate([d["l1"], d["l2"], d["prefix"].view(np.uint8)])
+                    if i in first:
+                        assert np.array_equal(got, first[i]), f"mesh {i} changed with block reuse"
+                    else:
+                        first[i] = got
+                    g.release()
+
+        s0 = c.pool_stats()
+        one_pass()
+        s1 = c.pool_stats()
+        one_pass()
+        s2 = c.pool_stats()
+        say(f"pool: {s0} -> {s1} -> {s2}")
+        assert s1["allocations"] > s0["allocations"]
+        assert s2["allocations"] - s1["allocations"] <= (s1["allocations"] - s0["allocations"]) // 4
+        assert s2["reuses"] > s1["reuses"]
+        assert s2["cached_bytes"] <= s2["cap_bytes"]
+        for p in ptrs:
+            c.free(p)
