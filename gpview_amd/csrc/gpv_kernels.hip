@@ -1222,6 +1222,15 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
     __shared__ unsigned long long s_acc[L2_WAVES][64];
     __shared__ float s_tv[L2_WAVES][64 * 9];
     __shared__ uint32_t s_q[L2_WAVES][L2_QCAP];
+    // per staged triangle: sub-box origin and extents, masks, slab, 1/dp, and
+    // the start of its sub-columns in the chunk's flattened (triangle,
+    // sub-column) item space
+    __shared__ uint32_t s_dec[L2_WAVES][64];
+    __shared__ unsigned long long s_mk[L2_WAVES][64];
+    __shared__ float4 s_sl[L2_WAVES][64];
+    __shared__ float s_idp[L2_WAVES][64];
+    __shared__ uint32_t s_off[L2_WAVES][64];
+    __shared__ float s_cz[L2_WAVES][16];
     const int wv = threadIdx.x >> 6;
     const int lane = __lane_id();
     const int64_t b = (int64_t)blockIdx.x * L2_WAVES + wv;
@@ -1234,19 +1243,27 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
     uint64_t* out = sat2 + (kM ? M.off[mesh].l2w0 + (uint64_t)(b - M.off[mesh].b0) * w2 : (uint64_t)b * w2);
     unsigned long long* acc = s_acc[wv];
     for (int i = lane; i < w2; i += 64) acc[i] = 0ull;
-    __builtin_amdgcn_wave_barrier();
     const int v = bidx[b];
     const int nxy = gnx * gny;
     const int k1 = v / nxy;
     const int ij = v - k1 * nxy;
     const int j1 = ij / gnx;
     const int i1 = ij - j1 * gnx;
-    // the parent's n2 sub-centers per axis in registers, entry p in lane p
+    // the parent's n2 sub-centers per axis in registers, entry p in lane p;
+    // z also in LDS for per-lane layer lookups
     const float cxl = lane < n2 ? g.c2x[(int64_t)i1 * n2 + lane] : 0.0f;
     const float cyl = lane < n2 ? g.c2y[(int64_t)j1 * n2 + lane] : 0.0f;
     const float czl = lane < n2 ? g.c2z[(int64_t)k1 * n2 + lane] : 0.0f;
+    float* cz = s_cz[wv];
+    if (lane < n2) cz[lane] = czl;
     float* stv = s_tv[wv];
     uint32_t* sq = s_q[wv];
+    uint32_t* dec = s_dec[wv];
+    unsigned long long* smk = s_mk[wv];
+    float4* ssl = s_sl[wv];
+    float* sidp = s_idp[wv];
+    uint32_t* soff = s_off[wv];
+    __builtin_amdgcn_wave_barrier();
     uint32_t qh = 0, qt = 0;   // ring head/tail (uniform)
     const uint32_t e0 = boff[b], e1 = boff[b + 1];
     for (uint32_t base = e0; base < e1; base += 64) {
@@ -1259,74 +1276,100 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
             qh += n;
         }
         __builtin_amdgcn_wave_barrier();
-        // staging: vertices to LDS; masks and slab stay in the lane's registers
+        // staging: lane j owns bin entry base + j -- vertices, masks, slab and
+        // its sub-box decode go to LDS
         const uint32_t i = base + lane;
-        uint64_t mk = 0;
-        float4 sl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t P = 0;
         if (i < e1) {
             const uint64_t t = list[i];
 #pragma unroll
             for (int c = 0; c < 9; ++c) stv[lane * 9 + c] = tris[t * 9 + c];
-            mk = pmask[i];
-            sl = slab2[t];
+            const uint64_t mk = pmask[i];
+            const uint32_t mx = (uint32_t)(mk & 0xffffu), my = (uint32_t)((mk >> 16) & 0xffffu),
+                           mz = (uint32_t)((mk >> 32) & 0xffffu);
+            if (mx && my && mz) {
+                const int pa = __ffs(mx) - 1, pb = 31 - __clz(mx);
+                const int qa = __ffs(my) - 1, qb = 31 - __clz(my);
+                const int ra = __ffs(mz) - 1, rb = 31 - __clz(mz);
+                const int dp = pb - pa + 1, dq = qb - qa + 1, dr = rb - ra + 1;
+                // contiguous passing sets (centers are monotone in the index) mean
+                // the box-axis predicates hold on the whole sub-box
+                const uint32_t boxbit =
+                    (__popc(mx) == dp && __popc(my) == dq && __popc(mz) == dr) ? 1u : 0u;
+                dec[lane] = (uint32_t)pa | ((uint32_t)qa << 4) | ((uint32_t)ra << 8) | ((uint32_t)rb << 12) |
+                            ((uint32_t)dp << 16) | (boxbit << 24);
+                smk[lane] = mk;
+                ssl[lane] = slab2[t];
+                sidp[lane] = 1.0f / (float)dp;
+                P = (uint32_t)(dp * dq);
+            }
         }
+        const uint32_t incl = wave_incl_scan<uint32_t>(P);
+        soff[lane] = incl - P;
+        const uint32_t S = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         __builtin_amdgcn_wave_barrier();
-        const int nv = (int)(e1 - base < 64 ? e1 - base : 64);
-        for (int j = 0; j < nv; ++j) {
-            const uint64_t mkj = lane_u64(mk, j);
-            const uint32_t mx = (uint32_t)(mkj & 0xffffu), my = (uint32_t)((mkj >> 16) & 0xffffu),
-                           mz = (uint32_t)((mkj >> 32) & 0xffffu);
-            if (!mx || !my || !mz) continue;
-            const int pa = __ffs(mx) - 1, pb = 31 - __clz(mx);
-            const int qa = __ffs(my) - 1, qb = 31 - __clz(my);
-            const int ra = __ffs(mz) - 1, rb = 31 - __clz(mz);
-            const int dp = pb - pa + 1, dq = qb - qa + 1, dr = rb - ra + 1;
-            // contiguous passing sets (centers are monotone in the index) mean the
-            // box-axis predicates hold on the whole sub-box
-            const uint32_t boxbit = (__popc(mx) == dp && __popc(my) == dq && __popc(mz) == dr) ? (1u << 18) : 0u;
-            const float Nx = lane_f(sl.x, j), Ny = lane_f(sl.y, j), Nz = lane_f(sl.z, j), lim = lane_f(sl.w, j);
-            const float t0x = stv[j * 9 + 0], t0y = stv[j * 9 + 1], t0z = stv[j * 9 + 2];
-            // the z term of float_slab_keeps for layer r, in lane r
-            const float zt = Nz * (czl - t0z);
-            const int P = dp * dq;
-            const float inv_dp = 1.0f / (float)dp;
-            for (int l0 = 0; l0 < P; l0 += 64) {
-                const int l = l0 + lane;
-                // exact small-integer division: (l + 0.5)/dp is >= 1/32 from an integer
-                const int rem = (int)(((float)l + 0.5f) * inv_dp);
-                const int p = pa + l - rem * dp;
-                const int q = qa + rem;
-                const float ccx = __shfl(cxl, p & 63), ccy = __shfl(cyl, q & 63);
-                uint32_t keep = 0;
-                if (l < P && ((mx >> p) & (my >> q) & 1u)) {
-                    // float_slab_keeps, with the (x, y) part shared by the sub-column
-                    const float A = Nx * (ccx - t0x) + Ny * (ccy - t0y);
-                    for (int r = ra; r <= rb; ++r) {
-                        const float sd = A + lane_f(zt, r);
-                        keep |= (uint32_t)(fabsf(sd) <= lim && ((mz >> r) & 1u)) << r;
-                    }
+        // the chunk's (triangle, sub-column) items, 64 per round
+        for (uint32_t l0 = 0; l0 < S; l0 += 64) {
+            const uint32_t l = l0 + (uint32_t)lane;
+            uint32_t keep = 0;
+            int p = 0, q = 0, j = 0, ra = 0, rb = -1;
+            uint32_t boxbit = 0, mz = 0;
+            bool in = false;
+            if (l < S) {
+                // the last triangle whose items start at or before l (it has items)
+                int lo = 0;
+#pragma unroll
+                for (int st = 32; st; st >>= 1)
+                    if (soff[lo + st] <= l) lo += st;
+                j = lo;
+                const uint32_t d = dec[j];
+                const int pa = d & 15, qa = (d >> 4) & 15;
+                ra = (d >> 8) & 15;
+                rb = (d >> 12) & 15;
+                const int dp = (d >> 16) & 255;
+                boxbit = (d >> 24) & 1u;
+                const int lj = (int)(l - soff[j]);
+                // exact small-integer division: (lj + 0.5)/dp is >= 1/32 from an integer
+                const int rem = (int)(((float)lj + 0.5f) * sidp[j]);
+                p = pa + lj - rem * dp;
+                q = qa + rem;
+                const uint64_t mk = smk[j];
+                const uint32_t mx = (uint32_t)(mk & 0xffffu), my = (uint32_t)((mk >> 16) & 0xffffu);
+                mz = (uint32_t)((mk >> 32) & 0xffffu);
+                in = ((mx >> p) & (my >> q) & 1u) != 0;
+            }
+            // every lane takes part in the cross-lane reads
+            const float ccx = __shfl(cxl, p), ccy = __shfl(cyl, q);
+            if (in) {
+                const float4 sl = ssl[j];
+                const float t0x = stv[j * 9 + 0], t0y = stv[j * 9 + 1], t0z = stv[j * 9 + 2];
+                // float_slab_keeps, with the (x, y) part shared by the sub-column
+                const float A = sl.x * (ccx - t0x) + sl.y * (ccy - t0y);
+                for (int r = ra; r <= rb; ++r) {
+                    const float sd = A + sl.z * (cz[r] - t0z);
+                    keep |= (uint32_t)(fabsf(sd) <= sl.w && ((mz >> r) & 1u)) << r;
                 }
-                // queue the kept sub-voxels not yet set, one r per lane per round
-                while (__ballot(keep != 0)) {
-                    const int r = keep ? __ffs(keep) - 1 : 0;
-                    const bool has = keep != 0;
-                    keep &= keep - 1;
-                    const int sv = (r * n2 + q) * n2 + p;
-                    const bool push = has && !((acc[(sv & 4095) >> 6] >> (sv & 63)) & 1ull);
-                    const uint64_t m = __ballot(push);
-                    if (push) {
-                        const uint32_t at = qt + (uint32_t)__popcll(m & lanes_below_mask());
-                        sq[at % L2_QCAP] = (uint32_t)p | ((uint32_t)q << 4) | ((uint32_t)r << 8) |
-                                           ((uint32_t)j << 12) | boxbit;
-                    }
-                    qt += (uint32_t)__popcll(m);
+            }
+            // queue the kept sub-voxels not yet set, one r per lane per round
+            while (__ballot(keep != 0)) {
+                const int r = keep ? __ffs(keep) - 1 : 0;
+                const bool has = keep != 0;
+                keep &= keep - 1;
+                const int sv = (r * n2 + q) * n2 + p;
+                const bool push = has && !((acc[(sv & 4095) >> 6] >> (sv & 63)) & 1ull);
+                const uint64_t m = __ballot(push);
+                if (push) {
+                    const uint32_t at = qt + (uint32_t)__popcll(m & lanes_below_mask());
+                    sq[at % L2_QCAP] = (uint32_t)p | ((uint32_t)q << 4) | ((uint32_t)r << 8) |
+                                       ((uint32_t)j << 12) | (boxbit << 18);
+                }
+                qt += (uint32_t)__popcll(m);
+                __builtin_amdgcn_wave_barrier();
+                if (qt - qh >= 64) {   // a full wave of work
+                    const uint32_t ent = sq[(qh + lane) % L2_QCAP];
+                    l2_sat_run(ent, true, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
+                    qh += 64;
                     __builtin_amdgcn_wave_barrier();
-                    if (qt - qh >= 64) {   // a full wave of work
-                        const uint32_t ent = sq[(qh + lane) % L2_QCAP];
-                        l2_sat_run(ent, true, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
-                        qh += 64;
-                        __builtin_amdgcn_wave_barrier();
-                    }
                 }
             }
         }
@@ -1391,20 +1434,166 @@ __device__ __forceinline__ bool colrec_xy(const ColRec& r, float ox, float oy, f
     return mt_column(m, r.v1x, r.v1y, ox, oy, tx, ty);
 }
 
+// Lane l holds row l of a 64x64 bit matrix (bit j = entry (l, j)); returns
+// column l (bit j = entry (j, l)).  Six block swaps of halving size.
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t v)
+{
+    const int lane = __lane_id();
+    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int s = 32 >> k;
+        const uint64_t m = masks[k];
+        const uint64_t x = shfl64(v, lane ^ s);
+        v = (lane & s) ? ((v & ~m) | ((x & ~m) >> s)) : ((v & m) | ((x & m) << s));
+    }
+    return v;
+}
+
+// the mesh whose columns hold global column c (c uniform)
+__device__ __forceinline__ int mesh_of_col(const MeshOff* __restrict__ off, int nm, uint32_t c)
+{
+    int lo = 0, hi = nm;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid].col0 <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The z-independent half of the MT test (det/u/v, src/TriRayIntersection.cpp:
+// 78-131 with D = +z) depends on the column entry and the sub-column's (x, y)
+// only, so it runs once per (column entry, sub-column) here: one wave per xy
+// column, lane = column entry, looping over the column's n2^2 sub-columns
+// (sc = q*n2 + p); entry e's pass bits go to cpass[e * stride + sc / 64], bit
+// sc % 64.  Every boundary voxel of the column then reads them instead of
+// repeating the test.
+constexpr int CP_WAVES = 4;
+
+template <bool kM>
+__global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass(const uint32_t* __restrict__ coff,
+                                                            const float4* __restrict__ rec, GridDev g0, Meshes M,
+                                                            int64_t ncol, int stride, uint64_t* __restrict__ cpass)
+{
+    const int64_t c = (int64_t)blockIdx.x * CP_WAVES + (threadIdx.x >> 6);
+    if (c >= ncol) return;
+    const uint32_t e0 = coff[c], e1 = coff[c + 1];
+    if (e0 == e1) return;
+    const int lane = __lane_id();
+    const int mesh = kM ? uniform_i(mesh_of_col(M.off, M.nm, (uint32_t)c)) : 0;
+    const GridDev& g = grid_of<kM>(g0, M, mesh);
+    const int n2 = g.n2;
+    if (n2 == 0) return;
+    const uint32_t col = (uint32_t)c - (kM ? M.off[mesh].col0 : 0u);
+    const int i1 = (int)(col % (uint32_t)g.nx), j1 = (int)(col / (uint32_t)g.nx);
+    // sub-column origins: x of sub-index p in lane p, y of q in lane q
+    const float rxl = lane < n2 ? g.r2x[(int64_t)i1 * n2 + lane] : 0.0f;
+    const float ryl = lane < n2 ? g.r2y[(int64_t)j1 * n2 + lane] : 0.0f;
+    for (uint32_t base = e0; base < e1; base += 64) {
+        const uint32_t e = base + (uint32_t)lane;
+        const bool own = e < e1;
+        const ColRec r = load_colrec(rec, own ? e : e0);
+        uint64_t* dst = cpass + (uint64_t)e * stride;
+        uint64_t w = 0;
+        int k = 0, slot = 0;
+        for (int q = 0; q < n2; ++q) {
+            const float oy = lane_f(ryl, q);
+            for (int p = 0; p < n2; ++p) {
+                const float ox = lane_f(rxl, p);
+                float tx, ty;
+                w |= (uint64_t)colrec_xy(r, ox, oy, tx, ty) << k;
+                if (++k == 64) {
+                    if (own) dst[slot] = w;
+                    ++slot;
+                    w = 0;
+                    k = 0;
+                }
+            }
+        }
+        if (k && own) dst[slot] = w;
+    }
+}
+
+void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M, int64_t ncol,
+                     int stride, uint64_t* cpass, hipStream_t s)
+{
+    if (ncol <= 0) return;
+    const int64_t blocks = (ncol + CP_WAVES - 1) / CP_WAVES;
+    if (M.nm > 1)
+        k_col_pass<true><<<(unsigned)blocks, 64 * CP_WAVES, 0, s>>>(coff, rec, g0, M, ncol, stride, cpass);
+    else
+        k_col_pass<false><<<(unsigned)blocks, 64 * CP_WAVES, 0, s>>>(coff, rec, g0, M, ncol, stride, cpass);
+}
+
+// Level2 parity launch order: boundary voxels grouped by xy column (counting
+// sort on the global column), so the waves of one column run close together
+// and read its entries and pass bits from cache.
+template <bool kM>
+__global__ void k_bcol_count(const int32_t* __restrict__ bidx, int64_t B, GridDev g0, Meshes M,
+                             uint32_t* __restrict__ cnt)
+{
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < B; b += (int64_t)gridDim.x * blockDim.x) {
+        const int mesh = kM ? (int)M.bmesh[b] : 0;
+        const GridDev& g = grid_of<kM>(g0, M, mesh);
+        const uint32_t gcol = (kM ? M.off[mesh].col0 : 0u) + (uint32_t)(bidx[b] % (g.nx * g.ny));
+        atomicAdd(&cnt[gcol], 1u);
+    }
+}
+
+template <bool kM>
+__global__ void k_bcol_scatter(const int32_t* __restrict__ bidx, int64_t B, GridDev g0, Meshes M,
+                               const uint32_t* __restrict__ off, uint32_t* __restrict__ cur,
+                               uint32_t* __restrict__ ord)
+{
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < B; b += (int64_t)gridDim.x * blockDim.x) {
+        const int mesh = kM ? (int)M.bmesh[b] : 0;
+        const GridDev& g = grid_of<kM>(g0, M, mesh);
+        const uint32_t gcol = (kM ? M.off[mesh].col0 : 0u) + (uint32_t)(bidx[b] % (g.nx * g.ny));
+        ord[off[gcol] + atomicAdd(&cur[gcol], 1u)] = (uint32_t)b;
+    }
+}
+
+void launch_bcol_count(const int32_t* bidx, int64_t B, const GridDev& g0, const Meshes& M, uint32_t* cnt,
+                       hipStream_t s)
+{
+    if (B <= 0) return;
+    const int blocks = grid_for(B, 256, 16384);
+    if (M.nm > 1)
+        k_bcol_count<true><<<blocks, 256, 0, s>>>(bidx, B, g0, M, cnt);
+    else
+        k_bcol_count<false><<<blocks, 256, 0, s>>>(bidx, B, g0, M, cnt);
+}
+
+void launch_bcol_scatter(const int32_t* bidx, int64_t B, const GridDev& g0, const Meshes& M, const uint32_t* off,
+                         uint32_t* cur, uint32_t* ord, hipStream_t s)
+{
+    if (B <= 0) return;
+    const int blocks = grid_for(B, 256, 16384);
+    if (M.nm > 1)
+        k_bcol_scatter<true><<<blocks, 256, 0, s>>>(bidx, B, g0, M, off, cur, ord);
+    else
+        k_bcol_scatter<false><<<blocks, 256, 0, s>>>(bidx, B, g0, M, off, cur, ord);
+}
+
 constexpr int L2P_WAVES = 4;
 
 template <bool kM>
 __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __restrict__ bidx, int64_t B,
                                                               const uint32_t* __restrict__ coff,
-                                                              const float4* __restrict__ rec, GridDev g0,
+                                                              const float4* __restrict__ rec,
+                                                              const uint64_t* __restrict__ cpass, int stride,
+                                                              const uint32_t* __restrict__ ord, GridDev g0,
                                                               Meshes M, uint64_t* __restrict__ par2)
 {
     __shared__ unsigned long long s_acc[L2P_WAVES][64];
     __shared__ ColRec s_rec[L2P_WAVES][64];
     const int wv = threadIdx.x >> 6;
     const int lane = __lane_id();
-    const int64_t b = (int64_t)blockIdx.x * L2P_WAVES + wv;
-    if (b >= B) return;
+    const int64_t slot = (int64_t)blockIdx.x * L2P_WAVES + wv;
+    if (slot >= B) return;
+    const int64_t b = (int64_t)(uint32_t)uniform_i((int)ord[slot]);
     const int mesh = kM ? uniform_i((int)M.bmesh[b]) : 0;
     const GridDev& g = grid_of<kM>(g0, M, mesh);
     const int n2 = g.n2, w2 = g.w2, gnx = g.nx, gny = g.ny;
@@ -1438,27 +1627,23 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
     const uint32_t e0 = coff[col0 + ij], e1 = coff[col0 + ij + 1];
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t i = base + lane;
-        bool valid = false;
+        uint64_t row[4] = {0, 0, 0, 0};   // candidate's det/u/v pass bits over the sub-columns
         if (i < e1) {
             const ColRec r = load_colrec(rec, i);
-            valid = r.zmax >= k1;
             cr[lane] = r;
-        }
-        __builtin_amdgcn_wave_barrier();
-        const uint64_t vm = __ballot(valid);
-        // det/u/v per (candidate, sub-column): pass bits per sub-column slot
-        uint64_t pass[4] = {0, 0, 0, 0};
-        for (uint64_t m = vm; m; m &= m - 1) {
-            const int j = __ffsll((unsigned long long)m) - 1;
-            const ColRec r = cr[j];
+            if (r.zmax >= k1) {
 #pragma unroll
-            for (int mm = 0; mm < 4; ++mm) {
-                if (mm < nm && lane + 64 * mm < nsc) {
-                    float tx, ty;
-                    if (colrec_xy(r, ox[mm], oy[mm], tx, ty)) pass[mm] |= 1ull << j;
-                }
+                for (int mm = 0; mm < 4; ++mm)
+                    if (mm < nm) row[mm] = cpass[(uint64_t)i * stride + mm];
             }
         }
+        __builtin_amdgcn_wave_barrier();
+        // pass bits per sub-column slot: bit j = candidate j passes at this
+        // lane's sub-column
+        uint64_t pass[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm)
+            if (mm < nm) pass[mm] = wave_transpose64(row[mm]);
         // t per sub-voxel for the passing pairs only
 #pragma unroll
         for (int mm = 0; mm < 4; ++mm) {
@@ -1489,15 +1674,18 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
     for (int i = lane; i < w2; i += 64) out[i] = acc[i];
 }
 
-void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec, const GridDev& g0,
-                      const Meshes& M, uint64_t* par2, hipStream_t s)
+void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec,
+                      const uint64_t* cpass, int stride, const uint32_t* ord, const GridDev& g0, const Meshes& M,
+                      uint64_t* par2, hipStream_t s)
 {
     if (B <= 0) return;
     const int64_t blocks = (B + L2P_WAVES - 1) / L2P_WAVES;
     if (M.nm > 1)
-        k_l2_parity<true><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, g0, M, par2);
+        k_l2_parity<true><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, cpass, stride, ord, g0,
+                                                                      M, par2);
     else
-        k_l2_parity<false><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, g0, M, par2);
+        k_l2_parity<false><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, cpass, stride, ord, g0,
+                                                                       M, par2);
 }
 
 // ---------------------------------------------------------------------------
