@@ -141,14 +141,9 @@ void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint
 // det/u/v pass bits per (column entry, sub-column): stride words per entry
 void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M, int64_t ncol,
                      int stride, uint64_t* cpass, hipStream_t s);
-// boundary ranks grouped by xy column (the Level2 parity launch order)
-void launch_bcol_count(const int32_t* bidx, int64_t B, const GridDev& g0, const Meshes& M, uint32_t* cnt,
-                       hipStream_t s);
-void launch_bcol_scatter(const int32_t* bidx, int64_t B, const GridDev& g0, const Meshes& M, const uint32_t* off,
-                         uint32_t* cur, uint32_t* ord, hipStream_t s);
 void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec,
-                      const uint64_t* cpass, int stride, const uint32_t* ord, const GridDev& g0, const Meshes& M,
-                      uint64_t* par2, hipStream_t s);
+                      const uint64_t* cpass, int stride, const GridDev& g0, const Meshes& M, uint64_t* par2,
+                      hipStream_t s);
 
 // normals (src/Object.cpp:3219-3253 Level1, 2372-2421 + 2705-2720 Level2)
 void launch_face_normals(const float* tris, int64_t T, float* fn, hipStream_t s);

@@ -1446,7 +1446,11 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t v)
         const int s = 32 >> k;
         const uint64_t m = masks[k];
         const uint64_t x = shfl64(v, lane ^ s);
-        v = (lane & s) ? ((v & ~m) | ((x & ~m) >> s)) : ((v & m) | ((x & m) << s));
+        // branch-free select: the upper lane of each pair keeps the ~m half
+        const uint64_t sel = (uint64_t)0 - (uint64_t)((lane >> (5 - k)) & 1);
+        const uint64_t lo = (v & m) | ((x & m) << s);
+        const uint64_t hi = (v & ~m) | ((x & ~m) >> s);
+        v = (lo & ~sel) | (hi & sel);
     }
     return v;
 }
@@ -1527,56 +1531,6 @@ void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0,
         k_col_pass<false><<<(unsigned)blocks, 64 * CP_WAVES, 0, s>>>(coff, rec, g0, M, ncol, stride, cpass);
 }
 
-// Level2 parity launch order: boundary voxels grouped by xy column (counting
-// sort on the global column), so the waves of one column run close together
-// and read its entries and pass bits from cache.
-template <bool kM>
-__global__ void k_bcol_count(const int32_t* __restrict__ bidx, int64_t B, GridDev g0, Meshes M,
-                             uint32_t* __restrict__ cnt)
-{
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < B; b += (int64_t)gridDim.x * blockDim.x) {
-        const int mesh = kM ? (int)M.bmesh[b] : 0;
-        const GridDev& g = grid_of<kM>(g0, M, mesh);
-        const uint32_t gcol = (kM ? M.off[mesh].col0 : 0u) + (uint32_t)(bidx[b] % (g.nx * g.ny));
-        atomicAdd(&cnt[gcol], 1u);
-    }
-}
-
-template <bool kM>
-__global__ void k_bcol_scatter(const int32_t* __restrict__ bidx, int64_t B, GridDev g0, Meshes M,
-                               const uint32_t* __restrict__ off, uint32_t* __restrict__ cur,
-                               uint32_t* __restrict__ ord)
-{
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < B; b += (int64_t)gridDim.x * blockDim.x) {
-        const int mesh = kM ? (int)M.bmesh[b] : 0;
-        const GridDev& g = grid_of<kM>(g0, M, mesh);
-        const uint32_t gcol = (kM ? M.off[mesh].col0 : 0u) + (uint32_t)(bidx[b] % (g.nx * g.ny));
-        ord[off[gcol] + atomicAdd(&cur[gcol], 1u)] = (uint32_t)b;
-    }
-}
-
-void launch_bcol_count(const int32_t* bidx, int64_t B, const GridDev& g0, const Meshes& M, uint32_t* cnt,
-                       hipStream_t s)
-{
-    if (B <= 0) return;
-    const int blocks = grid_for(B, 256, 16384);
-    if (M.nm > 1)
-        k_bcol_count<true><<<blocks, 256, 0, s>>>(bidx, B, g0, M, cnt);
-    else
-        k_bcol_count<false><<<blocks, 256, 0, s>>>(bidx, B, g0, M, cnt);
-}
-
-void launch_bcol_scatter(const int32_t* bidx, int64_t B, const GridDev& g0, const Meshes& M, const uint32_t* off,
-                         uint32_t* cur, uint32_t* ord, hipStream_t s)
-{
-    if (B <= 0) return;
-    const int blocks = grid_for(B, 256, 16384);
-    if (M.nm > 1)
-        k_bcol_scatter<true><<<blocks, 256, 0, s>>>(bidx, B, g0, M, off, cur, ord);
-    else
-        k_bcol_scatter<false><<<blocks, 256, 0, s>>>(bidx, B, g0, M, off, cur, ord);
-}
-
 constexpr int L2P_WAVES = 4;
 
 template <bool kM>
@@ -1584,16 +1538,14 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
                                                               const uint32_t* __restrict__ coff,
                                                               const float4* __restrict__ rec,
                                                               const uint64_t* __restrict__ cpass, int stride,
-                                                              const uint32_t* __restrict__ ord, GridDev g0,
-                                                              Meshes M, uint64_t* __restrict__ par2)
+                                                              GridDev g0, Meshes M, uint64_t* __restrict__ par2)
 {
     __shared__ unsigned long long s_acc[L2P_WAVES][64];
     __shared__ ColRec s_rec[L2P_WAVES][64];
     const int wv = threadIdx.x >> 6;
     const int lane = __lane_id();
-    const int64_t slot = (int64_t)blockIdx.x * L2P_WAVES + wv;
-    if (slot >= B) return;
-    const int64_t b = (int64_t)(uint32_t)uniform_i((int)ord[slot]);
+    const int64_t b = (int64_t)blockIdx.x * L2P_WAVES + wv;
+    if (b >= B) return;
     const int mesh = kM ? uniform_i((int)M.bmesh[b]) : 0;
     const GridDev& g = grid_of<kM>(g0, M, mesh);
     const int n2 = g.n2, w2 = g.w2, gnx = g.nx, gny = g.ny;
@@ -1651,9 +1603,12 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
                 const int j = __ffsll((unsigned long long)m) - 1;
                 const ColRec r = cr[j];
                 const float tx = ox[mm] - r.v1x, ty = oy[mm] - r.v1y;
+                uint32_t hm = 0;   // bit rr: t > 1e-6 at layer rr
                 for (int rr = 0; rr < n2; ++rr)
-                    if (mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z, lane_f(ozl, rr)))
-                        bits ^= 1ull << (mm * n2 + rr);
+                    hm |= (uint32_t)mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z,
+                                             lane_f(ozl, rr))
+                          << rr;
+                bits ^= (uint64_t)hm << (mm * n2);
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -1675,17 +1630,17 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
 }
 
 void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec,
-                      const uint64_t* cpass, int stride, const uint32_t* ord, const GridDev& g0, const Meshes& M,
-                      uint64_t* par2, hipStream_t s)
+                      const uint64_t* cpass, int stride, const GridDev& g0, const Meshes& M, uint64_t* par2,
+                      hipStream_t s)
 {
     if (B <= 0) return;
     const int64_t blocks = (B + L2P_WAVES - 1) / L2P_WAVES;
     if (M.nm > 1)
-        k_l2_parity<true><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, cpass, stride, ord, g0,
-                                                                      M, par2);
+        k_l2_parity<true><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M,
+                                                                      par2);
     else
-        k_l2_parity<false><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, cpass, stride, ord, g0,
-                                                                       M, par2);
+        k_l2_parity<false><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M,
+                                                                       par2);
 }
 
 // ---------------------------------------------------------------------------

@@ -760,22 +760,8 @@ int run_l2_parity(Job* J)
     release(J, J->cpass);
     TRY(alloc(J, &J->cpass, (size_t)std::max<uint64_t>(J->n_cols, 1) * stride));
     launch_col_pass(J->coff, J->crec, J->g[0], J->view(), J->ncol, stride, J->cpass, s);
-    // launch order: boundary voxels grouped by column
-    uint32_t *bcnt = nullptr, *bcoff = nullptr, *ord = nullptr;
-    TRY(alloc(J, &bcnt, (size_t)J->ncol));
-    TRY(alloc(J, &bcoff, (size_t)J->ncol + 1));
-    TRY(alloc(J, &ord, (size_t)std::max<int64_t>(J->B, 1)));
-    launch_fill_u32(bcnt, 0, J->ncol, s);
-    launch_bcol_count(J->bidx, J->B, J->g[0], J->view(), bcnt, s);
-    TRY(ensure_scan_tmp(J, J->ncol));
-    launch_scan_u32_u32(bcnt, bcoff, J->ncol, J->scan32, nullptr, s);
-    launch_fill_u32(bcnt, 0, J->ncol, s);
-    launch_bcol_scatter(J->bidx, J->B, J->g[0], J->view(), bcoff, bcnt, ord, s);
-    launch_l2_parity(J->bidx, J->B, J->coff, J->crec, J->cpass, stride, ord, J->g[0], J->view(), J->par2, s);
+    launch_l2_parity(J->bidx, J->B, J->coff, J->crec, J->cpass, stride, J->g[0], J->view(), J->par2, s);
     release(J, J->cpass);
-    release(J, bcnt);
-    release(J, bcoff);
-    release(J, ord);
     HIP_TRY(hipGetLastError());
     J->l2p_done = true;
     return GPV_OK;
