@@ -611,7 +611,12 @@ __device__ __forceinline__ void expand_items(const float* __restrict__ tris, int
     const uint64_t total = off[T];
     const int tid = threadIdx.x;
     const uint64_t per = ((total + gridDim.x - 1) / gridDim.x + XB - 1) / XB * XB;
-    const uint64_t b0 = (uint64_t)blockIdx.x * per;
+    // workgroups are dealt to the 8 XCDs round-robin and each XCD has its own
+    // L2: give the blocks of one XCD one contiguous run of ranges, so the
+    // atomics on neighbouring bitplane words mostly stay in one L2
+    const uint32_t G = gridDim.x;
+    const uint32_t lb = (G % 8u == 0u) ? (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u : blockIdx.x;
+    const uint64_t b0 = (uint64_t)lb * per;
     const uint64_t b1 = b0 + per < total ? b0 + per : total;
     int64_t lo_hint = 0;
     for (uint64_t base = b0; base < b1; base += XB) {
