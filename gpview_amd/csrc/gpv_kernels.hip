@@ -1538,81 +1538,79 @@ void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0,
 
 constexpr int L2P_WAVES = 4;
 
-template <bool kM>
-__global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __restrict__ bidx, int64_t B,
-                                                              const uint32_t* __restrict__ coff,
-                                                              const float4* __restrict__ rec,
-                                                              const uint64_t* __restrict__ cpass, int stride,
-                                                              GridDev g0, Meshes M, uint64_t* __restrict__ par2)
+// One boundary voxel (i1, j1, k1) of the Level2 parity.  N2 > 0: the Level2
+// resolution as a compile-time constant (layer loops unrolled, the layer
+// origins held uniform); N2 == 0: any resolution n2 <= 16.
+template <int N2>
+__device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev& g, int i1, int j1, int k1,
+                                                uint32_t e0, uint32_t e1, const float4* __restrict__ rec,
+                                                const uint64_t* __restrict__ cpass, int stride, ColRec* cr,
+                                                unsigned long long* acc, uint64_t* __restrict__ out)
 {
-    __shared__ unsigned long long s_acc[L2P_WAVES][64];
-    __shared__ ColRec s_rec[L2P_WAVES][64];
-    const int wv = threadIdx.x >> 6;
     const int lane = __lane_id();
-    const int64_t b = (int64_t)blockIdx.x * L2P_WAVES + wv;
-    if (b >= B) return;
-    const int mesh = kM ? uniform_i((int)M.bmesh[b]) : 0;
-    const GridDev& g = grid_of<kM>(g0, M, mesh);
-    const int n2 = g.n2, w2 = g.w2, gnx = g.nx, gny = g.ny;
-    if (kM && n2 == 0) return;
-    uint64_t* out = par2 + (kM ? M.off[mesh].l2w0 + (uint64_t)(b - M.off[mesh].b0) * w2 : (uint64_t)b * w2);
-    const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
-    unsigned long long* acc = s_acc[wv];
-    ColRec* cr = s_rec[wv];
-    const int v = bidx[b];
-    const int nxy = gnx * gny;
-    const int k1 = v / nxy;
-    const int ij = v - k1 * nxy;
-    const int j1 = ij / gnx;
-    const int i1 = ij - j1 * gnx;
+    const int n2 = N2 > 0 ? N2 : n2_rt;
+    constexpr int NMX = N2 > 0 ? (N2 * N2 + 63) / 64 : 4;   // sub-column slots per lane
     const int nsc = n2 * n2;
-    const int nm = (nsc + 63) >> 6;
+    const int nm = N2 > 0 ? NMX : (nsc + 63) >> 6;
     const float* rx = g.r2x + (int64_t)i1 * n2;
     const float* ry = g.r2y + (int64_t)j1 * n2;
     const float* rz = g.r2z + (int64_t)k1 * n2;
     // this lane's sub-columns sc = lane + 64*mm: ray origins
-    float ox[4], oy[4];
+    float ox[NMX], oy[NMX];
 #pragma unroll
-    for (int mm = 0; mm < 4; ++mm) {
+    for (int mm = 0; mm < NMX; ++mm) {
         const int sc = lane + 64 * mm;
         ox[mm] = (mm < nm && sc < nsc) ? rx[sc % n2] : 0.0f;
         oy[mm] = (mm < nm && sc < nsc) ? ry[sc / n2] : 0.0f;
     }
-    // layer origins of the sub-voxels: entry r in lane r
+    // layer origins of the sub-voxels: entry r in lane r (and, for a fixed
+    // resolution, in uniform registers)
     const float ozl = lane < n2 ? rz[lane] : 0.0f;
+    float oz[N2 > 0 ? N2 : 1];
+#pragma unroll
+    for (int rr = 0; rr < (N2 > 0 ? N2 : 1); ++rr) oz[rr] = lane_f(ozl, rr);
     uint64_t bits = 0;   // bit (mm*n2 + r) for sub-column lane + 64*mm
-    const uint32_t e0 = coff[col0 + ij], e1 = coff[col0 + ij + 1];
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t i = base + lane;
-        uint64_t row[4] = {0, 0, 0, 0};   // candidate's det/u/v pass bits over the sub-columns
+        const int k1c = k1;
+        uint64_t row[NMX];   // candidate's det/u/v pass bits over the sub-columns
+#pragma unroll
+        for (int mm = 0; mm < NMX; ++mm) row[mm] = 0;
         if (i < e1) {
             const ColRec r = load_colrec(rec, i);
             cr[lane] = r;
-            if (r.zmax >= k1) {
+            if (r.zmax >= k1c) {
 #pragma unroll
-                for (int mm = 0; mm < 4; ++mm)
+                for (int mm = 0; mm < NMX; ++mm)
                     if (mm < nm) row[mm] = cpass[(uint64_t)i * stride + mm];
             }
         }
         __builtin_amdgcn_wave_barrier();
         // pass bits per sub-column slot: bit j = candidate j passes at this
         // lane's sub-column
-        uint64_t pass[4] = {0, 0, 0, 0};
+        uint64_t pass[NMX];
 #pragma unroll
-        for (int mm = 0; mm < 4; ++mm)
-            if (mm < nm) pass[mm] = wave_transpose64(row[mm]);
+        for (int mm = 0; mm < NMX; ++mm) pass[mm] = mm < nm ? wave_transpose64(row[mm]) : 0;
         // t per sub-voxel for the passing pairs only
 #pragma unroll
-        for (int mm = 0; mm < 4; ++mm) {
+        for (int mm = 0; mm < NMX; ++mm) {
             for (uint64_t m = pass[mm]; m; m &= m - 1) {
                 const int j = __ffsll((unsigned long long)m) - 1;
                 const ColRec r = cr[j];
                 const float tx = ox[mm] - r.v1x, ty = oy[mm] - r.v1y;
                 uint32_t hm = 0;   // bit rr: t > 1e-6 at layer rr
-                for (int rr = 0; rr < n2; ++rr)
-                    hm |= (uint32_t)mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z,
-                                             lane_f(ozl, rr))
-                          << rr;
+                if constexpr (N2 > 0) {
+#pragma unroll
+                    for (int rr = 0; rr < N2; ++rr)
+                        hm |= (uint32_t)mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z,
+                                                 oz[rr])
+                              << rr;
+                } else {
+                    for (int rr = 0; rr < n2; ++rr)
+                        hm |= (uint32_t)mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z,
+                                                 lane_f(ozl, rr))
+                              << rr;
+                }
                 bits ^= (uint64_t)hm << (mm * n2);
             }
         }
@@ -1634,18 +1632,52 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
     for (int i = lane; i < w2; i += 64) out[i] = acc[i];
 }
 
+// kM: batch (resolution per mesh, N2 = 0); single mesh: N2 = its resolution
+// when it is 4 or 8 (a kernel of its own, so each keeps its own register
+// budget), else 0
+template <bool kM, int N2>
+__global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __restrict__ bidx, int64_t B,
+                                                              const uint32_t* __restrict__ coff,
+                                                              const float4* __restrict__ rec,
+                                                              const uint64_t* __restrict__ cpass, int stride,
+                                                              GridDev g0, Meshes M, uint64_t* __restrict__ par2)
+{
+    __shared__ unsigned long long s_acc[L2P_WAVES][64];
+    __shared__ ColRec s_rec[L2P_WAVES][64];
+    const int wv = threadIdx.x >> 6;
+    const int64_t b = (int64_t)blockIdx.x * L2P_WAVES + wv;
+    if (b >= B) return;
+    const int mesh = kM ? uniform_i((int)M.bmesh[b]) : 0;
+    const GridDev& g = grid_of<kM>(g0, M, mesh);
+    const int n2 = g.n2, w2 = g.w2, gnx = g.nx, gny = g.ny;
+    if (kM && n2 == 0) return;
+    uint64_t* out = par2 + (kM ? M.off[mesh].l2w0 + (uint64_t)(b - M.off[mesh].b0) * w2 : (uint64_t)b * w2);
+    const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
+    const int v = bidx[b];
+    const int nxy = gnx * gny;
+    const int k1 = v / nxy;
+    const int ij = v - k1 * nxy;
+    const int j1 = ij / gnx;
+    const int i1 = ij - j1 * gnx;
+    const uint32_t e0 = coff[col0 + ij], e1 = coff[col0 + ij + 1];
+    l2_parity_voxel<N2>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
+}
+
 void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec,
                       const uint64_t* cpass, int stride, const GridDev& g0, const Meshes& M, uint64_t* par2,
                       hipStream_t s)
 {
     if (B <= 0) return;
-    const int64_t blocks = (B + L2P_WAVES - 1) / L2P_WAVES;
+    const unsigned blocks = (unsigned)((B + L2P_WAVES - 1) / L2P_WAVES);
+    const dim3 bs(64 * L2P_WAVES);
     if (M.nm > 1)
-        k_l2_parity<true><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M,
-                                                                      par2);
+        k_l2_parity<true, 0><<<blocks, bs, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M, par2);
+    else if (g0.n2 == 8)
+        k_l2_parity<false, 8><<<blocks, bs, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M, par2);
+    else if (g0.n2 == 4)
+        k_l2_parity<false, 4><<<blocks, bs, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M, par2);
     else
-        k_l2_parity<false><<<(unsigned)blocks, 64 * L2P_WAVES, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M,
-                                                                       par2);
+        k_l2_parity<false, 0><<<blocks, bs, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M, par2);
 }
 
 // ---------------------------------------------------------------------------
