@@ -36,9 +36,29 @@ __device__ __forceinline__ bool sat_separated(float pa, float pb, float rad)
     return (mn > rad) | (mx < -rad);
 }
 
+// The three edge-axis predicates along e x z (the third AXISTEST of each
+// edge) read only the x and y components of v_i = tri_i - center and of the
+// edges, which depend on (cx, cy) alone: they hold or fail for a whole column
+// of boxes.  Same float expressions as in tri_box_overlap.
+__device__ __forceinline__ bool xy_edges_ok(float cx, float cy, float hx, float hy, const float* __restrict__ tv)
+{
+    const float v0x = tv[0] - cx, v0y = tv[1] - cy;
+    const float v1x = tv[3] - cx, v1y = tv[4] - cy;
+    const float v2x = tv[6] - cx, v2y = tv[7] - cy;
+    const float e0x = v1x - v0x, e0y = v1y - v0y;
+    const float e1x = v2x - v1x, e1y = v2y - v1y;
+    const float e2x = v0x - v2x, e2y = v0y - v2y;
+    if (sat_separated(e0y * v1x - e0x * v1y, e0y * v2x - e0x * v2y, fabsf(e0y) * hx + fabsf(e0x) * hy)) return false;
+    if (sat_separated(e1y * v0x - e1x * v0y, e1y * v1x - e1x * v1y, fabsf(e1y) * hx + fabsf(e1x) * hy)) return false;
+    if (sat_separated(e2y * v1x - e2x * v1y, e2y * v2x - e2x * v2y, fabsf(e2y) * hx + fabsf(e2x) * hy)) return false;
+    return true;
+}
+
 // kAxes = false skips the three box-axis predicates; only for callers that
 // have already established them with sat_axis_ok on the same center.
-template <bool kAxes = true>
+// kZEdges = false skips the three e x z edge predicates; only for callers that
+// have established them with xy_edges_ok on the same (cx, cy).
+template <bool kAxes = true, bool kZEdges = true>
 __device__ __forceinline__ bool tri_box_overlap(float cx, float cy, float cz,
                                                 float hx, float hy, float hz,
                                                 const float* __restrict__ tv)
@@ -94,15 +114,15 @@ __device__ __forceinline__ bool tri_box_overlap(float cx, float cy, float cz,
     float fx = fabsf(e0x), fy = fabsf(e0y), fz = fabsf(e0z);
     if (sat_separated(e0z * v0y - e0y * v0z, e0z * v2y - e0y * v2z, fz * hy + fy * hz)) return false;
     if (sat_separated(-e0z * v0x + e0x * v0z, -e0z * v2x + e0x * v2z, fz * hx + fx * hz)) return false;
-    if (sat_separated(e0y * v1x - e0x * v1y, e0y * v2x - e0x * v2y, fy * hx + fx * hy)) return false;
+    if (kZEdges && sat_separated(e0y * v1x - e0x * v1y, e0y * v2x - e0x * v2y, fy * hx + fx * hy)) return false;
     fx = fabsf(e1x); fy = fabsf(e1y); fz = fabsf(e1z);
     if (sat_separated(e1z * v0y - e1y * v0z, e1z * v2y - e1y * v2z, fz * hy + fy * hz)) return false;
     if (sat_separated(-e1z * v0x + e1x * v0z, -e1z * v2x + e1x * v2z, fz * hx + fx * hz)) return false;
-    if (sat_separated(e1y * v0x - e1x * v0y, e1y * v1x - e1x * v1y, fy * hx + fx * hy)) return false;
+    if (kZEdges && sat_separated(e1y * v0x - e1x * v0y, e1y * v1x - e1x * v1y, fy * hx + fx * hy)) return false;
     fx = fabsf(e2x); fy = fabsf(e2y); fz = fabsf(e2z);
     if (sat_separated(e2z * v0y - e2y * v0z, e2z * v1y - e2y * v1z, fz * hy + fy * hz)) return false;
     if (sat_separated(-e2z * v0x + e2x * v0z, -e2z * v1x + e2x * v1z, fz * hx + fx * hz)) return false;
-    if (sat_separated(e2y * v1x - e2x * v1y, e2y * v2x - e2x * v2y, fy * hx + fx * hy)) return false;
+    if (kZEdges && sat_separated(e2y * v1x - e2x * v1y, e2y * v2x - e2x * v2y, fy * hx + fx * hy)) return false;
     return true;
 }
 

@@ -1210,8 +1210,9 @@ __device__ __forceinline__ void l2_sat_run(uint32_t ent, bool valid, const float
     float tv[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) tv[c] = stv[j * 9 + c];
-    const bool hit = box ? tri_box_overlap<false>(ccx, ccy, ccz, h2x, h2y, h2z, tv)
-                         : tri_box_overlap<true>(ccx, ccy, ccz, h2x, h2y, h2z, tv);
+    // every queued entry's column passed xy_edges_ok
+    const bool hit = box ? tri_box_overlap<false, false>(ccx, ccy, ccz, h2x, h2y, h2z, tv)
+                         : tri_box_overlap<true, false>(ccx, ccy, ccz, h2x, h2y, h2z, tv);
     if (hit) atomicOr(&acc[s >> 6], 1ull << (s & 63));
 }
 
@@ -1345,6 +1346,8 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
             }
             // every lane takes part in the cross-lane reads
             const float ccx = __shfl(cxl, p), ccy = __shfl(cyl, q);
+            // the e x z edge predicates hold or fail for the whole sub-column
+            if (in) in = xy_edges_ok(ccx, ccy, h2x, h2y, stv + j * 9);
             if (in) {
                 const float4 sl = ssl[j];
                 const float t0x = stv[j * 9 + 0], t0y = stv[j * 9 + 1], t0z = stv[j * 9 + 2];
