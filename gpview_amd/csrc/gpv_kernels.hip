@@ -712,9 +712,11 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
             const double4 sl = slab1[t];
             slab_layers(sl.x, sl.y, sl.z, sl.w, cx, cy, tv, c1z, b0.z, b0.z + b1.z, za, zb);
         }
+        // the e x z edge predicates hold or fail for the whole column
+        if (!xy_edges_ok(cx, cy, hx, hy, tv)) zb = za;
         int zmax = -1;
         for (int z = za; z < zb; ++z) {
-            const bool hit = tri_box_overlap(cx, cy, c1z[z], hx, hy, hz, tv);
+            const bool hit = tri_box_overlap<true, false>(cx, cy, c1z[z], hx, hy, hz, tv);
             const uint64_t w = (uint64_t)z * plane + rowword;
             if (hit) {
                 zmax = z;
