@@ -112,8 +112,20 @@ void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const 
 void launch_count_keys_hi(const uint64_t* key, uint64_t n, uint32_t* cnt, hipStream_t s);
 void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n, const GridDev& g0, const Meshes& M,
                          const uint32_t* off, uint32_t* cnt, float4* rec, hipStream_t s);
-void launch_scatter_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n,
-                         const uint32_t* off, uint32_t* cnt, float4* rec, hipStream_t s);
+// scratch of a two-level counting sort (ts_tiles(nkeys) tiles): total and
+// fill counters [2*tiles + 1], tile offsets [tiles + 1], scan temporary
+// [scan_tmp_elems(tiles)], bucket [items]
+struct TileSortTmp {
+    uint32_t* total;
+    uint32_t* off;
+    uint32_t* scan;
+    uint2* bucket;
+};
+size_t ts_tiles(uint64_t nkeys);
+// column entries (key = column << 32 | triangle) -> CSR by column with the MT
+// records; ncol <= 2^27
+void launch_bin_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n, uint32_t ncol,
+                     const TileSortTmp& tmp, uint32_t* coff, float4* rec, hipStream_t s);
 
 // Level1 parity tasks: L1P_Z consecutive layers of one 64-column row strip
 constexpr int L1P_Z = 16;
@@ -127,11 +139,12 @@ void launch_mesh_bounds(const uint32_t* wprefix, const Meshes& M, hipStream_t s)
 void launch_boundary_index(const uint64_t* bnd, const uint32_t* wprefix, const GridDev& g0, const Meshes& M,
                            int64_t W, int32_t* bidx, hipStream_t s);
 // pairs: key = batch-wide boundary bit index, val = triangle
-void launch_pair_rank(const uint64_t* bits, uint64_t n, const uint64_t* bnd, const uint32_t* wprefix, uint32_t* pb,
-                      uint32_t* cntb, hipStream_t s);
-void launch_pair_scatter(const uint32_t* tri, const uint32_t* pb, uint64_t n, const uint32_t* boff, uint32_t* cntb,
-                         uint32_t* list, const float* tris, const int32_t* bidx, const GridDev& g0, const Meshes& M,
-                         uint64_t* pmask, hipStream_t s);
+// bin pairs -> CSR by boundary rank (boff[B + 1], list, Level2 box-axis
+// masks); B <= 2^27
+void launch_bin_pairs(const uint64_t* bits, const uint32_t* tri, uint64_t n, const uint64_t* bnd,
+                      const uint32_t* wprefix, uint32_t B, const TileSortTmp& tmp, uint32_t* boff,
+                      unsigned int* max_out, uint32_t* list, const float* tris, const int32_t* bidx,
+                      const GridDev& g0, const Meshes& M, uint64_t* pmask, hipStream_t s);
 // per-mesh bin totals and largest bin (multi-mesh runs)
 void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStream_t s);
 

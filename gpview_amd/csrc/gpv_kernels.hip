@@ -6,9 +6,9 @@
 //     cuda/CUDAClassifyTessellation.cu:320-401 ...... k_sat_expand
 //   Level1 ray parity, src/Object.cpp:716-779 ....... k_ray_expand, k_scatter_hits, k_l1_parity
 //   compaction + prefix, src/Object.cpp:3270-3279 ... k_popc_words, scan, k_boundary_index
-//   bins by boundary voxel (the CSR of :2137-2219) .. k_pair_rank, scan, k_pair_scatter
+//   bins by boundary voxel (the CSR of :2137-2219) .. k_ts_count, k_ts_bucket, k_ts_place
 //   Level2 SAT, src/Object.cpp:2344-2424 ............ k_l2_sat
-//   Level2 ray parity, src/Object.cpp:1111-1207 ..... k_scatter_cols, k_l2_parity
+//   Level2 ray parity, src/Object.cpp:1111-1207 ..... k_ts_* (column entries), k_col_pass, k_l2_parity
 //   counts, src/Object.cpp:3353-3378 ................ k_count_states
 //   .raw layout, src/Object.cpp:2934-3075 ........... k_export_*
 //
@@ -850,16 +850,19 @@ void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n, con
 }
 
 // column entries -> CSR by column, each with the MT record its candidates need
-__global__ void k_scatter_cols(const float* __restrict__ tris, const uint64_t* __restrict__ key,
-                               const uint32_t* __restrict__ zmax, uint64_t n, const uint32_t* __restrict__ off,
-                               uint32_t* __restrict__ cnt, float4* __restrict__ rec)
-{
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = key[i];
-        const uint32_t col = (uint32_t)(k >> 32);
-        const uint32_t t = (uint32_t)(k & 0xffffffffu);
-        const uint32_t pos = off[col] + atomicSub(&cnt[col], 1u) - 1u;
+struct ColKey {
+    const uint64_t* key;
+    __device__ uint32_t operator()(uint64_t i) const { return (uint32_t)(key[i] >> 32); }
+};
+
+struct ColPlace {
+    const float* tris;
+    const uint64_t* key;
+    const uint32_t* zmax;
+    float4* rec;
+    __device__ void operator()(uint32_t pos, uint32_t, uint32_t i) const
+    {
+        const uint32_t t = (uint32_t)(key[i] & 0xffffffffu);
         float tv[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) tv[c] = tris[(uint64_t)t * 9 + c];
@@ -869,14 +872,7 @@ __global__ void k_scatter_cols(const float* __restrict__ tris, const uint64_t* _
         rec[3 * (uint64_t)pos + 1] = make_float4(m.e2y, m.e2z, m.inv_det, tv[0]);
         rec[3 * (uint64_t)pos + 2] = make_float4(tv[1], tv[2], __int_as_float((int)zmax[i]), __uint_as_float(t));
     }
-}
-
-void launch_scatter_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n,
-                         const uint32_t* off, uint32_t* cnt, float4* rec, hipStream_t s)
-{
-    if (!n) return;
-    k_scatter_cols<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(tris, key, zmax, n, off, cnt, rec);
-}
+};
 
 // ---------------------------------------------------------------------------
 // Level1 parity: one wave per task = L1P_Z consecutive layers of a 64-column
@@ -1052,46 +1048,181 @@ __device__ __forceinline__ uint32_t rank_of(const uint64_t* __restrict__ bnd,
     return wprefix[w] + (uint32_t)__popcll(bnd[w] & below);
 }
 
-__global__ void k_pair_rank(const uint64_t* __restrict__ bits, uint64_t n, const uint64_t* __restrict__ bnd,
-                            const uint32_t* __restrict__ wprefix, uint32_t* __restrict__ pb,
-                            uint32_t* __restrict__ cntb)
+// ---------------------------------------------------------------------------
+// Counting sort of n items by a key in [0, nkeys) in two levels: tiles of
+// 2^shift consecutive keys.  Pass 1 (count) and pass 2 (bucket) walk the same
+// contiguous item range per block and histogram the tile ids in LDS, so the
+// global counters see one atomic per (block, tile) instead of one per item;
+// pass 3 (one block per tile) counts its keys in LDS, writes their CSR
+// offsets and places every item at its final position.  The order inside one
+// key is unspecified (as with any atomic placement).
+// ---------------------------------------------------------------------------
+constexpr int TS_NT = 256;
+
+__device__ __forceinline__ void ts_range(uint64_t n, uint64_t& lo, uint64_t& hi)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t b = rank_of(bnd, wprefix, bits[i]);
-        pb[i] = b;
-        atomicAdd(&cntb[b], 1u);
+    const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+    lo = (uint64_t)blockIdx.x * per;
+    hi = lo + per < n ? lo + per : n;
+}
+
+template <typename Key>
+__global__ __launch_bounds__(TS_NT) void k_ts_count(Key key, uint64_t n, int shift, int ntiles,
+                                                    uint32_t* __restrict__ tile_total)
+{
+    extern __shared__ uint32_t s_h[];   // [ntiles]
+    for (int t = threadIdx.x; t < ntiles; t += TS_NT) s_h[t] = 0;
+    __syncthreads();
+    uint64_t lo, hi;
+    ts_range(n, lo, hi);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) atomicAdd(&s_h[key(i) >> shift], 1u);
+    __syncthreads();
+    for (int t = threadIdx.x; t < ntiles; t += TS_NT)
+        if (s_h[t]) atomicAdd(&tile_total[t], s_h[t]);
+}
+
+template <typename Key>
+__global__ __launch_bounds__(TS_NT) void k_ts_bucket(Key key, uint64_t n, int shift, int ntiles,
+                                                     const uint32_t* __restrict__ tile_off,
+                                                     uint32_t* __restrict__ tile_fill, uint2* __restrict__ bucket)
+{
+    extern __shared__ uint32_t s_h[];   // [ntiles]: counts, then this block's next slot per tile
+    for (int t = threadIdx.x; t < ntiles; t += TS_NT) s_h[t] = 0;
+    __syncthreads();
+    uint64_t lo, hi;
+    ts_range(n, lo, hi);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) atomicAdd(&s_h[key(i) >> shift], 1u);
+    __syncthreads();
+    for (int t = threadIdx.x; t < ntiles; t += TS_NT)
+        if (s_h[t]) s_h[t] = tile_off[t] + atomicAdd(&tile_fill[t], s_h[t]);
+    __syncthreads();
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) {
+        const uint32_t k = key(i);
+        bucket[atomicAdd(&s_h[k >> shift], 1u)] = make_uint2(k, (uint32_t)i);
     }
 }
 
-void launch_pair_rank(const uint64_t* bits, uint64_t n, const uint64_t* bnd, const uint32_t* wprefix, uint32_t* pb,
-                      uint32_t* cntb, hipStream_t s)
+template <typename Place>
+__global__ __launch_bounds__(TS_NT) void k_ts_place(const uint2* __restrict__ bucket,
+                                                    const uint32_t* __restrict__ tile_off, int shift,
+                                                    uint32_t nkeys, uint32_t* __restrict__ off,
+                                                    unsigned int* __restrict__ max_out, Place place)
 {
-    if (!n) return;
-    k_pair_rank<<<grid_for((int64_t)n, 256, 16384), 256, 0, s>>>(bits, n, bnd, wprefix, pb, cntb);
+    extern __shared__ uint32_t s_c[];   // [2^shift] key counts, then next slots
+    __shared__ uint32_t s_wave[TS_NT / 64];
+    __shared__ uint32_t s_max;
+    const uint32_t T = 1u << shift;
+    const uint32_t k0 = (uint32_t)blockIdx.x << shift;
+    const uint32_t b0 = tile_off[blockIdx.x], b1 = tile_off[blockIdx.x + 1];
+    for (uint32_t j = threadIdx.x; j < T; j += TS_NT) s_c[j] = 0;
+    if (threadIdx.x == 0) s_max = 0;
+    __syncthreads();
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += TS_NT) atomicAdd(&s_c[bucket[i].x - k0], 1u);
+    __syncthreads();
+    // exclusive scan of the T counts: thread r owns counts [r*C, (r+1)*C)
+    const uint32_t C = T / TS_NT;
+    uint32_t sum = 0, mx = 0;
+    for (uint32_t c = 0; c < C; ++c) {
+        const uint32_t v = s_c[threadIdx.x * C + c];
+        sum += v;
+        mx = v > mx ? v : mx;
+    }
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan<uint32_t>(sum);
+    if (lane == 63) s_wave[wid] = inc;
+    for (unsigned d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(mx, d);
+        mx = o > mx ? o : mx;
+    }
+    if (lane == 0 && mx) atomicMax(&s_max, mx);
+    __syncthreads();
+    uint32_t run = b0 + inc - sum;
+    for (int w = 0; w < wid; ++w) run += s_wave[w];
+    for (uint32_t c = 0; c < C; ++c) {
+        const uint32_t j = threadIdx.x * C + c;
+        const uint32_t v = s_c[j];
+        s_c[j] = run;
+        if (k0 + j < nkeys) off[k0 + j] = run;
+        run += v;
+    }
+    if (k0 + T >= nkeys && threadIdx.x == TS_NT - 1) off[nkeys] = b1;
+    if (threadIdx.x == 0 && max_out && s_max) atomicMax(max_out, s_max);
+    __syncthreads();
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += TS_NT) {
+        const uint2 e = bucket[i];
+        place(atomicAdd(&s_c[e.x - k0], 1u), e.x, e.y);
+    }
 }
 
-// Bins by boundary voxel, and for each bin entry the Level2 box-axis masks:
-// bit p of mask x says the SAT's x box-axis predicate holds for the triangle
-// and the Level2 sub-index p of that voxel (the same float test the full SAT
-// runs), likewise y, z; packed x | y << 16 | z << 32.
-template <bool kM>
-__global__ void k_pair_scatter(const uint32_t* __restrict__ tri, const uint32_t* __restrict__ pb, uint64_t n,
-                               const uint32_t* __restrict__ boff, uint32_t* __restrict__ cntb,
-                               uint32_t* __restrict__ list, const float* __restrict__ tris,
-                               const int32_t* __restrict__ bidx, GridDev g0, Meshes M,
-                               uint64_t* __restrict__ pmask)
+// tile size 2^shift (>= TS_NT) so that at most 16384 tiles cover nkeys keys;
+// nkeys <= 2^27 keeps a tile's counters within 32 KiB of LDS
+int ts_shift(uint64_t nkeys)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t b = pb[i];
+    int lg = 0;
+    while ((1ull << lg) < nkeys) ++lg;
+    return lg - 14 > 8 ? lg - 14 : 8;
+}
+
+template <typename Key, typename Place>
+static void ts_sort(Key key, uint64_t n, uint32_t nkeys, const TileSortTmp& tmp, uint32_t* off,
+                    unsigned int* max_out, Place place, hipStream_t s)
+{
+    const int shift = ts_shift(nkeys);
+    const int ntiles = (int)((nkeys + (1u << shift) - 1) >> shift);
+    if (ntiles == 0) {   // no keys, hence no items
+        launch_fill_u32(off, 0, 1, s);
+        return;
+    }
+    launch_fill_u32(tmp.total, 0, 2 * (int64_t)ntiles + 1, s);   // total[ntiles], fill[ntiles + 1]
+    uint32_t* fill = tmp.total + ntiles;
+    if (n > 0) {
+        const int blocks = (int)std::min<uint64_t>(2048, std::max<uint64_t>(1, (n + 4095) / 4096));
+        const size_t lds = (size_t)ntiles * 4;
+        k_ts_count<Key><<<blocks, TS_NT, lds, s>>>(key, n, shift, ntiles, tmp.total);
+        launch_scan_u32_u32(tmp.total, tmp.off, ntiles, tmp.scan, nullptr, s);
+        k_ts_bucket<Key><<<blocks, TS_NT, lds, s>>>(key, n, shift, ntiles, tmp.off, fill, tmp.bucket);
+    } else {
+        launch_fill_u32(tmp.off, 0, (int64_t)ntiles + 1, s);
+    }
+    k_ts_place<Place><<<ntiles, TS_NT, (size_t)4 << shift, s>>>(tmp.bucket, tmp.off, shift, nkeys, off, max_out,
+                                                              place);
+}
+
+size_t ts_tiles(uint64_t nkeys)
+{
+    const int shift = ts_shift(nkeys);
+    return (size_t)((nkeys + (1u << shift) - 1) >> shift);
+}
+
+// key of a bin pair: the rank of its voxel among the boundary voxels
+struct PairRankKey {
+    const uint64_t* bits;
+    const uint64_t* bnd;
+    const uint32_t* wprefix;
+    __device__ uint32_t operator()(uint64_t i) const { return rank_of(bnd, wprefix, bits[i]); }
+};
+
+// Bin entry at pos: the triangle, and the Level2 box-axis masks: bit p of
+// mask x says the SAT's x box-axis predicate holds for the triangle and the
+// Level2 sub-index p of that voxel (the same float test the full SAT runs),
+// likewise y, z; packed x | y << 16 | z << 32.
+template <bool kM>
+struct PairPlace {
+    const uint32_t* tri;
+    uint32_t* list;
+    const float* tris;
+    const int32_t* bidx;
+    GridDev g0;
+    Meshes M;
+    uint64_t* pmask;
+    __device__ void operator()(uint32_t pos, uint32_t b, uint32_t i) const
+    {
         const uint32_t t = tri[i];
-        const uint32_t pos = boff[b] + atomicSub(&cntb[b], 1u) - 1u;
         list[pos] = t;
         const int mesh = kM ? (int)M.bmesh[b] : 0;
         const GridDev& g = grid_of<kM>(g0, M, mesh);
         const int n2 = g.n2;
-        if (!pmask || n2 == 0) continue;
+        if (!pmask || n2 == 0) return;
         const int v = bidx[b];
         const int nxy = g.nx * g.ny;
         const int k1 = v / nxy;
@@ -1112,18 +1243,24 @@ __global__ void k_pair_scatter(const uint32_t* __restrict__ tri, const uint32_t*
         }
         pmask[pos] = mx | (my << 16) | (mz << 32);
     }
+};
+
+void launch_bin_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n, uint32_t ncol,
+                     const TileSortTmp& tmp, uint32_t* coff, float4* rec, hipStream_t s)
+{
+    ts_sort(ColKey{key}, n, ncol, tmp, coff, nullptr, ColPlace{tris, key, zmax, rec}, s);
 }
 
-void launch_pair_scatter(const uint32_t* tri, const uint32_t* pb, uint64_t n, const uint32_t* boff, uint32_t* cntb,
-                         uint32_t* list, const float* tris, const int32_t* bidx, const GridDev& g0, const Meshes& M,
-                         uint64_t* pmask, hipStream_t s)
+void launch_bin_pairs(const uint64_t* bits, const uint32_t* tri, uint64_t n, const uint64_t* bnd,
+                      const uint32_t* wprefix, uint32_t B, const TileSortTmp& tmp, uint32_t* boff,
+                      unsigned int* max_out, uint32_t* list, const float* tris, const int32_t* bidx,
+                      const GridDev& g0, const Meshes& M, uint64_t* pmask, hipStream_t s)
 {
-    if (!n) return;
-    const int blocks = grid_for((int64_t)n, 256, 16384);
+    const PairRankKey key{bits, bnd, wprefix};
     if (M.nm > 1)
-        k_pair_scatter<true><<<blocks, 256, 0, s>>>(tri, pb, n, boff, cntb, list, tris, bidx, g0, M, pmask);
+        ts_sort(key, n, B, tmp, boff, max_out, PairPlace<true>{tri, list, tris, bidx, g0, M, pmask}, s);
     else
-        k_pair_scatter<false><<<blocks, 256, 0, s>>>(tri, pb, n, boff, cntb, list, tris, bidx, g0, M, pmask);
+        ts_sort(key, n, B, tmp, boff, max_out, PairPlace<false>{tri, list, tris, bidx, g0, M, pmask}, s);
 }
 
 // Per-mesh bin totals (pairs) and largest bin of a batch.  bmesh is

@@ -209,9 +209,9 @@ struct Job {
     uint32_t *wcnt = nullptr, *wprefix = nullptr;
     int32_t* bidx = nullptr;
     int64_t B = 0, W2 = 0;
-    uint32_t *pb = nullptr, *cntb = nullptr, *boff = nullptr, *list = nullptr;
+    uint32_t *boff = nullptr, *list = nullptr;
     uint64_t* pmask = nullptr;   // Level2 box-axis masks per bin entry
-    uint32_t *ccnt = nullptr, *coff = nullptr;
+    uint32_t* coff = nullptr;
     float4* crec = nullptr;   // 3 float4 per column entry
     uint64_t* cpass = nullptr;   // det/u/v pass bits per column entry and sub-column
     uint32_t *hcnt = nullptr, *hoff = nullptr;
@@ -434,7 +434,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         hit_cap += (uint64_t)(8 * d[m].ntris + 4 * nx * ny);
     }
     if (T >= (int64_t(1) << 31)) return fail(GPV_ERR_INVALID, "at most 2^31-1 triangles per batch");
-    if (ncol >= (int64_t(1) << 32) - 1) return fail(GPV_ERR_INVALID, "at most 2^32-2 xy columns per batch");
+    if (ncol > (int64_t(1) << 27)) return fail(GPV_ERR_INVALID, "at most 2^27 xy columns per batch");
     if (vox >= (int64_t(1) << 32)) return fail(GPV_ERR_INVALID, "at most 2^32-1 Level1 voxels per batch");
     if (T > 0 && on_device && nm == 1 && !d[0].tris) return fail(GPV_ERR_INVALID, "NULL soup");
     HIP_TRY(hipSetDevice(c->device));
@@ -658,6 +658,9 @@ int complete_sat(Job* J, int64_t* needed)
         J->off[1].l2n0 = J->g[0].normals ? (uint64_t)B * J->g[0].n2 * J->g[0].n2 * J->g[0].n2 : 0;
     }
     J->B = J->off[J->nm].b0;
+    if (J->B > (int64_t(1) << 27))
+        return fail(GPV_ERR_INVALID, "%lld boundary voxels: at most 2^27 per batch (split the batch)",
+                    (long long)J->B);
     J->W2 = (int64_t)J->off[J->nm].l2w0;
     J->NL2 = (int64_t)J->off[J->nm].l2n0;
     J->sat_done = true;
@@ -712,34 +715,36 @@ int launch_compact(Job* J)
     TRY(alloc(J, &J->bidx, Bn));
     if (J->nm > 1) TRY(alloc(J, &J->bmesh, Bn));
     launch_boundary_index(J->bnd, J->wprefix, J->g[0], J->view(), J->W, J->bidx, s);
-    release(J, J->pb);
-    release(J, J->cntb);
     release(J, J->boff);
     release(J, J->list);
     release(J, J->pmask);
-    TRY(alloc(J, &J->pb, (size_t)std::max<uint64_t>(J->n_pairs, 1)));
-    TRY(alloc(J, &J->cntb, Bn));
     TRY(alloc(J, &J->boff, (size_t)J->B + 1));
     TRY(alloc(J, &J->list, (size_t)std::max<uint64_t>(J->n_pairs, 1)));
     if (J->n2max > 0) TRY(alloc(J, &J->pmask, (size_t)std::max<uint64_t>(J->n_pairs, 1)));
-    launch_fill_u32(J->cntb, 0, J->B, s);
-    launch_pair_rank(J->pair_key, J->n_pairs, J->bnd, J->wprefix, J->pb, J->cntb, s);
-    TRY(ensure_scan_tmp(J, J->B));
-    launch_scan_u32_u32(J->cntb, J->boff, J->B, J->scan32, J->nm > 1 ? nullptr : &J->mctr[0].max_per_voxel, s);
-    launch_pair_scatter(J->pair_tri, J->pb, J->n_pairs, J->boff, J->cntb, J->list, J->tris, J->bidx, J->g[0],
-                        J->view(), J->pmask, s);
+    // one scratch for both counting sorts (bins by boundary rank, column
+    // entries by column)
+    const size_t tiles = std::max(ts_tiles((uint64_t)J->B), ts_tiles((uint64_t)J->ncol));
+    TileSortTmp ts{};
+    uint32_t* ts_cnt = nullptr;
+    uint2* ts_bucket = nullptr;
+    TRY(alloc(J, &ts_cnt, 3 * tiles + 2));
+    TRY(alloc(J, &ts_bucket, (size_t)std::max<uint64_t>(std::max(J->n_pairs, J->n_cols), 1)));
+    TRY(ensure_scan_tmp(J, (int64_t)tiles));
+    ts.total = ts_cnt;
+    ts.off = ts_cnt + 2 * tiles + 1;
+    ts.scan = J->scan32;
+    ts.bucket = ts_bucket;
+    launch_bin_pairs(J->pair_key, J->pair_tri, J->n_pairs, J->bnd, J->wprefix, (uint32_t)J->B, ts, J->boff,
+                     J->nm > 1 ? nullptr : &J->mctr[0].max_per_voxel, J->list, J->tris, J->bidx, J->g[0],
+                     J->view(), J->pmask, s);
     if (J->nm > 1) launch_mesh_bins(J->boff, J->B, J->view(), s);
-    release(J, J->ccnt);
     release(J, J->coff);
     release(J, J->crec);
-    TRY(alloc(J, &J->ccnt, (size_t)J->ncol));
     TRY(alloc(J, &J->coff, (size_t)J->ncol + 1));
     TRY(alloc(J, &J->crec, (size_t)std::max<uint64_t>(J->n_cols, 1) * 3));
-    launch_fill_u32(J->ccnt, 0, J->ncol, s);
-    launch_count_keys_hi(J->col_key, J->n_cols, J->ccnt, s);
-    TRY(ensure_scan_tmp(J, J->ncol));
-    launch_scan_u32_u32(J->ccnt, J->coff, J->ncol, J->scan32, nullptr, s);
-    launch_scatter_cols(J->tris, J->col_key, J->col_z, J->n_cols, J->coff, J->ccnt, J->crec, s);
+    launch_bin_cols(J->tris, J->col_key, J->col_z, J->n_cols, (uint32_t)J->ncol, ts, J->coff, J->crec, s);
+    release(J, ts_cnt);
+    release(J, ts_bucket);
     HIP_TRY(hipGetLastError());
     if (J->n2max > 0) {
         release(J, J->sat2);
@@ -858,8 +863,8 @@ int job_finish(Job* J)
     release(J, J->sat_items); release(J, J->ray_items); release(J, J->sat_off); release(J, J->ray_off);
     release(J, J->scan64); J->scan32 = nullptr; J->scan_cap = 0;
     release(J, J->pair_key); release(J, J->pair_tri); release(J, J->col_key); release(J, J->col_z);
-    release(J, J->hit_key); release(J, J->wcnt); release(J, J->pb); release(J, J->cntb); release(J, J->boff);
-    release(J, J->list); release(J, J->ccnt); release(J, J->coff); release(J, J->crec); release(J, J->hcnt);
+    release(J, J->hit_key); release(J, J->wcnt); release(J, J->boff);
+    release(J, J->list); release(J, J->coff); release(J, J->crec); release(J, J->hcnt);
     release(J, J->hoff); release(J, J->hrec); release(J, J->tri_mesh); release(J, J->bmesh);
     release(J, J->d_grids); release(J, J->d_off); release(J, J->own_soup); release(J, J->mctr);
     release(J, J->fnorm); release(J, J->slist);
