@@ -103,8 +103,10 @@ void launch_scan_u32_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t*
                          hipStream_t s);
 
 void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const double4* slab1,
-                       const uint64_t* off, const GridDev& g0, const Meshes& M, uint64_t* bnd, Append64 pairs,
+                       const uint64_t* off, const GridDev& g0, const Meshes& M, uint8_t* bbytes, Append64 pairs,
                        Append64 cols, int blocks, hipStream_t s);
+// boundary bytes (W * 64) -> boundary bitplane (W words)
+void launch_pack_bytes(const uint8_t* by, int64_t W, uint64_t* bits, hipStream_t s);
 void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off, const GridDev& g0,
                        const Meshes& M, Append64 hits, int blocks, hipStream_t s);
 

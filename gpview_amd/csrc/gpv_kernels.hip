@@ -668,7 +668,7 @@ __device__ __forceinline__ void expand_items(const float* __restrict__ tris, int
 }
 
 // Level1 SAT: one item = (triangle, xy column of its index-AABB); the lane
-// walks the AABB's z range.  Hits set the boundary bit, append the
+// walks the AABB's z range.  Hits set the voxel's boundary byte, append the
 // (voxel, triangle) bin pair -- key = batch-wide bit index, value = triangle --
 // and the lane's highest hit z becomes the (column, triangle, zmax) entry used
 // by the Level2 parity candidates.
@@ -678,7 +678,7 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
                                                    const int4* __restrict__ box1,
                                                    const double4* __restrict__ slab1,
                                                    const uint64_t* __restrict__ off, GridDev g0, Meshes M,
-                                                   unsigned long long* __restrict__ bnd, Append64 pairs,
+                                                   uint8_t* __restrict__ bbytes, Append64 pairs,
                                                    Append64 cols)
 {
     constexpr uint32_t PB = 1536, CB = 768;
@@ -705,7 +705,6 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
         const float hx = g.h1x, hy = g.h1y, hz = g.h1z;
         const uint64_t plane = (uint64_t)g.ny * g.nxw;
         const uint64_t rowword = word0 + (uint64_t)q * g.nxw + (uint64_t)(p >> 6);
-        const uint64_t bit = 1ull << (p & 63);
         const float* c1z = g.c1z;
         int za = b0.z, zb = b0.z + b1.z;
         if (b1.z >= kZSlabMin) {
@@ -720,7 +719,7 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
             const uint64_t w = (uint64_t)z * plane + rowword;
             if (hit) {
                 zmax = z;
-                atomicOr(&bnd[w], (unsigned long long)bit);
+                bbytes[w * 64 + (uint64_t)(p & 63)] = 1;   // every writer stores the same byte value
             }
             lds_append(lp, pairs, hit, w * 64 + (uint64_t)(p & 63), (uint32_t)t);
         }
@@ -738,15 +737,31 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
 }
 
 void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const double4* slab1,
-                       const uint64_t* off, const GridDev& g0, const Meshes& M, uint64_t* bnd, Append64 pairs,
+                       const uint64_t* off, const GridDev& g0, const Meshes& M, uint8_t* b, Append64 pairs,
                        Append64 cols, int blocks, hipStream_t s)
 {
     if (T <= 0) return;
-    unsigned long long* b = reinterpret_cast<unsigned long long*>(bnd);
     if (M.nm > 1)
         k_sat_expand<true><<<blocks, XB, 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols);
     else
         k_sat_expand<false><<<blocks, XB, 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols);
+}
+
+// boundary flags (one byte per padded voxel, written without atomics by the
+// Level1 SAT) -> the boundary bitplane, one word per wave
+__global__ void k_pack_bytes(const uint8_t* __restrict__ by, int64_t W, uint64_t* __restrict__ bits)
+{
+    const int64_t n = W * 64;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t m = __ballot(by[i] != 0);
+        if (__lane_id() == 0) bits[i >> 6] = m;
+    }
+}
+
+void launch_pack_bytes(const uint8_t* by, int64_t W, uint64_t* bits, hipStream_t s)
+{
+    if (W <= 0) return;
+    k_pack_bytes<<<grid_for(W * 64, 256, 65536), 256, 0, s>>>(by, W, bits);
 }
 
 // Level1 ray parity, first half: one item = (triangle, column inside its

@@ -573,12 +573,17 @@ int launch_sat_half(Job* J)
     gpv_ctx* c = J->ctx;
     hipStream_t s = c->stream;
     HIP_TRY(hipEventRecord(J->ev[1], s));
-    launch_fill_u64(J->bnd, 0, J->W, s);
+    // boundary flags as bytes (plain stores from the SAT), packed to bits after
+    uint8_t* bbytes = nullptr;
+    TRY(alloc(J, &bbytes, (size_t)J->W * 64));
+    launch_fill_u64(reinterpret_cast<uint64_t*>(bbytes), 0, J->W * 8, s);
     launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->pairs), 0, 2, s);   // pairs, colents
     Append64 pa{&J->ctr->pairs, J->pair_key, J->pair_tri, J->pair_cap};
     Append64 co{&J->ctr->colents, J->col_key, J->col_z, J->col_cap};
-    launch_sat_expand(J->tris, J->T, J->box0, J->box1, J->slab1, J->sat_off, J->g[0], J->view(), J->bnd, pa, co,
+    launch_sat_expand(J->tris, J->T, J->box0, J->box1, J->slab1, J->sat_off, J->g[0], J->view(), bbytes, pa, co,
                       expand_blocks(c), s);
+    launch_pack_bytes(bbytes, J->W, J->bnd, s);
+    release(J, bbytes);
     launch_popc_words(J->bnd, J->W, J->wcnt, s);
     TRY(ensure_scan_tmp(J, J->W));
     launch_scan_u32_u32(J->wcnt, J->wprefix, J->W, J->scan32, nullptr, s);

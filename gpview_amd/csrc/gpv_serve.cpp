@@ -183,6 +183,7 @@ int gpv_predict_footprint(const gpv_mesh_desc* d, int64_t* bytes)
     const int64_t tables = 4 * (2 * (nx + ny + nz) * (1 + (I.n2 > 0 ? I.n2 : 0)));
     *bytes = 36 * T + 80 * T            // soup copy and per-triangle set-up
              + 24 * W                   // boundary, inside, word counts, word prefix
+             + 64 * W                   // boundary flag bytes of the Level1 SAT (transient)
              + 80 * pairs               // bin pairs, column entries, ranks, lists, column records
              + 56 * hits                // ray hits and their records
              + 16 * nx * ny             // column counts and offsets
