@@ -121,7 +121,7 @@ struct TileSortTmp {
     uint32_t* total;
     uint32_t* off;
     uint32_t* scan;
-    uint2* bucket;
+    uint4* bucket;
 };
 size_t ts_tiles(uint64_t nkeys);
 // column entries (key = column << 32 | triangle) -> CSR by column with the MT

@@ -865,19 +865,24 @@ void launch_scatter_hits(const float* tris, const uint64_t* key, uint64_t n, con
 }
 
 // column entries -> CSR by column, each with the MT record its candidates need
-struct ColKey {
+struct ColSrc {
     const uint64_t* key;
-    __device__ uint32_t operator()(uint64_t i) const { return (uint32_t)(key[i] >> 32); }
+    const uint32_t* zmax;
+    __device__ uint32_t sort_key(uint64_t i) const { return (uint32_t)(key[i] >> 32); }
+    // (column, triangle, zmax)
+    __device__ uint4 item(uint64_t i) const
+    {
+        const uint64_t k = key[i];
+        return make_uint4((uint32_t)(k >> 32), (uint32_t)(k & 0xffffffffu), zmax[i], 0u);
+    }
 };
 
 struct ColPlace {
     const float* tris;
-    const uint64_t* key;
-    const uint32_t* zmax;
     float4* rec;
-    __device__ void operator()(uint32_t pos, uint32_t, uint32_t i) const
+    __device__ void operator()(uint32_t pos, const uint4& it) const
     {
-        const uint32_t t = (uint32_t)(key[i] & 0xffffffffu);
+        const uint32_t t = it.y;
         float tv[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) tv[c] = tris[(uint64_t)t * 9 + c];
@@ -885,7 +890,7 @@ struct ColPlace {
         mt_setup(tv, m);   // entries exist only for triangles whose det passes
         rec[3 * (uint64_t)pos + 0] = make_float4(m.e1x, m.e1y, m.e1z, m.e2x);
         rec[3 * (uint64_t)pos + 1] = make_float4(m.e2y, m.e2z, m.inv_det, tv[0]);
-        rec[3 * (uint64_t)pos + 2] = make_float4(tv[1], tv[2], __int_as_float((int)zmax[i]), __uint_as_float(t));
+        rec[3 * (uint64_t)pos + 2] = make_float4(tv[1], tv[2], __int_as_float((int)it.z), __uint_as_float(t));
     }
 };
 
@@ -1081,8 +1086,8 @@ __device__ __forceinline__ void ts_range(uint64_t n, uint64_t& lo, uint64_t& hi)
     hi = lo + per < n ? lo + per : n;
 }
 
-template <typename Key>
-__global__ __launch_bounds__(TS_NT) void k_ts_count(Key key, uint64_t n, int shift, int ntiles,
+template <typename Src>
+__global__ __launch_bounds__(TS_NT) void k_ts_count(Src src, uint64_t n, int shift, int ntiles,
                                                     uint32_t* __restrict__ tile_total)
 {
     extern __shared__ uint32_t s_h[];   // [ntiles]
@@ -1090,35 +1095,37 @@ __global__ __launch_bounds__(TS_NT) void k_ts_count(Key key, uint64_t n, int shi
     __syncthreads();
     uint64_t lo, hi;
     ts_range(n, lo, hi);
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) atomicAdd(&s_h[key(i) >> shift], 1u);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) atomicAdd(&s_h[src.sort_key(i) >> shift], 1u);
     __syncthreads();
     for (int t = threadIdx.x; t < ntiles; t += TS_NT)
         if (s_h[t]) atomicAdd(&tile_total[t], s_h[t]);
 }
 
-template <typename Key>
-__global__ __launch_bounds__(TS_NT) void k_ts_bucket(Key key, uint64_t n, int shift, int ntiles,
+// items (16 bytes, .x = key) to their tile's bucket range; the item is built
+// here, where the block reads its inputs in their original order
+template <typename Src>
+__global__ __launch_bounds__(TS_NT) void k_ts_bucket(Src src, uint64_t n, int shift, int ntiles,
                                                      const uint32_t* __restrict__ tile_off,
-                                                     uint32_t* __restrict__ tile_fill, uint2* __restrict__ bucket)
+                                                     uint32_t* __restrict__ tile_fill, uint4* __restrict__ bucket)
 {
     extern __shared__ uint32_t s_h[];   // [ntiles]: counts, then this block's next slot per tile
     for (int t = threadIdx.x; t < ntiles; t += TS_NT) s_h[t] = 0;
     __syncthreads();
     uint64_t lo, hi;
     ts_range(n, lo, hi);
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) atomicAdd(&s_h[key(i) >> shift], 1u);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) atomicAdd(&s_h[src.sort_key(i) >> shift], 1u);
     __syncthreads();
     for (int t = threadIdx.x; t < ntiles; t += TS_NT)
         if (s_h[t]) s_h[t] = tile_off[t] + atomicAdd(&tile_fill[t], s_h[t]);
     __syncthreads();
     for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) {
-        const uint32_t k = key(i);
-        bucket[atomicAdd(&s_h[k >> shift], 1u)] = make_uint2(k, (uint32_t)i);
+        const uint4 it = src.item(i);
+        bucket[atomicAdd(&s_h[it.x >> shift], 1u)] = it;
     }
 }
 
 template <typename Place>
-__global__ __launch_bounds__(TS_NT) void k_ts_place(const uint2* __restrict__ bucket,
+__global__ __launch_bounds__(TS_NT) void k_ts_place(const uint4* __restrict__ bucket,
                                                     const uint32_t* __restrict__ tile_off, int shift,
                                                     uint32_t nkeys, uint32_t* __restrict__ off,
                                                     unsigned int* __restrict__ max_out, Place place)
@@ -1164,8 +1171,8 @@ __global__ __launch_bounds__(TS_NT) void k_ts_place(const uint2* __restrict__ bu
     if (threadIdx.x == 0 && max_out && s_max) atomicMax(max_out, s_max);
     __syncthreads();
     for (uint32_t i = b0 + threadIdx.x; i < b1; i += TS_NT) {
-        const uint2 e = bucket[i];
-        place(atomicAdd(&s_c[e.x - k0], 1u), e.x, e.y);
+        const uint4 it = bucket[i];
+        place(atomicAdd(&s_c[it.x - k0], 1u), it);
     }
 }
 
@@ -1178,8 +1185,8 @@ int ts_shift(uint64_t nkeys)
     return lg - 14 > 8 ? lg - 14 : 8;
 }
 
-template <typename Key, typename Place>
-static void ts_sort(Key key, uint64_t n, uint32_t nkeys, const TileSortTmp& tmp, uint32_t* off,
+template <typename Src, typename Place>
+static void ts_sort(Src src, uint64_t n, uint32_t nkeys, const TileSortTmp& tmp, uint32_t* off,
                     unsigned int* max_out, Place place, hipStream_t s)
 {
     const int shift = ts_shift(nkeys);
@@ -1193,9 +1200,9 @@ static void ts_sort(Key key, uint64_t n, uint32_t nkeys, const TileSortTmp& tmp,
     if (n > 0) {
         const int blocks = (int)std::min<uint64_t>(2048, std::max<uint64_t>(1, (n + 4095) / 4096));
         const size_t lds = (size_t)ntiles * 4;
-        k_ts_count<Key><<<blocks, TS_NT, lds, s>>>(key, n, shift, ntiles, tmp.total);
+        k_ts_count<Src><<<blocks, TS_NT, lds, s>>>(src, n, shift, ntiles, tmp.total);
         launch_scan_u32_u32(tmp.total, tmp.off, ntiles, tmp.scan, nullptr, s);
-        k_ts_bucket<Key><<<blocks, TS_NT, lds, s>>>(key, n, shift, ntiles, tmp.off, fill, tmp.bucket);
+        k_ts_bucket<Src><<<blocks, TS_NT, lds, s>>>(src, n, shift, ntiles, tmp.off, fill, tmp.bucket);
     } else {
         launch_fill_u32(tmp.off, 0, (int64_t)ntiles + 1, s);
     }
@@ -1209,35 +1216,31 @@ size_t ts_tiles(uint64_t nkeys)
     return (size_t)((nkeys + (1u << shift) - 1) >> shift);
 }
 
-// key of a bin pair: the rank of its voxel among the boundary voxels
-struct PairRankKey {
+// A bin pair sorts by the rank of its voxel among the boundary voxels; its
+// item carries the triangle and the Level2 box-axis masks: bit p of mask x
+// says the SAT's x box-axis predicate holds for the triangle and the Level2
+// sub-index p of that voxel (the same float test the full SAT runs), likewise
+// y, z; packed x | y << 16 | z << 32.
+template <bool kM>
+struct PairSrc {
     const uint64_t* bits;
+    const uint32_t* tri;
     const uint64_t* bnd;
     const uint32_t* wprefix;
-    __device__ uint32_t operator()(uint64_t i) const { return rank_of(bnd, wprefix, bits[i]); }
-};
-
-// Bin entry at pos: the triangle, and the Level2 box-axis masks: bit p of
-// mask x says the SAT's x box-axis predicate holds for the triangle and the
-// Level2 sub-index p of that voxel (the same float test the full SAT runs),
-// likewise y, z; packed x | y << 16 | z << 32.
-template <bool kM>
-struct PairPlace {
-    const uint32_t* tri;
-    uint32_t* list;
     const float* tris;
     const int32_t* bidx;
     GridDev g0;
     Meshes M;
-    uint64_t* pmask;
-    __device__ void operator()(uint32_t pos, uint32_t b, uint32_t i) const
+    bool masks;
+    __device__ uint32_t sort_key(uint64_t i) const { return rank_of(bnd, wprefix, bits[i]); }
+    __device__ uint4 item(uint64_t i) const
     {
+        const uint32_t b = rank_of(bnd, wprefix, bits[i]);
         const uint32_t t = tri[i];
-        list[pos] = t;
         const int mesh = kM ? (int)M.bmesh[b] : 0;
         const GridDev& g = grid_of<kM>(g0, M, mesh);
         const int n2 = g.n2;
-        if (!pmask || n2 == 0) return;
+        if (!masks || n2 == 0) return make_uint4(b, t, 0u, 0u);
         const int v = bidx[b];
         const int nxy = g.nx * g.ny;
         const int k1 = v / nxy;
@@ -1250,20 +1253,30 @@ struct PairPlace {
         const float* cx = g.c2x + (int64_t)i1 * n2;
         const float* cy = g.c2y + (int64_t)j1 * n2;
         const float* cz = g.c2z + (int64_t)k1 * n2;
-        uint64_t mx = 0, my = 0, mz = 0;
+        uint32_t mx = 0, my = 0, mz = 0;
         for (int p = 0; p < n2; ++p) {
-            mx |= (uint64_t)sat_axis_ok(a0, a3, a6, cx[p], g.h2x) << p;
-            my |= (uint64_t)sat_axis_ok(a1, a4, a7, cy[p], g.h2y) << p;
-            mz |= (uint64_t)sat_axis_ok(a2, a5, a8, cz[p], g.h2z) << p;
+            mx |= (uint32_t)sat_axis_ok(a0, a3, a6, cx[p], g.h2x) << p;
+            my |= (uint32_t)sat_axis_ok(a1, a4, a7, cy[p], g.h2y) << p;
+            mz |= (uint32_t)sat_axis_ok(a2, a5, a8, cz[p], g.h2z) << p;
         }
-        pmask[pos] = mx | (my << 16) | (mz << 32);
+        return make_uint4(b, t, mx | (my << 16), mz);
+    }
+};
+
+struct PairPlace {
+    uint32_t* list;
+    uint64_t* pmask;
+    __device__ void operator()(uint32_t pos, const uint4& it) const
+    {
+        list[pos] = it.y;
+        if (pmask) pmask[pos] = (uint64_t)it.z | ((uint64_t)it.w << 32);
     }
 };
 
 void launch_bin_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n, uint32_t ncol,
                      const TileSortTmp& tmp, uint32_t* coff, float4* rec, hipStream_t s)
 {
-    ts_sort(ColKey{key}, n, ncol, tmp, coff, nullptr, ColPlace{tris, key, zmax, rec}, s);
+    ts_sort(ColSrc{key, zmax}, n, ncol, tmp, coff, nullptr, ColPlace{tris, rec}, s);
 }
 
 void launch_bin_pairs(const uint64_t* bits, const uint32_t* tri, uint64_t n, const uint64_t* bnd,
@@ -1271,11 +1284,13 @@ void launch_bin_pairs(const uint64_t* bits, const uint32_t* tri, uint64_t n, con
                       unsigned int* max_out, uint32_t* list, const float* tris, const int32_t* bidx,
                       const GridDev& g0, const Meshes& M, uint64_t* pmask, hipStream_t s)
 {
-    const PairRankKey key{bits, bnd, wprefix};
+    const PairPlace place{list, pmask};
     if (M.nm > 1)
-        ts_sort(key, n, B, tmp, boff, max_out, PairPlace<true>{tri, list, tris, bidx, g0, M, pmask}, s);
+        ts_sort(PairSrc<true>{bits, tri, bnd, wprefix, tris, bidx, g0, M, pmask != nullptr}, n, B, tmp, boff,
+                max_out, place, s);
     else
-        ts_sort(key, n, B, tmp, boff, max_out, PairPlace<false>{tri, list, tris, bidx, g0, M, pmask}, s);
+        ts_sort(PairSrc<false>{bits, tri, bnd, wprefix, tris, bidx, g0, M, pmask != nullptr}, n, B, tmp, boff,
+                max_out, place, s);
 }
 
 // Per-mesh bin totals (pairs) and largest bin of a batch.  bmesh is

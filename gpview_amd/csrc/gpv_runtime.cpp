@@ -731,7 +731,7 @@ int launch_compact(Job* J)
     const size_t tiles = std::max(ts_tiles((uint64_t)J->B), ts_tiles((uint64_t)J->ncol));
     TileSortTmp ts{};
     uint32_t* ts_cnt = nullptr;
-    uint2* ts_bucket = nullptr;
+    uint4* ts_bucket = nullptr;
     TRY(alloc(J, &ts_cnt, 3 * tiles + 2));
     TRY(alloc(J, &ts_bucket, (size_t)std::max<uint64_t>(std::max(J->n_pairs, J->n_cols), 1)));
     TRY(ensure_scan_tmp(J, (int64_t)tiles));
