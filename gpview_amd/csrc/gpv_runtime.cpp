@@ -153,6 +153,7 @@ struct gpv_ctx {
     std::recursive_mutex mu;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;   // second stream: Level2 parity beside the Level2 SAT
     Pool pool;
     char* h_buf = nullptr;   // pinned read-back staging
     size_t h_cap = 0;
@@ -230,8 +231,8 @@ struct Job {
          l2p_done = false, l2s_done = false, finished = false;
     // stream events: 0 setup, 1 L1 SAT, 2 ray expansion, 3 parity fill,
     // 4 compaction+bins, 5 L2 parity, 6 L2 SAT start, 7 L2 SAT end, 8 finish,
-    // 9 normals start, 10 normals end
-    hipEvent_t ev[11] = {};
+    // 9 normals start, 10 normals end, 11 L2 parity end
+    hipEvent_t ev[12] = {};
     std::vector<void*> bufs;
 
     Meshes view() const { return Meshes{d_grids, d_off, tri_mesh, bmesh, mctr, nm}; }
@@ -381,6 +382,7 @@ void job_free(Job* J)
     gpv_ctx* c = J->ctx;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);   // pooled blocks may be reused right away
+    (void)hipStreamSynchronize(c->side);
     for (void* p : J->bufs) c->pool.put(p);
     for (auto& e : J->ev)
         if (e) (void)hipEventDestroy(e);
@@ -761,18 +763,32 @@ int launch_compact(Job* J)
     return GPV_OK;
 }
 
+// Level2 parity kernels on stream s (cpass already allocated)
+int l2_parity_kernels(Job* J, hipStream_t s)
+{
+    const int stride = (J->n2max * J->n2max + 63) / 64;
+    launch_col_pass(J->coff, J->crec, J->g[0], J->view(), J->ncol, stride, J->cpass, s);
+    launch_l2_parity(J->bidx, J->B, J->coff, J->crec, J->cpass, stride, J->g[0], J->view(), J->par2, s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(J->ev[11], s));
+    return GPV_OK;
+}
+
+int alloc_cpass(Job* J)
+{
+    const int stride = (J->n2max * J->n2max + 63) / 64;
+    release(J, J->cpass);
+    return alloc(J, &J->cpass, (size_t)std::max<uint64_t>(J->n_cols, 1) * stride);
+}
+
 int run_l2_parity(Job* J)
 {
     if (J->n2max <= 0) return GPV_OK;
     hipStream_t s = J->ctx->stream;
     HIP_TRY(hipEventRecord(J->ev[5], s));
-    const int stride = (J->n2max * J->n2max + 63) / 64;
+    TRY(alloc_cpass(J));
+    TRY(l2_parity_kernels(J, s));
     release(J, J->cpass);
-    TRY(alloc(J, &J->cpass, (size_t)std::max<uint64_t>(J->n_cols, 1) * stride));
-    launch_col_pass(J->coff, J->crec, J->g[0], J->view(), J->ncol, stride, J->cpass, s);
-    launch_l2_parity(J->bidx, J->B, J->coff, J->crec, J->cpass, stride, J->g[0], J->view(), J->par2, s);
-    release(J, J->cpass);
-    HIP_TRY(hipGetLastError());
     J->l2p_done = true;
     return GPV_OK;
 }
@@ -786,6 +802,26 @@ int run_l2_sat(Job* J)
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(J->ev[7], s));
     J->l2s_done = true;
+    return GPV_OK;
+}
+
+// Both Level2 stages of the fused pipeline: the parity on the side stream
+// beside the SAT on the main one (they read the same inputs and write
+// different planes).  Nothing is allocated or released between the fork and
+// the join, so the pool's blocks stay in one stream order.
+int run_level2(Job* J)
+{
+    if (J->n2max <= 0) return GPV_OK;
+    gpv_ctx* c = J->ctx;
+    hipStream_t s = c->stream;
+    TRY(alloc_cpass(J));
+    HIP_TRY(hipEventRecord(J->ev[5], s));
+    HIP_TRY(hipStreamWaitEvent(c->side, J->ev[5], 0));
+    TRY(l2_parity_kernels(J, c->side));
+    TRY(run_l2_sat(J));
+    HIP_TRY(hipStreamWaitEvent(s, J->ev[11], 0));
+    release(J, J->cpass);
+    J->l2p_done = true;
     return GPV_OK;
 }
 
@@ -838,7 +874,7 @@ int job_finish(Job* J)
     float ms[6] = {el(0, 1), el(1, 2), el(2, 4), 0.0f, 0.0f, 0.0f};
     if (J->n2max > 0) {
         ms[3] = el(4, 5);
-        ms[4] = el(5, 6);
+        ms[4] = el(5, 11);   // in the fused pipeline the two Level2 stages overlap
         ms[5] = el(6, 7);
     } else {
         ms[3] = el(4, 8);
@@ -899,8 +935,7 @@ int job_run(Job* J)
     if (needed > 0 || !ray_ok) return fail(GPV_ERR_CAPACITY, "buffer capacity did not converge");
     TRY(launch_parity(J));
     TRY(launch_compact(J));
-    TRY(run_l2_parity(J));
-    TRY(run_l2_sat(J));
+    TRY(run_level2(J));
     TRY(run_normals(J));
     return job_finish(J);
 }
@@ -978,8 +1013,14 @@ int gpv_ctx_create(int device, unsigned flags, gpv_ctx** out)
         delete c;
         return fail(GPV_ERR_HIP, "stream creation failed");
     }
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(GPV_ERR_HIP, "stream creation failed");
+    }
     if (ensure_hbuf(c, 4096) != GPV_OK) {
         (void)hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->side);
         delete c;
         return fail(GPV_ERR_HIP, "pinned staging buffer allocation failed");
     }
@@ -994,9 +1035,11 @@ int gpv_ctx_destroy(gpv_ctx* c)
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->side);
         c->pool.trim();
         if (c->h_buf) (void)hipHostFree(c->h_buf);
         (void)hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->side);
     }
     delete c;
     return GPV_OK;
