@@ -203,7 +203,7 @@ def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48, plain_sample=
 
 # stage -> its kernel in the single-mesh pipeline (each stage's event pair
 # brackets exactly these launches)
-STAGE_KERNEL = {"l2_sat": "k_l2_sat<false>", "l2_parity": "k_l2_parity<false, 8>", "l1_sat": "k_sat_expand<false>"}
+STAGE_KERNEL = {"l2_sat": "k_l2_sat_voxel<false>", "l2_parity": "k_l2_parity<false, 8>", "l1_sat": "k_sat_expand<false>"}
 PROFILE = os.path.join(ROOT, "profiles", "traffic_torus_knot_plain.json")
 
 

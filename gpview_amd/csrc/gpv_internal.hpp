@@ -150,9 +150,13 @@ void launch_bin_pairs(const uint64_t* bits, const uint32_t* tri, uint64_t n, con
 // per-mesh bin totals and largest bin (multi-mesh runs)
 void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStream_t s);
 
+// Level2 SAT work groups of consecutive boundary voxels: flag[B], gidx[B+1]
+// (gidx[B] = group count), gstart[B+1]
+void launch_l2_groups(const uint32_t* boff, int64_t B, const GridDev& g0, const Meshes& M, uint32_t* flag,
+                      uint32_t* gidx, uint32_t* scan_tmp, uint32_t* gstart, hipStream_t s);
 void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
-                   const uint64_t* pmask, const float4* slab2, const GridDev& g0, const Meshes& M, uint64_t* sat2,
-                   hipStream_t s);
+                   const uint64_t* pmask, const float4* slab2, const uint32_t* gstart, const uint32_t* ngroups,
+                   int blocks, const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s);
 // det/u/v pass bits per (column entry, sub-column): stride words per entry
 void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M, int64_t ncol,
                      int stride, uint64_t* cpass, hipStream_t s);

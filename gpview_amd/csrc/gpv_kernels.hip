@@ -1330,18 +1330,29 @@ void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStrea
 }
 
 // ---------------------------------------------------------------------------
-// Level2 SAT (src/Object.cpp:2344-2424): one wave per boundary voxel.  Each
-// bin entry carries its box-axis masks (the SAT's own box-axis predicate per
-// sub-index, computed with the bins) and its triangle a plane slab
-// (gpv_slab.hpp) valid over every Level2 center of its index AABB.  For each
-// triangle the lanes take the sub-columns (p, q) its x/y masks allow and keep
-// the sub-voxels r whose float |N.(c - t0)| the slab admits (the bound is
-// grown by that evaluation's error, so the SAT rejects every other one);
-// those queue for the full float SAT in
-// 64-lane batches.  Sub-voxels already set are skipped (the result is an OR).
+// Level2 SAT (src/Object.cpp:2344-2424).  Two forms: one wave per boundary
+// voxel (k_l2_sat_voxel), and one wave per group of consecutive boundary
+// voxels (k_l2_sat_grp: same mesh, first bin entries in one 64-entry chunk,
+// at most L2_GMAX voxels and 64 Level2 words), so voxels with few bin entries
+// share the wave's rounds and SAT batches.  Each bin entry carries its box-axis masks
+// (the SAT's own box-axis predicate per sub-index, computed with the bins)
+// and its triangle a plane slab (gpv_slab.hpp) valid over every Level2
+// center of its index AABB.  The lanes take the (entry, sub-column) items its
+// x/y masks allow, drop sub-columns whose e x z edge predicates separate, and
+// keep the sub-voxels r whose float |N.(c - t0)| the slab admits (the bound
+// is grown by that evaluation's error, so the SAT rejects every other one);
+// those queue for the full float SAT in 64-lane batches.  Sub-voxels already
+// set are skipped (the result is an OR).
 // ---------------------------------------------------------------------------
 constexpr int L2_WAVES = 4;
 constexpr int L2_QCAP = 128;   // ring of queued (triangle, sub-voxel) entries per wave
+constexpr int L2_GMAX = 8;     // voxels per group
+constexpr int L2_CMAX = 192;   // center floats per group: 3 * voxels * n2 <= 192 for n2 <= 16
+
+// 1/d for the sub-box widths d = 1..16 (float quotients, as 1.0f / d)
+__constant__ float kInvSmall[17] = {0.0f,        1.0f,        0.5f,        1.0f / 3.0f, 0.25f,       0.2f,
+                                    1.0f / 6.0f, 1.0f / 7.0f, 0.125f,      1.0f / 9.0f, 0.1f,        1.0f / 11.0f,
+                                    1.0f / 12.0f, 1.0f / 13.0f, 1.0f / 14.0f, 1.0f / 15.0f, 0.0625f};
 
 // value of a float register in lane `l` (l uniform across the wave)
 __device__ __forceinline__ float lane_f(float v, int l)
@@ -1364,10 +1375,267 @@ __device__ __forceinline__ uint64_t lane_u64(uint64_t v, int l)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// voxels of a group: the capacity for a mesh's Level2 word count
+__device__ __forceinline__ uint32_t l2_group_cap(int w2)
+{
+    const uint32_t c = w2 > 0 ? 64u / (uint32_t)w2 : 64u;
+    return c < (uint32_t)L2_GMAX ? c : (uint32_t)L2_GMAX;
+}
+
+// Group starts: flag[b] = 1 where a new group begins (mesh start, a new
+// 64-entry chunk of first bin entries, or a full group).
+template <bool kM>
+__global__ void k_l2_group_flags(const uint32_t* __restrict__ boff, int64_t B, GridDev g0, Meshes M,
+                                 uint32_t* __restrict__ flag)
+{
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < B; b += (int64_t)gridDim.x * blockDim.x) {
+        const int mesh = kM ? (int)M.bmesh[b] : 0;
+        const GridDev& g = grid_of<kM>(g0, M, mesh);
+        const int64_t lb = b - (int64_t)(kM ? M.off[mesh].b0 : 0u);
+        const bool st = lb == 0 || lb % (int64_t)l2_group_cap(g.w2) == 0 || (boff[b] >> 6) != (boff[b - 1] >> 6);
+        flag[b] = st ? 1u : 0u;
+    }
+}
+
+__global__ void k_l2_group_starts(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ gidx, int64_t B,
+                                  uint32_t* __restrict__ gstart)
+{
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < B; b += (int64_t)gridDim.x * blockDim.x) {
+        if (flag[b]) gstart[gidx[b]] = (uint32_t)b;
+        if (b == B - 1) gstart[gidx[B]] = (uint32_t)B;
+    }
+}
+
+void launch_l2_groups(const uint32_t* boff, int64_t B, const GridDev& g0, const Meshes& M, uint32_t* flag,
+                      uint32_t* gidx, uint32_t* scan_tmp, uint32_t* gstart, hipStream_t s)
+{
+    if (B <= 0) return;
+    const int blocks = grid_for(B, 256, 16384);
+    if (M.nm > 1)
+        k_l2_group_flags<true><<<blocks, 256, 0, s>>>(boff, B, g0, M, flag);
+    else
+        k_l2_group_flags<false><<<blocks, 256, 0, s>>>(boff, B, g0, M, flag);
+    launch_scan_u32_u32(flag, gidx, B, scan_tmp, nullptr, s);
+    k_l2_group_starts<<<blocks, 256, 0, s>>>(flag, gidx, B, gstart);
+}
+
+// LDS of one wave of the Level2 SAT
+struct L2Lds {
+    unsigned long long acc[64];   // Level2 boundary words of the group's voxels
+    float tv[64 * 9];             // staged triangles
+    uint32_t q[L2_QCAP];          // ring of queued SAT entries
+    uint32_t dec[64];             // staged entry: sub-box origin/extent and box bit
+    unsigned long long mk[64];    // staged entry: box-axis masks
+    float4 sl[64];                // staged entry: Level2 slab
+    uint32_t off[64];             // staged entry: first item in the chunk's item space
+    uint8_t ev[64];               // staged entry: its voxel in the group
+    uint32_t vb[L2_GMAX + 1];     // group voxel v: first bin entry
+    float c[L2_CMAX];             // centers: x [v*n2+p], y [nv*n2 + ...], z [2*nv*n2 + ...]
+};
+
+// Full float SAT for queued entries (one per lane); entry = p | q<<4 | r<<8 |
+// j<<12 | box<<18 where j is the staged triangle (its voxel is ev[j]) and box
+// says the box-axis predicates are already known to hold.
+__device__ __forceinline__ void l2_sat_run(uint32_t ent, bool valid, const L2Lds& L, unsigned long long* acc, int n2,
+                                           int nvn2, int w2, float h2x, float h2y, float h2z)
+{
+    const int p = ent & 15, q = (ent >> 4) & 15, r = (ent >> 8) & 15, j = (ent >> 12) & 63;
+    const bool box = (ent >> 18) & 1;
+    const int v = valid ? (int)L.ev[j] : 0;
+    const int s = (r * n2 + q) * n2 + p;
+    const int wi = v * w2 + (s >> 6);
+    if (!valid || ((acc[wi] >> (s & 63)) & 1ull)) return;
+    const float ccx = L.c[v * n2 + p], ccy = L.c[nvn2 + v * n2 + q], ccz = L.c[2 * nvn2 + v * n2 + r];
+    float tv[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) tv[c] = L.tv[j * 9 + c];
+    // every queued entry's column passed xy_edges_ok
+    const bool hit = box ? tri_box_overlap<false, false>(ccx, ccy, ccz, h2x, h2y, h2z, tv)
+                         : tri_box_overlap<true, false>(ccx, ccy, ccz, h2x, h2y, h2z, tv);
+    if (hit) atomicOr(&acc[wi], 1ull << (s & 63));
+}
+
+template <bool kM>
+__global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_grp(const float* __restrict__ tris,
+                                                          const int32_t* __restrict__ bidx,
+                                                          const uint32_t* __restrict__ boff,
+                                                          const uint32_t* __restrict__ list,
+                                                          const uint64_t* __restrict__ pmask,
+                                                          const float4* __restrict__ slab2,
+                                                          const uint32_t* __restrict__ gstart,
+                                                          const uint32_t* __restrict__ ngroups, GridDev g0,
+                                                          Meshes M, uint64_t* __restrict__ sat2)
+{
+    __shared__ L2Lds s_l[L2_WAVES];
+    const int wv = threadIdx.x >> 6;
+    const int lane = __lane_id();
+    L2Lds& L = s_l[wv];
+    unsigned long long* acc = L.acc;
+    const uint32_t G = *ngroups;
+    for (uint32_t grp = blockIdx.x * L2_WAVES + wv; grp < G; grp += gridDim.x * L2_WAVES) {
+        const uint32_t vb0 = gstart[grp];
+        const int nv = (int)(gstart[grp + 1] - vb0);
+        const int mesh = kM ? uniform_i((int)M.bmesh[vb0]) : 0;
+        const GridDev& g = grid_of<kM>(g0, M, mesh);
+        const int n2 = g.n2, w2 = g.w2, gnx = g.nx, gny = g.ny;
+        const float h2x = g.h2x, h2y = g.h2y, h2z = g.h2z;
+        if (kM && n2 == 0) continue;
+        const int nvn2 = nv * n2;
+        uint64_t* out =
+            sat2 + (kM ? M.off[mesh].l2w0 + (uint64_t)(vb0 - M.off[mesh].b0) * w2 : (uint64_t)vb0 * w2);
+        for (int i = lane; i < nv * w2; i += 64) acc[i] = 0ull;
+        if (lane <= nv) L.vb[lane] = boff[vb0 + lane];
+        // the group's Level2 centers: x, y, z tables of each voxel
+        for (int i = lane; i < nvn2; i += 64) {
+            const int v = i / n2, pp = i - v * n2;
+            const int vox = bidx[vb0 + v];
+            const int nxy = gnx * gny;
+            const int k1 = vox / nxy;
+            const int ij = vox - k1 * nxy;
+            const int j1 = ij / gnx;
+            const int i1 = ij - j1 * gnx;
+            L.c[i] = g.c2x[(int64_t)i1 * n2 + pp];
+            L.c[nvn2 + i] = g.c2y[(int64_t)j1 * n2 + pp];
+            L.c[2 * nvn2 + i] = g.c2z[(int64_t)k1 * n2 + pp];
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t qh = 0, qt = 0;   // ring head/tail (uniform)
+        const uint32_t e0 = L.vb[0], e1 = L.vb[nv];
+        for (uint32_t base = e0; base < e1; base += 64) {
+            // the queue refers to staged triangles by slot: drain it before restaging
+            while (qt != qh) {
+                const uint32_t n = qt - qh < 64 ? qt - qh : 64;
+                const bool valid = (uint32_t)lane < n;
+                const uint32_t ent = valid ? L.q[(qh + lane) % L2_QCAP] : 0u;
+                l2_sat_run(ent, valid, L, acc, n2, nvn2, w2, h2x, h2y, h2z);
+                qh += n;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // staging: lane j owns bin entry base + j -- vertices, masks, slab,
+            // sub-box decode and its voxel go to LDS
+            const uint32_t i = base + lane;
+            uint32_t P = 0;
+            if (i < e1) {
+                const uint64_t t = list[i];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) L.tv[lane * 9 + c] = tris[t * 9 + c];
+                int lo = 0, hi = nv;   // the voxel whose entries hold i: vb[lo] <= i < vb[lo + 1]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L.vb[mid] <= i) lo = mid;
+                    else hi = mid;
+                }
+                L.ev[lane] = (uint8_t)lo;
+                const uint64_t mk = pmask[i];
+                const uint32_t mx = (uint32_t)(mk & 0xffffu), my = (uint32_t)((mk >> 16) & 0xffffu),
+                               mz = (uint32_t)((mk >> 32) & 0xffffu);
+                if (mx && my && mz) {
+                    const int pa = __ffs(mx) - 1, pb = 31 - __clz(mx);
+                    const int qa = __ffs(my) - 1, qb = 31 - __clz(my);
+                    const int ra = __ffs(mz) - 1, rb = 31 - __clz(mz);
+                    const int dp = pb - pa + 1, dq = qb - qa + 1, dr = rb - ra + 1;
+                    // contiguous passing sets (centers are monotone in the index) mean
+                    // the box-axis predicates hold on the whole sub-box
+                    const uint32_t boxbit =
+                        (__popc(mx) == dp && __popc(my) == dq && __popc(mz) == dr) ? 1u : 0u;
+                    L.dec[lane] = (uint32_t)pa | ((uint32_t)qa << 4) | ((uint32_t)ra << 8) | ((uint32_t)rb << 12) |
+                                  ((uint32_t)dp << 16) | (boxbit << 24);
+                    L.mk[lane] = mk;
+                    L.sl[lane] = slab2[t];
+                    P = (uint32_t)(dp * dq);
+                }
+            }
+            const uint32_t incl = wave_incl_scan<uint32_t>(P);
+            L.off[lane] = incl - P;
+            const uint32_t S = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            __builtin_amdgcn_wave_barrier();
+            // the chunk's (entry, sub-column) items, 64 per round
+            for (uint32_t l0 = 0; l0 < S; l0 += 64) {
+                const uint32_t l = l0 + (uint32_t)lane;
+                uint32_t keep = 0;
+                int p = 0, q = 0, j = 0, v = 0, ra = 0, rb = -1;
+                uint32_t boxbit = 0, mz = 0;
+                if (l < S) {
+                    // the last entry whose items start at or before l (it has items)
+                    int lo = 0;
+#pragma unroll
+                    for (int st = 32; st; st >>= 1)
+                        if (L.off[lo + st] <= l) lo += st;
+                    j = lo;
+                    v = (int)L.ev[j];
+                    const uint32_t d = L.dec[j];
+                    const int pa = d & 15, qa = (d >> 4) & 15;
+                    ra = (d >> 8) & 15;
+                    rb = (d >> 12) & 15;
+                    const int dp = (d >> 16) & 255;
+                    boxbit = (d >> 24) & 1u;
+                    const int lj = (int)(l - L.off[j]);
+                    // exact small-integer division: (lj + 0.5)/dp is >= 1/32 from an integer
+                    const int rem = (int)(((float)lj + 0.5f) * kInvSmall[dp]);
+                    p = pa + lj - rem * dp;
+                    q = qa + rem;
+                    const uint64_t mk = L.mk[j];
+                    const uint32_t mx = (uint32_t)(mk & 0xffffu), my = (uint32_t)((mk >> 16) & 0xffffu);
+                    mz = (uint32_t)((mk >> 32) & 0xffffu);
+                    if ((mx >> p) & (my >> q) & 1u) {
+                        const float ccx = L.c[v * n2 + p], ccy = L.c[nvn2 + v * n2 + q];
+                        // the e x z edge predicates hold or fail for the whole sub-column
+                        if (xy_edges_ok(ccx, ccy, h2x, h2y, L.tv + j * 9)) {
+                            const float4 sl = L.sl[j];
+                            const float t0x = L.tv[j * 9 + 0], t0y = L.tv[j * 9 + 1], t0z = L.tv[j * 9 + 2];
+                            // float_slab_keeps, with the (x, y) part shared by the sub-column
+                            const float A = sl.x * (ccx - t0x) + sl.y * (ccy - t0y);
+                            const float* cz = L.c + 2 * nvn2 + v * n2;
+                            for (int r = ra; r <= rb; ++r) {
+                                const float sd = A + sl.z * (cz[r] - t0z);
+                                keep |= (uint32_t)(fabsf(sd) <= sl.w && ((mz >> r) & 1u)) << r;
+                            }
+                        }
+                    }
+                }
+                // queue the kept sub-voxels not yet set, one r per lane per round
+                while (__ballot(keep != 0)) {
+                    const int r = keep ? __ffs(keep) - 1 : 0;
+                    const bool has = keep != 0;
+                    keep &= keep - 1;
+                    const int sv = (r * n2 + q) * n2 + p;
+                    const bool push = has && !((acc[(v * w2 + (sv >> 6)) & 63] >> (sv & 63)) & 1ull);
+                    const uint64_t m = __ballot(push);
+                    if (push) {
+                        const uint32_t at = qt + (uint32_t)__popcll(m & lanes_below_mask());
+                        L.q[at % L2_QCAP] = (uint32_t)p | ((uint32_t)q << 4) | ((uint32_t)r << 8) |
+                                            ((uint32_t)j << 12) | (boxbit << 18);
+                    }
+                    qt += (uint32_t)__popcll(m);
+                    __builtin_amdgcn_wave_barrier();
+                    if (qt - qh >= 64) {   // a full wave of work
+                        const uint32_t ent = L.q[(qh + lane) % L2_QCAP];
+                        l2_sat_run(ent, true, L, acc, n2, nvn2, w2, h2x, h2y, h2z);
+                        qh += 64;
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
+            }
+        }
+        while (qt != qh) {
+            const uint32_t n = qt - qh < 64 ? qt - qh : 64;
+            const bool valid = (uint32_t)lane < n;
+            const uint32_t ent = valid ? L.q[(qh + lane) % L2_QCAP] : 0u;
+            l2_sat_run(ent, valid, L, acc, n2, nvn2, w2, h2x, h2y, h2z);
+            qh += n;
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < nv * w2; i += 64) out[i] = acc[i];
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// One wave per boundary voxel: the form for voxels with many bin entries
+// (the parent's sub-centers stay in lane registers, entry p in lane p).
 // Full float SAT for queued entries (one per lane); entry = p | q<<4 | r<<8 |
 // j<<12 | box<<18 where j is the staged triangle and box says the box-axis
 // predicates are already known to hold.
-__device__ __forceinline__ void l2_sat_run(uint32_t ent, bool valid, const float* stv, float cxl, float cyl,
+__device__ __forceinline__ void l2_sat_run_v(uint32_t ent, bool valid, const float* stv, float cxl, float cyl,
                                            float czl, int n2, float h2x, float h2y, float h2z,
                                            unsigned long long* acc)
 {
@@ -1386,7 +1654,7 @@ __device__ __forceinline__ void l2_sat_run(uint32_t ent, bool valid, const float
 }
 
 template <bool kM>
-__global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restrict__ tris,
+__global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_voxel(const float* __restrict__ tris,
                                                           const int32_t* __restrict__ bidx, int64_t B,
                                                           const uint32_t* __restrict__ boff,
                                                           const uint32_t* __restrict__ list,
@@ -1447,7 +1715,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
             const uint32_t n = qt - qh < 64 ? qt - qh : 64;
             const bool valid = (uint32_t)lane < n;
             const uint32_t ent = valid ? sq[(qh + lane) % L2_QCAP] : 0u;
-            l2_sat_run(ent, valid, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
+            l2_sat_run_v(ent, valid, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
             qh += n;
         }
         __builtin_amdgcn_wave_barrier();
@@ -1544,7 +1812,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
                 __builtin_amdgcn_wave_barrier();
                 if (qt - qh >= 64) {   // a full wave of work
                     const uint32_t ent = sq[(qh + lane) % L2_QCAP];
-                    l2_sat_run(ent, true, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
+                    l2_sat_run_v(ent, true, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
                     qh += 64;
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -1555,7 +1823,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
         const uint32_t n = qt - qh < 64 ? qt - qh : 64;
         const bool valid = (uint32_t)lane < n;
         const uint32_t ent = valid ? sq[(qh + lane) % L2_QCAP] : 0u;
-        l2_sat_run(ent, valid, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
+        l2_sat_run_v(ent, valid, stv, cxl, cyl, czl, n2, h2x, h2y, h2z, acc);
         qh += n;
     }
     __builtin_amdgcn_wave_barrier();
@@ -1563,17 +1831,25 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat(const float* __restric
 }
 
 void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
-                   const uint64_t* pmask, const float4* slab2, const GridDev& g0, const Meshes& M, uint64_t* sat2,
-                   hipStream_t s)
+                   const uint64_t* pmask, const float4* slab2, const uint32_t* gstart, const uint32_t* ngroups,
+                   int blocks, const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s)
 {
     if (B <= 0) return;
-    const int64_t blocks = (B + L2_WAVES - 1) / L2_WAVES;
+    const dim3 bs(64 * L2_WAVES);
+    if (gstart) {
+        if (M.nm > 1)
+            k_l2_sat_grp<true><<<(unsigned)blocks, bs, 0, s>>>(tris, bidx, boff, list, pmask, slab2, gstart, ngroups,
+                                                              g0, M, sat2);
+        else
+            k_l2_sat_grp<false><<<(unsigned)blocks, bs, 0, s>>>(tris, bidx, boff, list, pmask, slab2, gstart, ngroups,
+                                                               g0, M, sat2);
+        return;
+    }
+    const unsigned vblocks = (unsigned)((B + L2_WAVES - 1) / L2_WAVES);
     if (M.nm > 1)
-        k_l2_sat<true><<<(unsigned)blocks, 64 * L2_WAVES, 0, s>>>(tris, bidx, B, boff, list, pmask, slab2, g0, M,
-                                                                  sat2);
+        k_l2_sat_voxel<true><<<vblocks, bs, 0, s>>>(tris, bidx, B, boff, list, pmask, slab2, g0, M, sat2);
     else
-        k_l2_sat<false><<<(unsigned)blocks, 64 * L2_WAVES, 0, s>>>(tris, bidx, B, boff, list, pmask, slab2, g0, M,
-                                                                   sat2);
+        k_l2_sat_voxel<false><<<vblocks, bs, 0, s>>>(tris, bidx, B, boff, list, pmask, slab2, g0, M, sat2);
 }
 
 // ---------------------------------------------------------------------------

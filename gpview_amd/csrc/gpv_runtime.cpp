@@ -215,6 +215,7 @@ struct Job {
     uint32_t* coff = nullptr;
     float4* crec = nullptr;   // 3 float4 per column entry
     uint64_t* cpass = nullptr;   // det/u/v pass bits per column entry and sub-column
+    uint32_t *gflag = nullptr, *gidx = nullptr, *gstart = nullptr;   // Level2 SAT voxel groups
     uint32_t *hcnt = nullptr, *hoff = nullptr;
     float4* hrec = nullptr;
     uint64_t *sat2 = nullptr, *par2 = nullptr;
@@ -793,14 +794,64 @@ int run_l2_parity(Job* J)
     return GPV_OK;
 }
 
+// group arrays of the Level2 SAT (allocated outside any two-stream region)
+int alloc_l2_groups(Job* J)
+{
+    const size_t n = (size_t)std::max<int64_t>(J->B, 1) + 1;
+    release(J, J->gflag);
+    release(J, J->gidx);
+    release(J, J->gstart);
+    TRY(alloc(J, &J->gflag, n));
+    TRY(alloc(J, &J->gidx, n));
+    TRY(alloc(J, &J->gstart, n));
+    return ensure_scan_tmp(J, J->B);
+}
+
+void release_l2_groups(Job* J)
+{
+    release(J, J->gflag);
+    release(J, J->gidx);
+    release(J, J->gstart);
+}
+
+// the Level2 SAT kernels on the main stream (group arrays allocated)
+int l2_sat_kernels(Job* J)
+{
+    hipStream_t s = J->ctx->stream;
+    HIP_TRY(hipEventRecord(J->ev[6], s));
+    // voxels with few bin entries share waves (k_l2_sat_grp); with many (an
+    // average of 8 or more per voxel) one voxel per wave is faster
+    const bool grouped = J->B > 0 && (int64_t)J->n_pairs < 8 * J->B;
+    if (grouped) launch_l2_groups(J->boff, J->B, J->g[0], J->view(), J->gflag, J->gidx, J->scan32, J->gstart, s);
+    // one wave per group, for an upper bound on the group count computed
+    // here (a group starts at a mesh start, at a new 64-entry chunk of first
+    // bin entries, or after a full group), so the grid is sized without
+    // reading the count back; waves past the real count exit
+    int64_t gmax = J->B;
+    if (grouped) {
+        gmax = (int64_t)(J->n_pairs / 64) + 1;
+        for (int m = 0; m < J->nm; ++m) {
+            const int64_t nb = J->nb(m);
+            const int w2 = J->g[m].w2;
+            const int64_t cap = std::min<int64_t>(w2 > 0 ? 64 / w2 : 64, 8);
+            gmax += (nb + cap - 1) / cap + 1;
+        }
+        gmax = std::min<int64_t>(gmax, J->B);
+    }
+    const int blocks = (int)std::max<int64_t>(1, (gmax + 3) / 4);
+    launch_l2_sat(J->tris, J->bidx, J->B, J->boff, J->list, J->pmask, J->slab2, grouped ? J->gstart : nullptr,
+                  J->gidx + J->B, blocks, J->g[0], J->view(), J->sat2, s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(J->ev[7], s));
+    return GPV_OK;
+}
+
 int run_l2_sat(Job* J)
 {
     if (J->n2max <= 0) return GPV_OK;
-    hipStream_t s = J->ctx->stream;
-    HIP_TRY(hipEventRecord(J->ev[6], s));
-    launch_l2_sat(J->tris, J->bidx, J->B, J->boff, J->list, J->pmask, J->slab2, J->g[0], J->view(), J->sat2, s);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(J->ev[7], s));
+    TRY(alloc_l2_groups(J));
+    TRY(l2_sat_kernels(J));
+    release_l2_groups(J);
     J->l2s_done = true;
     return GPV_OK;
 }
@@ -815,13 +866,16 @@ int run_level2(Job* J)
     gpv_ctx* c = J->ctx;
     hipStream_t s = c->stream;
     TRY(alloc_cpass(J));
+    TRY(alloc_l2_groups(J));
     HIP_TRY(hipEventRecord(J->ev[5], s));
     HIP_TRY(hipStreamWaitEvent(c->side, J->ev[5], 0));
     TRY(l2_parity_kernels(J, c->side));
-    TRY(run_l2_sat(J));
+    TRY(l2_sat_kernels(J));
     HIP_TRY(hipStreamWaitEvent(s, J->ev[11], 0));
     release(J, J->cpass);
+    release_l2_groups(J);
     J->l2p_done = true;
+    J->l2s_done = true;
     return GPV_OK;
 }
 
