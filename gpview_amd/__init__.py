@@ -28,7 +28,7 @@ class GpvError(RuntimeError):
 class Params(ctypes.Structure):
     """gpv_params (include/gpv.h)."""
     _fields_ = [("voxel_count", ctypes.c_int), ("voxel_count2", ctypes.c_int), ("level2", ctypes.c_int),
-                ("normals", ctypes.c_int), ("initial_capacity", ctypes.c_int64)]
+                ("normals", ctypes.c_int), ("initial_capacity", ctypes.c_int64), ("cuda_level2", ctypes.c_int)]
 
 
 class GridInfo(ctypes.Structure):
@@ -312,11 +312,13 @@ class Context:
         return s.as_dict()
 
     def voxelize(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0,
-                 device_ptr=None, normals=False):
-        """Object::PerformVoxelization on a flat soup; returns a Grid."""
+                 device_ptr=None, normals=False, cuda_level2=False):
+        """Object::PerformVoxelization on a flat soup; returns a Grid.
+        cuda_level2: the Level2 arithmetic of the reference's CUDA kernels."""
         soup = np.ascontiguousarray(soup, np.float32).reshape(-1, 9)
         p = Params(voxel_count=int(voxel_count), voxel_count2=int(voxel_count2), level2=1 if level2 else 0,
-                   normals=1 if normals else 0, initial_capacity=int(initial_capacity))
+                   normals=1 if normals else 0, initial_capacity=int(initial_capacity),
+                   cuda_level2=1 if cuda_level2 else 0)
         g = _vp()
         ptr = device_ptr if device_ptr is not None else soup.ctypes.data
         _check(lib().gpv_voxelize(self.h, ptr, len(soup), 1 if device_ptr is not None else 0,
@@ -367,7 +369,8 @@ def mesh_desc(m, device_ptr=None):
     d.max_model_size = float(m["mms"])
     d.params = Params(voxel_count=int(m["voxel_count"]), voxel_count2=int(m.get("voxel_count2", 4)),
                       level2=1 if m.get("level2", True) else 0, normals=1 if m.get("normals", False) else 0,
-                      initial_capacity=int(m.get("initial_capacity", 0)))
+                      initial_capacity=int(m.get("initial_capacity", 0)),
+                      cuda_level2=1 if m.get("cuda_level2", False) else 0)
     return d, soup
 
 

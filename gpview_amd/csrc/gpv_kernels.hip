@@ -2027,7 +2027,8 @@ __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev
         if (i < e1) {
             const ColRec r = load_colrec(rec, i);
             cr[lane] = r;
-            if (r.zmax >= k1c) {
+            // the CUDA kernels' candidates: the whole column's triangle list
+            if (g.l2cuda || r.zmax >= k1c) {
 #pragma unroll
                 for (int mm = 0; mm < NMX; ++mm)
                     if (mm < nm) row[mm] = cpass[(uint64_t)i * stride + mm];

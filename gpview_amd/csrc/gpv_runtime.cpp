@@ -322,7 +322,9 @@ int derive_grid(const float bmin[3], const float bmax[3], float mms, const gpv_p
 //   L1 ray  min + (i + 0.5)*g              src/Object.cpp:743-745
 //   L2 SAT  (p + 0.5)*g2 + mid - ext1      src/Object.cpp:2369-2371, 2389-2391
 //   L2 ray  minPoint + (p + 0.5)*g2        src/Object.cpp:1134-1136, 1158-1160
-void build_tables(const gpv_grid_info& I, std::vector<float>& tab, size_t off[12])
+// cuda_l2: both Level2 tables in the CUDA kernels' float expression
+//   (2p+1)*ext2 + mid - ext1               cuda/CUDAClassifyTessellation.cu:423-425, 472-474
+void build_tables(const gpv_grid_info& I, bool cuda_l2, std::vector<float>& tab, size_t off[12])
 {
     const int n2 = I.n2 > 0 ? I.n2 : 0;
     size_t pos = 0;
@@ -340,7 +342,17 @@ void build_tables(const gpv_grid_info& I, std::vector<float>& tab, size_t off[12
         for (int i = 0; i < I.n[a] && n2 > 0; i++) {
             const float mid = (float)((i + 0.5) * (double)I.g[a] + (double)I.bmin[a]);
             const float minp = (float)i * I.g[a] + I.bmin[a];
+            const float ext2 = (float)((double)I.g2[a] / 2.0);
             for (int p = 0; p < n2; p++) {
+                if (cuda_l2) {
+                    const float two_p1 = (float)(2 * p + 1);
+                    const float prod = two_p1 * ext2;
+                    const float sum = prod + mid;
+                    const float c = sum - ext1;
+                    tab[off[6 + a] + (size_t)i * n2 + p] = c;
+                    tab[off[9 + a] + (size_t)i * n2 + p] = c;
+                    continue;
+                }
                 tab[off[6 + a] + (size_t)i * n2 + p] =
                     (float)((p + 0.5) * (double)I.g2[a] + (double)mid - (double)ext1);
                 tab[off[9 + a] + (size_t)i * n2 + p] = (float)((double)minp + (p + 0.5) * (double)I.g2[a]);
@@ -467,6 +479,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         GridDev& g = J->g[m];
         g = grid_dev(J->info[m]);
         g.normals = d[m].params.normals ? 1 : 0;
+        g.l2cuda = d[m].params.cuda_level2 ? 1 : 0;
         J->info[m].has_normals = g.normals;
         J->any_normals = J->any_normals || g.normals;
         J->n2max = std::max(J->n2max, g.n2);
@@ -480,7 +493,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         tri0 += (uint32_t)d[m].ntris;
         std::vector<float> tab;
         size_t o[12];
-        build_tables(J->info[m], tab, o);
+        build_tables(J->info[m], g.l2cuda != 0, tab, o);
         for (int k = 0; k < 12; ++k) tab_at[(size_t)m * 12 + k] = J->h_tab.size() + o[k];
         J->h_tab.insert(J->h_tab.end(), tab.begin(), tab.end());
     }

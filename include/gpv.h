@@ -51,6 +51,17 @@ typedef struct gpv_params {
                                  analogue of PerformVoxelization's bufferSize
                                  (src/Object.cpp:3088-3089); overflow regrows and
                                  reruns Level1 binning (src/Object.cpp:3198-3214) */
+    int cuda_level2;          /* 0: the CPU composition (the contract).  1: the
+                                 Level2 arithmetic of the reference's CUDA kernels
+                                 (cuda/CUDAClassifyTessellation.cu:403-504), for
+                                 comparing with output of its GPU build: Level2
+                                 SAT centers and ray origins in float,
+                                 (2p+1)*ext2 + mid - ext1 (no FMA contraction), and
+                                 Level2 parity over the whole xy column's triangle
+                                 list instead of the bins at z >= k1.  Level1 and
+                                 the normals keep the CPU semantics.  Not pinned
+                                 against the CUDA build (it needs nvcc and an
+                                 NVIDIA GPU). */
 } gpv_params;
 
 /* ---- grid description (VoxelData, includes/Utilities.h:79-154) ----------- */

@@ -30,6 +30,9 @@ struct GLParameters {
     bool level2Voxels = true;
     bool saveVoxels = true;
     std::string outputDir = ".";
+    // the Level2 arithmetic of the reference's CUDACLASSIFYTESSELLATION build
+    // (src/Object.cpp:3342-3347; gpv_params::cuda_level2)
+    bool cudaClassifyTessellation = false;
 };
 
 class Object {

@@ -315,6 +315,19 @@ void orc_l1_ray_origins(const orc_grid *g, int a, float *out)
         out[i] = (float)((double)g->bmin[a] + (i + 0.5) * (double)g->g[a]);
 }
 
+/* The CUDA kernels' Level2 center and ray origin, one float expression for
+ * both (cuda/CUDAClassifyTessellation.cu:423-425, 472-474):
+ * (2p+1)*boxExtentsLevel2 + midPoint - boxExtentsLevel1, evaluated left to
+ * right in float, midPoint = (float)((i + 0.5)*gridSize + bBoxMin) from the
+ * host (src/Object.cpp:2566-2569), extents (float)(gridSize/2.0)
+ * (src/Object.cpp:2553-2554).  No multiply-add contraction. */
+float orc_l2_cuda_center(const orc_grid *g, int a, float mid, int p)
+{
+    float prod = (float)(2 * p + 1) * g->ext2[a];
+    float sum = prod + mid;
+    return sum - g->ext1[a];
+}
+
 /* Level2 SAT centers (src/Object.cpp:2369-2371, 2389-2391):
  * mid = (i + 0.5)*g + bBoxMin; center = (p + 0.5)*g2 + mid - ext1. */
 void orc_l2_centers(const orc_grid *g, int a, float *out)
@@ -323,8 +336,9 @@ void orc_l2_centers(const orc_grid *g, int a, float *out)
     for (int i = 0; i < g->n[a]; i++) {
         float mid = (float)((i + 0.5) * (double)g->g[a] + (double)g->bmin[a]);
         for (int p = 0; p < n2; p++)
-            out[i * n2 + p] =
-                (float)((p + 0.5) * (double)g->g2[a] + (double)mid - (double)g->ext1[a]);
+            out[i * n2 + p] = g->cuda_l2 ? orc_l2_cuda_center(g, a, mid, p)
+                                         : (float)((p + 0.5) * (double)g->g2[a] + (double)mid -
+                                                   (double)g->ext1[a]);
     }
 }
 
@@ -335,8 +349,10 @@ void orc_l2_ray_origins(const orc_grid *g, int a, float *out)
     int n2 = g->n2;
     for (int i = 0; i < g->n[a]; i++) {
         float minp = (float)i * g->g[a] + g->bmin[a];
+        float mid = (float)((i + 0.5) * (double)g->g[a] + (double)g->bmin[a]);
         for (int p = 0; p < n2; p++)
-            out[i * n2 + p] = (float)((double)minp + (p + 0.5) * (double)g->g2[a]);
+            out[i * n2 + p] = g->cuda_l2 ? orc_l2_cuda_center(g, a, mid, p)
+                                         : (float)((double)minp + (p + 0.5) * (double)g->g2[a]);
     }
 }
 
@@ -623,7 +639,8 @@ static void l2_parity_exact(const ctx_t *c, const orc_result *r, const float *ox
                     for (int p = 0; p < n2; p++) {
                         float o[3] = {ox[i1 * n2 + p], oy[j1 * n2 + q], oz[k1 * n2 + s]};
                         int hits = 0;
-                        for (int64_t z = k1; z < nz; z++) {
+                        /* the CUDA kernel's candidates: every bin of the column */
+                        for (int64_t z = c->g->cuda_l2 ? 0 : k1; z < nz; z++) {
                             int64_t v = z * nx * ny + ij;
                             for (int64_t e = r->list_off[v]; e < r->list_off[v + 1]; e++) {
                                 int32_t t = r->list_tri[e];
@@ -712,7 +729,7 @@ static void l2_parity_fast(const ctx_t *c, const orc_result *r, const float *ox,
                         for (int s = 0; s < n2; s++) {
                             int pp = 0;
                             for (int64_t h = 0; h < nh; h++)
-                                if (hz[h] >= k1) pp ^= mt_t(&hs[h], oz[k1 * n2 + s]);
+                                if (c->g->cuda_l2 || hz[h] >= k1) pp ^= mt_t(&hs[h], oz[k1 * n2 + s]);
                             par[b * n23 + ((int64_t)s * n2 + q) * n2 + p] = (uint8_t)pp;
                         }
                     }

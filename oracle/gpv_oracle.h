@@ -41,6 +41,10 @@ typedef struct orc_grid {
     int n2;               /* Level2 divisions per axis, 0 = Level2 off */
     float g2[3];          /* Level2 grid size gridSizeX2/Y2/Z2 */
     float ext2[3];        /* Level2 half extents (float)(g2/2.0) */
+    int cuda_l2;          /* 1: Level2 in the arithmetic of the reference's CUDA
+                             kernels (cuda/CUDAClassifyTessellation.cu:403-504):
+                             float centers/origins (2p+1)*ext2 + mid - ext1 and
+                             parity over the whole xy column's triangle list */
 } orc_grid;
 
 typedef struct orc_result {
@@ -87,6 +91,7 @@ void orc_l1_centers(const orc_grid *g, int axis, float *out);      /* [n]  SAT c
 void orc_l1_ray_origins(const orc_grid *g, int axis, float *out);  /* [n]  ray origins   */
 void orc_l2_centers(const orc_grid *g, int axis, float *out);      /* [n*n2] SAT centers */
 void orc_l2_ray_origins(const orc_grid *g, int axis, float *out);  /* [n*n2] ray origins */
+float orc_l2_cuda_center(const orc_grid *g, int axis, float mid, int p);
 
 /* Integer index range of one vertex coordinate (src/Object.cpp:2272-2300). */
 int orc_vertex_index(float v, float mn, float mx, int n);

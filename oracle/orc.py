@@ -28,6 +28,7 @@ class Grid(ctypes.Structure):
         ("n2", ctypes.c_int),
         ("g2", ctypes.c_float * 3),
         ("ext2", ctypes.c_float * 3),
+        ("cuda_l2", ctypes.c_int),
     ]
 
 
@@ -155,10 +156,13 @@ def bbox(verts):
     return bmin, bmax, np.float32(mms.value)
 
 
-def grid_setup(bmin, bmax, mms, voxel_count, voxel_count2):
+def grid_setup(bmin, bmax, mms, voxel_count, voxel_count2, cuda_l2=False):
+    """orc_grid_setup; cuda_l2 selects the Level2 arithmetic of the
+    reference's CUDA kernels (orc_grid.cuda_l2)."""
     g = Grid()
     lib().orc_grid_setup(_f32(bmin), _f32(bmax), float(mms), int(voxel_count), int(voxel_count2),
                          ctypes.byref(g))
+    g.cuda_l2 = 1 if cuda_l2 else 0
     return g
 
 
