@@ -38,7 +38,7 @@ def make_workload(gv):
     return v, f, gv.flatten(v, f)
 
 
-def bench_batched(gv, ctx, n_meshes, steps, warmup, check, small=False):
+def bench_batched(gv, ctx, n_meshes, steps, warmup, check, small=False, autotune=False):
     """The mixed-size stream: n_meshes meshes (triangles log-uniform in
     [1k, 200k], Level1 {64,128,256}, Level2 {4,8}), device-resident soups.
     Plain = one gpv_voxelize per mesh; batched = one gpv_voxelize_batch over
@@ -52,6 +52,10 @@ def bench_batched(gv, ctx, n_meshes, steps, warmup, check, small=False):
         meshes = workloads.mixed_stream(gv, n_meshes)
     ptrs = [ctx.upload(m["soup"]) for m in meshes]
     T = sum(len(m["soup"]) for m in meshes)
+    tune = None
+    if autotune:
+        shapes, tms = ctx.autotune(meshes, device_ptrs=ptrs, repeats=2)
+        tune = {"shapes": shapes, "device_ms": tms}
 
     def plain_pass():
         for m, p in zip(meshes, ptrs):
@@ -118,6 +122,7 @@ def bench_batched(gv, ctx, n_meshes, steps, warmup, check, small=False):
         "batched_stage_ms": stage_ms,
         "totals": tot,
         "bit_exact_vs_plain": exact,
+        "tune": tune,
     }
 
 
@@ -257,6 +262,8 @@ def main():
     ap.add_argument("--batch-meshes", type=int, default=256, help="meshes of the mixed-stream batch (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-check", action="store_true", help="skip the batched-vs-plain bit-exact check")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick launch shapes with gpv_ctx_autotune on each workload before timing it")
     ap.add_argument("--serve-requests", type=int, default=1000, help="served-stream requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="served-stream arrivals per second")
     ap.add_argument("--serve-lanes", type=int, default=2)
@@ -284,6 +291,11 @@ def main():
     bmin, bmax, mms = gv.mesh_bbox(v)
     ctx = gv.Context(0)
     dsoup = ctx.upload(soup)
+    tune = None
+    if args.autotune:
+        shapes, tms = ctx.autotune([{"soup": soup, "bmin": bmin, "bmax": bmax, "mms": mms, "voxel_count": 256,
+                                     "voxel_count2": 8}], device_ptrs=[dsoup], repeats=3)
+        tune = {"shapes": shapes, "device_ms": tms}
 
     def step():
         return ctx.voxelize(soup, bmin, bmax, mms, 256, 8, True, device_ptr=dsoup)
@@ -309,8 +321,10 @@ def main():
     ctx.free(dsoup)
     batched = batched_small = None
     if args.batch_meshes > 0:
-        batched = bench_batched(gv, ctx, args.batch_meshes, args.batch_steps, 1, not args.no_check)
-        batched_small = bench_batched(gv, ctx, args.batch_meshes, args.batch_steps, 1, not args.no_check, small=True)
+        batched = bench_batched(gv, ctx, args.batch_meshes, args.batch_steps, 1, not args.no_check,
+                                autotune=args.autotune)
+        batched_small = bench_batched(gv, ctx, args.batch_meshes, args.batch_steps, 1, not args.no_check, small=True,
+                                      autotune=args.autotune)
     ctx.close()
     served = None
     if args.serve_requests > 0:
@@ -383,6 +397,7 @@ def main():
         "roofline": rl,
         "roofline_valu": valu,
         "cpu_baseline": cpu,
+        "tune": tune,
         "batched": batched,
         "batched_small_meshes": batched_small,
         "served": served,

@@ -62,6 +62,15 @@ class PoolStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Tune(ctypes.Structure):
+    """gpv_tune (include/gpv.h)."""
+    _fields_ = [("expand_blocks_per_cu", ctypes.c_int), ("l2_group_below", ctypes.c_int),
+                ("sort_items_per_block", ctypes.c_int), ("l2_overlap", ctypes.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class MeshDesc(ctypes.Structure):
     """gpv_mesh_desc (include/gpv.h)."""
     _fields_ = [("tris", ctypes.c_void_p), ("ntris", ctypes.c_int64), ("bmin", ctypes.c_float * 3),
@@ -96,7 +105,8 @@ _lib = None
 # every symbol include/gpv.h and include/gpv_mesh.h declare
 EXPORTED = [
     "gpv_ctx_create", "gpv_ctx_destroy", "gpv_last_error", "gpv_status_string",
-    "gpv_device_upload", "gpv_device_free", "gpv_ctx_pool_stats",
+    "gpv_device_upload", "gpv_device_free", "gpv_ctx_pool_stats", "gpv_ctx_get_tune", "gpv_ctx_set_tune",
+    "gpv_ctx_autotune",
     "gpv_mesh_bbox", "gpv_grid_derive", "gpv_voxelize", "gpv_voxelize_batch", "gpv_grid_create",
     "gpv_classify_tessellation", "gpv_classify_inout", "gpv_compact_boundary",
     "gpv_classify_inout_level2", "gpv_classify_tessellation_level2", "gpv_grid_finish",
@@ -128,6 +138,9 @@ def lib():
     L.gpv_device_upload.argtypes = [_vp, _vp, c_i64, P(_vp)]
     L.gpv_device_free.argtypes = [_vp, _vp]
     L.gpv_ctx_pool_stats.argtypes = [_vp, P(PoolStats)]
+    L.gpv_ctx_get_tune.argtypes = [_vp, P(Tune)]
+    L.gpv_ctx_set_tune.argtypes = [_vp, P(Tune)]
+    L.gpv_ctx_autotune.argtypes = [_vp, P(MeshDesc), c_int, c_int, c_int, P(Tune), P(c_float)]
     L.gpv_server_pool_stats.argtypes = [_vp, P(PoolStats)]
     L.gpv_mesh_bbox.argtypes = [_f32p, c_i64, _f32p, _f32p, P(c_float)]
     L.gpv_grid_derive.argtypes = [_f32p, _f32p, c_float, P(Params), P(GridInfo)]
@@ -310,6 +323,32 @@ class Context:
         s = PoolStats()
         _check(lib().gpv_ctx_pool_stats(self.h, ctypes.byref(s)), "gpv_ctx_pool_stats")
         return s.as_dict()
+
+    def get_tune(self):
+        """gpv_ctx_get_tune as a dict."""
+        t = Tune()
+        _check(lib().gpv_ctx_get_tune(self.h, ctypes.byref(t)), "gpv_ctx_get_tune")
+        return t.as_dict()
+
+    def set_tune(self, **fields):
+        """gpv_ctx_set_tune: the current shapes with `fields` replaced."""
+        t = Tune(**dict(self.get_tune(), **fields))
+        _check(lib().gpv_ctx_set_tune(self.h, ctypes.byref(t)), "gpv_ctx_set_tune")
+
+    def autotune(self, meshes, device_ptrs=None, repeats=3):
+        """gpv_ctx_autotune over a batch of mesh dicts; returns (shapes, device ms)."""
+        n = len(meshes)
+        descs = (MeshDesc * n)()
+        keep = []
+        for i, m in enumerate(meshes):
+            d, soup = mesh_desc(m, device_ptrs[i] if device_ptrs is not None else None)
+            descs[i] = d
+            keep.append(soup)
+        t = Tune()
+        ms = ctypes.c_float()
+        _check(lib().gpv_ctx_autotune(self.h, descs, n, 1 if device_ptrs is not None else 0, int(repeats),
+                                      ctypes.byref(t), ctypes.byref(ms)), "gpv_ctx_autotune")
+        return t.as_dict(), ms.value
 
     def voxelize(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0,
                  device_ptr=None, normals=False, cuda_level2=False):

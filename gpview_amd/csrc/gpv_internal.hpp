@@ -128,7 +128,7 @@ size_t ts_tiles(uint64_t nkeys);
 // column entries (key = column << 32 | triangle) -> CSR by column with the MT
 // records; ncol <= 2^27
 void launch_bin_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n, uint32_t ncol,
-                     const TileSortTmp& tmp, uint32_t* coff, float4* rec, hipStream_t s);
+                     const TileSortTmp& tmp, int items_per_block, uint32_t* coff, float4* rec, hipStream_t s);
 
 // Level1 parity tasks: L1P_Z consecutive layers of one 64-column row strip
 constexpr int L1P_Z = 16;
@@ -145,9 +145,9 @@ void launch_boundary_index(const uint64_t* bnd, const uint32_t* wprefix, const G
 // bin pairs -> CSR by boundary rank (boff[B + 1], list, Level2 box-axis
 // masks); B <= 2^27
 void launch_bin_pairs(const uint64_t* bits, const uint32_t* tri, uint64_t n, const uint64_t* bnd,
-                      const uint32_t* wprefix, uint32_t B, const TileSortTmp& tmp, uint32_t* boff,
-                      unsigned int* max_out, uint32_t* list, const float* tris, const int32_t* bidx,
-                      const GridDev& g0, const Meshes& M, uint64_t* pmask, hipStream_t s);
+                      const uint32_t* wprefix, uint32_t B, const TileSortTmp& tmp, int items_per_block,
+                      uint32_t* boff, unsigned int* max_out, uint32_t* list, const float* tris,
+                      const int32_t* bidx, const GridDev& g0, const Meshes& M, uint64_t* pmask, hipStream_t s);
 // per-mesh bin totals and largest bin (multi-mesh runs)
 void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStream_t s);
 

@@ -1186,7 +1186,7 @@ int ts_shift(uint64_t nkeys)
 }
 
 template <typename Src, typename Place>
-static void ts_sort(Src src, uint64_t n, uint32_t nkeys, const TileSortTmp& tmp, uint32_t* off,
+static void ts_sort(Src src, uint64_t n, uint32_t nkeys, const TileSortTmp& tmp, int items_per_block, uint32_t* off,
                     unsigned int* max_out, Place place, hipStream_t s)
 {
     const int shift = ts_shift(nkeys);
@@ -1198,7 +1198,8 @@ static void ts_sort(Src src, uint64_t n, uint32_t nkeys, const TileSortTmp& tmp,
     launch_fill_u32(tmp.total, 0, 2 * (int64_t)ntiles + 1, s);   // total[ntiles], fill[ntiles + 1]
     uint32_t* fill = tmp.total + ntiles;
     if (n > 0) {
-        const int blocks = (int)std::min<uint64_t>(2048, std::max<uint64_t>(1, (n + 4095) / 4096));
+        const uint64_t ipb = (uint64_t)items_per_block;
+        const int blocks = (int)std::min<uint64_t>(2048, std::max<uint64_t>(1, (n + ipb - 1) / ipb));
         const size_t lds = (size_t)ntiles * 4;
         k_ts_count<Src><<<blocks, TS_NT, lds, s>>>(src, n, shift, ntiles, tmp.total);
         launch_scan_u32_u32(tmp.total, tmp.off, ntiles, tmp.scan, nullptr, s);
@@ -1274,23 +1275,23 @@ struct PairPlace {
 };
 
 void launch_bin_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n, uint32_t ncol,
-                     const TileSortTmp& tmp, uint32_t* coff, float4* rec, hipStream_t s)
+                     const TileSortTmp& tmp, int items_per_block, uint32_t* coff, float4* rec, hipStream_t s)
 {
-    ts_sort(ColSrc{key, zmax}, n, ncol, tmp, coff, nullptr, ColPlace{tris, rec}, s);
+    ts_sort(ColSrc{key, zmax}, n, ncol, tmp, items_per_block, coff, nullptr, ColPlace{tris, rec}, s);
 }
 
 void launch_bin_pairs(const uint64_t* bits, const uint32_t* tri, uint64_t n, const uint64_t* bnd,
-                      const uint32_t* wprefix, uint32_t B, const TileSortTmp& tmp, uint32_t* boff,
-                      unsigned int* max_out, uint32_t* list, const float* tris, const int32_t* bidx,
-                      const GridDev& g0, const Meshes& M, uint64_t* pmask, hipStream_t s)
+                      const uint32_t* wprefix, uint32_t B, const TileSortTmp& tmp, int items_per_block,
+                      uint32_t* boff, unsigned int* max_out, uint32_t* list, const float* tris,
+                      const int32_t* bidx, const GridDev& g0, const Meshes& M, uint64_t* pmask, hipStream_t s)
 {
     const PairPlace place{list, pmask};
     if (M.nm > 1)
-        ts_sort(PairSrc<true>{bits, tri, bnd, wprefix, tris, bidx, g0, M, pmask != nullptr}, n, B, tmp, boff,
-                max_out, place, s);
+        ts_sort(PairSrc<true>{bits, tri, bnd, wprefix, tris, bidx, g0, M, pmask != nullptr}, n, B, tmp,
+                items_per_block, boff, max_out, place, s);
     else
-        ts_sort(PairSrc<false>{bits, tri, bnd, wprefix, tris, bidx, g0, M, pmask != nullptr}, n, B, tmp, boff,
-                max_out, place, s);
+        ts_sort(PairSrc<false>{bits, tri, bnd, wprefix, tris, bidx, g0, M, pmask != nullptr}, n, B, tmp,
+                items_per_block, boff, max_out, place, s);
 }
 
 // Per-mesh bin totals (pairs) and largest bin of a batch.  bmesh is

@@ -158,6 +158,7 @@ struct gpv_ctx {
     char* h_buf = nullptr;   // pinned read-back staging
     size_t h_cap = 0;
     int num_cu = 256;
+    gpv_tune tune{8, 8, 4096, 1};
 };
 
 namespace gpv {
@@ -361,7 +362,7 @@ void build_tables(const gpv_grid_info& I, bool cuda_l2, std::vector<float>& tab,
     }
 }
 
-int expand_blocks(const gpv_ctx* c) { return std::max(1, c->num_cu) * 8; }
+int expand_blocks(const gpv_ctx* c) { return std::max(1, c->num_cu) * std::max(1, c->tune.expand_blocks_per_cu); }
 
 int ensure_hbuf(gpv_ctx* c, size_t bytes)
 {
@@ -755,7 +756,8 @@ int launch_compact(Job* J)
     ts.off = ts_cnt + 2 * tiles + 1;
     ts.scan = J->scan32;
     ts.bucket = ts_bucket;
-    launch_bin_pairs(J->pair_key, J->pair_tri, J->n_pairs, J->bnd, J->wprefix, (uint32_t)J->B, ts, J->boff,
+    const int ipb = std::max(256, J->ctx->tune.sort_items_per_block);
+    launch_bin_pairs(J->pair_key, J->pair_tri, J->n_pairs, J->bnd, J->wprefix, (uint32_t)J->B, ts, ipb, J->boff,
                      J->nm > 1 ? nullptr : &J->mctr[0].max_per_voxel, J->list, J->tris, J->bidx, J->g[0],
                      J->view(), J->pmask, s);
     if (J->nm > 1) launch_mesh_bins(J->boff, J->B, J->view(), s);
@@ -763,7 +765,7 @@ int launch_compact(Job* J)
     release(J, J->crec);
     TRY(alloc(J, &J->coff, (size_t)J->ncol + 1));
     TRY(alloc(J, &J->crec, (size_t)std::max<uint64_t>(J->n_cols, 1) * 3));
-    launch_bin_cols(J->tris, J->col_key, J->col_z, J->n_cols, (uint32_t)J->ncol, ts, J->coff, J->crec, s);
+    launch_bin_cols(J->tris, J->col_key, J->col_z, J->n_cols, (uint32_t)J->ncol, ts, ipb, J->coff, J->crec, s);
     release(J, ts_cnt);
     release(J, ts_bucket);
     HIP_TRY(hipGetLastError());
@@ -834,7 +836,7 @@ int l2_sat_kernels(Job* J)
     HIP_TRY(hipEventRecord(J->ev[6], s));
     // voxels with few bin entries share waves (k_l2_sat_grp); with many (an
     // average of 8 or more per voxel) one voxel per wave is faster
-    const bool grouped = J->B > 0 && (int64_t)J->n_pairs < 8 * J->B;
+    const bool grouped = J->B > 0 && (int64_t)J->n_pairs < (int64_t)J->ctx->tune.l2_group_below * J->B;
     if (grouped) launch_l2_groups(J->boff, J->B, J->g[0], J->view(), J->gflag, J->gidx, J->scan32, J->gstart, s);
     // one wave per group, for an upper bound on the group count computed
     // here (a group starts at a mesh start, at a new 64-entry chunk of first
@@ -1002,7 +1004,12 @@ int job_run(Job* J)
     if (needed > 0 || !ray_ok) return fail(GPV_ERR_CAPACITY, "buffer capacity did not converge");
     TRY(launch_parity(J));
     TRY(launch_compact(J));
-    TRY(run_level2(J));
+    if (J->ctx->tune.l2_overlap) {
+        TRY(run_level2(J));
+    } else {
+        TRY(run_l2_parity(J));
+        TRY(run_l2_sat(J));
+    }
     TRY(run_normals(J));
     return job_finish(J);
 }
@@ -1130,6 +1137,86 @@ int gpv_device_upload(gpv_ctx* c, const void* host, int64_t bytes, void** dev)
         }
     }
     *dev = p;
+    return GPV_OK;
+}
+
+int gpv_ctx_get_tune(gpv_ctx* c, gpv_tune* out)
+{
+    if (!c || !out) return fail(GPV_ERR_INVALID, "NULL argument");
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    *out = c->tune;
+    return GPV_OK;
+}
+
+int gpv_ctx_set_tune(gpv_ctx* c, const gpv_tune* t)
+{
+    if (!c || !t) return fail(GPV_ERR_INVALID, "NULL argument");
+    if (t->expand_blocks_per_cu < 1 || t->expand_blocks_per_cu > 64 || t->l2_group_below < 0 ||
+        t->sort_items_per_block < 256 || t->sort_items_per_block > (1 << 20))
+        return fail(GPV_ERR_INVALID, "launch shape out of range");
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->tune = *t;
+    return GPV_OK;
+}
+
+// device time of one batch run (ms_total of the batch), grids released
+static int tune_trial(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, int repeats, float* ms)
+{
+    std::vector<gpv_grid*> g((size_t)nm, nullptr);
+    float best = 0.0f;
+    for (int r = 0; r < repeats; ++r) {
+        TRY(gpv_voxelize_batch(c, d, nm, on_device, g.data()));
+        gpv_grid_info I;
+        gpv_grid_get_info(g[0], &I);
+        for (gpv_grid* x : g) gpv_grid_release(c, x);
+        if (r == 0 || I.ms_total < best) best = I.ms_total;
+    }
+    *ms = best;
+    return GPV_OK;
+}
+
+int gpv_ctx_autotune(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, int repeats, gpv_tune* best,
+                     float* best_ms)
+{
+    if (!c || !d || nm < 1) return fail(GPV_ERR_INVALID, "bad autotune arguments");
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    repeats = std::max(1, repeats);
+    const gpv_tune start = c->tune;
+    gpv_tune cur = start;
+    float base = 0.0f;
+    // one untimed run so that allocation and first-launch costs stay out
+    float warm = 0.0f;
+    int st = tune_trial(c, d, nm, on_device, 1, &warm);
+    if (st == GPV_OK) st = tune_trial(c, d, nm, on_device, repeats, &base);
+    const std::vector<int> expand = {4, 8, 16}, group = {0, 4, 8, 16, 1 << 20}, items = {2048, 4096, 8192},
+                           overlap = {0, 1};
+    auto sweep = [&](int gpv_tune::*field, const std::vector<int>& vals) -> int {
+        for (int v : vals) {
+            if (v == cur.*field) continue;
+            gpv_tune t = cur;
+            t.*field = v;
+            c->tune = t;
+            float ms = 0.0f;
+            TRY(tune_trial(c, d, nm, on_device, repeats, &ms));
+            if (ms < base * 0.99f) {
+                base = ms;
+                cur = t;
+            }
+        }
+        c->tune = cur;
+        return GPV_OK;
+    };
+    if (st == GPV_OK) st = sweep(&gpv_tune::expand_blocks_per_cu, expand);
+    if (st == GPV_OK) st = sweep(&gpv_tune::l2_group_below, group);
+    if (st == GPV_OK) st = sweep(&gpv_tune::sort_items_per_block, items);
+    if (st == GPV_OK) st = sweep(&gpv_tune::l2_overlap, overlap);
+    if (st != GPV_OK) {
+        c->tune = start;
+        return st;
+    }
+    c->tune = cur;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = base;
     return GPV_OK;
 }
 

@@ -122,6 +122,20 @@ typedef struct gpv_pool_stats {
 } gpv_pool_stats;
 int gpv_ctx_pool_stats(gpv_ctx *ctx, gpv_pool_stats *out);
 
+/* Launch shapes of a context, picked by gpv_ctx_autotune or set by hand.  No
+ * counterpart in the reference (its launch shapes are fixed,
+ * cuda/CUDAHelper.h:23-38).  None of them changes a result. */
+typedef struct gpv_tune {
+    int expand_blocks_per_cu;  /* persistent blocks per CU of the Level1 SAT and ray expansion (8) */
+    int l2_group_below;        /* the Level2 SAT shares a wave among consecutive boundary voxels
+                                  when the bins average fewer entries than this per voxel (8);
+                                  0 never, a large value always */
+    int sort_items_per_block;  /* bin entries per block in the counting sorts (4096) */
+    int l2_overlap;            /* run the Level2 parity on a second stream beside the Level2 SAT (1) */
+} gpv_tune;
+int gpv_ctx_get_tune(gpv_ctx *ctx, gpv_tune *out);
+int gpv_ctx_set_tune(gpv_ctx *ctx, const gpv_tune *t);
+
 /* ---- host-side geometry set-up ----------------------------------------- */
 /* bbox over vertices + 0.1% pad + maxModelSize: Object::ReadObject
  * (src/Object.cpp:441-450, 572-583).  verts: nverts*3 floats (host). */
@@ -166,6 +180,14 @@ typedef struct gpv_mesh_desc {
  * 2^32-1 Level1 voxels (rows padded to 64), 2^32-1 bin entries. */
 int gpv_voxelize_batch(gpv_ctx *ctx, const gpv_mesh_desc *meshes, int nmeshes, int tris_on_device,
                        gpv_grid **out);
+
+/* Launch shapes for a workload (gpv_tune): measures the device time of
+ * gpv_voxelize_batch over `meshes` (min over `repeats` runs) for candidate
+ * values of each field in turn, keeps a value only when it is at least 1%
+ * faster, and leaves the best shapes set on the context.  best / best_ms may
+ * be NULL. */
+int gpv_ctx_autotune(gpv_ctx *ctx, const gpv_mesh_desc *meshes, int nmeshes, int tris_on_device, int repeats,
+                     gpv_tune *best, float *best_ms);
 
 /* ---- stage entry points (same order PerformVoxelization calls them) ------- */
 /* Allocates the grid, uploads the soup and the center tables: the set-up of

@@ -229,3 +229,32 @@ def test_pool_reuses_blocks_across_batch_sizes(gv):
         assert s2["cached_bytes"] <= s2["cap_bytes"]
         for p in ptrs:
             c.free(p)
+
+
+def test_autotune_keeps_results(gv):
+    """gpv_ctx_autotune picks launch shapes by measurement; no shape changes a
+    result, so grids before and after tuning (and under hand-set extreme
+    shapes) are identical."""
+    from gpview_amd import workloads
+    ms = workloads.mixed_stream(gv, 6, seed=95, tri_range=(1000, 12000), level1=(32, 64), level2=(4, 8))
+    with gv.Context(0) as c:
+        def run():
+            gs = c.voxelize_batch(ms)
+            out = [np.concatenate([d["l1"], d["l2"], d["prefix"].view(np.uint8)]) for d in (g.download() for g in gs)]
+            for g in gs:
+                g.release()
+            return out
+        base = run()
+        shapes, best_ms = c.autotune(ms, repeats=2)
+        say(f"autotune: {shapes} {best_ms:.3f} ms")
+        assert shapes == c.get_tune() and best_ms > 0
+        assert shapes["expand_blocks_per_cu"] in (4, 8, 16) and shapes["l2_overlap"] in (0, 1)
+        for a, b in zip(base, run()):
+            assert np.array_equal(a, b)
+        for extreme in ({"l2_group_below": 0, "l2_overlap": 0, "expand_blocks_per_cu": 1, "sort_items_per_block": 256},
+                        {"l2_group_below": 1 << 20, "sort_items_per_block": 1 << 20}):
+            c.set_tune(**extreme)
+            for a, b in zip(base, run()):
+                assert np.array_equal(a, b), extreme
+        with pytest.raises(gv.GpvError):
+            c.set_tune(sort_items_per_block=1)
