@@ -829,15 +829,28 @@ void release_l2_groups(Job* J)
     release(J, J->gstart);
 }
 
-// the Level2 SAT kernels on the main stream (group arrays allocated)
+// whether the Level2 SAT shares waves among voxels: few bin entries per voxel
+bool l2_grouped(const Job* J)
+{
+    return J->B > 0 && (int64_t)J->n_pairs < (int64_t)J->ctx->tune.l2_group_below * J->B;
+}
+
+// group formation of the Level2 SAT (main stream, group arrays allocated)
+void l2_sat_groups(Job* J)
+{
+    if (l2_grouped(J))
+        launch_l2_groups(J->boff, J->B, J->g[0], J->view(), J->gflag, J->gidx, J->scan32, J->gstart,
+                         J->ctx->stream);
+}
+
+// the Level2 SAT kernel on the main stream (groups formed)
 int l2_sat_kernels(Job* J)
 {
     hipStream_t s = J->ctx->stream;
     HIP_TRY(hipEventRecord(J->ev[6], s));
-    // voxels with few bin entries share waves (k_l2_sat_grp); with many (an
-    // average of 8 or more per voxel) one voxel per wave is faster
-    const bool grouped = J->B > 0 && (int64_t)J->n_pairs < (int64_t)J->ctx->tune.l2_group_below * J->B;
-    if (grouped) launch_l2_groups(J->boff, J->B, J->g[0], J->view(), J->gflag, J->gidx, J->scan32, J->gstart, s);
+    // voxels with few bin entries share waves (k_l2_sat_grp); with many one
+    // voxel per wave is faster
+    const bool grouped = l2_grouped(J);
     // one wave per group, for an upper bound on the group count computed
     // here (a group starts at a mesh start, at a new 64-entry chunk of first
     // bin entries, or after a full group), so the grid is sized without
@@ -865,6 +878,7 @@ int run_l2_sat(Job* J)
 {
     if (J->n2max <= 0) return GPV_OK;
     TRY(alloc_l2_groups(J));
+    l2_sat_groups(J);
     TRY(l2_sat_kernels(J));
     release_l2_groups(J);
     J->l2s_done = true;
@@ -882,6 +896,9 @@ int run_level2(Job* J)
     hipStream_t s = c->stream;
     TRY(alloc_cpass(J));
     TRY(alloc_l2_groups(J));
+    // the SAT's small group-formation kernels go first: launched beside the
+    // parity they would wait for its waves to drain
+    l2_sat_groups(J);
     HIP_TRY(hipEventRecord(J->ev[5], s));
     HIP_TRY(hipStreamWaitEvent(c->side, J->ev[5], 0));
     TRY(l2_parity_kernels(J, c->side));
