@@ -2110,7 +2110,18 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
     const int j1 = ij / gnx;
     const int i1 = ij - j1 * gnx;
     const uint32_t e0 = coff[col0 + ij], e1 = coff[col0 + ij + 1];
-    l2_parity_voxel<N2>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
+    if constexpr (kM && N2 == 0) {
+        // a batch mixes resolutions: the common ones get their unrolled body
+        // (the branch is uniform per wave)
+        if (n2 == 8)
+            l2_parity_voxel<8>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
+        else if (n2 == 4)
+            l2_parity_voxel<4>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
+        else
+            l2_parity_voxel<0>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
+    } else {
+        l2_parity_voxel<N2>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
+    }
 }
 
 void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec,
