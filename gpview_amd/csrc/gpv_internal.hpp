@@ -39,6 +39,8 @@ struct MeshOff {
     uint32_t tri0;
     uint32_t b0;
     uint32_t ptask0;   // first Level1 parity task (16 layers x 64 columns of one row)
+    uint32_t pw0;      // first Level2 parity wave (set on the host once boundary counts are known)
+    uint32_t pad;
 };
 
 // Per-mesh results accumulated on the device.
@@ -161,7 +163,10 @@ void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint
 // det/u/v pass bits per (column entry, sub-column): stride words per entry
 void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M, int64_t ncol,
                      int stride, uint64_t* cpass, hipStream_t s);
-void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec,
+// Level2 parity waves of one mesh: one per boundary voxel, one per four when
+// n2 <= 4, none when Level2 is off
+int64_t l2_parity_waves(int n2, int64_t nb);
+void launch_l2_parity(const int32_t* bidx, int64_t B, int64_t nwaves, const uint32_t* coff, const float4* rec,
                       const uint64_t* cpass, int stride, const GridDev& g0, const Meshes& M, uint64_t* par2,
                       hipStream_t s);
 

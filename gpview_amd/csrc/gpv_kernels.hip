@@ -2085,9 +2085,92 @@ __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev
 // kM: batch (resolution per mesh, N2 = 0); single mesh: N2 = its resolution
 // when it is 4 or 8 (a kernel of its own, so each keeps its own register
 // budget), else 0
+// Four boundary voxels of one mesh with n2 <= 4 (n2^2 <= 16 sub-columns) in
+// one wave: lane 16v + sc is sub-column sc of voxel v.  Each quarter stages
+// 16 of its own column's candidates per round; a candidate's pass bits sit at
+// bits [16v, 16v + 16) of its lane's row, so one 64x64 transpose gives every
+// lane the pass bits of its own voxel's candidates (the rows of the other
+// quarters have no bits there).  N2 = 4: layers unrolled; 0: any n2 <= 4.
+template <int N2>
+__device__ __forceinline__ void l2_parity_pack4(int n2rt, const GridDev& g, uint32_t col0, int64_t bfirst, int nv,
+                                                const int32_t* __restrict__ bidx, const uint32_t* __restrict__ coff,
+                                                const float4* __restrict__ rec, const uint64_t* __restrict__ cpass,
+                                                int stride, ColRec* cr, uint64_t* __restrict__ out)
+{
+    const int lane = __lane_id();
+    const int n2 = N2 > 0 ? N2 : n2rt;
+    const int nsc = n2 * n2;
+    const int v = lane >> 4, sc = lane & 15;
+    const bool lv = v < nv;
+    int k1 = 0, ij = 0, i1 = 0, j1 = 0;
+    if (lv) {
+        const int vox = bidx[bfirst + v];
+        const int nxy = g.nx * g.ny;
+        k1 = vox / nxy;
+        ij = vox - k1 * nxy;
+        j1 = ij / g.nx;
+        i1 = ij - j1 * g.nx;
+    }
+    const uint32_t e0 = lv ? coff[col0 + ij] : 0u, e1 = lv ? coff[col0 + ij + 1] : 0u;
+    const bool ls = lv && sc < nsc;
+    const float ox = ls ? g.r2x[(int64_t)i1 * n2 + sc % n2] : 0.0f;
+    const float oy = ls ? g.r2y[(int64_t)j1 * n2 + sc / n2] : 0.0f;
+    float oz[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) oz[r] = (lv && r < n2) ? g.r2z[(int64_t)k1 * n2 + r] : 0.0f;
+    // rounds of 16 candidates: the most any quarter needs
+    uint32_t nr = (e1 - e0 + 15) >> 4;
+#pragma unroll
+    for (int d = 16; d <= 32; d <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_xor((int)nr, d);
+        nr = o > nr ? o : nr;
+    }
+    const uint32_t rounds = (uint32_t)__builtin_amdgcn_readfirstlane((int)nr);
+    uint32_t par = 0;   // bit r: parity at layer r of this lane's sub-column
+    for (uint32_t rd = 0; rd < rounds; ++rd) {
+        const uint32_t i = e0 + rd * 16 + (uint32_t)sc;
+        uint64_t row = 0;
+        if (lv && i < e1) {
+            const ColRec r = load_colrec(rec, i);
+            cr[lane] = r;
+            if (r.zmax >= k1) row = (cpass[(uint64_t)i * stride] & 0xffffull) << (16 * v);
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint64_t m = wave_transpose64(row); m; m &= m - 1) {
+            const int j = __ffsll((unsigned long long)m) - 1;
+            const ColRec r = cr[j];
+            const float tx = ox - r.v1x, ty = oy - r.v1y;
+            uint32_t hm = 0;
+            if constexpr (N2 > 0) {
+#pragma unroll
+                for (int rr = 0; rr < N2; ++rr)
+                    hm |= (uint32_t)mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z, oz[rr])
+                          << rr;
+            } else {
+                for (int rr = 0; rr < n2; ++rr)
+                    hm |= (uint32_t)mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z,
+                                             rr == 0 ? oz[0] : rr == 1 ? oz[1] : rr == 2 ? oz[2] : oz[3])
+                          << rr;
+            }
+            par ^= hm;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // voxel v's word: bit r*nsc + sc for layer r (n2^3 <= 64, one word)
+    const uint64_t smask = nsc >= 64 ? ~0ull : ((1ull << nsc) - 1);
+    uint64_t word = 0;
+    for (int r = 0; r < n2; ++r) {
+        const uint64_t m = __ballot((par >> r) & 1u);
+        word |= ((m >> (16 * v)) & smask) << (r * nsc);
+    }
+    if (lv && sc == 0) out[v] = word;
+}
+
+// Waves: single mesh, one per boundary voxel, or (n2 <= 4) one per four;
+// batch, per mesh from MeshOff::pw0 likewise.
 template <bool kM, int N2>
 __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __restrict__ bidx, int64_t B,
-                                                              const uint32_t* __restrict__ coff,
+                                                              int64_t nwaves, const uint32_t* __restrict__ coff,
                                                               const float4* __restrict__ rec,
                                                               const uint64_t* __restrict__ cpass, int stride,
                                                               GridDev g0, Meshes M, uint64_t* __restrict__ par2)
@@ -2095,14 +2178,38 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
     __shared__ unsigned long long s_acc[L2P_WAVES][64];
     __shared__ ColRec s_rec[L2P_WAVES][64];
     const int wv = threadIdx.x >> 6;
-    const int64_t b = (int64_t)blockIdx.x * L2P_WAVES + wv;
-    if (b >= B) return;
-    const int mesh = kM ? uniform_i((int)M.bmesh[b]) : 0;
+    const int64_t w = (int64_t)blockIdx.x * L2P_WAVES + wv;
+    if (w >= nwaves) return;
+    int mesh = 0;
+    if constexpr (kM) {
+        int lo = 0, hi = M.nm;   // the mesh whose waves hold w (w uniform)
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((int64_t)M.off[mid].pw0 <= w) lo = mid;
+            else hi = mid;
+        }
+        mesh = uniform_i(lo);
+    }
     const GridDev& g = grid_of<kM>(g0, M, mesh);
     const int n2 = g.n2, w2 = g.w2, gnx = g.nx, gny = g.ny;
     if (kM && n2 == 0) return;
-    uint64_t* out = par2 + (kM ? M.off[mesh].l2w0 + (uint64_t)(b - M.off[mesh].b0) * w2 : (uint64_t)b * w2);
+    const int64_t mb0 = kM ? (int64_t)M.off[mesh].b0 : 0;
+    const int64_t mb1 = kM ? (int64_t)M.off[mesh + 1].b0 : B;
+    const int64_t lw = w - (kM ? (int64_t)M.off[mesh].pw0 : 0);
+    const uint64_t l2w0 = kM ? M.off[mesh].l2w0 : 0;
     const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
+    if ((N2 == 0 || N2 == 4) && n2 <= 4) {
+        const int64_t bfirst = mb0 + 4 * lw;
+        const int nv = (int)(mb1 - bfirst < 4 ? mb1 - bfirst : 4);
+        uint64_t* out = par2 + l2w0 + (uint64_t)(bfirst - mb0);   // w2 == 1
+        if (n2 == 4)
+            l2_parity_pack4<4>(n2, g, col0, bfirst, nv, bidx, coff, rec, cpass, stride, s_rec[wv], out);
+        else
+            l2_parity_pack4<0>(n2, g, col0, bfirst, nv, bidx, coff, rec, cpass, stride, s_rec[wv], out);
+        return;
+    }
+    const int64_t b = mb0 + lw;
+    uint64_t* out = par2 + l2w0 + (uint64_t)(b - mb0) * w2;
     const int v = bidx[b];
     const int nxy = gnx * gny;
     const int k1 = v / nxy;
@@ -2111,34 +2218,39 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
     const int i1 = ij - j1 * gnx;
     const uint32_t e0 = coff[col0 + ij], e1 = coff[col0 + ij + 1];
     if constexpr (kM && N2 == 0) {
-        // a batch mixes resolutions: the common ones get their unrolled body
+        // a batch mixes resolutions: the common one gets its unrolled body
         // (the branch is uniform per wave)
         if (n2 == 8)
             l2_parity_voxel<8>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
-        else if (n2 == 4)
-            l2_parity_voxel<4>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
         else
             l2_parity_voxel<0>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
-    } else {
+    } else if constexpr (N2 != 4) {
         l2_parity_voxel<N2>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
     }
 }
 
-void launch_l2_parity(const int32_t* bidx, int64_t B, const uint32_t* coff, const float4* rec,
+// waves of the Level2 parity for nb boundary voxels at resolution n2
+int64_t l2_parity_waves(int n2, int64_t nb)
+{
+    if (n2 <= 0) return 0;
+    return n2 <= 4 ? (nb + 3) / 4 : nb;
+}
+
+void launch_l2_parity(const int32_t* bidx, int64_t B, int64_t nwaves, const uint32_t* coff, const float4* rec,
                       const uint64_t* cpass, int stride, const GridDev& g0, const Meshes& M, uint64_t* par2,
                       hipStream_t s)
 {
-    if (B <= 0) return;
-    const unsigned blocks = (unsigned)((B + L2P_WAVES - 1) / L2P_WAVES);
+    if (B <= 0 || nwaves <= 0) return;
+    const unsigned blocks = (unsigned)((nwaves + L2P_WAVES - 1) / L2P_WAVES);
     const dim3 bs(64 * L2P_WAVES);
     if (M.nm > 1)
-        k_l2_parity<true, 0><<<blocks, bs, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M, par2);
+        k_l2_parity<true, 0><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
     else if (g0.n2 == 8)
-        k_l2_parity<false, 8><<<blocks, bs, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M, par2);
+        k_l2_parity<false, 8><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
     else if (g0.n2 == 4)
-        k_l2_parity<false, 4><<<blocks, bs, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M, par2);
+        k_l2_parity<false, 4><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
     else
-        k_l2_parity<false, 0><<<blocks, bs, 0, s>>>(bidx, B, coff, rec, cpass, stride, g0, M, par2);
+        k_l2_parity<false, 0><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
 }
 
 // ---------------------------------------------------------------------------

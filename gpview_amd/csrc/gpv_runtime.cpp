@@ -217,6 +217,7 @@ struct Job {
     float4* crec = nullptr;   // 3 float4 per column entry
     uint64_t* cpass = nullptr;   // det/u/v pass bits per column entry and sub-column
     uint32_t *gflag = nullptr, *gidx = nullptr, *gstart = nullptr;   // Level2 SAT voxel groups
+    int64_t l2p_waves = 0;   // Level2 parity waves (l2_parity_plan)
     uint32_t *hcnt = nullptr, *hoff = nullptr;
     float4* hrec = nullptr;
     uint64_t *sat2 = nullptr, *par2 = nullptr;
@@ -779,12 +780,37 @@ int launch_compact(Job* J)
     return GPV_OK;
 }
 
-// Level2 parity kernels on stream s (cpass already allocated)
+// Level2 parity waves of the job; in a batch each mesh's first wave goes to
+// MeshOff::pw0 on the device (main stream, before any parity launch)
+int l2_parity_plan(Job* J)
+{
+    int64_t w = 0;
+    for (int m = 0; m < J->nm; ++m) {
+        J->off[m].pw0 = (uint32_t)w;
+        w += l2_parity_waves(J->g[m].n2, J->nb(m));
+    }
+    J->off[J->nm].pw0 = (uint32_t)w;
+    if (w >= (int64_t(1) << 32)) return fail(GPV_ERR_INVALID, "too many Level2 parity waves");
+    J->l2p_waves = w;
+    if (J->nm > 1) {
+        hipStream_t s = J->ctx->stream;
+        // the MeshOff staging region was last filled by the size read-back,
+        // which the host has already waited for; a later job refills it only
+        // after its own read-back, which this stream orders after this copy
+        MeshOff* h = reinterpret_cast<MeshOff*>(J->ctx->h_buf + kHOff);
+        memcpy(h, J->off.data(), sizeof(MeshOff) * ((size_t)J->nm + 1));
+        HIP_TRY(hipMemcpyAsync(J->d_off, h, sizeof(MeshOff) * ((size_t)J->nm + 1), hipMemcpyHostToDevice, s));
+    }
+    return GPV_OK;
+}
+
+// Level2 parity kernels on stream s (cpass allocated, waves planned)
 int l2_parity_kernels(Job* J, hipStream_t s)
 {
     const int stride = (J->n2max * J->n2max + 63) / 64;
     launch_col_pass(J->coff, J->crec, J->g[0], J->view(), J->ncol, stride, J->cpass, s);
-    launch_l2_parity(J->bidx, J->B, J->coff, J->crec, J->cpass, stride, J->g[0], J->view(), J->par2, s);
+    launch_l2_parity(J->bidx, J->B, J->l2p_waves, J->coff, J->crec, J->cpass, stride, J->g[0], J->view(), J->par2,
+                     s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(J->ev[11], s));
     return GPV_OK;
@@ -803,6 +829,7 @@ int run_l2_parity(Job* J)
     hipStream_t s = J->ctx->stream;
     HIP_TRY(hipEventRecord(J->ev[5], s));
     TRY(alloc_cpass(J));
+    TRY(l2_parity_plan(J));
     TRY(l2_parity_kernels(J, s));
     release(J, J->cpass);
     J->l2p_done = true;
@@ -896,6 +923,7 @@ int run_level2(Job* J)
     hipStream_t s = c->stream;
     TRY(alloc_cpass(J));
     TRY(alloc_l2_groups(J));
+    TRY(l2_parity_plan(J));
     // the SAT's small group-formation kernels go first: launched beside the
     // parity they would wait for its waves to drain
     l2_sat_groups(J);
