@@ -159,6 +159,7 @@ struct gpv_ctx {
     size_t h_cap = 0;
     int num_cu = 256;
     gpv_tune tune{8, 8, 4096, 1};
+    std::vector<hipEvent_t> events;   // recycled stage events of finished jobs
 };
 
 namespace gpv {
@@ -400,7 +401,7 @@ void job_free(Job* J)
     (void)hipStreamSynchronize(c->side);
     for (void* p : J->bufs) c->pool.put(p);
     for (auto& e : J->ev)
-        if (e) (void)hipEventDestroy(e);
+        if (e) c->events.push_back(e);   // every use is complete (the streams were synchronised)
     delete J;
 }
 
@@ -468,8 +469,15 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         job_free(J);
         return r;
     };
-    for (auto& e : J->ev)
-        if (hipEventCreate(&e) != hipSuccess) return bail(fail(GPV_ERR_HIP, "event creation failed"));
+    for (auto& e : J->ev) {
+        if (!c->events.empty()) {
+            e = c->events.back();
+            c->events.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            return bail(fail(GPV_ERR_HIP, "event creation failed"));
+        }
+    }
 
     // per-mesh grids, offsets and the concatenated center tables
     J->g.resize((size_t)nm);
@@ -1155,6 +1163,8 @@ int gpv_ctx_destroy(gpv_ctx* c)
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->side);
+        for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+        c->events.clear();
         c->pool.trim();
         if (c->h_buf) (void)hipHostFree(c->h_buf);
         (void)hipStreamDestroy(c->stream);
