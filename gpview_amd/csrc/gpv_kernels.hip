@@ -470,6 +470,37 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const TI* __restrict__ i
     }
 }
 
+// apply with the block prefix summed in place from the block sums (two
+// launches per scan instead of a recursion, for up to SCAN_TILE blocks)
+template <typename TI, typename TO>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_apply_direct(const TI* __restrict__ in, int64_t n,
+                                                              const TO* __restrict__ sums, TO* __restrict__ out)
+{
+    __shared__ TO s_wave[SCAN_NT / 64];
+    __shared__ TO s_total;
+    // this block's prefix: the sum of the sums of the blocks before it
+    TO pre = 0;
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += SCAN_NT) pre += sums[k];
+    (void)block_excl_scan<TO>(pre, s_wave, &s_total);   // leaves the block-wide sum in s_total
+    const TO block_prefix = s_total;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_IPT;
+    TO v[SCAN_IPT];
+    TO acc = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; ++k) {
+        v[k] = (base + k < n) ? (TO)in[base + k] : (TO)0;
+        acc += v[k];
+    }
+    TO run = block_excl_scan<TO>(acc, s_wave, &s_total) + block_prefix;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+        if (base + k == n - 1) out[n] = run;
+    }
+}
+
 // scan_rec(n) uses sums[nb+1], then (nb > 1) rest[nb+1] and scan_rec(nb)'s own
 size_t scan_tmp_elems(int64_t n)
 {
@@ -491,6 +522,10 @@ static void scan_rec(const TI* in, TO* out, int64_t n, TO* tmp, unsigned int* ma
     TO* sums = tmp;            // nb + 1
     TO* rest = tmp + nb + 1;
     k_scan_reduce<TI, TO><<<(unsigned)nb, SCAN_NT, 0, s>>>(in, n, sums, max_out);
+    if (nb <= SCAN_TILE) {
+        k_scan_apply_direct<TI, TO><<<(unsigned)nb, SCAN_NT, 0, s>>>(in, n, sums, out);
+        return;
+    }
     if (nb == 1) {
         // block prefix of the single block is 0
         if constexpr (sizeof(TO) == 8) launch_fill_u64(reinterpret_cast<uint64_t*>(sums), 0, 1, s);
