@@ -215,6 +215,7 @@ struct Job {
     uint32_t *boff = nullptr, *list = nullptr;
     uint64_t* pmask = nullptr;   // Level2 box-axis masks per bin entry
     uint32_t* coff = nullptr;
+    uint32_t* scan_side = nullptr;   // scan temporary of the parity fill (it may run beside compaction)
     float4* crec = nullptr;   // 3 float4 per column entry
     uint64_t* cpass = nullptr;   // det/u/v pass bits per column entry and sub-column
     uint32_t *gflag = nullptr, *gidx = nullptr, *gstart = nullptr;   // Level2 SAT voxel groups
@@ -235,8 +236,9 @@ struct Job {
          l2p_done = false, l2s_done = false, finished = false;
     // stream events: 0 setup, 1 L1 SAT, 2 ray expansion, 3 parity fill,
     // 4 compaction+bins, 5 L2 parity, 6 L2 SAT start, 7 L2 SAT end, 8 finish,
-    // 9 normals start, 10 normals end, 11 L2 parity end
-    hipEvent_t ev[12] = {};
+    // 9 normals start, 10 normals end, 11 L2 parity end, 12 L1 SAT end,
+    // 13 ray expansion end, 14 parity fill end, 15 fork of the ray expansion
+    hipEvent_t ev[16] = {};
     std::vector<void*> bufs;
 
     Meshes view() const { return Meshes{d_grids, d_off, tri_mesh, bmesh, mctr, nm}; }
@@ -615,20 +617,24 @@ int launch_sat_half(Job* J)
     launch_scan_u32_u32(J->wcnt, J->wprefix, J->W, J->scan32, nullptr, s);
     if (J->nm > 1) launch_mesh_bounds(J->wprefix, J->view(), s);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(J->ev[12], s));
     return GPV_OK;
 }
 
-int launch_ray_half(Job* J)
+// Level1 ray expansion on stream s (allocates nothing)
+int launch_ray_half(Job* J, hipStream_t s)
 {
     gpv_ctx* c = J->ctx;
-    hipStream_t s = c->stream;
     HIP_TRY(hipEventRecord(J->ev[2], s));
     launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->hits), 0, 1, s);
     Append64 hi{&J->ctr->hits, J->hit_key, nullptr, J->hit_cap};
     launch_ray_expand(J->tris, J->T, J->rng, J->ray_off, J->g[0], J->view(), hi, expand_blocks(c), s);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(J->ev[13], s));
     return GPV_OK;
 }
+
+int launch_ray_half(Job* J) { return launch_ray_half(J, J->ctx->stream); }
 
 // one round trip: the counters and, when asked, the boundary offsets
 int read_counts(Job* J, bool with_b)
@@ -714,23 +720,38 @@ int complete_ray(Job* J, bool* ok)
 }
 
 // hits grouped by column, then the parity of every Level1 voxel
+// buffers of the parity fill (allocated before it may fork onto the side stream)
+int alloc_parity(Job* J)
+{
+    release(J, J->hcnt);
+    release(J, J->hoff);
+    release(J, J->hrec);
+    release(J, J->scan_side);
+    TRY(alloc(J, &J->hcnt, (size_t)J->ncol));
+    TRY(alloc(J, &J->hoff, (size_t)J->ncol + 1));
+    TRY(alloc(J, &J->hrec, (size_t)std::max<uint64_t>(J->n_hits, 1) * 3));
+    return alloc(J, &J->scan_side, scan_tmp_elems(J->ncol));
+}
+
+// the parity fill on stream s: hits by column, their MT records, the fill
+int parity_kernels(Job* J, hipStream_t s)
+{
+    launch_fill_u32(J->hcnt, 0, J->ncol, s);
+    launch_count_keys_hi(J->hit_key, J->n_hits, J->hcnt, s);
+    launch_scan_u32_u32(J->hcnt, J->hoff, J->ncol, J->scan_side, nullptr, s);
+    launch_scatter_hits(J->tris, J->hit_key, J->n_hits, J->g[0], J->view(), J->hoff, J->hcnt, J->hrec, s);
+    launch_l1_parity(J->hrec, J->hoff, J->g[0], J->view(), (int64_t)J->off[J->nm].ptask0, J->ins, s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(J->ev[14], s));
+    return GPV_OK;
+}
+
 int launch_parity(Job* J)
 {
     hipStream_t s = J->ctx->stream;
     HIP_TRY(hipEventRecord(J->ev[3], s));
-    release(J, J->hcnt);
-    release(J, J->hoff);
-    release(J, J->hrec);
-    TRY(alloc(J, &J->hcnt, (size_t)J->ncol));
-    TRY(alloc(J, &J->hoff, (size_t)J->ncol + 1));
-    TRY(alloc(J, &J->hrec, (size_t)std::max<uint64_t>(J->n_hits, 1) * 3));
-    launch_fill_u32(J->hcnt, 0, J->ncol, s);
-    launch_count_keys_hi(J->hit_key, J->n_hits, J->hcnt, s);
-    TRY(ensure_scan_tmp(J, J->ncol));
-    launch_scan_u32_u32(J->hcnt, J->hoff, J->ncol, J->scan32, nullptr, s);
-    launch_scatter_hits(J->tris, J->hit_key, J->n_hits, J->g[0], J->view(), J->hoff, J->hcnt, J->hrec, s);
-    launch_l1_parity(J->hrec, J->hoff, J->g[0], J->view(), (int64_t)J->off[J->nm].ptask0, J->ins, s);
-    HIP_TRY(hipGetLastError());
+    TRY(alloc_parity(J));
+    TRY(parity_kernels(J, s));
     J->inout_done = true;
     return GPV_OK;
 }
@@ -990,10 +1011,11 @@ int job_finish(Job* J)
         }
         return v;
     };
-    // setup, L1 SAT (+ boundary scan), L1 parity (ray expansion, the size
-    // read-back and the parity fill), compaction + bins, L2 parity, L2 SAT;
-    // for a batch these are the batch's stage times
-    float ms[6] = {el(0, 1), el(1, 2), el(2, 4), 0.0f, 0.0f, 0.0f};
+    // setup, L1 SAT (+ boundary scan), L1 parity (ray expansion + parity
+    // fill), compaction + bins, L2 parity, L2 SAT, each on its own stream;
+    // stages on the side stream overlap the main one's.  For a batch these
+    // are the batch's stage times
+    float ms[6] = {el(0, 1), el(1, 12), el(2, 13) + el(3, 14), 0.0f, 0.0f, 0.0f};
     if (J->n2max > 0) {
         ms[3] = el(4, 5);
         ms[4] = el(5, 11);   // in the fused pipeline the two Level2 stages overlap
@@ -1028,7 +1050,8 @@ int job_finish(Job* J)
     release(J, J->pair_key); release(J, J->pair_tri); release(J, J->col_key); release(J, J->col_z);
     release(J, J->hit_key); release(J, J->wcnt); release(J, J->boff);
     release(J, J->list); release(J, J->coff); release(J, J->crec); release(J, J->hcnt);
-    release(J, J->hoff); release(J, J->hrec); release(J, J->tri_mesh); release(J, J->bmesh);
+    release(J, J->hoff); release(J, J->hrec); release(J, J->scan_side); release(J, J->tri_mesh);
+    release(J, J->bmesh);
     release(J, J->d_grids); release(J, J->d_off); release(J, J->own_soup); release(J, J->mctr);
     release(J, J->fnorm); release(J, J->slist);
     J->tris = nullptr;
@@ -1040,8 +1063,23 @@ int job_finish(Job* J)
 int job_run(Job* J)
 {
     TRY(stage_setup(J));
-    TRY(launch_sat_half(J));
-    TRY(launch_ray_half(J));
+    gpv_ctx* c = J->ctx;
+    // independent stages go beside each other on the side stream: the ray
+    // expansion beside the Level1 SAT, the parity fill beside compaction (and,
+    // in run_level2, the Level2 parity beside the Level2 SAT).  Side-stream
+    // work allocates nothing and its buffers live until the job finishes, so
+    // pool blocks keep one stream order.
+    const bool side = c->tune.l2_overlap != 0;
+    if (side) {
+        HIP_TRY(hipEventRecord(J->ev[15], c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->side, J->ev[15], 0));
+        TRY(launch_ray_half(J, c->side));
+        TRY(launch_sat_half(J));
+        HIP_TRY(hipStreamWaitEvent(c->stream, J->ev[13], 0));
+    } else {
+        TRY(launch_sat_half(J));
+        TRY(launch_ray_half(J));
+    }
     TRY(read_counts(J, true));
     int64_t needed = 0;
     bool ray_ok = false;
@@ -1055,11 +1093,18 @@ int job_run(Job* J)
         if (!ray_ok) TRY(complete_ray(J, &ray_ok));
     }
     if (needed > 0 || !ray_ok) return fail(GPV_ERR_CAPACITY, "buffer capacity did not converge");
-    TRY(launch_parity(J));
-    TRY(launch_compact(J));
-    if (J->ctx->tune.l2_overlap) {
+    if (side) {
+        TRY(alloc_parity(J));
+        HIP_TRY(hipEventRecord(J->ev[3], c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->side, J->ev[3], 0));
+        TRY(parity_kernels(J, c->side));
+        J->inout_done = true;
+        TRY(launch_compact(J));
         TRY(run_level2(J));
+        HIP_TRY(hipStreamWaitEvent(c->stream, J->ev[14], 0));
     } else {
+        TRY(launch_parity(J));
+        TRY(launch_compact(J));
         TRY(run_l2_parity(J));
         TRY(run_l2_sat(J));
     }

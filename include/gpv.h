@@ -131,7 +131,9 @@ typedef struct gpv_tune {
                                   when the bins average fewer entries than this per voxel (8);
                                   0 never, a large value always */
     int sort_items_per_block;  /* bin entries per block in the counting sorts (4096) */
-    int l2_overlap;            /* run the Level2 parity on a second stream beside the Level2 SAT (1) */
+    int l2_overlap;            /* run independent stages on the context's second stream: the ray
+                                  expansion beside the Level1 SAT, the parity fill beside compaction,
+                                  the Level2 parity beside the Level2 SAT (1) */
 } gpv_tune;
 int gpv_ctx_get_tune(gpv_ctx *ctx, gpv_tune *out);
 int gpv_ctx_set_tune(gpv_ctx *ctx, const gpv_tune *t);
