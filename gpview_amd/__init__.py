@@ -110,7 +110,8 @@ EXPORTED = [
     "gpv_mesh_bbox", "gpv_grid_derive", "gpv_voxelize", "gpv_voxelize_batch", "gpv_grid_create",
     "gpv_classify_tessellation", "gpv_classify_inout", "gpv_compact_boundary",
     "gpv_classify_inout_level2", "gpv_classify_tessellation_level2", "gpv_grid_finish",
-    "gpv_grid_get_info", "gpv_grid_download", "gpv_grid_download_normals", "gpv_grid_device_views", "gpv_save_raw",
+    "gpv_grid_get_info", "gpv_grid_download", "gpv_grid_download_normals", "gpv_grid_device_views", "gpv_grid_download_bits",
+    "gpv_save_raw",
     "gpv_grid_release",
     "gpv_mesh_free", "gpv_mesh_flatten", "gpv_mesh_bounds", "gpv_mesh_icosphere",
     "gpv_mesh_torus_knot", "gpv_mesh_torus", "gpv_mesh_drilled_box", "gpv_mesh_mixed",
@@ -162,6 +163,7 @@ def lib():
     L.gpv_grid_download.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
     L.gpv_grid_download_normals.argtypes = [_vp, _vp, _vp, _vp]
     L.gpv_grid_device_views.argtypes = [_vp, P(_vp), P(_vp), P(_vp), P(_vp), P(_vp)]
+    L.gpv_grid_download_bits.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
     L.gpv_save_raw.argtypes = [_vp, _vp, c_int, ctypes.c_char_p]
     L.gpv_mesh_free.argtypes = [P(Mesh)]
     L.gpv_mesh_flatten.argtypes = [P(Mesh), _f32p]
@@ -527,6 +529,20 @@ class Grid:
                                        bidx.ctypes.data if nb else None,
                                        l2.ctypes.data if (nb and n2) else None), "gpv_grid_download")
         return {"l1": l1, "prefix": prefix, "bidx": bidx[:nb], "l2": l2[:nb * n2 ** 3]}
+
+    def download_bits(self):
+        """gpv_grid_download_bits: (l1 boundary, l1 inside, l2 boundary, l2
+        inside) uint64 bitplanes (Level2 ones empty without Level2)."""
+        i = self.info()
+        nx, ny, nz = i["n"]
+        w1 = nz * ny * ((nx + 63) // 64)
+        n2 = i["n2"]
+        w2 = i["nb"] * ((n2 ** 3 + 63) // 64) if n2 else 0
+        out = [np.zeros(max(w1, 1), np.uint64), np.zeros(max(w1, 1), np.uint64),
+               np.zeros(max(w2, 1), np.uint64), np.zeros(max(w2, 1), np.uint64)]
+        _check(lib().gpv_grid_download_bits(self.ctx.h, self.h, *(a.ctypes.data for a in out)),
+               "gpv_grid_download_bits")
+        return out[0][:w1], out[1][:w1], out[2][:w2], out[3][:w2]
 
     def download_normals(self):
         """(l1n [nb,3], l2n [nb*n2^3,3]) float32 of a grid made with normals."""

@@ -1533,6 +1533,28 @@ int gpv_grid_device_views(const gpv_grid* G, const uint64_t** l1b, const uint64_
     return GPV_OK;
 }
 
+int gpv_grid_download_bits(gpv_ctx* c, const gpv_grid* G, uint64_t* l1b, uint64_t* l1i, uint64_t* l2b,
+                           uint64_t* l2i)
+{
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    TRY(check_grid(c, G));
+    const Job* J = G->job;
+    if (!J->finished) return fail(GPV_ERR_STATE, "grid is not finished");
+    HIP_TRY(hipSetDevice(c->device));
+    const GridDev& g = J->g[G->m];
+    const MeshOff& o = J->off[G->m];
+    const size_t w1 = (size_t)g.nz * g.ny * g.nxw * 8;
+    const size_t w2 = g.n2 > 0 ? (size_t)J->nb(G->m) * g.w2 * 8 : 0;
+    hipStream_t s = c->stream;
+    if (l1b && w1) HIP_TRY(hipMemcpyAsync(l1b, J->bnd + o.word0, w1, hipMemcpyDeviceToHost, s));
+    if (l1i && w1) HIP_TRY(hipMemcpyAsync(l1i, J->ins + o.word0, w1, hipMemcpyDeviceToHost, s));
+    if (l2b && w2) HIP_TRY(hipMemcpyAsync(l2b, J->sat2 + o.l2w0, w2, hipMemcpyDeviceToHost, s));
+    if (l2i && w2) HIP_TRY(hipMemcpyAsync(l2i, J->par2 + o.l2w0, w2, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return GPV_OK;
+}
+
 static int download_impl(gpv_ctx* c, const gpv_grid* G, uint8_t unit, uint8_t* l1, int32_t* prefix,
                          int32_t* bidx, uint8_t* l2)
 {

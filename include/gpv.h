@@ -243,6 +243,12 @@ int gpv_grid_download_normals(gpv_ctx *ctx, const gpv_grid *grid, float *l1_norm
 int gpv_grid_device_views(const gpv_grid *grid, const uint64_t **l1_boundary_bits,
                           const uint64_t **l1_inside_bits, const uint64_t **l2_boundary_bits,
                           const uint64_t **l2_inside_bits, const int32_t **bidx);
+/* The same bitplanes copied to host memory (1 bit per voxel instead of the
+ * state bytes of gpv_grid_download): l1 buffers hold nz*ny*ceil(nx/64) words,
+ * l2 buffers nb*ceil(n2^3/64) words; any pointer may be NULL.  No
+ * counterpart in the reference, which copies float grids. */
+int gpv_grid_download_bits(gpv_ctx *ctx, const gpv_grid *grid, uint64_t *l1_boundary_bits,
+                           uint64_t *l1_inside_bits, uint64_t *l2_boundary_bits, uint64_t *l2_inside_bits);
 /* Object::SaveVoxelization (src/Object.cpp:2934-3075): writes
  * Obj<obj_id>VoxelConfig.txt, Obj<obj_id>Level1InOut.raw and, with Level2,
  * Obj<obj_id>Level1BoundaryPrefixSum.raw and Obj<obj_id>Level2InOut.raw

@@ -258,3 +258,55 @@ def test_autotune_keeps_results(gv):
                 assert np.array_equal(a, b), extreme
         with pytest.raises(gv.GpvError):
             c.set_tune(sort_items_per_block=1)
+
+
+def test_bits_download_matches_states(gv, ctx):
+    """gpv_grid_download_bits returns the bitplanes behind gpv_grid_download's
+    state bytes: rebuilt states are equal."""
+    from gpview_amd import workloads
+    ms = workloads.mixed_stream(gv, 3, seed=97, tri_range=(2000, 9000), level1=(40, 64), level2=(4, 8))
+    gs = ctx.voxelize_batch(ms)
+    try:
+        for g in gs:
+            i = g.info()
+            nx, ny, nz = i["n"]
+            nxw = (nx + 63) // 64
+            b1, n1, b2, n2b = g.download_bits()
+            d = g.download()
+            bits = lambda w, n: np.unpackbits(w.view(np.uint8), bitorder="little")[:n]
+            bnd = np.unpackbits(b1.view(np.uint8), bitorder="little").reshape(nz, ny, nxw * 64)[:, :, :nx].ravel()
+            ins = np.unpackbits(n1.view(np.uint8), bitorder="little").reshape(nz, ny, nxw * 64)[:, :, :nx].ravel()
+            assert np.array_equal(np.where(bnd == 1, 2, ins).astype(np.uint8), d["l1"])
+            n23 = i["n2"] ** 3
+            w2 = (n23 + 63) // 64
+            sb = np.unpackbits(b2.view(np.uint8), bitorder="little").reshape(i["nb"], w2 * 64)[:, :n23].ravel()
+            pb = np.unpackbits(n2b.view(np.uint8), bitorder="little").reshape(i["nb"], w2 * 64)[:, :n23].ravel()
+            assert np.array_equal(np.where(sb == 1, 2, pb).astype(np.uint8), d["l2"])
+    finally:
+        for g in gs:
+            g.release()
+
+
+def test_stress_level1_512_batch_equals_alone(gv, ctx):
+    """Exactness stress at the stress workload's resolution (Level1 512,
+    Level2 16; lattice-snapped, sliver and face-coplanar triangles): each
+    mesh of a batch has the same bitplanes as the mesh alone (the single-mesh
+    path itself is pinned to the oracle by the parity tests)."""
+    from gpview_amd import workloads
+    ms = workloads.stress(gv, 3, seed=101, tri_range=(60000, 250000))
+    say(f"stress 512/16: {[len(m['soup']) for m in ms]} triangles")
+    gs = ctx.voxelize_batch(ms)
+    try:
+        for k, (m, gb) in enumerate(zip(ms, gs)):
+            ga = ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], 512, 16, True)
+            try:
+                ib, ia = gb.info(), ga.info()
+                for key in ("nb", "pairs", "l1_inside", "l2_inside", "l2_boundary", "max_tris_per_voxel"):
+                    assert ib[key] == ia[key], (k, key)
+                for x, y in zip(gb.download_bits(), ga.download_bits()):
+                    assert np.array_equal(x, y), k
+            finally:
+                ga.release()
+    finally:
+        for g in gs:
+            g.release()
