@@ -17,6 +17,8 @@ struct GridDev {
     int n2, w2;
     int normals;           // this mesh asks for Level1/Level2 normals
     int l2cuda;            // Level2 in the reference's CUDA arithmetic (gpv_params::cuda_level2)
+    int r2mono;            // each Level2 ray-origin table non-decreasing over its whole length
+    int pad;
     float bminx, bminy, bminz, bmaxx, bmaxy, bmaxz;
     float h1x, h1y, h1z;   // Level1 half extents
     float h2x, h2y, h2z;   // Level2 half extents

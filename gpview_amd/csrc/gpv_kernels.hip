@@ -137,26 +137,6 @@ __device__ __forceinline__ int mesh_of_word(const MeshOff* __restrict__ off, int
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// The same lookup by a whole wave for a wave-uniform w: 64-ary search, two
-// rounds of loads for a few hundred meshes instead of log2(nm) dependent ones.
-// Every lane of the wave must call it.
-__device__ __forceinline__ int wave_mesh_of_word(const MeshOff* __restrict__ off, int nm, uint64_t w)
-{
-    const int lane = __lane_id();
-    int lo = 0, hi = nm;   // off[lo].word0 <= w; the answer lies in [lo, hi)
-    while (hi - lo > 1) {
-        const int step = (hi - lo + 63) / 64;
-        const int idx = lo + lane * step;
-        const bool le = idx < hi && off[idx].word0 <= w;
-        const uint64_t m = __ballot(le);
-        const int L = 63 - __clzll((long long)m);
-        const int nlo = lo + L * step;
-        hi = nlo + step < hi ? nlo + step : hi;
-        lo = nlo;
-    }
-    return uniform_i(lo);
-}
-
 // ---------------------------------------------------------------------------
 // fills (used instead of memset so that the same code can sit in a graph)
 // ---------------------------------------------------------------------------
@@ -1396,21 +1376,6 @@ __device__ __forceinline__ float lane_f(float v, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-__device__ __forceinline__ double lane_d(double v, int l)
-{
-    const uint64_t u = (uint64_t)__double_as_longlong(v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
-__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int l)
-{
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // voxels of a group: the capacity for a mesh's Level2 word count
 __device__ __forceinline__ uint32_t l2_group_cap(int w2)
 {
@@ -1962,7 +1927,14 @@ __device__ __forceinline__ int mesh_of_col(const MeshOff* __restrict__ off, int 
 // column, lane = column entry, looping over the column's n2^2 sub-columns
 // (sc = q*n2 + p); entry e's pass bits go to cpass[e * stride + sc / 64], bit
 // sc % 64.  Every boundary voxel of the column then reads them instead of
-// repeating the test.
+// repeating the test.  The entry's last word, cpass[e * stride + stride - 1],
+// is its header: bits 0-31 hold kh + 1 and bits 32-63 zc + 1, where zc is
+// the last layer whose voxels take the entry as a candidate (zmax, or the
+// top layer for the CUDA kernels' whole-column candidates) and kh < zc a
+// layer such that t > 1e-6 holds for every ray origin of the column in
+// layers [0, kh] (t_decided over that box; -1 if none found).  For a voxel at
+// k1 <= kh the entry then flips every layer of each sub-column it passes
+// without a t test.
 constexpr int CP_WAVES = 4;
 
 template <bool kM>
@@ -1984,11 +1956,34 @@ __global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass(const uint32_t* __re
     // sub-column origins: x of sub-index p in lane p, y of q in lane q
     const float rxl = lane < n2 ? g.r2x[(int64_t)i1 * n2 + lane] : 0.0f;
     const float ryl = lane < n2 ? g.r2y[(int64_t)j1 * n2 + lane] : 0.0f;
+    // the column's box of ray origins (tables non-decreasing)
+    const bool tbox = g.r2mono != 0;
+    const float oxlo = g.r2x[(int64_t)i1 * n2], oxhi = g.r2x[(int64_t)i1 * n2 + n2 - 1];
+    const float oylo = g.r2y[(int64_t)j1 * n2], oyhi = g.r2y[(int64_t)j1 * n2 + n2 - 1];
+    const float oz0 = g.r2z[0];
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t e = base + (uint32_t)lane;
         const bool own = e < e1;
         const ColRec r = load_colrec(rec, own ? e : e0);
         uint64_t* dst = cpass + (uint64_t)e * stride;
+        {
+            const int zc = g.l2cuda ? g.nz - 1 : r.zmax;
+            // kh: the largest of the layers zmax - 1, - 2, - 4, - 8 whose box
+            // [layer 0, that layer] t_decided passes (below its lowest layer a
+            // triangle lies above every origin; the few probes are independent).
+            // A probe that returns +1 is itself the proof for its box.
+            int kh = -1;
+            if (tbox) {
+#pragma unroll
+                for (int d = 8; d >= 1; d >>= 1) {
+                    const int k = r.zmax - d;
+                    if (k >= 0 && t_decided(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, r.v1x, r.v1y, r.v1z, oxlo,
+                                            oxhi, oylo, oyhi, oz0, g.r2z[(int64_t)k * n2 + n2 - 1], eps_above()) > 0)
+                        kh = k;
+                }
+            }
+            if (own) dst[stride - 1] = (uint64_t)(uint32_t)(kh + 1) | ((uint64_t)(uint32_t)(zc + 1) << 32);
+        }
         uint64_t w = 0;
         int k = 0, slot = 0;
         for (int q = 0; q < n2; ++q) {
@@ -2053,6 +2048,7 @@ __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev
     float oz[N2 > 0 ? N2 : 1];
 #pragma unroll
     for (int rr = 0; rr < (N2 > 0 ? N2 : 1); ++rr) oz[rr] = lane_f(ozl, rr);
+    const uint64_t layers = (1ull << n2) - 1;
     uint64_t bits = 0;   // bit (mm*n2 + r) for sub-column lane + 64*mm
     for (uint32_t base = e0; base < e1; base += 64) {
         const uint32_t i = base + lane;
@@ -2060,26 +2056,31 @@ __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev
         uint64_t row[NMX];   // candidate's det/u/v pass bits over the sub-columns
 #pragma unroll
         for (int mm = 0; mm < NMX; ++mm) row[mm] = 0;
+        bool hit = false;   // every layer's t passes: no t test, no record
         if (i < e1) {
-            const ColRec r = load_colrec(rec, i);
-            cr[lane] = r;
-            // the CUDA kernels' candidates: the whole column's triangle list
-            if (g.l2cuda || r.zmax >= k1c) {
+            // header (k_col_pass): candidate while k1 <= zc
+            const uint64_t hw = cpass[(uint64_t)i * stride + stride - 1];
+            const int kh = (int)(uint32_t)hw - 1, zc = (int)(uint32_t)(hw >> 32) - 1;
+            if (k1c <= zc) {
 #pragma unroll
                 for (int mm = 0; mm < NMX; ++mm)
                     if (mm < nm) row[mm] = cpass[(uint64_t)i * stride + mm];
+                hit = k1c <= kh;
+                if (!hit) cr[lane] = load_colrec(rec, i);
             }
         }
+        const uint64_t all_hit = __ballot(hit);
         __builtin_amdgcn_wave_barrier();
         // pass bits per sub-column slot: bit j = candidate j passes at this
         // lane's sub-column
         uint64_t pass[NMX];
 #pragma unroll
         for (int mm = 0; mm < NMX; ++mm) pass[mm] = mm < nm ? wave_transpose64(row[mm]) : 0;
-        // t per sub-voxel for the passing pairs only
+        // t per sub-voxel for the passing pairs not decided for the voxel
 #pragma unroll
         for (int mm = 0; mm < NMX; ++mm) {
-            for (uint64_t m = pass[mm]; m; m &= m - 1) {
+            if (__popcll(pass[mm] & all_hit) & 1) bits ^= layers << (mm * n2);
+            for (uint64_t m = pass[mm] & ~all_hit; m; m &= m - 1) {
                 const int j = __ffsll((unsigned long long)m) - 1;
                 const ColRec r = cr[j];
                 const float tx = ox[mm] - r.v1x, ty = oy[mm] - r.v1y;
@@ -2100,6 +2101,21 @@ __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev
             }
         }
         __builtin_amdgcn_wave_barrier();
+    }
+    if (nsc % 64 == 0) {
+        // word r * (nsc / 64) + mm holds layer r of sub-columns 64mm + lane:
+        // one ballot per word, lane w keeps word w, one coalesced store
+        uint64_t mine = 0;
+#pragma unroll
+        for (int mm = 0; mm < NMX; ++mm) {
+            if (mm >= nm) break;
+            for (int r = 0; r < n2; ++r) {
+                const uint64_t wd = __ballot((bits >> (mm * n2 + r)) & 1ull);
+                if (lane == r * nm + mm) mine = wd;
+            }
+        }
+        if (lane < w2) out[lane] = mine;
+        return;
     }
     for (int i = lane; i < w2; i += 64) acc[i] = 0ull;
     __builtin_amdgcn_wave_barrier();
@@ -2153,6 +2169,7 @@ __device__ __forceinline__ void l2_parity_pack4(int n2rt, const GridDev& g, uint
     float oz[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) oz[r] = (lv && r < n2) ? g.r2z[(int64_t)k1 * n2 + r] : 0.0f;
+    const uint32_t layers = (1u << n2) - 1;
     // rounds of 16 candidates: the most any quarter needs
     uint32_t nr = (e1 - e0 + 15) >> 4;
 #pragma unroll
@@ -2165,13 +2182,24 @@ __device__ __forceinline__ void l2_parity_pack4(int n2rt, const GridDev& g, uint
     for (uint32_t rd = 0; rd < rounds; ++rd) {
         const uint32_t i = e0 + rd * 16 + (uint32_t)sc;
         uint64_t row = 0;
+        bool hit = false;   // every layer's t passes for this slot's voxel
         if (lv && i < e1) {
-            const ColRec r = load_colrec(rec, i);
-            cr[lane] = r;
-            if (r.zmax >= k1) row = (cpass[(uint64_t)i * stride] & 0xffffull) << (16 * v);
+            // header (k_col_pass): candidate while k1 <= zc
+            const uint64_t hw = cpass[(uint64_t)i * stride + stride - 1];
+            const int kh = (int)(uint32_t)hw - 1, zc = (int)(uint32_t)(hw >> 32) - 1;
+            if (k1 <= zc) {
+                row = (cpass[(uint64_t)i * stride] & 0xffffull) << (16 * v);
+                hit = k1 <= kh;
+                if (!hit) cr[lane] = load_colrec(rec, i);
+            }
         }
+        // slot j's flag is for slot j's own voxel, the only one whose lanes
+        // see bit j after the transpose
+        const uint64_t all_hit = __ballot(hit);
         __builtin_amdgcn_wave_barrier();
-        for (uint64_t m = wave_transpose64(row); m; m &= m - 1) {
+        const uint64_t pm = wave_transpose64(row);
+        if (__popcll(pm & all_hit) & 1) par ^= layers;
+        for (uint64_t m = pm & ~all_hit; m; m &= m - 1) {
             const int j = __ffsll((unsigned long long)m) - 1;
             const ColRec r = cr[j];
             const float tx = ox - r.v1x, ty = oy - r.v1y;

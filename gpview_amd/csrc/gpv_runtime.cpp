@@ -329,7 +329,10 @@ int derive_grid(const float bmin[3], const float bmax[3], float mms, const gpv_p
 //   L2 ray  minPoint + (p + 0.5)*g2        src/Object.cpp:1134-1136, 1158-1160
 // cuda_l2: both Level2 tables in the CUDA kernels' float expression
 //   (2p+1)*ext2 + mid - ext1               cuda/CUDAClassifyTessellation.cu:423-425, 472-474
-void build_tables(const gpv_grid_info& I, bool cuda_l2, std::vector<float>& tab, size_t off[12])
+// Returns whether each Level2 ray-origin table is non-decreasing over its
+// whole length (the parity kernels then bound a range of origins by its
+// first and last entries).
+bool build_tables(const gpv_grid_info& I, bool cuda_l2, std::vector<float>& tab, size_t off[12])
 {
     const int n2 = I.n2 > 0 ? I.n2 : 0;
     size_t pos = 0;
@@ -364,6 +367,10 @@ void build_tables(const gpv_grid_info& I, bool cuda_l2, std::vector<float>& tab,
             }
         }
     }
+    bool mono = true;
+    for (int a = 0; a < 3; a++)
+        for (size_t k = 1; k < (size_t)I.n[a] * n2; k++) mono = mono && tab[off[9 + a] + k - 1] <= tab[off[9 + a] + k];
+    return mono;
 }
 
 int expand_blocks(const gpv_ctx* c) { return std::max(1, c->num_cu) * std::max(1, c->tune.expand_blocks_per_cu); }
@@ -505,7 +512,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         tri0 += (uint32_t)d[m].ntris;
         std::vector<float> tab;
         size_t o[12];
-        build_tables(J->info[m], g.l2cuda != 0, tab, o);
+        g.r2mono = build_tables(J->info[m], g.l2cuda != 0, tab, o) ? 1 : 0;
         for (int k = 0; k < 12; ++k) tab_at[(size_t)m * 12 + k] = J->h_tab.size() + o[k];
         J->h_tab.insert(J->h_tab.end(), tab.begin(), tab.end());
     }
@@ -833,10 +840,13 @@ int l2_parity_plan(Job* J)
     return GPV_OK;
 }
 
+// cpass words per column entry: the pass bits of its sub-columns and a header
+int cpass_stride(const Job* J) { return (J->n2max * J->n2max + 63) / 64 + 1; }
+
 // Level2 parity kernels on stream s (cpass allocated, waves planned)
 int l2_parity_kernels(Job* J, hipStream_t s)
 {
-    const int stride = (J->n2max * J->n2max + 63) / 64;
+    const int stride = cpass_stride(J);
     launch_col_pass(J->coff, J->crec, J->g[0], J->view(), J->ncol, stride, J->cpass, s);
     launch_l2_parity(J->bidx, J->B, J->l2p_waves, J->coff, J->crec, J->cpass, stride, J->g[0], J->view(), J->par2,
                      s);
@@ -847,7 +857,7 @@ int l2_parity_kernels(Job* J, hipStream_t s)
 
 int alloc_cpass(Job* J)
 {
-    const int stride = (J->n2max * J->n2max + 63) / 64;
+    const int stride = cpass_stride(J);
     release(J, J->cpass);
     return alloc(J, &J->cpass, (size_t)std::max<uint64_t>(J->n_cols, 1) * stride);
 }

@@ -157,4 +157,55 @@ __host__ __device__ inline void slab_layers(double nx, double ny, double nz, dou
     zb = c + 1 < z1 ? c + 1 : z1;
 }
 
+// ---------------------------------------------------------------------------
+// Level2 parity: the +z ray test's t > 1e-6 decided for a whole voxel.
+// For a column record (edges e1, e2, 1/det inv, vertex v1) the parity kernels
+// evaluate per ray origin o, in float (src/TriRayIntersection.cpp:112-122):
+//   tx = o.x - v1x, ty = o.y - v1y, tz = o.z - v1z
+//   q  = (ty e1z - tz e1y, tz e1x - tx e1z, tx e1y - ty e1x)
+//   s  = (e2x qx + e2y qy) + e2z qz,  t = s * inv,  pass iff t >= eps.
+// Rounding is monotone, so for every origin of a voxel T_a lies in
+// [tl_a, th_a] = [fl(olo_a - v1_a), fl(ohi_a - v1_a)].  Let n = e1 x e2,
+// N_a = |e1_b e2_c| + |e1_c e2_b| (b, c the other axes), M_a = max(|tl_a|,
+// |th_a|) and B = sum_a M_a N_a.  S = e2 . (T x e1) = T . n is the exact value
+// on the float T, affine in T, so g S (g = sign(inv)) ranges over
+// [sum_a min(g n_a tl_a, g n_a th_a), sum_a max(...)] on the box.
+//   - each q_a is within 2U of its two products, and the dot product adds 3U
+//     per term: |s - S| <= (5U + 6U^2) B (the bound K of the products is B);
+//   - the float n_a is within 2U N_a, each float product adds U, the sum of
+//     three terms 2U: the computed range ends are within 6.0001U B.
+// So g s lies in [lo - E, hi + E] with E = 16U B_f (B_f the float B, itself
+// within 5U of B) plus 2^-100 for gradual underflow.  t = fl(g s |inv|):
+// pass for every origin when fl(fl(lo - E) |inv|) >= eps (1 + 2^-18), fail
+// for every origin when fl(hi + E) <= 0 or fl(fl(hi + E) |inv|) <= eps
+// (1 - 2^-18); the 2^-18 margins absorb the last few roundings.  Returns +1,
+// -1, or 0 when undecided.  `slack` scales E (1 in the kernels; the tests
+// lower it to show the terms are needed).  All in float: few registers.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline void t_axis(float n, float tl, float th, float nabs, float& lo, float& hi, float& B)
+{
+    const float p = n * tl, q = n * th;
+    lo += fminf(p, q);
+    hi += fmaxf(p, q);
+    B += fmaxf(fabsf(tl), fabsf(th)) * nabs;
+}
+
+__host__ __device__ inline int t_decided(float e1x, float e1y, float e1z, float e2x, float e2y, float e2z, float inv,
+                                         float v1x, float v1y, float v1z, float oxlo, float oxhi, float oylo,
+                                         float oyhi, float ozlo, float ozhi, float eps, float slack = 1.0f)
+{
+    const float g = inv > 0.0f ? 1.0f : -1.0f;
+    float lo = 0.0f, hi = 0.0f, B = 0.0f;
+    t_axis(g * (e1y * e2z - e1z * e2y), oxlo - v1x, oxhi - v1x, fabsf(e1y * e2z) + fabsf(e1z * e2y), lo, hi, B);
+    t_axis(g * (e1z * e2x - e1x * e2z), oylo - v1y, oyhi - v1y, fabsf(e1z * e2x) + fabsf(e1x * e2z), lo, hi, B);
+    t_axis(g * (e1x * e2y - e1y * e2x), ozlo - v1z, ozhi - v1z, fabsf(e1x * e2y) + fabsf(e1y * e2x), lo, hi, B);
+    if (!(B <= 3.0e38f) || !(fabsf(lo) <= 3.0e38f) || !(fabsf(hi) <= 3.0e38f)) return 0;
+    const float E = slack * (B * 9.5367431640625e-07f + 7.888609052210118e-31f);   // 16U B + 2^-100
+    const float ai = fabsf(inv);
+    const float l = lo - E, m = hi + E;
+    if (l > 0.0f && l * ai >= eps * 1.0000038f) return 1;
+    if (m <= 0.0f || m * ai <= eps * 0.9999962f) return -1;
+    return 0;
+}
+
 }  // namespace gpv

@@ -51,3 +51,27 @@ extern "C" void float_slab_keep_batch(int64_t n, const float* tv, const float* c
         keep[i] = (uint8_t)gpv::float_slab_keeps(f, tv + 9 * i, c[3 * i], c[3 * i + 1], c[3 * i + 2]);
     }
 }
+
+// Level2 parity: gpv::t_decided for the column record of triangle i (edges,
+// det and 1/det in the float expressions of the +z Moller-Trumbore test,
+// src/TriRayIntersection.cpp:86-101) over the origin box
+// box[6i..6i+6) = (oxlo, oxhi, oylo, oyhi, ozlo, ozhi).  dec = 2 where the
+// record's det fails (no ray can hit).
+extern "C" void t_decided_batch(int64_t n, const float* tv, const float* box, float eps, double slack, int8_t* dec)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const float* t = tv + 9 * i;
+        const float e1x = t[3] - t[0], e1y = t[4] - t[1], e1z = t[5] - t[2];
+        const float e2x = t[6] - t[0], e2y = t[7] - t[1], e2z = t[8] - t[2];
+        const float px = 0.0f * e2z - e2y, py = e2x - 0.0f * e2z, pz = 0.0f * e2y - 0.0f * e2x;
+        const float det = e1x * px + e1y * py + e1z * pz;
+        if ((double)det > -1e-6 && (double)det < 1e-6) {
+            dec[i] = 2;
+            continue;
+        }
+        const float inv = 1.f / det;
+        const float* b = box + 6 * i;
+        dec[i] = (int8_t)gpv::t_decided(e1x, e1y, e1z, e2x, e2y, e2z, inv, t[0], t[1], t[2], b[0], b[1], b[2], b[3],
+                                        b[4], b[5], eps, slack);
+    }
+}
