@@ -217,6 +217,10 @@ struct Job {
     uint32_t* coff = nullptr;
     uint32_t* scan_side = nullptr;   // scan temporary of the parity fill (it may run beside compaction)
     float4* crec = nullptr;   // 3 float4 per column entry
+    // column-entry sort on the side stream (beside the bins): its own scratch
+    bool cols_side = false;
+    uint32_t* cs_cnt = nullptr;
+    uint4* cs_bucket = nullptr;
     uint64_t* cpass = nullptr;   // det/u/v pass bits per column entry and sub-column
     uint32_t *gflag = nullptr, *gidx = nullptr, *gstart = nullptr;   // Level2 SAT voxel groups
     int64_t l2p_waves = 0;   // Level2 parity waves (l2_parity_plan)
@@ -740,6 +744,32 @@ int alloc_parity(Job* J)
     return alloc(J, &J->scan_side, scan_tmp_elems(J->ncol));
 }
 
+// buffers of the column-entry sort when it runs on the side stream
+// (allocated before the fork, released after the Level2 join)
+int alloc_cols(Job* J)
+{
+    release(J, J->coff);
+    release(J, J->crec);
+    release(J, J->cs_cnt);
+    release(J, J->cs_bucket);
+    TRY(alloc(J, &J->coff, (size_t)J->ncol + 1));
+    TRY(alloc(J, &J->crec, (size_t)std::max<uint64_t>(J->n_cols, 1) * 3));
+    TRY(alloc(J, &J->cs_cnt, 3 * ts_tiles((uint64_t)J->ncol) + 2));
+    return alloc(J, &J->cs_bucket, (size_t)std::max<uint64_t>(J->n_cols, 1));
+}
+
+// column entries by column with their MT records, on stream s; scan is a
+// scan temporary of at least scan_tmp_elems(ncol) that no other stream uses
+int col_sort_kernels(Job* J, hipStream_t s, uint32_t* scan)
+{
+    const size_t tiles = ts_tiles((uint64_t)J->ncol);
+    const TileSortTmp ts{J->cs_cnt, J->cs_cnt + 2 * tiles + 1, scan, J->cs_bucket};
+    const int ipb = std::max(256, J->ctx->tune.sort_items_per_block);
+    launch_bin_cols(J->tris, J->col_key, J->col_z, J->n_cols, (uint32_t)J->ncol, ts, ipb, J->coff, J->crec, s);
+    HIP_TRY(hipGetLastError());
+    return GPV_OK;
+}
+
 // the parity fill on stream s: hits by column, their MT records, the fill
 int parity_kernels(Job* J, hipStream_t s)
 {
@@ -782,12 +812,14 @@ int launch_compact(Job* J)
     if (J->n2max > 0) TRY(alloc(J, &J->pmask, (size_t)std::max<uint64_t>(J->n_pairs, 1)));
     // one scratch for both counting sorts (bins by boundary rank, column
     // entries by column)
-    const size_t tiles = std::max(ts_tiles((uint64_t)J->B), ts_tiles((uint64_t)J->ncol));
+    const size_t tiles = J->cols_side ? ts_tiles((uint64_t)J->B)
+                                      : std::max(ts_tiles((uint64_t)J->B), ts_tiles((uint64_t)J->ncol));
     TileSortTmp ts{};
     uint32_t* ts_cnt = nullptr;
     uint4* ts_bucket = nullptr;
     TRY(alloc(J, &ts_cnt, 3 * tiles + 2));
-    TRY(alloc(J, &ts_bucket, (size_t)std::max<uint64_t>(std::max(J->n_pairs, J->n_cols), 1)));
+    TRY(alloc(J, &ts_bucket,
+              (size_t)std::max<uint64_t>(J->cols_side ? J->n_pairs : std::max(J->n_pairs, J->n_cols), 1)));
     TRY(ensure_scan_tmp(J, (int64_t)tiles));
     ts.total = ts_cnt;
     ts.off = ts_cnt + 2 * tiles + 1;
@@ -798,11 +830,13 @@ int launch_compact(Job* J)
                      J->nm > 1 ? nullptr : &J->mctr[0].max_per_voxel, J->list, J->tris, J->bidx, J->g[0],
                      J->view(), J->pmask, s);
     if (J->nm > 1) launch_mesh_bins(J->boff, J->B, J->view(), s);
-    release(J, J->coff);
-    release(J, J->crec);
-    TRY(alloc(J, &J->coff, (size_t)J->ncol + 1));
-    TRY(alloc(J, &J->crec, (size_t)std::max<uint64_t>(J->n_cols, 1) * 3));
-    launch_bin_cols(J->tris, J->col_key, J->col_z, J->n_cols, (uint32_t)J->ncol, ts, ipb, J->coff, J->crec, s);
+    if (!J->cols_side) {
+        release(J, J->coff);
+        release(J, J->crec);
+        TRY(alloc(J, &J->coff, (size_t)J->ncol + 1));
+        TRY(alloc(J, &J->crec, (size_t)std::max<uint64_t>(J->n_cols, 1) * 3));
+        launch_bin_cols(J->tris, J->col_key, J->col_z, J->n_cols, (uint32_t)J->ncol, ts, ipb, J->coff, J->crec, s);
+    }
     release(J, ts_cnt);
     release(J, ts_bucket);
     HIP_TRY(hipGetLastError());
@@ -972,6 +1006,8 @@ int run_level2(Job* J)
     TRY(l2_sat_kernels(J));
     HIP_TRY(hipStreamWaitEvent(s, J->ev[11], 0));
     release(J, J->cpass);
+    release(J, J->cs_cnt);
+    release(J, J->cs_bucket);
     release_l2_groups(J);
     J->l2p_done = true;
     J->l2s_done = true;
@@ -1060,6 +1096,7 @@ int job_finish(Job* J)
     release(J, J->pair_key); release(J, J->pair_tri); release(J, J->col_key); release(J, J->col_z);
     release(J, J->hit_key); release(J, J->wcnt); release(J, J->boff);
     release(J, J->list); release(J, J->coff); release(J, J->crec); release(J, J->hcnt);
+    release(J, J->cs_cnt); release(J, J->cs_bucket);
     release(J, J->hoff); release(J, J->hrec); release(J, J->scan_side); release(J, J->tri_mesh);
     release(J, J->bmesh);
     release(J, J->d_grids); release(J, J->d_off); release(J, J->own_soup); release(J, J->mctr);
@@ -1105,9 +1142,14 @@ int job_run(Job* J)
     if (needed > 0 || !ray_ok) return fail(GPV_ERR_CAPACITY, "buffer capacity did not converge");
     if (side) {
         TRY(alloc_parity(J));
+        // the column entries (read only by the Level2 parity, which runs on
+        // the side stream too) sort there after the fill, beside the bins
+        J->cols_side = J->n2max > 0;
+        if (J->cols_side) TRY(alloc_cols(J));
         HIP_TRY(hipEventRecord(J->ev[3], c->stream));
         HIP_TRY(hipStreamWaitEvent(c->side, J->ev[3], 0));
         TRY(parity_kernels(J, c->side));
+        if (J->cols_side) TRY(col_sort_kernels(J, c->side, J->scan_side));
         J->inout_done = true;
         TRY(launch_compact(J));
         TRY(run_level2(J));
