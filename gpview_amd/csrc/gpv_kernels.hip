@@ -892,10 +892,11 @@ struct ColSrc {
     }
 };
 
+// place(pos, it, ok): loads for any valid item, stores only when ok
 struct ColPlace {
     const float* tris;
     float4* rec;
-    __device__ void operator()(uint32_t pos, const uint4& it) const
+    __device__ void operator()(uint32_t pos, const uint4& it, bool ok) const
     {
         const uint32_t t = it.y;
         float tv[9];
@@ -903,6 +904,7 @@ struct ColPlace {
         for (int c = 0; c < 9; ++c) tv[c] = tris[(uint64_t)t * 9 + c];
         MtTri m;
         mt_setup(tv, m);   // entries exist only for triangles whose det passes
+        if (!ok) return;
         rec[3 * (uint64_t)pos + 0] = make_float4(m.e1x, m.e1y, m.e1z, m.e2x);
         rec[3 * (uint64_t)pos + 1] = make_float4(m.e2y, m.e2z, m.inv_det, tv[0]);
         rec[3 * (uint64_t)pos + 2] = make_float4(tv[1], tv[2], __int_as_float((int)it.z), __uint_as_float(t));
@@ -1090,15 +1092,36 @@ __device__ __forceinline__ uint32_t rank_of(const uint64_t* __restrict__ bnd,
 // global counters see one atomic per (block, tile) instead of one per item;
 // pass 3 (one block per tile) counts its keys in LDS, writes their CSR
 // offsets and places every item at its final position.  The order inside one
-// key is unspecified (as with any atomic placement).
+// key is unspecified (as with any atomic placement).  Each thread keeps TS_U
+// items in flight: their loads are issued together (indices clamped to a
+// valid item, the LDS atomics and stores predicated), since the per-item
+// chains of dependent loads, not bandwidth, bound these passes.
 // ---------------------------------------------------------------------------
 constexpr int TS_NT = 256;
+constexpr int TS_U = 4;
 
 __device__ __forceinline__ void ts_range(uint64_t n, uint64_t& lo, uint64_t& hi)
 {
     const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
     lo = (uint64_t)blockIdx.x * per;
     hi = lo + per < n ? lo + per : n;
+}
+
+// LDS histogram of the tile ids of items [lo, hi) of this block
+template <typename Src>
+__device__ __forceinline__ void ts_histogram(const Src& src, uint64_t lo, uint64_t hi, int shift, uint32_t* s_h)
+{
+    for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)TS_U * TS_NT) {
+        uint32_t k[TS_U];
+#pragma unroll
+        for (int u = 0; u < TS_U; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * TS_NT;
+            k[u] = src.sort_key(i < hi ? i : i0);
+        }
+#pragma unroll
+        for (int u = 0; u < TS_U; ++u)
+            if (i0 + (uint64_t)u * TS_NT < hi) atomicAdd(&s_h[k[u] >> shift], 1u);
+    }
 }
 
 template <typename Src>
@@ -1110,7 +1133,7 @@ __global__ __launch_bounds__(TS_NT) void k_ts_count(Src src, uint64_t n, int shi
     __syncthreads();
     uint64_t lo, hi;
     ts_range(n, lo, hi);
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) atomicAdd(&s_h[src.sort_key(i) >> shift], 1u);
+    ts_histogram(src, lo, hi, shift, s_h);
     __syncthreads();
     for (int t = threadIdx.x; t < ntiles; t += TS_NT)
         if (s_h[t]) atomicAdd(&tile_total[t], s_h[t]);
@@ -1128,14 +1151,21 @@ __global__ __launch_bounds__(TS_NT) void k_ts_bucket(Src src, uint64_t n, int sh
     __syncthreads();
     uint64_t lo, hi;
     ts_range(n, lo, hi);
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) atomicAdd(&s_h[src.sort_key(i) >> shift], 1u);
+    ts_histogram(src, lo, hi, shift, s_h);
     __syncthreads();
     for (int t = threadIdx.x; t < ntiles; t += TS_NT)
         if (s_h[t]) s_h[t] = tile_off[t] + atomicAdd(&tile_fill[t], s_h[t]);
     __syncthreads();
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += TS_NT) {
-        const uint4 it = src.item(i);
-        bucket[atomicAdd(&s_h[it.x >> shift], 1u)] = it;
+    for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)TS_U * TS_NT) {
+        uint4 it[TS_U];
+#pragma unroll
+        for (int u = 0; u < TS_U; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * TS_NT;
+            it[u] = src.item(i < hi ? i : i0);
+        }
+#pragma unroll
+        for (int u = 0; u < TS_U; ++u)
+            if (i0 + (uint64_t)u * TS_NT < hi) bucket[atomicAdd(&s_h[it[u].x >> shift], 1u)] = it[u];
     }
 }
 
@@ -1154,7 +1184,17 @@ __global__ __launch_bounds__(TS_NT) void k_ts_place(const uint4* __restrict__ bu
     for (uint32_t j = threadIdx.x; j < T; j += TS_NT) s_c[j] = 0;
     if (threadIdx.x == 0) s_max = 0;
     __syncthreads();
-    for (uint32_t i = b0 + threadIdx.x; i < b1; i += TS_NT) atomicAdd(&s_c[bucket[i].x - k0], 1u);
+    for (uint32_t i0 = b0 + threadIdx.x; i0 < b1; i0 += TS_U * TS_NT) {
+        uint32_t k[TS_U];
+#pragma unroll
+        for (int u = 0; u < TS_U; ++u) {
+            const uint32_t i = i0 + (uint32_t)u * TS_NT;
+            k[u] = bucket[i < b1 ? i : i0].x;
+        }
+#pragma unroll
+        for (int u = 0; u < TS_U; ++u)
+            if (i0 + (uint32_t)u * TS_NT < b1) atomicAdd(&s_c[k[u] - k0], 1u);
+    }
     __syncthreads();
     // exclusive scan of the T counts: thread r owns counts [r*C, (r+1)*C)
     const uint32_t C = T / TS_NT;
@@ -1185,9 +1225,19 @@ __global__ __launch_bounds__(TS_NT) void k_ts_place(const uint4* __restrict__ bu
     if (k0 + T >= nkeys && threadIdx.x == TS_NT - 1) off[nkeys] = b1;
     if (threadIdx.x == 0 && max_out && s_max) atomicMax(max_out, s_max);
     __syncthreads();
-    for (uint32_t i = b0 + threadIdx.x; i < b1; i += TS_NT) {
-        const uint4 it = bucket[i];
-        place(atomicAdd(&s_c[it.x - k0], 1u), it);
+    for (uint32_t i0 = b0 + threadIdx.x; i0 < b1; i0 += TS_U * TS_NT) {
+        uint4 it[TS_U];
+        uint32_t pos[TS_U];
+#pragma unroll
+        for (int u = 0; u < TS_U; ++u) {
+            const uint32_t i = i0 + (uint32_t)u * TS_NT;
+            it[u] = bucket[i < b1 ? i : i0];
+        }
+#pragma unroll
+        for (int u = 0; u < TS_U; ++u)
+            pos[u] = i0 + (uint32_t)u * TS_NT < b1 ? atomicAdd(&s_c[it[u].x - k0], 1u) : 0u;
+#pragma unroll
+        for (int u = 0; u < TS_U; ++u) place(pos[u], it[u], i0 + (uint32_t)u * TS_NT < b1);
     }
 }
 
@@ -1282,8 +1332,9 @@ struct PairSrc {
 struct PairPlace {
     uint32_t* list;
     uint64_t* pmask;
-    __device__ void operator()(uint32_t pos, const uint4& it) const
+    __device__ void operator()(uint32_t pos, const uint4& it, bool ok) const
     {
+        if (!ok) return;
         list[pos] = it.y;
         if (pmask) pmask[pos] = (uint64_t)it.z | ((uint64_t)it.w << 32);
     }
