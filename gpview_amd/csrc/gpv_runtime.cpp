@@ -1112,10 +1112,11 @@ int job_run(Job* J)
     TRY(stage_setup(J));
     gpv_ctx* c = J->ctx;
     // independent stages go beside each other on the side stream: the ray
-    // expansion beside the Level1 SAT, the parity fill beside compaction (and,
-    // in run_level2, the Level2 parity beside the Level2 SAT).  Side-stream
-    // work allocates nothing and its buffers live until the job finishes, so
-    // pool blocks keep one stream order.
+    // expansion beside the Level1 SAT, the parity fill and the column-entry
+    // sort beside the bins (and, in run_level2, the Level2 parity beside the
+    // Level2 SAT).  Side-stream work allocates nothing and its buffers are
+    // allocated before the fork and live until the join, so pool blocks keep
+    // one stream order.
     const bool side = c->tune.l2_overlap != 0;
     if (side) {
         HIP_TRY(hipEventRecord(J->ev[15], c->stream));
@@ -1146,12 +1147,14 @@ int job_run(Job* J)
         // the side stream too) sort there after the fill, beside the bins
         J->cols_side = J->n2max > 0;
         if (J->cols_side) TRY(alloc_cols(J));
+        // the main stream's compaction (the critical path) is enqueued first;
+        // the side stream waits for the point before it
         HIP_TRY(hipEventRecord(J->ev[3], c->stream));
+        TRY(launch_compact(J));
         HIP_TRY(hipStreamWaitEvent(c->side, J->ev[3], 0));
         TRY(parity_kernels(J, c->side));
         if (J->cols_side) TRY(col_sort_kernels(J, c->side, J->scan_side));
         J->inout_done = true;
-        TRY(launch_compact(J));
         TRY(run_level2(J));
         HIP_TRY(hipStreamWaitEvent(c->stream, J->ev[14], 0));
     } else {
