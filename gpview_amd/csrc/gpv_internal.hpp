@@ -96,7 +96,7 @@ void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
 
 void launch_mesh_ids(const MeshOff* off, int nm, int64_t T, uint32_t* tri_mesh, hipStream_t s);
 void launch_tri_setup(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
-                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, float4* slab2,
+                      uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, float4* slab2, int part,
                       hipStream_t s);
 
 // exclusive scan of n u32 into out[0..n] (out[n] = total); tmp sized by

@@ -270,7 +270,10 @@ void launch_mesh_ids(const MeshOff* off, int nm, int64_t T, uint32_t* tri_mesh, 
 // then visits, per column, only the layers the slab can keep.
 constexpr int kZSlabMin = 3;
 
-template <bool kM>
+// kPart: 1 the SAT half (index AABB, items, slabs, test counts), 2 the ray
+// half (column bound, items), 3 both.  The halves share nothing but the
+// vertices, so they can run on two streams.
+template <bool kM, int kPart>
 __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tris, int64_t T, GridDev g0, Meshes M,
                                                    int4* __restrict__ box0, int4* __restrict__ box1,
                                                    uint32_t* __restrict__ sat_items,
@@ -287,45 +290,48 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
         float tv[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) tv[c] = tris[t * 9 + c];
-        int lo[3], dd[3];
-        const int n[3] = {g.nx, g.ny, g.nz};
-        const float mn[3] = {g.bminx, g.bminy, g.bminz};
-        const float mx[3] = {g.bmaxx, g.bmaxy, g.bmaxz};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int i0 = vertex_index(tv[a], mn[a], mx[a], n[a]);
-            const int i1 = vertex_index(tv[3 + a], mn[a], mx[a], n[a]);
-            const int i2 = vertex_index(tv[6 + a], mn[a], mx[a], n[a]);
-            int l = min(i0, min(i1, i2));
-            int h = max(i0, max(i1, i2));
-            l = max(l, 0);
-            h = min(h, n[a] - 1);
-            lo[a] = l;
-            dd[a] = h >= l ? h - l + 1 : 0;
-        }
         MtTri m;
         const bool vis = mt_setup(tv, m);
-        box0[t] = make_int4(lo[0], lo[1], lo[2], vis ? 1 : 0);
-        box1[t] = make_int4(dd[0], dd[1], dd[2], 0);
-        const bool any = dd[0] > 0 && dd[1] > 0 && dd[2] > 0;
-        sat_items[t] = any ? (uint32_t)(dd[0] * dd[1]) : 0u;
-        tests += any ? (uint64_t)dd[0] * dd[1] * dd[2] : 0;
-        if (any && dd[2] >= kZSlabMin) {
-            const double clo[3] = {g.c1x[lo[0]], g.c1y[lo[1]], g.c1z[lo[2]]};
-            const double chi[3] = {g.c1x[lo[0] + dd[0] - 1], g.c1y[lo[1] + dd[1] - 1], g.c1z[lo[2] + dd[2] - 1]};
-            const PlaneSlab sl = plane_slab(tv, g.h1x, g.h1y, g.h1z, clo, chi);
-            slab1[t] = make_double4(sl.nx, sl.ny, sl.nz, sl.lim);
+        if constexpr ((kPart & 1) != 0) {
+            int lo[3], dd[3];
+            const int n[3] = {g.nx, g.ny, g.nz};
+            const float mn[3] = {g.bminx, g.bminy, g.bminz};
+            const float mx[3] = {g.bmaxx, g.bmaxy, g.bmaxz};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int i0 = vertex_index(tv[a], mn[a], mx[a], n[a]);
+                const int i1 = vertex_index(tv[3 + a], mn[a], mx[a], n[a]);
+                const int i2 = vertex_index(tv[6 + a], mn[a], mx[a], n[a]);
+                int l = min(i0, min(i1, i2));
+                int h = max(i0, max(i1, i2));
+                l = max(l, 0);
+                h = min(h, n[a] - 1);
+                lo[a] = l;
+                dd[a] = h >= l ? h - l + 1 : 0;
+            }
+            box0[t] = make_int4(lo[0], lo[1], lo[2], vis ? 1 : 0);
+            box1[t] = make_int4(dd[0], dd[1], dd[2], 0);
+            const bool any = dd[0] > 0 && dd[1] > 0 && dd[2] > 0;
+            sat_items[t] = any ? (uint32_t)(dd[0] * dd[1]) : 0u;
+            tests += any ? (uint64_t)dd[0] * dd[1] * dd[2] : 0;
+            if (any && dd[2] >= kZSlabMin) {
+                const double clo[3] = {g.c1x[lo[0]], g.c1y[lo[1]], g.c1z[lo[2]]};
+                const double chi[3] = {g.c1x[lo[0] + dd[0] - 1], g.c1y[lo[1] + dd[1] - 1], g.c1z[lo[2] + dd[2] - 1]};
+                const PlaneSlab sl = plane_slab(tv, g.h1x, g.h1y, g.h1z, clo, chi);
+                slab1[t] = make_double4(sl.nx, sl.ny, sl.nz, sl.lim);
+            }
+            if (any && g.n2 > 0) {
+                // over every Level2 center of the index AABB's Level1 voxels (the
+                // tables are monotone, so the extreme entries bound them)
+                const int n2 = g.n2;
+                const double clo[3] = {g.c2x[(int64_t)lo[0] * n2], g.c2y[(int64_t)lo[1] * n2], g.c2z[(int64_t)lo[2] * n2]};
+                const double chi[3] = {g.c2x[(int64_t)(lo[0] + dd[0]) * n2 - 1], g.c2y[(int64_t)(lo[1] + dd[1]) * n2 - 1],
+                                       g.c2z[(int64_t)(lo[2] + dd[2]) * n2 - 1]};
+                const FloatSlab f = float_slab(plane_slab(tv, g.h2x, g.h2y, g.h2z, clo, chi), tv, clo, chi);
+                slab2[t] = make_float4(f.nx, f.ny, f.nz, f.lim);
+            }
         }
-        if (any && g.n2 > 0) {
-            // over every Level2 center of the index AABB's Level1 voxels (the
-            // tables are monotone, so the extreme entries bound them)
-            const int n2 = g.n2;
-            const double clo[3] = {g.c2x[(int64_t)lo[0] * n2], g.c2y[(int64_t)lo[1] * n2], g.c2z[(int64_t)lo[2] * n2]};
-            const double chi[3] = {g.c2x[(int64_t)(lo[0] + dd[0]) * n2 - 1], g.c2y[(int64_t)(lo[1] + dd[1]) * n2 - 1],
-                                   g.c2z[(int64_t)(lo[2] + dd[2]) * n2 - 1]};
-            const FloatSlab f = float_slab(plane_slab(tv, g.h2x, g.h2y, g.h2z, clo, chi), tv, clo, chi);
-            slab2[t] = make_float4(f.nx, f.ny, f.nz, f.lim);
-        }
+        if constexpr ((kPart & 2) == 0) continue;
         uint32_t rit = 0;
         int4 rr = make_int4(0, 0, 0, 0);
         if (vis) {
@@ -345,6 +351,7 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
         ray_rng[t] = rr;
         ray_items[t] = rit;
     }
+    if constexpr ((kPart & 1) == 0) return;
     if constexpr (!kM) {
         block_add_u64(tests, &M.ctr[0].tests_l1);
     } else {
@@ -358,17 +365,30 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
     }
 }
 
+template <int kPart>
+static void tri_setup_part(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
+                           uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, float4* slab2,
+                           hipStream_t s)
+{
+    if (M.nm > 1)
+        k_tri_setup<true, kPart><<<grid_for(T, 256, 1 << 30), 256, 0, s>>>(tris, T, g0, M, box0, box1, sat_items,
+                                                                            ray_rng, ray_items, slab1, slab2);
+    else
+        k_tri_setup<false, kPart><<<grid_for(T, 256, 1024), 256, 0, s>>>(tris, T, g0, M, box0, box1, sat_items,
+                                                                          ray_rng, ray_items, slab1, slab2);
+}
+
 void launch_tri_setup(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
                       uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, float4* slab2,
-                      hipStream_t s)
+                      int part, hipStream_t s)
 {
     if (T <= 0) return;
-    if (M.nm > 1)
-        k_tri_setup<true><<<grid_for(T, 256, 1 << 30), 256, 0, s>>>(tris, T, g0, M, box0, box1, sat_items, ray_rng,
-                                                                     ray_items, slab1, slab2);
+    if (part == 1)
+        tri_setup_part<1>(tris, T, g0, M, box0, box1, sat_items, ray_rng, ray_items, slab1, slab2, s);
+    else if (part == 2)
+        tri_setup_part<2>(tris, T, g0, M, box0, box1, sat_items, ray_rng, ray_items, slab1, slab2, s);
     else
-        k_tri_setup<false><<<grid_for(T, 256, 1024), 256, 0, s>>>(tris, T, g0, M, box0, box1, sat_items, ray_rng,
-                                                                   ray_items, slab1, slab2);
+        tri_setup_part<3>(tris, T, g0, M, box0, box1, sat_items, ray_rng, ray_items, slab1, slab2, s);
 }
 
 // ---------------------------------------------------------------------------

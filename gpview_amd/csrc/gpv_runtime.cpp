@@ -191,6 +191,7 @@ struct Job {
     float4* slab2 = nullptr;    // plane slab over the Level2 centers of the index AABB (float form)
     uint32_t *sat_items = nullptr, *ray_items = nullptr;
     uint64_t *sat_off = nullptr, *ray_off = nullptr, *scan64 = nullptr;
+    uint64_t* scan_ray = nullptr;   // scan temporary of the ray half of the set-up (side stream)
     uint32_t* scan32 = nullptr;
     size_t scan_cap = 0;
     Counters* ctr = nullptr;
@@ -597,10 +598,35 @@ int stage_setup(Job* J)
     launch_fill_u64(reinterpret_cast<uint64_t*>(J->mctr), 0, (int64_t)(sizeof(MeshCtr) / 8) * J->nm, s);
     if (J->nm > 1) launch_mesh_ids(J->d_off, J->nm, J->T, J->tri_mesh, s);
     launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items,
-                     J->slab1, J->slab2, s);
+                     J->slab1, J->slab2, 3, s);
     TRY(ensure_scan_tmp(J, J->T));
     launch_scan_u32_u64(J->sat_items, J->sat_off, J->T, J->scan64, s);
     launch_scan_u32_u64(J->ray_items, J->ray_off, J->T, J->scan64, s);
+    HIP_TRY(hipGetLastError());
+    J->setup_done = true;
+    return GPV_OK;
+}
+
+// The set-up in two halves: the SAT half on the main stream, the ray half
+// (and, after it, the ray expansion) on the side stream, forked at ev[15].
+int stage_setup_split(Job* J)
+{
+    gpv_ctx* c = J->ctx;
+    hipStream_t s = c->stream;
+    HIP_TRY(hipEventRecord(J->ev[0], s));
+    launch_fill_u64(reinterpret_cast<uint64_t*>(J->mctr), 0, (int64_t)(sizeof(MeshCtr) / 8) * J->nm, s);
+    if (J->nm > 1) launch_mesh_ids(J->d_off, J->nm, J->T, J->tri_mesh, s);
+    TRY(ensure_scan_tmp(J, J->T));
+    release(J, J->scan_ray);
+    TRY(alloc(J, &J->scan_ray, scan_tmp_elems(J->T)));
+    HIP_TRY(hipEventRecord(J->ev[15], s));
+    launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items,
+                     J->slab1, J->slab2, 1, s);
+    launch_scan_u32_u64(J->sat_items, J->sat_off, J->T, J->scan64, s);
+    HIP_TRY(hipStreamWaitEvent(c->side, J->ev[15], 0));
+    launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items,
+                     J->slab1, J->slab2, 2, c->side);
+    launch_scan_u32_u64(J->ray_items, J->ray_off, J->T, J->scan_ray, c->side);
     HIP_TRY(hipGetLastError());
     J->setup_done = true;
     return GPV_OK;
@@ -1092,7 +1118,7 @@ int job_finish(Job* J)
     release(J, J->box0); release(J, J->box1); release(J, J->rng); release(J, J->slab1); release(J, J->slab2);
     release(J, J->pmask);
     release(J, J->sat_items); release(J, J->ray_items); release(J, J->sat_off); release(J, J->ray_off);
-    release(J, J->scan64); J->scan32 = nullptr; J->scan_cap = 0;
+    release(J, J->scan64); J->scan32 = nullptr; J->scan_cap = 0; release(J, J->scan_ray);
     release(J, J->pair_key); release(J, J->pair_tri); release(J, J->col_key); release(J, J->col_z);
     release(J, J->hit_key); release(J, J->wcnt); release(J, J->boff);
     release(J, J->list); release(J, J->coff); release(J, J->crec); release(J, J->hcnt);
@@ -1109,7 +1135,6 @@ int job_finish(Job* J)
 // the fused pipeline: every stage back to back, one size read-back
 int job_run(Job* J)
 {
-    TRY(stage_setup(J));
     gpv_ctx* c = J->ctx;
     // independent stages go beside each other on the side stream: the ray
     // expansion beside the Level1 SAT, the parity fill and the column-entry
@@ -1119,12 +1144,12 @@ int job_run(Job* J)
     // one stream order.
     const bool side = c->tune.l2_overlap != 0;
     if (side) {
-        HIP_TRY(hipEventRecord(J->ev[15], c->stream));
-        HIP_TRY(hipStreamWaitEvent(c->side, J->ev[15], 0));
+        TRY(stage_setup_split(J));
         TRY(launch_ray_half(J, c->side));
         TRY(launch_sat_half(J));
         HIP_TRY(hipStreamWaitEvent(c->stream, J->ev[13], 0));
     } else {
+        TRY(stage_setup(J));
         TRY(launch_sat_half(J));
         TRY(launch_ray_half(J));
     }
