@@ -1746,6 +1746,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_voxel(const float* __r
     __shared__ float s_idp[L2_WAVES][64];
     __shared__ uint32_t s_off[L2_WAVES][64];
     __shared__ float s_cz[L2_WAVES][16];
+    __shared__ uint8_t s_win[L2_WAVES][64];   // per round: staged entry starting at each item position
     const int wv = threadIdx.x >> 6;
     const int lane = __lane_id();
     const int64_t b = (int64_t)blockIdx.x * L2_WAVES + wv;
@@ -1778,6 +1779,7 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_voxel(const float* __r
     float4* ssl = s_sl[wv];
     float* sidp = s_idp[wv];
     uint32_t* soff = s_off[wv];
+    uint8_t* win = s_win[wv];
     __builtin_amdgcn_wave_barrier();
     uint32_t qh = 0, qt = 0;   // ring head/tail (uniform)
     const uint32_t e0 = boff[b], e1 = boff[b + 1];
@@ -1820,23 +1822,32 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_voxel(const float* __r
             }
         }
         const uint32_t incl = wave_incl_scan<uint32_t>(P);
-        soff[lane] = incl - P;
+        const uint32_t so = incl - P;   // this lane's entry: items [so, so + P)
+        soff[lane] = so;
         const uint32_t S = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         __builtin_amdgcn_wave_barrier();
         // the chunk's (triangle, sub-column) items, 64 per round
         for (uint32_t l0 = 0; l0 < S; l0 += 64) {
+            // each item's entry: every entry with items in [l0, l0 + 64) marks
+            // its first position there (the one running into the window marks
+            // 0; starts are distinct), and item l takes the mark at the highest
+            // marked position <= l
+            win[lane] = 0xff;
+            __builtin_amdgcn_wave_barrier();
+            if (P && so < l0 + 64u && so + P > l0) win[so > l0 ? so - l0 : 0u] = (uint8_t)lane;
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t wj = win[lane];
+            const uint64_t marks = __ballot(wj != 0xffu) & ((2ull << lane) - 1ull);
+            const int top = marks ? 63 - __clzll((long long)marks) : 0;
+            const int jw = __shfl((int)wj, top);
+            __builtin_amdgcn_wave_barrier();
             const uint32_t l = l0 + (uint32_t)lane;
             uint32_t keep = 0;
             int p = 0, q = 0, j = 0, ra = 0, rb = -1;
             uint32_t boxbit = 0, mz = 0;
             bool in = false;
             if (l < S) {
-                // the last triangle whose items start at or before l (it has items)
-                int lo = 0;
-#pragma unroll
-                for (int st = 32; st; st >>= 1)
-                    if (soff[lo + st] <= l) lo += st;
-                j = lo;
+                j = jw;
                 const uint32_t d = dec[j];
                 const int pa = d & 15, qa = (d >> 4) & 15;
                 ra = (d >> 8) & 15;
