@@ -1501,6 +1501,7 @@ struct L2Lds {
     float4 sl[64];                // staged entry: Level2 slab
     uint32_t off[64];             // staged entry: first item in the chunk's item space
     uint8_t ev[64];               // staged entry: its voxel in the group
+    uint8_t win[64];              // per round: staged entry starting at each item position
     uint32_t vb[L2_GMAX + 1];     // group voxel v: first bin entry
     float c[L2_CMAX];             // centers: x [v*n2+p], y [nv*n2 + ...], z [2*nv*n2 + ...]
 };
@@ -1618,22 +1619,30 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_grp(const float* __res
                 }
             }
             const uint32_t incl = wave_incl_scan<uint32_t>(P);
-            L.off[lane] = incl - P;
+            const uint32_t so = incl - P;   // this lane's entry: items [so, so + P)
+            L.off[lane] = so;
             const uint32_t S = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
             __builtin_amdgcn_wave_barrier();
             // the chunk's (entry, sub-column) items, 64 per round
             for (uint32_t l0 = 0; l0 < S; l0 += 64) {
+                // each item's entry, as in k_l2_sat_voxel: entries mark their
+                // first position in the window, an item takes the highest mark
+                // at or below it
+                L.win[lane] = 0xff;
+                __builtin_amdgcn_wave_barrier();
+                if (P && so < l0 + 64u && so + P > l0) L.win[so > l0 ? so - l0 : 0u] = (uint8_t)lane;
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t wj = L.win[lane];
+                const uint64_t marks = __ballot(wj != 0xffu) & ((2ull << lane) - 1ull);
+                const int top = marks ? 63 - __clzll((long long)marks) : 0;
+                const int jw = __shfl((int)wj, top);
+                __builtin_amdgcn_wave_barrier();
                 const uint32_t l = l0 + (uint32_t)lane;
                 uint32_t keep = 0;
                 int p = 0, q = 0, j = 0, v = 0, ra = 0, rb = -1;
                 uint32_t boxbit = 0, mz = 0;
                 if (l < S) {
-                    // the last entry whose items start at or before l (it has items)
-                    int lo = 0;
-#pragma unroll
-                    for (int st = 32; st; st >>= 1)
-                        if (L.off[lo + st] <= l) lo += st;
-                    j = lo;
+                    j = jw;
                     v = (int)L.ev[j];
                     const uint32_t d = L.dec[j];
                     const int pa = d & 15, qa = (d >> 4) & 15;
