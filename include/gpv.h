@@ -132,8 +132,9 @@ typedef struct gpv_tune {
                                   0 never, a large value always */
     int sort_items_per_block;  /* bin entries per block in the counting sorts (4096) */
     int l2_overlap;            /* run independent stages on the context's second stream: the ray
-                                  expansion beside the Level1 SAT, the parity fill beside compaction,
-                                  the Level2 parity beside the Level2 SAT (1) */
+                                  half of the set-up and the ray expansion beside the SAT half and
+                                  the Level1 SAT, the parity fill and the column-entry sort beside
+                                  the bins, the Level2 parity beside the Level2 SAT (1) */
 } gpv_tune;
 int gpv_ctx_get_tune(gpv_ctx *ctx, gpv_tune *out);
 int gpv_ctx_set_tune(gpv_ctx *ctx, const gpv_tune *t);
