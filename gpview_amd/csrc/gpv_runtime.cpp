@@ -158,7 +158,7 @@ struct gpv_ctx {
     char* h_buf = nullptr;   // pinned read-back staging
     size_t h_cap = 0;
     int num_cu = 256;
-    gpv_tune tune{8, 8, 4096, 1};
+    gpv_tune tune{16, 8, 2048, 1};
     std::vector<hipEvent_t> events;   // recycled stage events of finished jobs
 };
 
@@ -1368,7 +1368,7 @@ int gpv_ctx_autotune(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, 
     float warm = 0.0f;
     int st = tune_trial(c, d, nm, on_device, 1, &warm);
     if (st == GPV_OK) st = tune_trial(c, d, nm, on_device, repeats, &base);
-    const std::vector<int> expand = {4, 8, 16}, group = {0, 4, 8, 16, 1 << 20}, items = {2048, 4096, 8192},
+    const std::vector<int> expand = {4, 8, 16, 32}, group = {0, 4, 8, 16, 1 << 20}, items = {1024, 2048, 4096, 8192},
                            overlap = {0, 1};
     auto sweep = [&](int gpv_tune::*field, const std::vector<int>& vals) -> int {
         for (int v : vals) {

@@ -126,11 +126,11 @@ int gpv_ctx_pool_stats(gpv_ctx *ctx, gpv_pool_stats *out);
  * counterpart in the reference (its launch shapes are fixed,
  * cuda/CUDAHelper.h:23-38).  None of them changes a result. */
 typedef struct gpv_tune {
-    int expand_blocks_per_cu;  /* persistent blocks per CU of the Level1 SAT and ray expansion (8) */
+    int expand_blocks_per_cu;  /* persistent blocks per CU of the Level1 SAT and ray expansion (16) */
     int l2_group_below;        /* the Level2 SAT shares a wave among consecutive boundary voxels
                                   when the bins average fewer entries than this per voxel (8);
                                   0 never, a large value always */
-    int sort_items_per_block;  /* bin entries per block in the counting sorts (4096) */
+    int sort_items_per_block;  /* bin entries per block in the counting sorts (2048) */
     int l2_overlap;            /* run independent stages on the context's second stream: the ray
                                   half of the set-up and the ray expansion beside the SAT half and
                                   the Level1 SAT, the parity fill and the column-entry sort beside

@@ -248,7 +248,8 @@ def test_autotune_keeps_results(gv):
         shapes, best_ms = c.autotune(ms, repeats=2)
         say(f"autotune: {shapes} {best_ms:.3f} ms")
         assert shapes == c.get_tune() and best_ms > 0
-        assert shapes["expand_blocks_per_cu"] in (4, 8, 16) and shapes["l2_overlap"] in (0, 1)
+        assert shapes["expand_blocks_per_cu"] in (4, 8, 16, 32) and shapes["l2_overlap"] in (0, 1)
+        assert shapes["sort_items_per_block"] in (1024, 2048, 4096, 8192)
         for a, b in zip(base, run()):
             assert np.array_equal(a, b)
         for extreme in ({"l2_group_below": 0, "l2_overlap": 0, "expand_blocks_per_cu": 1, "sort_items_per_block": 256},
