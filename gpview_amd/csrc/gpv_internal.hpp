@@ -129,6 +129,19 @@ struct TileSortTmp {
     uint4* bucket;
 };
 size_t ts_tiles(uint64_t nkeys);
+
+// Level2 SAT voxel groups (k_l2_sat_grp): at most kL2GroupMax voxels, their
+// Level2 words within one wave's 64 and their 3 * n2 centers each within
+// kL2GroupCenters floats of LDS
+constexpr int kL2GroupMax = 16;
+constexpr int kL2GroupCenters = 192;
+__host__ __device__ inline int l2_group_voxels(int w2, int n2)
+{
+    int c = w2 > 0 ? 64 / w2 : 64;
+    if (c > kL2GroupMax) c = kL2GroupMax;
+    if (n2 > 0 && c > kL2GroupCenters / (3 * n2)) c = kL2GroupCenters / (3 * n2);
+    return c;
+}
 // column entries (key = column << 32 | triangle) -> CSR by column with the MT
 // records; ncol <= 2^27
 void launch_bin_cols(const float* tris, const uint64_t* key, const uint32_t* zmax, uint64_t n, uint32_t ncol,

@@ -1433,8 +1433,8 @@ void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStrea
 // ---------------------------------------------------------------------------
 constexpr int L2_WAVES = 4;
 constexpr int L2_QCAP = 128;   // ring of queued (triangle, sub-voxel) entries per wave
-constexpr int L2_GMAX = 8;     // voxels per group
-constexpr int L2_CMAX = 192;   // center floats per group: 3 * voxels * n2 <= 192 for n2 <= 16
+constexpr int L2_GMAX = kL2GroupMax;   // voxels per group
+constexpr int L2_CMAX = kL2GroupCenters;   // center floats per group: 3 * voxels * n2
 
 // 1/d for the sub-box widths d = 1..16 (float quotients, as 1.0f / d)
 __constant__ float kInvSmall[17] = {0.0f,        1.0f,        0.5f,        1.0f / 3.0f, 0.25f,       0.2f,
@@ -1447,11 +1447,10 @@ __device__ __forceinline__ float lane_f(float v, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// voxels of a group: the capacity for a mesh's Level2 word count
-__device__ __forceinline__ uint32_t l2_group_cap(int w2)
+// voxels of a group: the capacity for a mesh's resolution
+__device__ __forceinline__ uint32_t l2_group_cap(int w2, int n2)
 {
-    const uint32_t c = w2 > 0 ? 64u / (uint32_t)w2 : 64u;
-    return c < (uint32_t)L2_GMAX ? c : (uint32_t)L2_GMAX;
+    return (uint32_t)l2_group_voxels(w2, n2);
 }
 
 // Group starts: flag[b] = 1 where a new group begins (mesh start, a new
@@ -1464,7 +1463,8 @@ __global__ void k_l2_group_flags(const uint32_t* __restrict__ boff, int64_t B, G
         const int mesh = kM ? (int)M.bmesh[b] : 0;
         const GridDev& g = grid_of<kM>(g0, M, mesh);
         const int64_t lb = b - (int64_t)(kM ? M.off[mesh].b0 : 0u);
-        const bool st = lb == 0 || lb % (int64_t)l2_group_cap(g.w2) == 0 || (boff[b] >> 6) != (boff[b - 1] >> 6);
+        const bool st =
+            lb == 0 || lb % (int64_t)l2_group_cap(g.w2, g.n2) == 0 || (boff[b] >> 6) != (boff[b - 1] >> 6);
         flag[b] = st ? 1u : 0u;
     }
 }

@@ -987,7 +987,7 @@ int l2_sat_kernels(Job* J)
         for (int m = 0; m < J->nm; ++m) {
             const int64_t nb = J->nb(m);
             const int w2 = J->g[m].w2;
-            const int64_t cap = std::min<int64_t>(w2 > 0 ? 64 / w2 : 64, 8);
+            const int64_t cap = l2_group_voxels(w2, J->g[m].n2);
             gmax += (nb + cap - 1) / cap + 1;
         }
         gmax = std::min<int64_t>(gmax, J->B);
