@@ -558,7 +558,7 @@ void launch_scan_u32_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t*
 // triangles).
 // ---------------------------------------------------------------------------
 constexpr int XB = 256;
-constexpr int XCAP = 384;
+constexpr int XCAP = 256;
 
 // 64-ary search by one wave: largest t in [lo, hi) with off[t] <= x, given
 // off[lo] <= x and (hi == n or off[hi] > x).  ~log64 rounds of dependent
