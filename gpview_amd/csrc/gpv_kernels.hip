@@ -808,7 +808,7 @@ __global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tri
                                                    const uint64_t* __restrict__ off, GridDev g0, Meshes M,
                                                    Append64 hits)
 {
-    constexpr uint32_t HB = 1536;
+    constexpr uint32_t HB = 1024;
     __shared__ uint64_t s_hk[HB];
     __shared__ uint32_t s_n;
     __shared__ uint64_t s_gbase;
