@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgpv.so")
 
 GPV_OK, GPV_ERR_INVALID, GPV_ERR_OOM, GPV_ERR_CAPACITY, GPV_ERR_HIP, GPV_ERR_NO_DEVICE, GPV_ERR_IO, \
-    GPV_ERR_STATE = range(8)
+    GPV_ERR_STATE, GPV_ERR_TOO_LARGE = range(9)
+NOALLOC_SCRATCH_BYTES = 256 << 20   # GPV_NOALLOC_SCRATCH_BYTES
 
 
 class GpvError(RuntimeError):
@@ -107,7 +108,8 @@ EXPORTED = [
     "gpv_ctx_create", "gpv_ctx_destroy", "gpv_last_error", "gpv_status_string",
     "gpv_device_upload", "gpv_device_free", "gpv_ctx_pool_stats", "gpv_ctx_get_tune", "gpv_ctx_set_tune",
     "gpv_ctx_autotune",
-    "gpv_mesh_bbox", "gpv_grid_derive", "gpv_voxelize", "gpv_voxelize_batch", "gpv_grid_create",
+    "gpv_mesh_bbox", "gpv_grid_derive", "gpv_voxelize", "gpv_voxelize_noalloc", "gpv_voxelize_batch",
+    "gpv_grid_create",
     "gpv_classify_tessellation", "gpv_classify_inout", "gpv_compact_boundary",
     "gpv_classify_inout_level2", "gpv_classify_tessellation_level2", "gpv_grid_finish",
     "gpv_grid_get_info", "gpv_grid_download", "gpv_grid_download_normals", "gpv_grid_device_views", "gpv_grid_download_bits",
@@ -147,6 +149,8 @@ def lib():
     L.gpv_grid_derive.argtypes = [_f32p, _f32p, c_float, P(Params), P(GridInfo)]
     for fn in ("gpv_voxelize", "gpv_grid_create"):
         getattr(L, fn).argtypes = [_vp, _vp, c_i64, c_int, _f32p, _f32p, c_float, P(Params), P(_vp)]
+    L.gpv_voxelize_noalloc.argtypes = [_vp, _vp, c_i64, _f32p, _f32p, c_float, P(Params), _vp, _vp, c_i64, _vp,
+                                       _vp, _vp, c_i64, P(GridInfo)]
     L.gpv_voxelize_batch.argtypes = [_vp, P(MeshDesc), c_int, c_int, P(_vp)]
     L.gpv_predict_footprint.argtypes = [P(MeshDesc), P(c_i64)]
     L.gpv_server_create.argtypes = [P(ServerOpts), P(_vp)]
@@ -366,6 +370,18 @@ class Context:
                                   np.asarray(bmin, np.float32), np.asarray(bmax, np.float32), float(mms),
                                   ctypes.byref(p), ctypes.byref(g)), "gpv_voxelize")
         return Grid(self, g)
+
+    def voxelize_noalloc(self, device_ptr, ntris, bmin, bmax, mms, voxel_count, voxel_count2, level2, l1b, l1i,
+                         l1_words, bidx, l2b, l2i, nb_cap):
+        """gpv_voxelize_noalloc: device soup in, results into the caller's
+        device buffers (pointers); returns the grid info dict."""
+        p = Params(voxel_count=int(voxel_count), voxel_count2=int(voxel_count2), level2=1 if level2 else 0)
+        info = GridInfo()
+        _check(lib().gpv_voxelize_noalloc(self.h, device_ptr, int(ntris), np.asarray(bmin, np.float32),
+                                          np.asarray(bmax, np.float32), float(mms), ctypes.byref(p), l1b, l1i,
+                                          int(l1_words), bidx, l2b, l2i, int(nb_cap), ctypes.byref(info)),
+               "gpv_voxelize_noalloc")
+        return info.as_dict()
 
     def voxelize_batch(self, meshes, device_ptrs=None):
         """gpv_voxelize_batch.  meshes: sequence of dicts with soup, bmin, bmax,

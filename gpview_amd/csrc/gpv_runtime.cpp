@@ -145,6 +145,41 @@ private:
     int64_t allocs_ = 0, reuses_ = 0, evictions_ = 0;
 };
 
+// Working memory of gpv_voxelize_noalloc: every transient buffer of that
+// entry point is carved from this array, so it calls no hipMalloc.  Each
+// device that loads the library holds its own copy until the process ends.
+// In host code `scratch` only names the symbol; its device address comes
+// from hipGetSymbolAddress, once per device (scratch_setup), and only that
+// address is ever handed to kernels or copies.
+constexpr size_t kScratchBytes = size_t(GPV_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+// gpv_voxelize_noalloc calls on one device are serialised: they share the array
+std::mutex g_scratch_call_mu[kMaxDevices];
+
+// Looks up this device's copy of the scratch array (the device must be
+// current).  Called by gpv_ctx_create; a failure is kept and reported by
+// gpv_voxelize_noalloc, the only user.
+void scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        g_scratch_err[device] = e;
+        return;
+    }
+    g_scratch_ptr[device] = p;
+    g_scratch_err[device] = hipSuccess;
+}
+
 }  // namespace
 
 struct gpv_ctx {
@@ -160,6 +195,7 @@ struct gpv_ctx {
     int num_cu = 256;
     gpv_tune tune{16, 8, 2048, 1};
     std::vector<hipEvent_t> events;   // recycled stage events of finished jobs
+    hipEvent_t na_ev[16] = {};        // stage events of gpv_voxelize_noalloc (created with the context)
 };
 
 namespace gpv {
@@ -245,6 +281,15 @@ struct Job {
     // 13 ray expansion end, 14 parity fill end, 15 fork of the ray expansion
     hipEvent_t ev[16] = {};
     std::vector<void*> bufs;
+    // gpv_voxelize_noalloc: buffers are carved from the device scratch array
+    // (arena != nullptr) instead of the pool, the result planes are the
+    // caller's, and the events are the context's own set
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    bool borrowed_events = false;
+    int64_t nb_cap = 0;                 // caller's capacity in boundary voxels
+    int32_t* out_bidx = nullptr;
+    uint64_t *out_sat2 = nullptr, *out_par2 = nullptr;
 
     Meshes view() const { return Meshes{d_grids, d_off, tri_mesh, bmesh, mctr, nm}; }
     int64_t nb(int m) const { return (int64_t)off[m + 1].b0 - (int64_t)off[m].b0; }
@@ -264,6 +309,19 @@ template <typename T>
 int alloc(Job* J, T** p, size_t count)
 {
     void* q = nullptr;
+    if (J->arena) {
+        // bump allocation in 256-byte steps; nothing is returned before the
+        // call ends (release is a no-op), so the arena never fragments
+        const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + 255) / 256 * 256;
+        if (bytes > J->arena_cap - J->arena_used)
+            return fail(GPV_ERR_TOO_LARGE,
+                        "gpv_voxelize_noalloc: this mesh needs more than the %zu bytes of scratch "
+                        "(%zu in use, %zu more asked); use gpv_voxelize",
+                        J->arena_cap, J->arena_used, bytes);
+        *p = reinterpret_cast<T*>(J->arena + J->arena_used);
+        J->arena_used += bytes;
+        return GPV_OK;
+    }
     hipError_t e = J->ctx->pool.get(std::max<size_t>(count, 1) * sizeof(T), &q);
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? GPV_ERR_OOM : GPV_ERR_HIP, "device allocation of %zu bytes: %s",
@@ -277,6 +335,10 @@ template <typename T>
 void release(Job* J, T*& p)
 {
     if (!p) return;
+    if (J->arena) {
+        p = nullptr;
+        return;
+    }
     void* q = (void*)p;
     J->ctx->pool.put(q);
     auto it = std::find(J->bufs.begin(), J->bufs.end(), q);
@@ -414,8 +476,9 @@ void job_free(Job* J)
     (void)hipStreamSynchronize(c->stream);   // pooled blocks may be reused right away
     (void)hipStreamSynchronize(c->side);
     for (void* p : J->bufs) c->pool.put(p);
-    for (auto& e : J->ev)
-        if (e) c->events.push_back(e);   // every use is complete (the streams were synchronised)
+    if (!J->borrowed_events)
+        for (auto& e : J->ev)
+            if (e) c->events.push_back(e);   // every use is complete (the streams were synchronised)
     delete J;
 }
 
@@ -436,9 +499,24 @@ GridDev grid_dev(const gpv_grid_info& info)
     return g;
 }
 
+// What gpv_voxelize_noalloc hands a job: the scratch arena and the caller's
+// result buffers (Level1 planes of l1_words words each; bidx and the Level2
+// planes for nb_cap boundary voxels).
+struct NoAlloc {
+    char* arena;
+    size_t arena_cap;
+    uint64_t *l1b, *l1i;
+    int64_t l1_words;
+    int32_t* bidx;
+    uint64_t *l2b, *l2i;
+    int64_t nb_cap;
+};
+
 // Grids, offsets, tables, the concatenated soup and every size-independent
 // buffer of a job.  nm == 1 with a device soup uses the caller's pointer.
-int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** out)
+// With na (one mesh, device soup) nothing comes from the pool: buffers are
+// carved from na->arena and the Level1 planes are the caller's.
+int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** out, const NoAlloc* na = nullptr)
 {
     *out = nullptr;
     if (nm < 1) return fail(GPV_ERR_INVALID, "a batch needs at least one mesh");
@@ -470,9 +548,32 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
     if (vox >= (int64_t(1) << 32)) return fail(GPV_ERR_INVALID, "at most 2^32-1 Level1 voxels per batch");
     if (T > 0 && on_device && nm == 1 && !d[0].tris) return fail(GPV_ERR_INVALID, "NULL soup");
     HIP_TRY(hipSetDevice(c->device));
-    TRY(ensure_hbuf(c, hctr_at(nm) + sizeof(MeshCtr) * (size_t)nm + 64));
+    const size_t hneed = hctr_at(nm) + sizeof(MeshCtr) * (size_t)nm + 64;
+    if (na) {
+        if (nm != 1 || !on_device) return fail(GPV_ERR_INVALID, "gpv_voxelize_noalloc takes one device soup");
+        if (na->l1_words < W)
+            return fail(GPV_ERR_INVALID, "Level1 planes hold %lld words; this grid needs %lld",
+                        (long long)na->l1_words, (long long)W);
+        if (hneed > c->h_cap) return fail(GPV_ERR_STATE, "the context's read-back buffer is too small");
+    } else {
+        TRY(ensure_hbuf(c, hneed));
+    }
 
     Job* J = new Job();
+    if (na) {
+        J->arena = na->arena;
+        J->arena_cap = na->arena_cap;
+        J->borrowed_events = true;
+        for (int k = 0; k < 16; ++k) J->ev[k] = c->na_ev[k];
+        J->nb_cap = na->nb_cap;
+        J->out_bidx = na->bidx;
+        J->out_sat2 = na->l2b;
+        J->out_par2 = na->l2i;
+        // bin and hit buffers within a quarter and a sixteenth of the arena;
+        // a larger count regrows (from what is left) like any other run
+        pair_cap = std::min<uint64_t>(pair_cap, na->arena_cap / 4 / 24);
+        hit_cap = std::min<uint64_t>(hit_cap, na->arena_cap / 16 / 8);
+    }
     J->ctx = c;
     J->nm = nm;
     J->info = std::move(info);
@@ -484,6 +585,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         return r;
     };
     for (auto& e : J->ev) {
+        if (J->borrowed_events) break;
         if (!c->events.empty()) {
             e = c->events.back();
             c->events.pop_back();
@@ -574,9 +676,15 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         (r = alloc(J, &J->slab1, Tn)) || (r = alloc(J, &J->slab2, Tn)) ||
         (r = alloc(J, &J->sat_items, Tn)) || (r = alloc(J, &J->ray_items, Tn)) ||
         (r = alloc(J, &J->sat_off, Tn + 1)) || (r = alloc(J, &J->ray_off, Tn + 1)) ||
-        (r = alloc(J, &J->ctr, 1)) || (r = alloc(J, &J->bnd, (size_t)W)) || (r = alloc(J, &J->ins, (size_t)W)) ||
-        (r = alloc(J, &J->wcnt, (size_t)W)) || (r = alloc(J, &J->wprefix, (size_t)W + 1)))
+        (r = alloc(J, &J->ctr, 1)) || (r = alloc(J, &J->wcnt, (size_t)W)) ||
+        (r = alloc(J, &J->wprefix, (size_t)W + 1)))
         return bail(r);
+    if (na) {
+        J->bnd = na->l1b;
+        J->ins = na->l1i;
+    } else if ((r = alloc(J, &J->bnd, (size_t)W)) || (r = alloc(J, &J->ins, (size_t)W))) {
+        return bail(r);
+    }
     J->pair_cap = pair_cap;
     J->col_cap = J->pair_cap;
     J->hit_cap = std::max<uint64_t>(hit_cap, 1u << 16);
@@ -827,7 +935,14 @@ int launch_compact(Job* J)
     const size_t Bn = (size_t)std::max<int64_t>(J->B, 1);
     release(J, J->bidx);
     release(J, J->bmesh);
-    TRY(alloc(J, &J->bidx, Bn));
+    if (J->out_bidx) {
+        if (J->B > J->nb_cap)
+            return fail(GPV_ERR_TOO_LARGE, "%lld boundary voxels; the result buffers hold %lld",
+                        (long long)J->B, (long long)J->nb_cap);
+        J->bidx = J->out_bidx;
+    } else {
+        TRY(alloc(J, &J->bidx, Bn));
+    }
     if (J->nm > 1) TRY(alloc(J, &J->bmesh, Bn));
     launch_boundary_index(J->bnd, J->wprefix, J->g[0], J->view(), J->W, J->bidx, s);
     release(J, J->boff);
@@ -869,8 +984,13 @@ int launch_compact(Job* J)
     if (J->n2max > 0) {
         release(J, J->sat2);
         release(J, J->par2);
-        TRY(alloc(J, &J->sat2, (size_t)std::max<int64_t>(J->W2, 1)));
-        TRY(alloc(J, &J->par2, (size_t)std::max<int64_t>(J->W2, 1)));
+        if (J->out_bidx) {   // W2 = B * w2 fits: B <= nb_cap was checked above
+            J->sat2 = J->out_sat2;
+            J->par2 = J->out_par2;
+        } else {
+            TRY(alloc(J, &J->sat2, (size_t)std::max<int64_t>(J->W2, 1)));
+            TRY(alloc(J, &J->par2, (size_t)std::max<int64_t>(J->W2, 1)));
+        }
     }
     J->compact_done = true;
     return GPV_OK;
@@ -1236,6 +1356,7 @@ const char* gpv_status_string(int s)
     case GPV_ERR_NO_DEVICE: return "no usable device";
     case GPV_ERR_IO: return "I/O error";
     case GPV_ERR_STATE: return "stage called out of order";
+    case GPV_ERR_TOO_LARGE: return "input exceeds the fixed scratch or result buffers";
     default: return "unknown status";
     }
 }
@@ -1276,6 +1397,14 @@ int gpv_ctx_create(int device, unsigned flags, gpv_ctx** out)
         delete c;
         return fail(GPV_ERR_HIP, "pinned staging buffer allocation failed");
     }
+    for (hipEvent_t& e : c->na_ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            gpv_ctx_destroy(c);
+            return fail(GPV_ERR_HIP, "event creation failed");
+        }
+    }
+    scratch_setup(device);
     *out = c;
     return GPV_OK;
 }
@@ -1290,6 +1419,8 @@ int gpv_ctx_destroy(gpv_ctx* c)
         (void)hipStreamSynchronize(c->side);
         for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
         c->events.clear();
+        for (hipEvent_t e : c->na_ev)
+            if (e) (void)hipEventDestroy(e);
         c->pool.trim();
         if (c->h_buf) (void)hipHostFree(c->h_buf);
         (void)hipStreamDestroy(c->stream);
@@ -1571,6 +1702,64 @@ int gpv_voxelize(gpv_ctx* c, const float* tris, int64_t T, int on_device, const 
     }
     *out = G;
     return GPV_OK;
+}
+
+int gpv_voxelize_noalloc(gpv_ctx* c, const float* tris, int64_t T, const float bmin[3], const float bmax[3],
+                         float mms, const gpv_params* params, uint64_t* l1_boundary_bits, uint64_t* l1_inside_bits,
+                         int64_t l1_words, int32_t* bidx, uint64_t* l2_boundary_bits, uint64_t* l2_inside_bits,
+                         int64_t nb_cap, gpv_grid_info* info)
+{
+    std::unique_lock<std::recursive_mutex> lk_;
+    if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
+    if (!c || !bmin || !bmax || !params || !info || !l1_boundary_bits || !l1_inside_bits || l1_words < 0 ||
+        nb_cap < 0 || (nb_cap > 0 && !bidx))
+        return fail(GPV_ERR_INVALID, "gpv_voxelize_noalloc: NULL or negative argument");
+    if (T < 0 || (T > 0 && !tris)) return fail(GPV_ERR_INVALID, "bad triangle soup");
+    if (params->normals) return fail(GPV_ERR_INVALID, "gpv_voxelize_noalloc does not compute normals");
+    if (params->level2 && nb_cap > 0 && (!l2_boundary_bits || !l2_inside_bits))
+        return fail(GPV_ERR_INVALID, "gpv_voxelize_noalloc: Level2 planes are NULL");
+    if (c->device < 0 || c->device >= kMaxDevices) return fail(GPV_ERR_INVALID, "device index out of range");
+    char* arena = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[c->device])
+            return fail(GPV_ERR_HIP, "scratch address of device %d: %s", c->device,
+                        hipGetErrorString(g_scratch_err[c->device]));
+        arena = static_cast<char*>(g_scratch_ptr[c->device]);
+    }
+    // the size-independent part of the working set, checked before any
+    // launch: boundary flag bytes and word counts per Level1 word, the
+    // per-triangle set-up records
+    gpv_grid_info gi;
+    TRY(derive_grid(bmin, bmax, mms, params, &gi));
+    const int64_t W = (int64_t)gi.n[2] * gi.n[1] * ((gi.n[0] + 63) / 64);
+    const double base = 72.0 * (double)W + 140.0 * (double)T;
+    if (base > (double)kScratchBytes)
+        return fail(GPV_ERR_TOO_LARGE,
+                    "gpv_voxelize_noalloc: %lld triangles on a %dx%dx%d grid need more than the %zu bytes of "
+                    "scratch; use gpv_voxelize",
+                    (long long)T, gi.n[0], gi.n[1], gi.n[2], kScratchBytes);
+    std::lock_guard<std::mutex> call_lk(g_scratch_call_mu[c->device]);
+    HIP_TRY(hipSetDevice(c->device));
+    gpv_mesh_desc d{};
+    d.tris = tris;
+    d.ntris = T;
+    for (int q = 0; q < 3; ++q) {
+        d.bmin[q] = bmin[q];
+        d.bmax[q] = bmax[q];
+    }
+    d.max_model_size = mms;
+    d.params = *params;
+    const NoAlloc na{arena, kScratchBytes, l1_boundary_bits, l1_inside_bits, l1_words,
+                     bidx, l2_boundary_bits, l2_inside_bits, nb_cap};
+    Job* J = nullptr;
+    TRY(job_create(c, &d, 1, 1, &J, &na));
+    int r = job_run(J);
+    if (r == GPV_OK) *info = J->info[0];
+    std::string keep = g_err;
+    job_free(J);   // waits for both streams: the scratch is free for the next call
+    g_err = keep;
+    return r;
 }
 
 int gpv_voxelize_batch(gpv_ctx* c, const gpv_mesh_desc* meshes, int nmeshes, int on_device, gpv_grid** out)

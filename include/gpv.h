@@ -35,7 +35,9 @@ enum gpv_status {
     GPV_ERR_HIP = 4,          /* HIP runtime error (message via gpv_last_error) */
     GPV_ERR_NO_DEVICE = 5,    /* no usable gfx950 device */
     GPV_ERR_IO = 6,           /* file could not be written */
-    GPV_ERR_STATE = 7         /* stage called out of order */
+    GPV_ERR_STATE = 7,        /* stage called out of order */
+    GPV_ERR_TOO_LARGE = 8     /* input exceeds the fixed scratch or the caller's result
+                                 buffers of gpv_voxelize_noalloc */
 };
 
 /* ---- parameters: GLParameters fields used by the path
@@ -159,6 +161,36 @@ int gpv_grid_derive(const float bmin[3], const float bmax[3], float max_model_si
 int gpv_voxelize(gpv_ctx *ctx, const float *tris, int64_t ntris, int tris_on_device,
                  const float bmin[3], const float bmax[3], float max_model_size,
                  const gpv_params *params, gpv_grid **out);
+
+/* Device scratch of gpv_voxelize_noalloc, per GPU, held until the process ends. */
+#define GPV_NOALLOC_SCRATCH_BYTES (256ull << 20)
+
+/* gpv_voxelize for callers that cannot allocate device memory: code under
+ * HIP graph capture, serving loops and harnesses that forbid hipMalloc.  No
+ * counterpart in the reference (its wrappers cudaMalloc around every call,
+ * src/Object.cpp:2096-2099, 2583-2586).  It makes no hipMalloc, hipFree or
+ * other allocating call: every transient buffer comes from a fixed device
+ * array of GPV_NOALLOC_SCRATCH_BYTES per GPU inside the library, and the
+ * results go to the caller's device buffers, in the layout of
+ * gpv_grid_device_views:
+ *   l1_boundary_bits, l1_inside_bits   l1_words words each; the grid needs
+ *                                      nz*ny*ceil(nx/64) (gpv_grid_derive)
+ *   bidx                               nb_cap boundary voxel indices
+ *   l2_boundary_bits, l2_inside_bits   nb_cap*ceil(n2^3/64) words each
+ *                                      (may be NULL without Level2)
+ * info receives the grid, counts and stage times (info->nb boundary voxels).
+ * tris is a device soup (gpv_voxelize with tris_on_device = 1); normals are
+ * not computed (params->normals must be 0).  A mesh whose working set does
+ * not fit the scratch, or whose boundary count exceeds nb_cap, returns
+ * GPV_ERR_TOO_LARGE.  Returns when the results are complete.  Calls on one
+ * GPU share its scratch array: the library serialises them (overlapping
+ * calls on one GPU would race on it), and each call clears what it needs
+ * cleared, so the array's previous contents never matter. */
+int gpv_voxelize_noalloc(gpv_ctx *ctx, const float *tris, int64_t ntris, const float bmin[3],
+                         const float bmax[3], float max_model_size, const gpv_params *params,
+                         uint64_t *l1_boundary_bits, uint64_t *l1_inside_bits, int64_t l1_words,
+                         int32_t *bidx, uint64_t *l2_boundary_bits, uint64_t *l2_inside_bits,
+                         int64_t nb_cap, gpv_grid_info *info);
 
 /* ---- batches ---------------------------------------------------------------- */
 /* One mesh of a batch: the arguments gpv_voxelize takes for it. */

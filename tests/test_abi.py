@@ -29,8 +29,8 @@ def test_exports_every_declared_symbol(gv):
 
 def test_status_strings(gv):
     L = gv.lib()
-    for s in range(8):
-        assert L.gpv_status_string(s)
+    for s in range(9):
+        assert L.gpv_status_string(s) != b"unknown status"
 
 
 def test_bbox_and_grid_match_oracle(gv, orc):
