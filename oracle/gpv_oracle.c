@@ -135,6 +135,169 @@ int orc_tri_box_overlap(const float c[3], const float h[3], const float tv[9])
 }
 
 /* ------------------------------------------------------------------------ */
+/* The SAT factored by the center components it reads (fast mode only)      */
+/* ------------------------------------------------------------------------ */
+/* orc_tri_box_overlap is the conjunction of 13 side-effect-free predicates.
+ * With v_k = tri_k - c taken component by component and e_k the differences
+ * of the v_k, component q of every v_k and e_k depends on c[q] alone, so:
+ *   the AABB test of axis q reads c[q] only;
+ *   the three edge tests along e x X read components 1 and 2 (c.y, c.z),
+ *   along e x Y components 0 and 2, along e x Z components 0 and 1;
+ *   only the plane test reads all three.
+ * The helpers below evaluate exactly those float expressions (same operands,
+ * same order) for one or two center components, so a grid of centers can be
+ * screened by 1-D and 2-D tables; a center that passes every screen has
+ * passed 12 of the 13 predicates and is decided by the plane test alone.
+ * No tolerance or error bound is involved. */
+static int aabb_axis_ok(const float *tv, int q, float c, float h)
+{
+    float a = tv[q] - c, b = tv[3 + q] - c, d = tv[6 + q] - c;
+    float mn = a, mx = a;
+    if (b < mn) mn = b;
+    if (b > mx) mx = b;
+    if (d < mn) mn = d;
+    if (d > mx) mx = d;
+    return !(mn > h || mx < -h);
+}
+
+/* the three tests along e_k x X (AXISTEST_X01 e0, X01 e1, X2 e2) */
+static int edges_x_ok(const float *tv, float cy, float cz, const float h[3])
+{
+    float v0y = tv[1] - cy, v1y = tv[4] - cy, v2y = tv[7] - cy;
+    float v0z = tv[2] - cz, v1z = tv[5] - cz, v2z = tv[8] - cz;
+    float e0y = v1y - v0y, e1y = v2y - v1y, e2y = v0y - v2y;
+    float e0z = v1z - v0z, e1z = v2z - v1z, e2z = v0z - v2z;
+    if (separated(e0z * v0y - e0y * v0z, e0z * v2y - e0y * v2z, fabsf(e0z) * h[1] + fabsf(e0y) * h[2])) return 0;
+    if (separated(e1z * v0y - e1y * v0z, e1z * v2y - e1y * v2z, fabsf(e1z) * h[1] + fabsf(e1y) * h[2])) return 0;
+    if (separated(e2z * v0y - e2y * v0z, e2z * v1y - e2y * v1z, fabsf(e2z) * h[1] + fabsf(e2y) * h[2])) return 0;
+    return 1;
+}
+
+/* the three tests along e_k x Y (AXISTEST_Y02 e0, Y02 e1, Y1 e2) */
+static int edges_y_ok(const float *tv, float cx, float cz, const float h[3])
+{
+    float v0x = tv[0] - cx, v1x = tv[3] - cx, v2x = tv[6] - cx;
+    float v0z = tv[2] - cz, v1z = tv[5] - cz, v2z = tv[8] - cz;
+    float e0x = v1x - v0x, e1x = v2x - v1x, e2x = v0x - v2x;
+    float e0z = v1z - v0z, e1z = v2z - v1z, e2z = v0z - v2z;
+    if (separated(-e0z * v0x + e0x * v0z, -e0z * v2x + e0x * v2z, fabsf(e0z) * h[0] + fabsf(e0x) * h[2])) return 0;
+    if (separated(-e1z * v0x + e1x * v0z, -e1z * v2x + e1x * v2z, fabsf(e1z) * h[0] + fabsf(e1x) * h[2])) return 0;
+    if (separated(-e2z * v0x + e2x * v0z, -e2z * v1x + e2x * v1z, fabsf(e2z) * h[0] + fabsf(e2x) * h[2])) return 0;
+    return 1;
+}
+
+/* the three tests along e_k x Z (AXISTEST_Z12 e0, Z0 e1, Z12 e2) */
+static int edges_z_ok(const float *tv, float cx, float cy, const float h[3])
+{
+    float v0x = tv[0] - cx, v1x = tv[3] - cx, v2x = tv[6] - cx;
+    float v0y = tv[1] - cy, v1y = tv[4] - cy, v2y = tv[7] - cy;
+    float e0x = v1x - v0x, e1x = v2x - v1x, e2x = v0x - v2x;
+    float e0y = v1y - v0y, e1y = v2y - v1y, e2y = v0y - v2y;
+    if (separated(e0y * v1x - e0x * v1y, e0y * v2x - e0x * v2y, fabsf(e0y) * h[0] + fabsf(e0x) * h[1])) return 0;
+    if (separated(e1y * v0x - e1x * v0y, e1y * v1x - e1x * v1y, fabsf(e1y) * h[0] + fabsf(e1x) * h[1])) return 0;
+    if (separated(e2y * v1x - e2x * v1y, e2y * v2x - e2x * v2y, fabsf(e2y) * h[0] + fabsf(e2x) * h[1])) return 0;
+    return 1;
+}
+
+/* the plane predicate alone: n = e0 x e1, planeBoxOverlap(n, v0, h) */
+static int plane_ok(const float *tv, const float c[3], const float h[3])
+{
+    float v0[3], v1[3], v2[3], e0[3], e1[3], n[3];
+    for (int q = 0; q < 3; q++) {
+        v0[q] = tv[q] - c[q];
+        v1[q] = tv[3 + q] - c[q];
+        v2[q] = tv[6 + q] - c[q];
+    }
+    for (int q = 0; q < 3; q++) {
+        e0[q] = v1[q] - v0[q];
+        e1[q] = v2[q] - v1[q];
+    }
+    n[0] = e0[1] * e1[2] - e0[2] * e1[1];
+    n[1] = e0[2] * e1[0] - e0[0] * e1[2];
+    n[2] = e0[0] * e1[1] - e0[1] * e1[0];
+    return plane_box_overlap(n, v0, h);
+}
+
+/* The factored form on one case (for the tests: it must equal
+ * orc_tri_box_overlap everywhere). */
+int orc_tri_box_factored(const float c[3], const float h[3], const float tv[9])
+{
+    for (int q = 0; q < 3; q++)
+        if (!aabb_axis_ok(tv, q, c[q], h[q])) return 0;
+    if (!edges_z_ok(tv, c[0], c[1], h) || !edges_x_ok(tv, c[1], c[2], h) || !edges_y_ok(tv, c[0], c[2], h))
+        return 0;
+    return plane_ok(tv, c, h);
+}
+
+void orc_tri_box_factored_batch(int64_t n, const float *c, const float *h, const float *tv, uint8_t *res)
+{
+    for (int64_t i = 0; i < n; i++) res[i] = (uint8_t)orc_tri_box_factored(c + 3 * i, h + 3 * i, tv + 9 * i);
+}
+
+/* Per-thread tables of the sweep. */
+typedef struct { uint8_t *a; size_t cap; } bytes_t;
+
+static uint8_t *bytes_get(bytes_t *b, size_t n)
+{
+    if (n > b->cap) {
+        b->a = (uint8_t *)realloc(b->a, n);
+        if (!b->a) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+        b->cap = n;
+    }
+    return b->a;
+}
+
+/* Every (p, q, s) of the nx x ny x nz center grid cx[p], cy[q], cz[s] whose
+ * box overlaps triangle tv: the literal triple loop over
+ * orc_tri_box_overlap, screened by the factored predicates.  The plane
+ * tables are built only over the indices whose AABB test passes.
+ * emit(ud, p, q, s) is called for each overlap. */
+static void sat_sweep(const float *tv, const float h[3], const float *cx, int nx, const float *cy, int ny,
+                      const float *cz, int nz, bytes_t *tmp, void (*emit)(void *, int, int, int), void *ud)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0) return;
+    int *ix = (int *)bytes_get(tmp, sizeof(int) * ((size_t)nx + ny + nz));
+    int *iy = ix + nx, *iz = iy + ny;
+    int mx = 0, my = 0, mz = 0;
+    for (int p = 0; p < nx; p++)
+        if (aabb_axis_ok(tv, 0, cx[p], h[0])) ix[mx++] = p;
+    if (!mx) return;
+    for (int q = 0; q < ny; q++)
+        if (aabb_axis_ok(tv, 1, cy[q], h[1])) iy[my++] = q;
+    if (!my) return;
+    for (int k = 0; k < nz; k++)
+        if (aabb_axis_ok(tv, 2, cz[k], h[2])) iz[mz++] = k;
+    if (!mz) return;
+    size_t need = sizeof(int) * ((size_t)nx + ny + nz) + (size_t)mx * my + (size_t)my * mz + (size_t)mx * mz;
+    ix = (int *)bytes_get(tmp, need);   /* may move: re-derive the views */
+    iy = ix + nx;
+    iz = iy + ny;
+    uint8_t *txy = (uint8_t *)(iz + nz), *tyz = txy + (size_t)mx * my, *txz = tyz + (size_t)my * mz;
+    int any = 0;
+    for (int a = 0; a < mx; a++)
+        for (int b = 0; b < my; b++)
+            any |= (txy[(size_t)a * my + b] = (uint8_t)edges_z_ok(tv, cx[ix[a]], cy[iy[b]], h));
+    if (!any) return;
+    any = 0;
+    for (int b = 0; b < my; b++)
+        for (int k = 0; k < mz; k++)
+            any |= (tyz[(size_t)b * mz + k] = (uint8_t)edges_x_ok(tv, cy[iy[b]], cz[iz[k]], h));
+    if (!any) return;
+    for (int a = 0; a < mx; a++)
+        for (int k = 0; k < mz; k++)
+            txz[(size_t)a * mz + k] = (uint8_t)edges_y_ok(tv, cx[ix[a]], cz[iz[k]], h);
+    for (int a = 0; a < mx; a++)
+        for (int b = 0; b < my; b++) {
+            if (!txy[(size_t)a * my + b]) continue;
+            for (int k = 0; k < mz; k++) {
+                if (!tyz[(size_t)b * mz + k] || !txz[(size_t)a * mz + k]) continue;
+                float ctr[3] = {cx[ix[a]], cy[iy[b]], cz[iz[k]]};
+                if (plane_ok(tv, ctr, h)) emit(ud, ix[a], iy[b], iz[k]);
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Moller-Trumbore: src/TriRayIntersection.cpp:78-131 (EPSILON :38 is the   */
 /* double 1e-6; det and t are promoted to double for the compares)          */
 /* ------------------------------------------------------------------------ */
@@ -181,17 +344,29 @@ typedef struct {
 
 static const float RAY_D[3] = {0.0f, 0.0f, 1.0f};
 
-static int mt_xy(const float *tv, float ox, float oy, mt_col *s)
+/* the column-independent half: edges, P = D x e2, det and its reciprocal;
+ * returns 0 when the det test rejects every ray */
+typedef struct { float v1[3], e1[3], e2[3], P[3], inv_det; } mt_tri;
+
+static int mt_setup(const float *tv, mt_tri *m)
 {
-    float e1[3], e2[3], P[3];
-    for (int q = 0; q < 3; q++) { e1[q] = tv[3 + q] - tv[q]; e2[q] = tv[6 + q] - tv[q]; }
+    float *e1 = m->e1, *e2 = m->e2, *P = m->P;
+    for (int q = 0; q < 3; q++) { m->v1[q] = tv[q]; e1[q] = tv[3 + q] - tv[q]; e2[q] = tv[6 + q] - tv[q]; }
     P[0] = RAY_D[1] * e2[2] - RAY_D[2] * e2[1];
     P[1] = RAY_D[2] * e2[0] - RAY_D[0] * e2[2];
     P[2] = RAY_D[0] * e2[1] - RAY_D[1] * e2[0];
     float det = e1[0] * P[0] + e1[1] * P[1] + e1[2] * P[2];
     if ((double)det > -MT_EPS && (double)det < MT_EPS) return 0;
-    float inv_det = 1.f / det;
-    float tx = ox - tv[0], ty = oy - tv[1];
+    m->inv_det = 1.f / det;
+    return 1;
+}
+
+/* the u/v half for the column (ox, oy) */
+static int mt_xy_col(const mt_tri *m, float ox, float oy, mt_col *s)
+{
+    const float *e1 = m->e1, *P = m->P;
+    float inv_det = m->inv_det;
+    float tx = ox - m->v1[0], ty = oy - m->v1[1];
     /* T.z * P.z is an exact signed zero: it cannot change a nonzero sum, and
      * a zero sum gives u = +-0 which passes/fails every compare identically. */
     float u = (tx * P[0] + ty * P[1] + 0.0f * P[2]) * inv_det;
@@ -199,10 +374,44 @@ static int mt_xy(const float *tv, float ox, float oy, mt_col *s)
     float qz = tx * e1[1] - ty * e1[0];
     float v = (RAY_D[0] * 0.0f + RAY_D[1] * 0.0f + RAY_D[2] * qz) * inv_det;
     if (v < 0.f || u + v > 1.f) return 0;
-    for (int q = 0; q < 3; q++) { s->v1[q] = tv[q]; s->e1[q] = e1[q]; s->e2[q] = e2[q]; }
+    for (int q = 0; q < 3; q++) { s->v1[q] = m->v1[q]; s->e1[q] = e1[q]; s->e2[q] = m->e2[q]; }
     s->inv_det = inv_det;
     s->tx = tx;
     s->ty = ty;
+    return 1;
+}
+
+static int mt_xy(const float *tv, float ox, float oy, mt_col *s)
+{
+    mt_tri m;
+    return mt_setup(tv, &m) && mt_xy_col(&m, ox, oy, s);
+}
+
+static int sgnf(float x) { return (x > 0.0f) - (x < 0.0f); }
+
+/* Whether mt_t(s, oz) is monotone in oz over [oz_lo, oz_hi], and which way
+ * (*dir <= 0: the test passes on a prefix of increasing oz, > 0: on a
+ * suffix).  Rounding is monotone, so each float operation of mt_t is
+ * monotone in a monotone operand: tz = oz - v1.z rises with oz; qx falls
+ * with tz when e1.y > 0 (rises when < 0), qy rises with tz when e1.x > 0;
+ * e2.x*qx and e2.y*qy are then monotone with directions
+ * -sgn(e2.x)sgn(e1.y) and sgn(e2.y)sgn(e1.x), the left-to-right sum is
+ * monotone when those two directions do not oppose (e2.z*qz is constant),
+ * and the product by inv_det flips it when det < 0.  The magnitude guard
+ * keeps every intermediate finite (no overflow, no NaN) over the range. */
+static int mt_t_monotone(const mt_col *s, float oz_lo, float oz_hi, int *dir)
+{
+    const float lim = 1e15f;
+    float tz0 = oz_lo - s->v1[2], tz1 = oz_hi - s->v1[2];
+    if (!(fabsf(tz0) <= lim && fabsf(tz1) <= lim && fabsf(s->tx) <= lim && fabsf(s->ty) <= lim &&
+          fabsf(s->inv_det) <= 1e7f))
+        return 0;
+    for (int q = 0; q < 3; q++)
+        if (!(fabsf(s->e1[q]) <= lim && fabsf(s->e2[q]) <= lim)) return 0;
+    int da = -sgnf(s->e2[0]) * sgnf(s->e1[1]);
+    int db = sgnf(s->e2[1]) * sgnf(s->e1[0]);
+    if (da * db < 0) return 0;
+    *dir = (da ? da : db) * sgnf(s->inv_det);
     return 1;
 }
 
@@ -468,6 +677,7 @@ typedef struct {
     int64_t nx, ny, nz, n1;
     float *cx, *cy, *cz;     /* L1 SAT centers */
     float *rx, *ry, *rz;     /* L1 ray origins */
+    int mode;                /* ORC_EXACT: literal loops; ORC_FAST: the result-identical forms */
 } ctx_t;
 
 /* Level1 ray parity over ALL triangles (src/Object.cpp:716-779), literal. */
@@ -561,6 +771,16 @@ static void l1_parity_fast(const ctx_t *c, uint8_t *par)
     free(lst);
 }
 
+/* sat_sweep callback of the Level1 sweep: one bin entry per overlap */
+typedef struct { vec64 *out; int p0, q0, s0; int64_t nx, ny, t; } l1_emit_t;
+
+static void l1_emit(void *ud, int p, int q, int s)
+{
+    l1_emit_t *e = (l1_emit_t *)ud;
+    int64_t v = (int64_t)(e->s0 + s) * e->ny * e->nx + (int64_t)(e->q0 + q) * e->nx + (e->p0 + p);
+    v64_push(e->out, (v << 32) | e->t);
+}
+
 /* Level1 SAT sweep + bins (src/Object.cpp:2256-2342 with AddPointToVoxel
  * :2705-2748).  Bins list triangle ids in ascending order, as the
  * reference's triangle-outer loop appends them. */
@@ -576,10 +796,18 @@ static void l1_sat(const ctx_t *c, orc_result *r)
         int tid = omp_get_thread_num();
         int nth = omp_get_num_threads();
         int64_t t0 = c->T * tid / nth, t1 = c->T * (tid + 1) / nth;
+        bytes_t tmp = {0};
+        l1_emit_t em = {&loc[tid], 0, 0, 0, nx, ny, 0};
         for (int64_t t = t0; t < t1; t++) {
             const float *tv = c->tris + 9 * t;
             int lo[3], hi[3];
             tests += tri_index_box(tv, c->g, lo, hi);
+            if (c->mode == ORC_FAST) {
+                em.p0 = lo[0]; em.q0 = lo[1]; em.s0 = lo[2]; em.t = t;
+                sat_sweep(tv, h, c->cx + lo[0], hi[0] - lo[0] + 1, c->cy + lo[1], hi[1] - lo[1] + 1,
+                          c->cz + lo[2], hi[2] - lo[2] + 1, &tmp, l1_emit, &em);
+                continue;
+            }
             for (int p = lo[0]; p <= hi[0]; p++)
                 for (int q = lo[1]; q <= hi[1]; q++)
                     for (int s = lo[2]; s <= hi[2]; s++) {
@@ -588,6 +816,7 @@ static void l1_sat(const ctx_t *c, orc_result *r)
                             v64_push(&loc[tid], ((int64_t)(s * ny * nx + q * nx + p) << 32) | t);
                     }
         }
+        free(tmp.a);
     }
     r->tests_l1 = tests;
     int64_t n1 = c->n1;
@@ -690,6 +919,10 @@ static void l2_parity_fast(const ctx_t *c, const orc_result *r, const float *ox,
         if (q == 0 || r->bidx[order[q]] % (nx * ny) != r->bidx[order[q - 1]] % (nx * ny))
             starts[ng++] = q;
     starts[ng] = nb;
+    /* ray origins along z non-decreasing: then a column's layers, boundary
+     * voxels by ascending z and their sub-layers, have non-decreasing oz */
+    int zmono = 1;
+    for (int64_t k = 1; k < nz * n2; k++) zmono &= oz[k - 1] <= oz[k];
 #pragma omp parallel num_threads(c->nt)
     {
         int32_t *zmax = (int32_t *)xcalloc((size_t)c->T, sizeof(int32_t));
@@ -697,11 +930,28 @@ static void l2_parity_fast(const ctx_t *c, const orc_result *r, const float *ox,
         vec64 uniq = {0};
         mt_col *hs = NULL;
         int32_t *hz = NULL;
+        mt_tri *ut = NULL;      /* set-up of the column's candidates that pass the det test */
+        int32_t *uz = NULL;
         int64_t hcap = 0;
+        float *ozl = NULL;      /* the group's layers: l = (m - m0)*n2 + s */
+        int64_t *kk = NULL;     /* k1 of the group's boundary voxels, ascending */
+        uint8_t *flip = NULL;   /* parity toggles at layer boundaries */
+        int64_t lcap = 0;
 #pragma omp for schedule(dynamic, 4)
         for (int64_t gi = 0; gi < ng; gi++) {
             int64_t ij = r->bidx[order[starts[gi]]] % (nx * ny);
             int64_t i1 = ij % nx, j1 = ij / nx;
+            int64_t m0 = starts[gi], nbg = starts[gi + 1] - m0, L = nbg * n2;
+            if (L + 1 > lcap) {
+                lcap = L + 1;
+                ozl = (float *)realloc(ozl, (size_t)lcap * sizeof(float));
+                kk = (int64_t *)realloc(kk, (size_t)lcap * sizeof(int64_t));
+                flip = (uint8_t *)realloc(flip, (size_t)lcap);
+            }
+            for (int64_t m = 0; m < nbg; m++) {
+                kk[m] = r->bidx[order[m0 + m]] / (nx * ny);
+                for (int s = 0; s < n2; s++) ozl[m * n2 + s] = oz[kk[m] * n2 + s];
+            }
             for (int64_t z = 0; z < nz; z++) {
                 int64_t v = z * nx * ny + ij;
                 for (int64_t e = r->list_off[v]; e < r->list_off[v + 1]; e++) {
@@ -714,22 +964,65 @@ static void l2_parity_fast(const ctx_t *c, const orc_result *r, const float *ox,
                 hcap = uniq.n;
                 hs = (mt_col *)realloc(hs, (size_t)hcap * sizeof(mt_col));
                 hz = (int32_t *)realloc(hz, (size_t)hcap * sizeof(int32_t));
+                ut = (mt_tri *)realloc(ut, (size_t)hcap * sizeof(mt_tri));
+                uz = (int32_t *)realloc(uz, (size_t)hcap * sizeof(int32_t));
+            }
+            int64_t nu = 0;
+            for (int64_t e = 0; e < uniq.n; e++) {
+                int32_t t = (int32_t)uniq.a[e];
+                if (mt_setup(c->tris + 9 * (int64_t)t, &ut[nu])) uz[nu++] = zmax[t];
             }
             for (int q = 0; q < n2; q++)
                 for (int p = 0; p < n2; p++) {
                     int64_t nh = 0;
-                    for (int64_t e = 0; e < uniq.n; e++) {
-                        int32_t t = (int32_t)uniq.a[e];
-                        if (mt_xy(c->tris + 9 * (int64_t)t, ox[i1 * n2 + p], oy[j1 * n2 + q], &hs[nh]))
-                            hz[nh++] = zmax[t];
+                    for (int64_t e = 0; e < nu; e++)
+                        if (mt_xy_col(&ut[e], ox[i1 * n2 + p], oy[j1 * n2 + q], &hs[nh])) hz[nh++] = uz[e];
+                    /* parity at layer l = XOR over candidates h (hz[h] >= k1 of
+                     * l's voxel) of mt_t(h, ozl[l]); each candidate's passing
+                     * layers are one run when mt_t is monotone in oz (a
+                     * binary search finds it), else they are tested one by
+                     * one; runs are toggled at their ends and summed by a
+                     * prefix XOR */
+                    memset(flip, 0, (size_t)L + 1);
+                    for (int64_t h = 0; h < nh; h++) {
+                        int64_t lend = L;
+                        if (!c->g->cuda_l2) {   /* voxels with k1 <= hz[h]: a prefix */
+                            int64_t a = 0, bb = nbg;
+                            while (a < bb) {
+                                int64_t mid = (a + bb) / 2;
+                                if (kk[mid] <= hz[h]) a = mid + 1; else bb = mid;
+                            }
+                            lend = a * n2;
+                        }
+                        if (lend == 0) continue;
+                        int dir = 0;
+                        if (zmono && mt_t_monotone(&hs[h], ozl[0], ozl[lend - 1], &dir)) {
+                            int64_t a = 0, bb = lend;
+                            if (dir <= 0) {   /* passes on [0, first failure) */
+                                while (a < bb) {
+                                    int64_t mid = (a + bb) / 2;
+                                    if (mt_t(&hs[h], ozl[mid])) a = mid + 1; else bb = mid;
+                                }
+                                flip[0] ^= 1;
+                                flip[a] ^= 1;
+                            } else {          /* passes on [first pass, lend) */
+                                while (a < bb) {
+                                    int64_t mid = (a + bb) / 2;
+                                    if (mt_t(&hs[h], ozl[mid])) bb = mid; else a = mid + 1;
+                                }
+                                flip[a] ^= 1;
+                                flip[lend] ^= 1;
+                            }
+                        } else {
+                            for (int64_t l = 0; l < lend; l++)
+                                if (mt_t(&hs[h], ozl[l])) { flip[l] ^= 1; flip[l + 1] ^= 1; }
+                        }
                     }
-                    for (int64_t m = starts[gi]; m < starts[gi + 1]; m++) {
-                        int64_t b = order[m];
-                        int64_t k1 = r->bidx[b] / (nx * ny);
+                    int pp = 0;
+                    for (int64_t m = 0; m < nbg; m++) {
+                        int64_t b = order[m0 + m];
                         for (int s = 0; s < n2; s++) {
-                            int pp = 0;
-                            for (int64_t h = 0; h < nh; h++)
-                                if (c->g->cuda_l2 || hz[h] >= k1) pp ^= mt_t(&hs[h], oz[k1 * n2 + s]);
+                            pp ^= flip[m * n2 + s];
                             par[b * n23 + ((int64_t)s * n2 + q) * n2 + p] = (uint8_t)pp;
                         }
                     }
@@ -741,35 +1034,59 @@ static void l2_parity_fast(const ctx_t *c, const orc_result *r, const float *ox,
         free(uniq.a);
         free(hs);
         free(hz);
+        free(ut);
+        free(uz);
+        free(ozl);
+        free(kk);
+        free(flip);
     }
     free(order);
     free(starts);
 }
 
+/* sat_sweep callback of the Level2 sweep: state 2 for the sub-voxel */
+typedef struct { uint8_t *blk; int n2; } l2_emit_t;
+
+static void l2_emit(void *ud, int p, int q, int s)
+{
+    l2_emit_t *e = (l2_emit_t *)ud;
+    e->blk[((int64_t)s * e->n2 + q) * e->n2 + p] = 2;
+}
+
 /* Level2 SAT (src/Object.cpp:2344-2424): every triangle of the parent's bin
- * against every Level2 voxel; a hit sets 2. */
+ * against every Level2 voxel; a hit sets state 2 in l2 (over the parity). */
 static int64_t l2_sat(const ctx_t *c, const orc_result *r, const float *sx, const float *sy,
-                      const float *sz, uint8_t *sat)
+                      const float *sz, uint8_t *l2)
 {
     int n2 = c->g->n2;
     int64_t n23 = (int64_t)n2 * n2 * n2, nx = c->nx, ny = c->ny;
     int64_t tests = 0;
     const float *h = c->g->ext2;
-#pragma omp parallel for schedule(dynamic, 16) num_threads(c->nt) reduction(+ : tests)
-    for (int64_t b = 0; b < r->nb; b++) {
-        int64_t l1 = r->bidx[b];
-        int64_t k1 = l1 / (nx * ny), ij = l1 - k1 * nx * ny, j1 = ij / nx, i1 = ij % nx;
-        tests += (r->list_off[l1 + 1] - r->list_off[l1]) * n23;
-        for (int64_t e = r->list_off[l1]; e < r->list_off[l1 + 1]; e++) {
-            const float *tv = c->tris + 9 * (int64_t)r->list_tri[e];
-            for (int p = 0; p < n2; p++)
-                for (int q = 0; q < n2; q++)
-                    for (int s = 0; s < n2; s++) {
-                        float ctr[3] = {sx[i1 * n2 + p], sy[j1 * n2 + q], sz[k1 * n2 + s]};
-                        if (orc_tri_box_overlap(ctr, h, tv))
-                            sat[b * n23 + ((int64_t)s * n2 + q) * n2 + p] = 1;
-                    }
+#pragma omp parallel num_threads(c->nt) reduction(+ : tests)
+    {
+        bytes_t tmp = {0};
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t b = 0; b < r->nb; b++) {
+            int64_t l1 = r->bidx[b];
+            int64_t k1 = l1 / (nx * ny), ij = l1 - k1 * nx * ny, j1 = ij / nx, i1 = ij % nx;
+            tests += (r->list_off[l1 + 1] - r->list_off[l1]) * n23;
+            l2_emit_t em = {l2 + b * n23, n2};
+            for (int64_t e = r->list_off[l1]; e < r->list_off[l1 + 1]; e++) {
+                const float *tv = c->tris + 9 * (int64_t)r->list_tri[e];
+                if (c->mode == ORC_FAST) {
+                    sat_sweep(tv, h, sx + i1 * n2, n2, sy + j1 * n2, n2, sz + k1 * n2, n2, &tmp, l2_emit, &em);
+                    continue;
+                }
+                for (int p = 0; p < n2; p++)
+                    for (int q = 0; q < n2; q++)
+                        for (int s = 0; s < n2; s++) {
+                            float ctr[3] = {sx[i1 * n2 + p], sy[j1 * n2 + q], sz[k1 * n2 + s]};
+                            if (orc_tri_box_overlap(ctr, h, tv))
+                                l2[b * n23 + ((int64_t)s * n2 + q) * n2 + p] = 2;
+                        }
+            }
         }
+        free(tmp.a);
     }
     return tests;
 }
@@ -783,6 +1100,7 @@ int orc_voxelize(const float *tris, int64_t T, const orc_grid *g, int mode, int 
     c.T = T;
     c.g = g;
     c.nt = threads > 0 ? threads : omp_get_max_threads();
+    c.mode = mode;
     c.nx = g->n[0]; c.ny = g->n[1]; c.nz = g->n[2];
     c.n1 = c.nx * c.ny * c.nz;
     float *tab[6];
@@ -834,22 +1152,24 @@ int orc_voxelize(const float *tris, int64_t T, const orc_grid *g, int mode, int 
             orc_l2_centers(g, q, l2t[q]);
             orc_l2_ray_origins(g, q, l2t[3 + q]);
         }
-        uint8_t *p2 = (uint8_t *)xcalloc((size_t)(r->nb * n23), 1);
-        if (mode == ORC_EXACT) l2_parity_exact(&c, r, l2t[3], l2t[4], l2t[5], p2);
-        else l2_parity_fast(&c, r, l2t[3], l2t[4], l2t[5], p2);
-        t4 = omp_get_wtime();
-        uint8_t *s2 = (uint8_t *)xcalloc((size_t)(r->nb * n23), 1);
-        r->tests_l2 = l2_sat(&c, r, l2t[0], l2t[1], l2t[2], s2);
-        t5 = omp_get_wtime();
+        /* the parity writes 0/1 into every Level2 voxel, then the SAT
+         * overwrites its hits with 2: state = SAT hit ? 2 : parity */
         r->l2 = (uint8_t *)xcalloc((size_t)(r->nb * n23), 1);
-        for (int64_t q = 0; q < r->nb * n23; q++) {
-            uint8_t s = s2[q] ? 2 : p2[q];
-            r->l2[q] = s;
-            if (s == 1) r->l2_inside++;     /* int(v) % 2 == 1, src/Object.cpp:3366 */
-            if (s == 2) r->l2_boundary++;
+        if (mode == ORC_EXACT) l2_parity_exact(&c, r, l2t[3], l2t[4], l2t[5], r->l2);
+        else l2_parity_fast(&c, r, l2t[3], l2t[4], l2t[5], r->l2);
+        t4 = omp_get_wtime();
+        r->tests_l2 = l2_sat(&c, r, l2t[0], l2t[1], l2t[2], r->l2);
+        t5 = omp_get_wtime();
+        int64_t ins2 = 0, bnd2 = 0;
+        const int64_t nl2 = r->nb * n23;
+        const uint8_t *l2s = r->l2;
+#pragma omp parallel for schedule(static) num_threads(c.nt) reduction(+ : ins2, bnd2)
+        for (int64_t q = 0; q < nl2; q++) {
+            ins2 += l2s[q] == 1;     /* int(v) % 2 == 1, src/Object.cpp:3366 */
+            bnd2 += l2s[q] == 2;
         }
-        free(p2);
-        free(s2);
+        r->l2_inside = ins2;
+        r->l2_boundary = bnd2;
         for (int q = 0; q < 6; q++) free(l2t[q]);
     }
     r->t_stage[0] = t1 - t0;

@@ -74,6 +74,10 @@ int orc_tri_ray(const float v1[3], const float v2[3], const float v3[3],
 
 /* Batched forms for tests: n independent cases. */
 void orc_tri_box_batch(int64_t n, const float *c, const float *h, const float *tv, uint8_t *res);
+/* The SAT screened by its per-axis and per-plane predicate groups before the
+ * full test (the form ORC_FAST sweeps with); must equal orc_tri_box_overlap. */
+int orc_tri_box_factored(const float center[3], const float half[3], const float tv[9]);
+void orc_tri_box_factored_batch(int64_t n, const float *c, const float *h, const float *tv, uint8_t *res);
 void orc_tri_ray_batch(int64_t n, const float *tv, const float *o, const float *d,
                        uint8_t *hit, float *t);
 /* Column-factorised +z ray test (det/u/v on O.x,O.y; t on O.z), batched. */
@@ -103,8 +107,11 @@ int orc_vertex_index(float v, float mn, float mx, int n);
 int orc_ray_column_bounds(const float tv[9], const float bmin[3], const float bmax[3],
                           double safety, double lo[2], double hi[2]);
 
-/* Full pipeline.  mode ORC_EXACT runs the literal all-triangle parity loops;
- * ORC_FAST runs the result-identical column factorisation. threads<=0: all. */
+/* Full pipeline.  mode ORC_EXACT runs the literal loops (all-triangle
+ * parity, every box of every index AABB and bin through the full SAT);
+ * ORC_FAST runs the result-identical forms: the column-factorised parity and
+ * the SAT sweeps screened by orc_tri_box_factored's predicate groups.
+ * threads<=0: all. */
 int orc_voxelize(const float *tris, int64_t ntris, const orc_grid *g, int mode,
                  int threads, orc_result *out);
 void orc_result_free(orc_result *r);

@@ -73,6 +73,7 @@ def lib():
             build()
         L = ctypes.CDLL(path)
         L.orc_tri_box_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _f32p, _u8p]
+        L.orc_tri_box_factored_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _f32p, _u8p]
         L.orc_tri_ray_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _f32p, _u8p, _f32p]
         L.orc_tri_zray_split_batch.argtypes = [ctypes.c_int64, _f32p, _f32p, _u8p]
         L.orc_bbox.argtypes = [_f32p, ctypes.c_int64, _f32p, _f32p, ctypes.POINTER(ctypes.c_float)]
@@ -123,10 +124,11 @@ def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
-def tri_box(c, h, tv, use_ref=False):
+def tri_box(c, h, tv, use_ref=False, factored=False):
     c, h, tv = _f32(c).reshape(-1, 3), _f32(h).reshape(-1, 3), _f32(tv).reshape(-1, 9)
     out = np.zeros(len(c), np.uint8)
-    fn = ref().ref_tri_box_batch if use_ref else lib().orc_tri_box_batch
+    fn = (ref().ref_tri_box_batch if use_ref else
+          lib().orc_tri_box_factored_batch if factored else lib().orc_tri_box_batch)
     fn(len(c), c.ravel(), h.ravel(), tv.ravel(), out)
     return out
 
@@ -224,9 +226,11 @@ def quantize_normals(v):
     return out
 
 
-def voxelize(tris, g, mode=ORC_FAST, threads=0, normals=False):
+def voxelize(tris, g, mode=ORC_FAST, threads=0, normals=False, bins=True):
     """Run the oracle pipeline; returns a dict of numpy arrays and counters
-    (with normals=True also l1n [nb,3] and l2n [nb*n2^3,3])."""
+    (with normals=True also l1n [nb,3] and l2n [nb*n2^3,3]; with
+    bins=False the Level1 bin lists list_off/list_tri are not copied out,
+    which saves 8 bytes per Level1 voxel on large grids)."""
     tris = _f32(tris).reshape(-1)
     T = len(tris) // 9
     r = _Result()
@@ -239,9 +243,9 @@ def voxelize(tris, g, mode=ORC_FAST, threads=0, normals=False):
             "bidx": np.ctypeslib.as_array(r.bidx, (r.nb,)).copy() if r.nb else np.zeros(0, np.int32),
             "l2": (np.ctypeslib.as_array(r.l2, (r.nb * n23,)).copy()
                    if (r.nb and n23) else np.zeros(0, np.uint8)),
-            "list_off": np.ctypeslib.as_array(r.list_off, (r.n1 + 1,)).copy(),
-            "list_tri": (np.ctypeslib.as_array(r.list_tri, (r.pairs,)).copy()
-                         if r.pairs else np.zeros(0, np.int32)),
+            "list_off": np.ctypeslib.as_array(r.list_off, (r.n1 + 1,)).copy() if bins else None,
+            "list_tri": ((np.ctypeslib.as_array(r.list_tri, (r.pairs,)).copy()
+                          if r.pairs else np.zeros(0, np.int32)) if bins else None),
             "nb": r.nb, "pairs": r.pairs, "tests_l1": r.tests_l1, "tests_l2": r.tests_l2,
             "l1_inside": r.l1_inside, "l1_boundary": r.l1_boundary,
             "l2_inside": r.l2_inside, "l2_boundary": r.l2_boundary,
