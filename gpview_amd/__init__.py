@@ -108,7 +108,8 @@ EXPORTED = [
     "gpv_ctx_create", "gpv_ctx_destroy", "gpv_last_error", "gpv_status_string",
     "gpv_device_upload", "gpv_device_free", "gpv_ctx_pool_stats", "gpv_ctx_get_tune", "gpv_ctx_set_tune",
     "gpv_ctx_autotune",
-    "gpv_mesh_bbox", "gpv_grid_derive", "gpv_voxelize", "gpv_voxelize_noalloc", "gpv_voxelize_batch",
+    "gpv_mesh_bbox", "gpv_grid_derive", "gpv_grid_centers", "gpv_voxelize", "gpv_voxelize_noalloc",
+    "gpv_voxelize_batch",
     "gpv_grid_create",
     "gpv_classify_tessellation", "gpv_classify_inout", "gpv_compact_boundary",
     "gpv_classify_inout_level2", "gpv_classify_tessellation_level2", "gpv_grid_finish",
@@ -147,6 +148,7 @@ def lib():
     L.gpv_server_pool_stats.argtypes = [_vp, P(PoolStats)]
     L.gpv_mesh_bbox.argtypes = [_f32p, c_i64, _f32p, _f32p, P(c_float)]
     L.gpv_grid_derive.argtypes = [_f32p, _f32p, c_float, P(Params), P(GridInfo)]
+    L.gpv_grid_centers.argtypes = [P(GridInfo), c_int, c_int, c_int, _f32p]
     for fn in ("gpv_voxelize", "gpv_grid_create"):
         getattr(L, fn).argtypes = [_vp, _vp, c_i64, c_int, _f32p, _f32p, c_float, P(Params), P(_vp)]
     L.gpv_voxelize_noalloc.argtypes = [_vp, _vp, c_i64, _f32p, _f32p, c_float, P(Params), _vp, _vp, c_i64, _vp,
@@ -291,6 +293,17 @@ def grid_derive(bmin, bmax, mms, voxel_count, voxel_count2, level2=True):
     _check(lib().gpv_grid_derive(np.asarray(bmin, np.float32), np.asarray(bmax, np.float32), float(mms),
                                  ctypes.byref(p), ctypes.byref(info)), "gpv_grid_derive")
     return info
+
+
+def grid_centers(info, which, axis, cuda_level2=False):
+    """gpv_grid_centers: table `which` (0 Level1 SAT centers, 1 Level1 ray
+    origins, 2 Level2 SAT centers, 3 Level2 ray origins) along `axis` for a
+    GridInfo from grid_derive."""
+    n = info.n[axis] * (info.n2 if which >= 2 else 1) if 0 <= axis < 3 else 0
+    out = np.zeros(max(n, 1), np.float32)
+    _check(lib().gpv_grid_centers(ctypes.byref(info), 1 if cuda_level2 else 0, int(which), int(axis), out),
+           "gpv_grid_centers")
+    return out[:n]
 
 
 # ---------------------------------------------------------------------------

@@ -5,7 +5,7 @@
 Flags: -ffp-contract=off and no fast-math on host and device (occupancy is an
 exact quantity; no multiply/add pair may be fused), packed-fp32 instructions
 switched off, correctly rounded f32 division (the HIP default, checked by
-tests/test_codegen.py on the disassembly).
+tests/test_abi.py::test_codegen_exactness on the disassembly).
 """
 import os
 import shutil

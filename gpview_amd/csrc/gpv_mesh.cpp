@@ -442,8 +442,12 @@ int gpv_mesh_mixed(int kind, int64_t target, uint64_t seed, gpv_mesh* out)
     if (!out || target < 20) return GPV_ERR_INVALID;
     Rng R(seed ^ 0x5bd1e995ull);
     if (kind == 0) {
+        // nearest subdivision level in log scale (20 * 4^level triangles);
+        // levels 7 and 8 (327,680 and 1,310,720 triangles) only when they do
+        // not overshoot the target by more than half
         int level = (int)std::lround(std::log((double)target / 20.0) / std::log(4.0));
-        level = std::max(2, std::min(level, 6));
+        level = std::max(2, std::min(level, 8));
+        if (level >= 7 && 20.0 * std::pow(4.0, level) > 1.5 * (double)target) level--;
         const double r = R.uni(0.25, 0.45);
         float c[3];
         for (int q = 0; q < 3; q++) c[q] = (float)(0.5 + R.uni(-1, 1) * (0.48 - r));

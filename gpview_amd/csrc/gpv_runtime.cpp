@@ -1585,6 +1585,21 @@ int gpv_grid_derive(const float bmin[3], const float bmax[3], float mms, const g
     return derive_grid(bmin, bmax, mms, params, info);
 }
 
+int gpv_grid_centers(const gpv_grid_info* info, int cuda_level2, int which, int axis, float* out)
+{
+    if (!info || !out || which < 0 || which > 3 || axis < 0 || axis > 2)
+        return fail(GPV_ERR_INVALID, "gpv_grid_centers: bad argument");
+    if (which >= 2 && info->n2 <= 0) return fail(GPV_ERR_INVALID, "gpv_grid_centers: Level2 is off");
+    if (info->n[0] < 1 || info->n[1] < 1 || info->n[2] < 1 || info->n2 > 16)
+        return fail(GPV_ERR_INVALID, "gpv_grid_centers: not a derived grid");
+    std::vector<float> tab;
+    size_t off[12];
+    build_tables(*info, cuda_level2 != 0, tab, off);
+    const size_t n = (size_t)info->n[axis] * (which >= 2 ? (size_t)info->n2 : 1);
+    memcpy(out, tab.data() + off[3 * which + axis], n * sizeof(float));
+    return GPV_OK;
+}
+
 int gpv_grid_create(gpv_ctx* c, const float* tris, int64_t T, int on_device, const float bmin[3],
                     const float bmax[3], float mms, const gpv_params* params, gpv_grid** out)
 {

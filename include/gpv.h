@@ -151,6 +151,20 @@ int gpv_mesh_bbox(const float *verts, int64_t nverts, float bmin[3], float bmax[
 int gpv_grid_derive(const float bmin[3], const float bmax[3], float max_model_size,
                     const gpv_params *params, gpv_grid_info *info);
 
+/* The box centers and ray origins the path tests with, along one axis, as
+ * the reference computes them (the float result of its double expressions;
+ * these exact tables are uploaded for every grid):
+ *   which 0  Level1 SAT centers  [n[axis]]     (p + 0.5)*ext*2 + min    src/Object.cpp:2329-2331
+ *         1  Level1 ray origins  [n[axis]]     min + (i + 0.5)*g        src/Object.cpp:743-745
+ *         2  Level2 SAT centers  [n[axis]*n2]  (p + 0.5)*g2 + mid - ext1, mid = (i + 0.5)*g + min
+ *                                                                      src/Object.cpp:2369-2371, 2389-2391
+ *         3  Level2 ray origins  [n[axis]*n2]  minPoint + (p + 0.5)*g2, minPoint = i*g + min in float
+ *                                                                      src/Object.cpp:1134-1136, 1158-1160
+ * Entry i*n2 + p of a Level2 table belongs to Level1 index i.  info comes
+ * from gpv_grid_derive; cuda_level2 selects the CUDA kernels' float Level2
+ * expression as in gpv_params. */
+int gpv_grid_centers(const gpv_grid_info *info, int cuda_level2, int which, int axis, float *out);
+
 /* ---- whole path ----------------------------------------------------------- */
 /* Object::PerformVoxelization (src/Object.cpp:3077-3430) on a flat triangle
  * soup (Object::CreateFlatTriangleData layout, src/Object.cpp:3496-3527:

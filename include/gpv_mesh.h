@@ -59,7 +59,9 @@ int gpv_mesh_torus(int nu, int nv, float ratio, float extent, uint64_t seed, gpv
 int gpv_mesh_drilled_box(int64_t target_tris, float extent, uint64_t seed, gpv_mesh *out);
 
 /* Mixed workload member: kind 0 icosphere, 1 torus, 2 drilled box, sized to
- * about target_tris triangles. */
+ * about target_tris triangles (icospheres come in 20 * 4^level triangles,
+ * levels 2 to 8, the nearest level in log scale that does not overshoot
+ * targets above 163,840 by more than half). */
 int gpv_mesh_mixed(int kind, int64_t target_tris, uint64_t seed, gpv_mesh *out);
 
 /* Exactness stressors applied in place (used for the stress workload):

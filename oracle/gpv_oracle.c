@@ -1399,3 +1399,34 @@ int orc_save_raw(const orc_grid *g, const orc_result *r, int obj_id, const char 
     }
     return 0;
 }
+
+/* The two normal files of SaveVoxelization (src/Object.cpp:3024-3060):
+ * Obj<id>Level1Normal.raw, uint8[n1][3], the boundary voxels' normals (l1n,
+ * bidx order) and 0 elsewhere, and Obj<id>Level2Normal.raw,
+ * uint8[nb*n2^3][3], each component uint8(n*256/3 + 127). */
+int orc_save_raw_normals(const orc_grid *g, const orc_result *r, const float *l1n, const float *l2n, int obj_id,
+                         const char *dir)
+{
+    char path[4096];
+    FILE *f;
+    uint8_t *buf = (uint8_t *)xcalloc((size_t)r->n1 * 3, 1);
+    memset(buf, orc_quantize_normal(0.0f), (size_t)r->n1 * 3);
+    for (int64_t b = 0; b < r->nb; b++)
+        for (int q = 0; q < 3; q++) buf[(int64_t)r->bidx[b] * 3 + q] = orc_quantize_normal(l1n[b * 3 + q]);
+    snprintf(path, sizeof path, "%s/Obj%dLevel1Normal.raw", dir, obj_id);
+    if (!(f = fopen(path, "wb"))) { free(buf); return 1; }
+    fwrite(buf, 1, (size_t)r->n1 * 3, f);
+    fclose(f);
+    free(buf);
+    if (g->n2 > 0) {
+        int64_t n = r->nb * (int64_t)g->n2 * g->n2 * g->n2 * 3;
+        buf = (uint8_t *)xcalloc((size_t)n, 1);
+        for (int64_t v = 0; v < n; v++) buf[v] = orc_quantize_normal(l2n[v]);
+        snprintf(path, sizeof path, "%s/Obj%dLevel2Normal.raw", dir, obj_id);
+        if (!(f = fopen(path, "wb"))) { free(buf); return 1; }
+        fwrite(buf, 1, (size_t)n, f);
+        fclose(f);
+        free(buf);
+    }
+    return 0;
+}

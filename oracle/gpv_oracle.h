@@ -133,6 +133,10 @@ void orc_quantize_normals(int64_t n, const float *v, uint8_t *out);
 /* Writes Obj<obj_id>{VoxelConfig.txt,Level1InOut.raw,Level1BoundaryPrefixSum.raw,
  * Level2InOut.raw} under dir.  Returns 0 on success. */
 int orc_save_raw(const orc_grid *g, const orc_result *r, int obj_id, const char *dir);
+/* Obj<obj_id>Level1Normal.raw and (with Level2) Obj<obj_id>Level2Normal.raw
+ * from the normals of orc_normals.  Returns 0 on success. */
+int orc_save_raw_normals(const orc_grid *g, const orc_result *r, const float *l1n, const float *l2n, int obj_id,
+                         const char *dir);
 
 #ifdef __cplusplus
 }

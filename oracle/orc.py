@@ -98,6 +98,8 @@ def lib():
         L.orc_quantize_normals.argtypes = [ctypes.c_int64, _f32p, _u8p]
         L.orc_save_raw.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(_Result), ctypes.c_int,
                                    ctypes.c_char_p]
+        L.orc_save_raw_normals.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(_Result), _f32p, ctypes.c_void_p,
+                                           ctypes.c_int, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -262,3 +264,28 @@ def voxelize(tris, g, mode=ORC_FAST, threads=0, normals=False, bins=True):
     finally:
         lib().orc_result_free(ctypes.byref(r))
     return out
+
+
+def save_raw(tris, g, obj_id, directory, normals=True, threads=0):
+    """SaveVoxelization's files for the oracle's voxelization of tris on
+    grid g: VoxelConfig.txt, Level1InOut.raw and, with Level2,
+    Level1BoundaryPrefixSum.raw and Level2InOut.raw; with normals also
+    Level1Normal.raw and (with Level2) Level2Normal.raw."""
+    tris = _f32(tris).reshape(-1)
+    T = len(tris) // 9
+    r = _Result()
+    L = lib()
+    L.orc_voxelize(tris, T, ctypes.byref(g), ORC_FAST, int(threads), ctypes.byref(r))
+    try:
+        d = os.fsencode(directory)
+        assert L.orc_save_raw(ctypes.byref(g), ctypes.byref(r), int(obj_id), d) == 0
+        if normals:
+            n23 = g.n2 ** 3 if g.n2 > 0 else 0
+            l1n = np.zeros(max(r.nb, 1) * 3, np.float32)
+            l2n = np.zeros(max(r.nb * n23, 1) * 3, np.float32)
+            L.orc_normals(tris, ctypes.byref(g), ctypes.byref(r), int(threads), l1n,
+                          l2n.ctypes.data if n23 else None)
+            assert L.orc_save_raw_normals(ctypes.byref(g), ctypes.byref(r), l1n, l2n.ctypes.data, int(obj_id),
+                                          d) == 0
+    finally:
+        L.orc_result_free(ctypes.byref(r))

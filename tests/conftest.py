@@ -27,6 +27,9 @@ def _ensure_built():
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
     if not os.path.exists(os.path.join(ROOT, "gpview_amd", "libgpv.so")):
         subprocess.run([sys.executable, "-m", "gpview_amd.build"], check=True, cwd=ROOT)
+    native = os.path.join(ROOT, "tests", "native")
+    if not all(os.path.exists(os.path.join(native, f)) for f in ("libslabcheck.so", "libdevpredicates.so")):
+        subprocess.run(["make", "-s", "-C", native], check=True)
 
 
 _ensure_built()
