@@ -1,21 +1,33 @@
-"""Benchmark of the plain (one mesh per call) voxelizer on one MI355X.
+"""Benchmark of the voxelizer on one MI355X.
 
-Workload: the seeded (3,7) torus-knot tube, 2,048 segments x 256 ring
-vertices (1,048,576 FP32 triangles), scaled to 0.9 of the unit cube;
-Level1 voxelCount 256, Level2 8^3 per boundary voxel.  One step = one
-gpv_voxelize call on the device-resident soup (all stages, counts, sync).
+Headline (value): the plain path, one gpv_voxelize call per step on the
+seeded (3,7) torus-knot tube, 2,048 segments x 256 ring vertices
+(1,048,576 FP32 triangles) scaled to 0.9 of the unit cube, Level1
+voxelCount 256, Level2 8^3 per boundary voxel, the soup resident in HBM.
+One step = all stages, counts and the final synchronisation.
 
-Prints ONE JSON line: meshes/s (value), triangle-box tests/s, voxels/s,
-p50/p99 step latency, per-stage device times, the roofline of the dominant
-kernel (HIP events on the library's stream) and the CPU oracle timed on this
-host on the same mesh.
+Prints ONE JSON line with, beside the headline:
+- tri_box_tests_per_s: the tri-box tests/s metric as defined on the
+  reference's loop counts (every voxel of every triangle's index AABB at
+  Level1, |bin| * n2^3 at Level2); the kernels execute a pruned subset whose
+  outcome is the same, so the figure is reference-equivalent;
+- roofline (HBM) and roofline_valu of the dominant kernel, from HIP events
+  of this run and the committed counter summary of the same kernel;
+- cpu_baseline: the oracle in the CPU baseline's composition (column-
+  factorised parity, literal SAT loops) on the full mesh over all host
+  cores, with the single-thread run and the literal all-triangle
+  composition on the 5,120-triangle icosphere beside it;
+- batched: 256 meshes of the mixed-size stream, one gpv_voxelize per mesh
+  against one gpv_voxelize_batch, with the soups uploaded from pinned host
+  memory inside the timed region (and resident, for comparison);
+- stress: members of the 64-mesh exactness-stress stream at Level1 512 /
+  Level2 16, batched in runs, soups uploaded in the timed region, with a
+  bit-exact check against the oracle on a stated sample;
+- served: 10,000 Poisson requests through the serving loop at a stated rate
+  (0.8 of the measured saturated capacity), soups uploaded per request,
+  p50/p99 latency, beside the saturated capacity itself.
 
-Also measured (key "batched"): the mixed-size stream of 256 meshes, one
-gpv_voxelize per mesh against one gpv_voxelize_batch for all of them; and
-(key "served") a Poisson request stream through the serving loop, with
-p50/p99 latency at the stated arrival rate.
-
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--batch-meshes N]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] ...
 """
 import argparse
 import json
@@ -29,66 +41,56 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-VALU_PEAK_OPS = 78.65e12       # MI355X FP32 vector peak 157.3 TFLOP/s counts an FMA as 2: lane-ops per second
+# FP32 vector issue without packed instructions (the library is built with
+# packed FP32 off): 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-operations/s
+VALU_PEAK_OPS = 39.3e12
 STAGES = ["setup", "l1_sat", "l1_parity", "compaction_bins", "l2_parity", "l2_sat"]
+TESTS_BASIS = ("reference loop counts: sum of index-AABB volumes (Level1) + |bin| * n2^3 (Level2); "
+               "the kernels run a pruned subset with the same outcome")
 
 
-def make_workload(gv):
-    v, f = gv.torus_knot(p=3, q=7, segments=2048, ring=256, tube=0.3, jitter=0.01, extent=0.9, seed=2)
-    return v, f, gv.flatten(v, f)
+def stage_ms(infos):
+    return {s: float(np.median([i["ms_stage"][k] for i in infos])) for k, s in enumerate(STAGES)}
 
 
-def bench_batched(gv, ctx, n_meshes, steps, warmup, check, small=False, autotune=False):
-    """The mixed-size stream: n_meshes meshes (triangles log-uniform in
-    [1k, 200k], Level1 {64,128,256}, Level2 {4,8}), device-resident soups.
-    Plain = one gpv_voxelize per mesh; batched = one gpv_voxelize_batch over
-    all of them.  Both timed over whole passes, grids released each pass.
-    small=True: the same generators at 1k-20k triangles, Level1 64, Level2 4
-    (meshes whose single-mesh run is dominated by per-call costs)."""
-    from gpview_amd import workloads
-    if small:
-        meshes = workloads.mixed_stream(gv, n_meshes, seed=7, tri_range=(1000, 20000), level1=(64,), level2=(4,))
-    else:
-        meshes = workloads.mixed_stream(gv, n_meshes)
+def bench_batched(gv, ctx, meshes, steps, warmup, check, desc):
+    """Plain (one gpv_voxelize per mesh, resident soups) against
+    gpv_voxelize_batch, the latter both from pinned host soups (upload in the
+    timed region) and from resident soups."""
     ptrs = [ctx.upload(m["soup"]) for m in meshes]
+    pinned = [gv.pinned_copy(m["soup"]) for m in meshes]
+    host = [dict(m, soup=p.array) for m, p in zip(meshes, pinned)]
     T = sum(len(m["soup"]) for m in meshes)
-    tune = None
-    if autotune:
-        shapes, tms = ctx.autotune(meshes, device_ptrs=ptrs, repeats=2)
-        tune = {"shapes": shapes, "device_ms": tms}
+    n = len(meshes)
 
     def plain_pass():
         for m, p in zip(meshes, ptrs):
             ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], m["voxel_count"], m["voxel_count2"], True,
                          device_ptr=p).release()
 
-    def batch_pass():
-        gs = ctx.voxelize_batch(meshes, device_ptrs=ptrs)
+    def batch_pass(resident):
+        gs = ctx.voxelize_batch(meshes, device_ptrs=ptrs) if resident else ctx.voxelize_batch(host)
         info = gs[0].info()
         for g in gs:
             g.release()
         return info
 
+    def timed(fn, *a):
+        t0 = time.perf_counter()
+        out = [fn(*a) for _ in range(steps)]
+        return (time.perf_counter() - t0) / steps, out
+
     for _ in range(warmup):
         plain_pass()
-        batch_pass()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        plain_pass()
-    t_plain = (time.perf_counter() - t0) / steps
-    lat = []
-    binfo = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        s0 = time.perf_counter()
-        binfo.append(batch_pass())
-        lat.append(time.perf_counter() - s0)
-    t_batch = (time.perf_counter() - t0) / steps
+        batch_pass(False)
+        batch_pass(True)
+    t_plain, _ = timed(plain_pass)
+    t_host, binfo = timed(batch_pass, False)
+    t_res, _ = timed(batch_pass, True)
 
-    # totals and the per-mesh bit-exact check (outside the timed passes)
-    gs = ctx.voxelize_batch(meshes, device_ptrs=ptrs)
-    tests = 0
-    voxels = 0
+    # totals and the per-mesh bit-exact check against the single-mesh path
+    gs = ctx.voxelize_batch(host)
+    tests = voxels = 0
     tot = {"tests_l1": 0, "tests_l2": 0, "pairs": 0, "nb": 0, "n1": 0}
     exact = True if check else None
     for m, p, g in zip(meshes, ptrs, gs):
@@ -98,52 +100,107 @@ def bench_batched(gv, ctx, n_meshes, steps, warmup, check, small=False, autotune
         for k in tot:
             tot[k] += i[k]
         if check:
-            a = g.download()
+            a = g.download_bits()
             ga = ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], m["voxel_count"], m["voxel_count2"],
                               True, device_ptr=p)
-            b = ga.download()
+            b = ga.download_bits()
             ga.release()
-            exact = exact and all(np.array_equal(a[k], b[k]) for k in ("l1", "prefix", "bidx", "l2"))
+            exact = exact and all(np.array_equal(x, y) for x, y in zip(a, b))
         g.release()
     for p in ptrs:
         ctx.free(p)
-    stage_ms = {s: float(np.median([i["ms_stage"][k] for i in binfo])) for k, s in enumerate(STAGES)}
-    desc = ("log-uniform 1k-20k; icospheres, tori, drilled boxes; L1 64, L2 4" if small else
-            "log-uniform 1k-200k; icospheres, tori, drilled boxes; L1 {64,128,256}, L2 {4,8}")
+    for p in pinned:
+        p.close()
     return {
-        "workload": f"{'small-mesh' if small else 'mixed-size'} stream, {n_meshes} meshes, {T} triangles ({desc})",
-        "meshes": n_meshes, "triangles": T,
-        "plain_meshes_per_s": n_meshes / t_plain,
-        "batched_meshes_per_s": n_meshes / t_batch,
-        "speedup": t_plain / t_batch,
-        "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3,
-        "batched_tri_box_tests_per_s": tests / t_batch,
-        "batched_voxels_per_s": voxels / t_batch,
-        "batched_stage_ms": stage_ms,
+        "workload": f"{desc}: {n} meshes, {T} triangles",
+        "meshes": n, "triangles": T, "soup_bytes": 36 * T,
+        "plain_meshes_per_s": n / t_plain,
+        "batched_meshes_per_s": n / t_host,
+        "batched_resident_meshes_per_s": n / t_res,
+        "speedup": t_plain / t_host,
+        "speedup_resident": t_plain / t_res,
+        "plain_ms": t_plain * 1e3, "batched_ms": t_host * 1e3, "batched_resident_ms": t_res * 1e3,
+        "batched_tri_box_tests_per_s": tests / t_host,
+        "batched_voxels_per_s": voxels / t_host,
+        "batched_stage_ms": stage_ms(binfo),
+        "batched_device_ms": float(np.median([i["ms_total"] for i in binfo])),
         "totals": tot,
         "bit_exact_vs_plain": exact,
-        "tune": tune,
+        "note": "plain uses resident soups; batched_meshes_per_s uploads every soup from pinned host memory "
+                "inside the timed region",
+    }
+
+
+def bench_stress(gv, ctx, n_meshes, per_run, check_members):
+    """Stress members 0..n_meshes-1 (100k-2M triangles, lattice-snapped /
+    sliver / face-coplanar) at Level1 512 / Level2 16: gpv_voxelize_batch
+    over runs of per_run meshes from pinned host soups, grids released after
+    each run; the plain path mesh by mesh beside it; and the oracle on the
+    sampled members."""
+    from gpview_amd import workloads
+    from oracle import orc
+    t0 = time.perf_counter()
+    meshes = [workloads.stress_member(gv, i) for i in range(n_meshes)]
+    t_gen = time.perf_counter() - t0
+    pinned = [gv.pinned_copy(m["soup"]) for m in meshes]
+    host = [dict(m, soup=p.array) for m, p in zip(meshes, pinned)]
+    T = sum(len(m["soup"]) for m in meshes)
+    runs = [host[a:a + per_run] for a in range(0, n_meshes, per_run)]
+    ctx.voxelize_batch(runs[0][:1])[0].release()   # warm-up
+    infos = []
+    t0 = time.perf_counter()
+    for r in runs:
+        gs = ctx.voxelize_batch(r)
+        infos += [g.info() for g in gs]
+        for g in gs:
+            g.release()
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for m in host:
+        ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], 512, 16).release()
+    t_plain = time.perf_counter() - t0
+    tests = sum(i["tests_l1"] + i["tests_l2"] for i in infos)
+    voxels = sum(i["n1"] + i["nb"] * 16 ** 3 for i in infos)
+    checked = []
+    for k in check_members:
+        m = host[k]
+        g = orc.grid_setup(m["bmin"], m["bmax"], m["mms"], 512, 16)
+        want = orc.voxelize(m["soup"], g, orc.ORC_FAST, 0, bins=False)
+        gs = ctx.voxelize_batch(host[(k // per_run) * per_run:(k // per_run + 1) * per_run])
+        got = gs[k % per_run].download()
+        for x in gs:
+            x.release()
+        ok = all(np.array_equal(got[key], want[key]) for key in ("l1", "prefix", "bidx", "l2"))
+        checked.append({"member": k, "triangles": len(m["soup"]), "nb": int(want["nb"]), "bit_exact": ok})
+        del want, got
+    for p in pinned:
+        p.close()
+    return {
+        "workload": f"exactness-stress stream, members 0-{n_meshes - 1} of 64 ({T} triangles, 100k-2M each; "
+                    "lattice-snapped p=0.3, 10% slivers, 5% face-coplanar), Level1 512, Level2 16",
+        "meshes": n_meshes, "triangles": T, "per_run": per_run,
+        "batched_meshes_per_s": n_meshes / t_batch, "plain_meshes_per_s": n_meshes / t_plain,
+        "batched_s": t_batch, "plain_s": t_plain,
+        "batched_tri_box_tests_per_s": tests / t_batch, "batched_voxels_per_s": voxels / t_batch,
+        "boundary_voxels": int(sum(i["nb"] for i in infos)),
+        "oracle_sample": checked, "generation_s": t_gen,
+        "note": "soups uploaded from pinned host memory inside the timed region; grids released after each run",
     }
 
 
 def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48, plain_sample=300):
-    """The served stream: Poisson arrivals at `rate` requests/s (0: all at
-    once, the saturated case), each drawing
-    a mesh from a library made by the mixed-stream generators (triangles
-    log-uniform in [1k, 2M]) and a resolution (Level1 {64,128,256,512},
-    Level2 {4,8,16}).  Library soups are device-resident; results stay
-    resident until the client releases them (right after it has waited)."""
+    """The served stream: requests drawn from a library of the torus knot and
+    mixed-stream meshes of 1k-2M triangles, each at Level1 {64,128,256,512}
+    and Level2 {4,8,16}; soups in pinned host memory, uploaded per request
+    (tris_on_device = 0); results released by the client right after its
+    wait.  rate 0 submits everything at once (saturated)."""
     import threading
     from gpview_amd import workloads
-    lib_meshes = workloads.mixed_stream(gv, n_library, seed=seed, tri_range=(1000, 2000000))
-    rng = np.random.default_rng(seed)
-    pick = rng.integers(0, n_library, n_requests)
-    l1 = rng.choice([64, 128, 256, 512], n_requests, p=[0.3, 0.3, 0.3, 0.1])
-    l2 = rng.choice([4, 8, 16], n_requests, p=[0.45, 0.45, 0.1])
-    gaps = rng.exponential(1.0 / rate, n_requests) if rate > 0 else np.zeros(n_requests)
+    lib_meshes = workloads.served_library(gv, n_library, seed=seed)
+    pinned = [gv.pinned_copy(m["soup"]) for m in lib_meshes]
+    lib_host = [dict(m, soup=p.array) for m, p in zip(lib_meshes, pinned)]
+    pick, l1, l2, gaps = workloads.served_requests(n_requests, n_library, rate, seed=seed)
     srv = gv.Server(lanes=lanes, hbm_budget=200e9)
-    ctx = gv.Context(0)
-    ptrs = [ctx.upload(m["soup"]) for m in lib_meshes]
     ids = [None] * n_requests
     ready = threading.Semaphore(0)
     stats = [None] * n_requests
@@ -156,20 +213,23 @@ def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48, plain_sample=
             d = t_next - time.perf_counter()
             if d > 0:
                 time.sleep(d)
-            m = dict(lib_meshes[pick[i]], voxel_count=int(l1[i]), voxel_count2=int(l2[i]))
+            m = dict(lib_host[pick[i]], voxel_count=int(l1[i]), voxel_count2=int(l2[i]))
             tris[i] = len(m["soup"])
-            ids[i] = srv.submit(m, device_ptr=ptrs[pick[i]])
+            ids[i] = srv.submit(m)
             ready.release()
 
-    # the same request sequence through the single-mesh path, one call after
-    # another (a bounded prefix), for the served-vs-plain comparison
-    n_plain = min(n_requests, plain_sample)
-    t0 = time.perf_counter()
-    for i in range(n_plain):
-        m = lib_meshes[pick[i]]
-        ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], int(l1[i]), int(l2[i]), True,
-                     device_ptr=ptrs[pick[i]]).release()
-    plain_rate = n_plain / (time.perf_counter() - t0) if n_plain else None
+    plain_rate = None
+    if plain_sample:
+        # the same request sequence through the single-mesh path, one call
+        # after another (a bounded prefix), for the served-vs-plain ratio
+        ctx = gv.Context(0)
+        n_plain = min(n_requests, plain_sample)
+        t0 = time.perf_counter()
+        for i in range(n_plain):
+            m = lib_host[pick[i]]
+            ctx.voxelize(m["soup"], m["bmin"], m["bmax"], m["mms"], int(l1[i]), int(l2[i]), True).release()
+        plain_rate = n_plain / (time.perf_counter() - t0)
+        ctx.close()
     th = threading.Thread(target=submitter)
     t0 = time.perf_counter()
     th.start()
@@ -183,26 +243,29 @@ def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48, plain_sample=
     c = srv.counters()
     pool = srv.pool_stats()
     srv.close()
-    for p in ptrs:
-        ctx.free(p)
-    ctx.close()
+    for p in pinned:
+        p.close()
     lat = np.array([s["t_done"] - s["t_submit"] for s in stats])
     span = max(s["t_done"] for s in stats) - min(s["t_submit"] for s in stats)
     sizes = np.array([s["batch_size"] for s in stats])
-    pred = np.array([s["predicted_bytes"] for s in stats], np.float64)
-    res = np.array([s["resident_bytes"] for s in stats], np.float64)
+    batches = {}
+    for s in stats:
+        batches[s["batch_id"]] = (s["batch_peak_bytes"], s["batch_predicted_bytes"])
+    ratio = [pk / pr for pk, pr in batches.values() if pr > 0]
     return {
         "workload": f"served stream: {n_requests} " + (f"Poisson requests at {rate:g}/s" if rate > 0 else
-                                                          "requests submitted at once") + f" over a {n_library}-mesh "
-                    "device-resident library (1k-2M triangles), L1 {64,128,256,512}, L2 {4,8,16}",
+                                                          "requests submitted at once") +
+                    f" over a {n_library}-mesh library (the torus knot + mixed meshes of 1k-2M triangles), "
+                    "Level1 {64,128,256,512}, Level2 {4,8,16}; soups uploaded per request from pinned host memory",
         "requests": n_requests, "rate_per_s": rate, "lanes": lanes, "hbm_budget_bytes": 200e9,
         "meshes_per_s": n_requests / span, "wall_s": wall, "triangles": int(tris.sum()),
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "plain_meshes_per_s": plain_rate, "plain_sample": n_plain,
-        "mean_batch": float(sizes.mean()), "batches": c["batches"],
+        "plain_meshes_per_s": plain_rate, "plain_sample": min(n_requests, plain_sample),
+        "mean_batch": float(sizes.mean()), "max_batch": int(sizes.max()), "batches": c["batches"],
+        "readmitted": c["readmitted"], "failed": int(sum(s["status"] != 0 for s in stats)),
         "peak_charged_bytes": c["peak_charged"],
+        "batch_peak_over_predicted": {"max": float(max(ratio)), "median": float(np.median(ratio))} if ratio else None,
         "pool": {k: pool[k] for k in ("held_bytes", "allocations", "reuses", "evictions")},
-        "predicted_over_resident": float(np.median(pred / np.maximum(res, 1))),
     }
 
 
@@ -222,7 +285,7 @@ def measured(kernel):
     k = prof.get("kernels", {}).get(kernel)
     if not k:
         return None
-    return dict(k, source=os.path.relpath(PROFILE, ROOT))
+    return dict(k, source=os.path.relpath(PROFILE, ROOT), valu_peak=prof.get("valu_peak_lane_ops"))
 
 
 def copy_calibration():
@@ -252,6 +315,40 @@ def kernel_bytes(stage, info, T):
     return 36 * T + 2 * ((info["n1"] + 7) // 8)
 
 
+def cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads):
+    """The oracle on this host: the CPU baseline's composition on the full
+    headline mesh with all cores (value) and one thread, and the literal
+    composition (all-triangle parity) against it on the 5,120-triangle
+    icosphere."""
+    from gpview_amd import workloads
+    g0 = orc.grid_setup(bmin, bmax, mms, 256, 8)
+
+    def run(tris, g, mode, th):
+        c0 = time.perf_counter()
+        r = orc.voxelize(tris, g, mode, th, bins=False)
+        return time.perf_counter() - c0, r
+
+    t_all, want = run(soup, g0, orc.ORC_PLAN, threads)
+    t_one, _ = run(soup, g0, orc.ORC_PLAN, 1)
+    s = workloads.plain_small(gv)
+    g1 = orc.grid_setup(s["bmin"], s["bmax"], s["mms"], 64, 4)
+    legs = {}
+    for name, mode in (("literal", orc.ORC_EXACT), ("column_factorised", orc.ORC_PLAN)):
+        for th in (threads, 1):
+            t, _ = run(s["soup"], g1, mode, th)
+            legs[f"icosphere_5120_{name}_{th}t"] = {"seconds": t, "meshes_per_s": 1.0 / t, "cores": th}
+    tests = want["tests_l1"] + want["tests_l2"]
+    out = {"value": 1.0 / t_all, "unit": "meshes/s", "cores": threads, "kind": "port",
+           "sample": "the full headline mesh (1 mesh): oracle in the baseline composition (column-factorised "
+                     "parity, literal SAT loops over every index-AABB voxel and bin), -O2 -ffp-contract=off",
+           "seconds": t_all, "tri_box_tests_per_s": tests / t_all,
+           "single_thread": {"value": 1.0 / t_one, "seconds": t_one, "cores": 1,
+                             "tri_box_tests_per_s": tests / t_one},
+           "icosphere_5120_64_4": legs,
+           "stage_s": dict(zip(["l1_parity", "l1_sat", "compaction", "l2_parity", "l2_sat"], want["t_stage"]))}
+    return out, want
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,15 +359,15 @@ def main():
     ap.add_argument("--batch-meshes", type=int, default=256, help="meshes of the mixed-stream batch (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-check", action="store_true", help="skip the batched-vs-plain bit-exact check")
-    ap.add_argument("--autotune", action="store_true",
-                    help="pick launch shapes with gpv_ctx_autotune on each workload before timing it")
-    ap.add_argument("--serve-requests", type=int, default=1000, help="served-stream requests (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=100.0, help="served-stream arrivals per second")
+    ap.add_argument("--stress-meshes", type=int, default=16, help="stress-stream members (0: skip; 64: all)")
+    ap.add_argument("--stress-run", type=int, default=8, help="stress meshes per gpv_voxelize_batch")
+    ap.add_argument("--serve-requests", type=int, default=10000, help="served-stream requests (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=-1.0,
+                    help="served-stream arrivals per second (-1: 0.8 of the measured saturated capacity)")
     ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy that calibrates the traffic counters")
-    ap.add_argument("--serve-only", action="store_true",
-                    help="only the served stream (--serve-requests at --serve-rate); one JSON line")
+    ap.add_argument("--serve-only", action="store_true", help="only the served stream; one JSON line")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -281,21 +378,26 @@ def main():
         sys.exit(2)
 
     import gpview_amd as gv
+    from gpview_amd import workloads
+
+    def served():
+        if args.serve_requests <= 0:
+            return None, None
+        sat = bench_serve(gv, min(2000, args.serve_requests), 0.0, args.serve_lanes, plain_sample=300)
+        rate = args.serve_rate if args.serve_rate >= 0 else round(0.8 * sat["meshes_per_s"], -1)
+        return bench_serve(gv, args.serve_requests, rate, args.serve_lanes, plain_sample=0), sat
+
     if args.serve_only:
-        sv = bench_serve(gv, args.serve_requests, args.serve_rate, args.serve_lanes)
+        sv, sat = served()
         print(json.dumps({"metric": "served meshes/s; p50/p99 request latency", "value": sv["meshes_per_s"],
-                          "unit": "meshes/s", "higher_is_better": True, "n_gpus": 1, "served": sv}), flush=True)
+                          "unit": "meshes/s", "higher_is_better": True, "n_gpus": 1, "served": sv,
+                          "served_saturated": sat}), flush=True)
         return
-    v, f, soup = make_workload(gv)
+    m = workloads.plain_large(gv)
+    soup, bmin, bmax, mms = m["soup"], m["bmin"], m["bmax"], m["mms"]
     T = len(soup)
-    bmin, bmax, mms = gv.mesh_bbox(v)
     ctx = gv.Context(0)
     dsoup = ctx.upload(soup)
-    tune = None
-    if args.autotune:
-        shapes, tms = ctx.autotune([{"soup": soup, "bmin": bmin, "bmax": bmax, "mms": mms, "voxel_count": 256,
-                                     "voxel_count2": 8}], device_ptrs=[dsoup], repeats=3)
-        tune = {"shapes": shapes, "device_ms": tms}
 
     def step():
         return ctx.voxelize(soup, bmin, bmax, mms, 256, 8, True, device_ptr=dsoup)
@@ -319,55 +421,56 @@ def main():
     got = g.download()
     g.release()
     ctx.free(dsoup)
-    batched = batched_small = None
+    batched = batched_small = stress = None
     if args.batch_meshes > 0:
-        batched = bench_batched(gv, ctx, args.batch_meshes, args.batch_steps, 1, not args.no_check,
-                                autotune=args.autotune)
-        batched_small = bench_batched(gv, ctx, args.batch_meshes, args.batch_steps, 1, not args.no_check, small=True,
-                                      autotune=args.autotune)
+        batched = bench_batched(gv, ctx, workloads.mixed_stream(gv, args.batch_meshes), args.batch_steps, 1,
+                                not args.no_check,
+                                "mixed-size stream (log-uniform 1k-200k triangles; icospheres, tori, drilled "
+                                "boxes; Level1 {64,128,256}, Level2 {4,8})")
+        batched_small = bench_batched(
+            gv, ctx, workloads.mixed_stream(gv, args.batch_meshes, seed=7, tri_range=(1000, 20000), level1=(64,),
+                                            level2=(4,)), args.batch_steps, 1, False,
+            "small-mesh stream (log-uniform 1k-20k triangles, Level1 64, Level2 4)")
+    if args.stress_meshes > 0:
+        n = min(args.stress_meshes, 64)
+        sizes = [workloads.stress_spec(i)[1] for i in range(n)]
+        sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes))}) if not args.no_check else []
+        stress = bench_stress(gv, ctx, n, args.stress_run, sample)
     ctx.close()
-    served = None
-    if args.serve_requests > 0:
-        served = bench_serve(gv, args.serve_requests, args.serve_rate, args.serve_lanes)
+    served_stream, served_sat = served()
 
     meshes_per_s = args.steps / wall
     tests = info["tests_l1"] + info["tests_l2"]
     n2 = info["n2"]
     voxels = info["n1"] + info["nb"] * n2 ** 3
-    stage_ms = {s: float(np.median([i["ms_stage"][k] for i in infos])) for k, s in enumerate(STAGES)}
-    dom = max(STAGES, key=lambda s: stage_ms[s])
-    t_dom = stage_ms[dom] / 1e3
+    sms = stage_ms(infos)
+    dom = max(STAGES, key=lambda s: sms[s])
+    t_dom = sms[dom] / 1e3
     bytes_dom = kernel_bytes(dom, info, T)
     meas = measured(STAGE_KERNEL.get(dom))
     rl = {"bound": "hbm", "achieved": bytes_dom / t_dom / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
           "frac": bytes_dom / t_dom / 1e9 / HBM_PEAK_GBS,
           "traffic": meas["traffic_bytes"] if meas else None, "kernel": STAGE_KERNEL.get(dom, dom),
-          "algorithmic_bytes": bytes_dom, "ms": stage_ms[dom],
+          "algorithmic_bytes": bytes_dom, "ms": sms[dom],
           "traffic_source": meas["source"] if meas else None}
     # The path is not a contraction (no MFMA); its binding resource is VALU
-    # issue.  Measured from the SQ counters of the same kernel and workload:
-    # lane-ops/s = VALU wave instructions x 64 / launch time, against the
-    # published FP32 vector peak in lane-operations (FMA counted once).
+    # issue: VALU wave instructions of the same kernel and workload (SQ
+    # counters, committed summary) x 64 lanes / this run's launch time,
+    # against the non-packed FP32 issue peak.
     valu = None
     if meas and meas.get("valu_insts"):
+        peak = meas.get("valu_peak") or VALU_PEAK_OPS
         lane_ops = meas["valu_insts"] * 64 / t_dom
-        valu = {"bound": "valu", "achieved": lane_ops / 1e12, "peak": VALU_PEAK_OPS / 1e12, "unit": "Tlane-op/s",
-                "frac": lane_ops / VALU_PEAK_OPS, "kernel": rl["kernel"], "source": meas["source"]}
+        valu = {"bound": "valu", "achieved": lane_ops / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s",
+                "frac": lane_ops / peak, "kernel": rl["kernel"], "source": meas["source"],
+                "valu_insts": meas["valu_insts"]}
 
     cpu = None
     exact = None
     if not args.no_cpu:
         from oracle import orc
         threads = args.cpu_threads or os.cpu_count()
-        g0 = orc.grid_setup(bmin, bmax, mms, 256, 8)
-        c0 = time.perf_counter()
-        want = orc.voxelize(soup, g0, orc.ORC_FAST, threads)
-        cdt = time.perf_counter() - c0
-        cpu = {"value": 1.0 / cdt, "unit": "meshes/s", "cores": threads, "kind": "port",
-               "sample": "the full workload mesh (1 mesh), oracle column-factorised mode",
-               "seconds": cdt, "tests_per_s": (want["tests_l1"] + want["tests_l2"]) / cdt,
-               "stage_s": dict(zip(["l1_parity", "l1_sat", "compaction", "l2_parity", "l2_sat"],
-                                   want["t_stage"]))}
+        cpu, want = cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads)
         exact = all(np.array_equal(got[k], want[k]) for k in ("l1", "prefix", "bidx", "l2"))
 
     out = {
@@ -387,20 +490,22 @@ def main():
                    "triangles": T, "level1": info["n"], "level2": n2, "boundary_voxels": info["nb"],
                    "parallelism": "single GPU"},
         "tri_box_tests_per_s": tests / (wall / args.steps),
+        "tri_box_tests_basis": TESTS_BASIS,
         "voxels_per_s": voxels / (wall / args.steps),
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "device_ms_total": float(np.median([i["ms_total"] for i in infos])),
-        "stage_ms": stage_ms,
+        "stage_ms": sms,
         "tests_l1": info["tests_l1"], "tests_l2": info["tests_l2"], "pairs": info["pairs"],
         "bit_exact_vs_cpu": exact,
         "roofline": rl,
         "roofline_valu": valu,
         "cpu_baseline": cpu,
-        "tune": tune,
         "batched": batched,
         "batched_small_meshes": batched_small,
-        "served": served,
+        "stress": stress,
+        "served": served_stream,
+        "served_saturated": served_sat,
     }
     print(json.dumps(out), flush=True)
 

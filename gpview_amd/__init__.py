@@ -57,7 +57,9 @@ class GridInfo(ctypes.Structure):
 class PoolStats(ctypes.Structure):
     """gpv_pool_stats (include/gpv.h)."""
     _fields_ = [("held_bytes", ctypes.c_int64), ("cached_bytes", ctypes.c_int64), ("cap_bytes", ctypes.c_int64),
-                ("allocations", ctypes.c_int64), ("reuses", ctypes.c_int64), ("evictions", ctypes.c_int64)]
+                ("allocations", ctypes.c_int64), ("reuses", ctypes.c_int64), ("evictions", ctypes.c_int64),
+                ("live_bytes", ctypes.c_int64), ("peak_live_bytes", ctypes.c_int64), ("limit_bytes", ctypes.c_int64),
+                ("requested_bytes", ctypes.c_int64), ("peak_requested_bytes", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -81,7 +83,8 @@ class MeshDesc(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     """gpv_server_opts (include/gpv_serve.h)."""
     _fields_ = [("device", ctypes.c_int), ("lanes", ctypes.c_int), ("hbm_budget", ctypes.c_int64),
-                ("max_batch_meshes", ctypes.c_int), ("max_batch_tris", ctypes.c_int64)]
+                ("max_batch_meshes", ctypes.c_int), ("max_batch_tris", ctypes.c_int64),
+                ("lane_mem_limit", ctypes.c_int64)]
 
 
 class RequestStats(ctypes.Structure):
@@ -89,7 +92,8 @@ class RequestStats(ctypes.Structure):
     _fields_ = [("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
                 ("predicted_bytes", ctypes.c_int64), ("resident_bytes", ctypes.c_int64),
                 ("batch_id", ctypes.c_int32), ("batch_size", ctypes.c_int32), ("lane", ctypes.c_int32),
-                ("status", ctypes.c_int32)]
+                ("status", ctypes.c_int32), ("retries", ctypes.c_int32),
+                ("batch_predicted_bytes", ctypes.c_int64), ("batch_peak_bytes", ctypes.c_int64)]
 
 
 class Mesh(ctypes.Structure):
@@ -106,7 +110,8 @@ _lib = None
 # every symbol include/gpv.h and include/gpv_mesh.h declare
 EXPORTED = [
     "gpv_ctx_create", "gpv_ctx_destroy", "gpv_last_error", "gpv_status_string",
-    "gpv_device_upload", "gpv_device_free", "gpv_ctx_pool_stats", "gpv_ctx_get_tune", "gpv_ctx_set_tune",
+    "gpv_device_upload", "gpv_device_free", "gpv_host_alloc", "gpv_host_free", "gpv_ctx_pool_stats", "gpv_ctx_set_mem_limit", "gpv_ctx_get_tune",
+    "gpv_ctx_set_tune",
     "gpv_ctx_autotune",
     "gpv_mesh_bbox", "gpv_grid_derive", "gpv_grid_centers", "gpv_voxelize", "gpv_voxelize_noalloc",
     "gpv_voxelize_batch",
@@ -120,7 +125,8 @@ EXPORTED = [
     "gpv_mesh_torus_knot", "gpv_mesh_torus", "gpv_mesh_drilled_box", "gpv_mesh_mixed",
     "gpv_mesh_stress", "gpv_mesh_read_obj", "gpv_mesh_read_off",
     "gpv_predict_footprint", "gpv_server_create", "gpv_server_destroy", "gpv_server_submit",
-    "gpv_server_wait", "gpv_server_release", "gpv_server_counters", "gpv_server_pool_stats",
+    "gpv_server_wait", "gpv_server_release", "gpv_server_counters", "gpv_server_readmitted",
+    "gpv_server_pool_stats",
 ]
 
 
@@ -141,7 +147,11 @@ def lib():
     L.gpv_ctx_destroy.argtypes = [_vp]
     L.gpv_device_upload.argtypes = [_vp, _vp, c_i64, P(_vp)]
     L.gpv_device_free.argtypes = [_vp, _vp]
+    L.gpv_host_alloc.argtypes = [c_i64, P(_vp)]
+    L.gpv_host_free.argtypes = [_vp]
     L.gpv_ctx_pool_stats.argtypes = [_vp, P(PoolStats)]
+    L.gpv_ctx_set_mem_limit.argtypes = [_vp, c_i64]
+    L.gpv_server_readmitted.argtypes = [_vp, P(c_i64)]
     L.gpv_ctx_get_tune.argtypes = [_vp, P(Tune)]
     L.gpv_ctx_set_tune.argtypes = [_vp, P(Tune)]
     L.gpv_ctx_autotune.argtypes = [_vp, P(MeshDesc), c_int, c_int, c_int, P(Tune), P(c_float)]
@@ -343,6 +353,10 @@ class Context:
         _check(lib().gpv_ctx_pool_stats(self.h, ctypes.byref(s)), "gpv_ctx_pool_stats")
         return s.as_dict()
 
+    def set_mem_limit(self, nbytes):
+        """gpv_ctx_set_mem_limit: cap on the pool's bytes in use (0: none)."""
+        _check(lib().gpv_ctx_set_mem_limit(self.h, int(nbytes)), "gpv_ctx_set_mem_limit")
+
     def get_tune(self):
         """gpv_ctx_get_tune as a dict."""
         t = Tune()
@@ -427,6 +441,32 @@ class Context:
         return Grid(self, g)
 
 
+class PinnedArray:
+    """A numpy array in page-locked host memory (gpv_host_alloc); .array is
+    the view, close() returns the memory."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * self.dtype.itemsize
+        self.h = _vp()
+        _check(lib().gpv_host_alloc(n, ctypes.byref(self.h)), "gpv_host_alloc")
+        buf = (ctypes.c_char * max(n, 1)).from_address(self.h.value)
+        self.array = np.frombuffer(buf, self.dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def close(self):
+        if self.h:
+            self.array = None
+            lib().gpv_host_free(self.h)
+            self.h = _vp()
+
+
+def pinned_copy(a):
+    """PinnedArray holding a copy of numpy array a."""
+    p = PinnedArray(a.shape, a.dtype)
+    p.array[...] = a
+    return p
+
+
 def mesh_desc(m, device_ptr=None):
     """gpv_mesh_desc of a mesh dict (soup, bmin, bmax, mms, voxel_count, ...);
     returns (desc, soup) -- keep the soup alive while the desc is in use."""
@@ -461,9 +501,9 @@ class _CtxRef:
 class Server:
     """gpv_server: continuous-batching request loop (include/gpv_serve.h)."""
 
-    def __init__(self, lanes=2, hbm_budget=0, max_batch_meshes=0, max_batch_tris=0, device=-1):
+    def __init__(self, lanes=2, hbm_budget=0, max_batch_meshes=0, max_batch_tris=0, device=-1, lane_mem_limit=0):
         self.h = _vp()
-        o = ServerOpts(device, lanes, int(hbm_budget), max_batch_meshes, int(max_batch_tris))
+        o = ServerOpts(device, lanes, int(hbm_budget), max_batch_meshes, int(max_batch_tris), int(lane_mem_limit))
         _check(lib().gpv_server_create(ctypes.byref(o), ctypes.byref(self.h)), "gpv_server_create")
         self._keep = {}
 
@@ -503,7 +543,10 @@ class Server:
         a, b, c, d = (ctypes.c_int64() for _ in range(4))
         _check(lib().gpv_server_counters(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
                                          ctypes.byref(d)), "gpv_server_counters")
-        return {"completed": a.value, "batches": b.value, "charged": c.value, "peak_charged": d.value}
+        r = ctypes.c_int64()
+        _check(lib().gpv_server_readmitted(self.h, ctypes.byref(r)), "gpv_server_readmitted")
+        return {"completed": a.value, "batches": b.value, "charged": c.value, "peak_charged": d.value,
+                "readmitted": r.value}
 
     def pool_stats(self):
         """gpv_server_pool_stats: the lanes' pools, summed."""

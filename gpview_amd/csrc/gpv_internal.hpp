@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct gpv_ctx;
+
 namespace gpv {
 
 // Grid constants and per-axis center tables of one mesh.
@@ -87,6 +89,8 @@ struct Counters {
 
 // sets the calling thread's gpv_last_error text and returns code
 int set_error(int code, const char* msg);
+// restarts the context pool's peak of bytes in use from the bytes in use now
+void pool_reset_peak(gpv_ctx* c);
 
 // ---- launch wrappers (gpv_kernels.hip) ------------------------------------
 // Launchers that take the batch view M and a grid g0 run the single-mesh

@@ -6,6 +6,7 @@
 // citations per function) and this file is compiled with -ffp-contract=off.
 #include "gpv.h"
 #include "gpv_internal.hpp"
+#include "gpv_serve.h"
 
 #include <hip/hip_runtime.h>
 
@@ -80,18 +81,28 @@ public:
         return (n + gran - 1) / gran * gran;
     }
     void set_cap(size_t cap) { cap_ = cap; }
+    // device bytes the pool may hold in use at once (0: no limit); a request
+    // beyond it fails as if the device were out of memory
+    void set_limit(size_t limit) { limit_ = limit; }
+    void reset_peak()
+    {
+        peak_live_ = live_bytes_;
+        peak_req_ = req_bytes_;
+    }
     hipError_t get(size_t n, void** out)
     {
         const size_t c = size_class(n);
         auto it = free_.lower_bound(c);
-        if (it != free_.end() && it->first <= 2 * c) {
+        if (it != free_.end() && it->first <= 2 * c && (!limit_ || live_bytes_ + it->first <= limit_)) {
             *out = it->second;
-            live_[*out] = it->first;
+            live_[*out] = {it->first, n};
             cached_ -= it->first;
             free_.erase(it);
             ++reuses_;
+            note_live(it->first, n);
             return hipSuccess;
         }
+        if (limit_ && live_bytes_ + c > limit_) return hipErrorOutOfMemory;
         while (!free_.empty() && cached_ + c > cap_) evict(std::prev(free_.end()));
         void* p = nullptr;
         hipError_t e = hipMalloc(&p, c);
@@ -101,9 +112,10 @@ public:
             e = hipMalloc(&p, c);
             if (e != hipSuccess) return e;
         }
-        live_[p] = c;
+        live_[p] = {c, n};
         bytes_ += c;
         ++allocs_;
+        note_live(c, n);
         *out = p;
         return hipSuccess;
     }
@@ -112,8 +124,10 @@ public:
         if (!p) return;
         auto it = live_.find(p);
         if (it == live_.end()) return;
-        free_.emplace(it->second, p);
-        cached_ += it->second;
+        free_.emplace(it->second.first, p);
+        cached_ += it->second.first;
+        live_bytes_ -= it->second.first;
+        req_bytes_ -= it->second.second;
         live_.erase(it);
     }
     void trim()
@@ -128,9 +142,23 @@ public:
         s->allocations = allocs_;
         s->reuses = reuses_;
         s->evictions = evictions_;
+        s->live_bytes = (int64_t)live_bytes_;
+        s->peak_live_bytes = (int64_t)peak_live_;
+        s->limit_bytes = (int64_t)limit_;
+        s->requested_bytes = (int64_t)req_bytes_;
+        s->peak_requested_bytes = (int64_t)peak_req_;
     }
+    size_t live_bytes() const { return live_bytes_; }
+    size_t peak_live() const { return peak_live_; }
 
 private:
+    void note_live(size_t c, size_t n)
+    {
+        live_bytes_ += c;
+        peak_live_ = std::max(peak_live_, live_bytes_);
+        req_bytes_ += n;
+        peak_req_ = std::max(peak_req_, req_bytes_);
+    }
     void evict(std::multimap<size_t, void*>::iterator it)
     {
         (void)hipFree(it->second);
@@ -140,8 +168,9 @@ private:
         free_.erase(it);
     }
     std::multimap<size_t, void*> free_;
-    std::unordered_map<void*, size_t> live_;
-    size_t bytes_ = 0, cached_ = 0, cap_ = ~size_t(0);
+    std::unordered_map<void*, std::pair<size_t, size_t>> live_;   // block -> (class bytes, requested bytes)
+    size_t bytes_ = 0, cached_ = 0, cap_ = ~size_t(0), limit_ = 0, live_bytes_ = 0, peak_live_ = 0;
+    size_t req_bytes_ = 0, peak_req_ = 0;
     int64_t allocs_ = 0, reuses_ = 0, evictions_ = 0;
 };
 
@@ -199,6 +228,12 @@ struct gpv_ctx {
 };
 
 namespace gpv {
+
+void pool_reset_peak(gpv_ctx* c)
+{
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->pool.reset_peak();
+}
 
 // One pipeline run over nm meshes; nm = 1 is a plain voxelization.  Every
 // stage is one launch (or one short launch sequence) over all meshes.  The
@@ -1312,6 +1347,41 @@ int job_run(Job* J)
     return job_finish(J);
 }
 
+// Upper bound of the bytes a one-mesh run requests from the pool (every
+// buffer it allocates, as if none were returned before the run ends), with
+// the run's own capacities for the bin and hit buffers and an estimate of
+// the boundary voxels: twice the voxels on the faces of the grid's box, to
+// allow for folds.  Exceeded only when the bins regrow or the surface holds
+// more boundary voxels than that.
+int64_t predict_run_bytes(const gpv_mesh_desc& d, const gpv_grid_info& I)
+{
+    const int64_t nx = I.n[0], ny = I.n[1], nz = I.n[2];
+    const int64_t W = nz * ny * ((nx + 63) / 64);
+    const int64_t n1 = nx * ny * nz, ncol = nx * ny;
+    const int64_t n2 = I.n2 > 0 ? I.n2 : 0;
+    const int64_t w2 = n2 > 0 ? (n2 * n2 * n2 + 63) / 64 : 0;
+    const int64_t T = d.ntris;
+    const int64_t nmax = std::max(nx, std::max(ny, nz));
+    const int64_t P = d.params.initial_capacity > 0 ? d.params.initial_capacity
+                                                    : std::max<int64_t>(int64_t(1) << 16, 16 * T + 8 * nmax * nmax);
+    const int64_t H = std::max<int64_t>(8 * T + 4 * ncol, int64_t(1) << 16);
+    const int64_t B = std::min(n1, 4 * (nx * ny + ny * nz + nz * nx));
+    auto scan = [](int64_t n) { return 8 * (int64_t)scan_tmp_elems(n); };
+    int64_t b = 0;
+    b += 4 * (2 * (nx + ny + nz) * (1 + n2) + 1);               // center tables
+    b += 36 * T + 4 * T + 4 * B + 4096;                          // soup copy, batch mesh ids, descriptors
+    b += 120 * T + 16 + 2 * scan(std::max(T, W));               // per-triangle set-up, item scans
+    b += 24 * W + 64 * W;                                        // planes, word counts and prefix, flag bytes
+    b += 24 * P + 8 * H;                                         // bin pairs, column entries, ray hits
+    b += 8 * ncol + 8 + 48 * H + scan(ncol);                     // parity fill: hits by column, records
+    b += 4 * (ncol + 1) + 48 * P + 16 * P + 64 * (ncol / 64 + 1);   // column entries by column, records
+    b += 8 * B + 4 + 4 * P + 8 * P + 16 * P + 64 * (B / 64 + 1) + scan(B);   // boundary list, bins, masks
+    b += 16 * B * w2;                                            // Level2 planes
+    b += n2 > 0 ? 8 * P * ((n2 * n2 + 63) / 64 + 1) + 12 * (B + 1) : 0;   // Level2 parity bits, SAT groups
+    if (d.params.normals) b += 12 * T + 4 * P + 12 * B + 12 * B * n2 * n2 * n2;
+    return b;
+}
+
 gpv_grid* make_view(Job* J, int m)
 {
     gpv_grid* G = new gpv_grid();
@@ -1335,6 +1405,41 @@ int check_staged(gpv_ctx* c, const gpv_grid* G)
     if (G->job->nm != 1) return fail(GPV_ERR_STATE, "stage entry points take grids from gpv_grid_create");
     if (G->job->finished) return fail(GPV_ERR_STATE, "grid is already finished");
     return GPV_OK;
+}
+
+// Meshes [lo, hi) as one run.  A run that fails on a per-run limit found
+// after Level1 (bin entries, boundary voxels, ray hits) or on device memory
+// is split in halves and each half runs on its own, down to single meshes;
+// a single mesh's failure is returned.  On failure no grid of the range is
+// left behind.
+int run_batch_range(gpv_ctx* c, const gpv_mesh_desc* d, int lo, int hi, int on_device, gpv_grid** out)
+{
+    Job* J = nullptr;
+    int r = job_create(c, d + lo, hi - lo, on_device, &J);
+    if (r == GPV_OK) {
+        r = job_run(J);
+        if (r == GPV_OK) {
+            for (int m = lo; m < hi; ++m) out[m] = make_view(J, m - lo);
+            return GPV_OK;
+        }
+        std::string keep = g_err;
+        job_free(J);
+        g_err = keep;
+    }
+    const bool split = r == GPV_ERR_OOM || r == GPV_ERR_CAPACITY || r == GPV_ERR_INVALID;
+    if (hi - lo < 2 || !split) return r;
+    const int mid = lo + (hi - lo) / 2;
+    TRY(run_batch_range(c, d, lo, mid, on_device, out));
+    r = run_batch_range(c, d, mid, hi, on_device, out);
+    if (r != GPV_OK) {
+        std::string keep = g_err;
+        for (int m = lo; m < mid; ++m) {
+            gpv_grid_release(c, out[m]);
+            out[m] = nullptr;
+        }
+        g_err = keep;
+    }
+    return r;
 }
 
 }  // namespace
@@ -1451,6 +1556,25 @@ int gpv_device_upload(gpv_ctx* c, const void* host, int64_t bytes, void** dev)
     return GPV_OK;
 }
 
+int gpv_host_alloc(int64_t bytes, void** host)
+{
+    if (!host || bytes < 0) return fail(GPV_ERR_INVALID, "bad host allocation arguments");
+    *host = nullptr;
+    const hipError_t e = hipHostMalloc(host, (size_t)std::max<int64_t>(bytes, 1), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *host = nullptr;
+        return fail(GPV_ERR_OOM, "hipHostMalloc of %lld bytes: %s", (long long)bytes, hipGetErrorString(e));
+    }
+    return GPV_OK;
+}
+
+int gpv_host_free(void* host)
+{
+    if (host) HIP_TRY(hipHostFree(host));
+    return GPV_OK;
+}
+
 int gpv_ctx_get_tune(gpv_ctx* c, gpv_tune* out)
 {
     if (!c || !out) return fail(GPV_ERR_INVALID, "NULL argument");
@@ -1536,6 +1660,23 @@ int gpv_ctx_pool_stats(gpv_ctx* c, gpv_pool_stats* out)
     if (!c || !out) return fail(GPV_ERR_INVALID, "NULL argument");
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     c->pool.stats(out);
+    return GPV_OK;
+}
+
+int gpv_predict_footprint(const gpv_mesh_desc* d, int64_t* bytes)
+{
+    if (!d || !bytes || d->ntris < 0) return fail(GPV_ERR_INVALID, "bad mesh description");
+    gpv_grid_info I;
+    TRY(derive_grid(d->bmin, d->bmax, d->max_model_size, &d->params, &I));
+    *bytes = predict_run_bytes(*d, I);
+    return GPV_OK;
+}
+
+int gpv_ctx_set_mem_limit(gpv_ctx* c, int64_t bytes)
+{
+    if (!c || bytes < 0) return fail(GPV_ERR_INVALID, "bad memory limit");
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->pool.set_limit((size_t)bytes);
     return GPV_OK;
 }
 
@@ -1783,16 +1924,63 @@ int gpv_voxelize_batch(gpv_ctx* c, const gpv_mesh_desc* meshes, int nmeshes, int
     if (c) lk_ = std::unique_lock<std::recursive_mutex>(c->mu);
     if (!c || !meshes || !out || nmeshes < 1) return fail(GPV_ERR_INVALID, "bad batch arguments");
     for (int m = 0; m < nmeshes; ++m) out[m] = nullptr;
-    Job* J = nullptr;
-    TRY(job_create(c, meshes, nmeshes, on_device, &J));
-    const int r = job_run(J);
-    if (r != GPV_OK) {
-        std::string keep = g_err;
-        job_free(J);
-        g_err = keep;
-        return r;
+    // every mesh is checked first, so that a failure below is a batch limit
+    // or memory, which splitting resolves
+    std::vector<int64_t> vox((size_t)nmeshes), cols((size_t)nmeshes), foot((size_t)nmeshes);
+    for (int m = 0; m < nmeshes; ++m) {
+        const gpv_mesh_desc& d = meshes[m];
+        if (d.ntris < 0 || (d.ntris > 0 && !d.tris)) return fail(GPV_ERR_INVALID, "mesh %d: bad triangle soup", m);
+        gpv_grid_info I;
+        TRY(derive_grid(d.bmin, d.bmax, d.max_model_size, &d.params, &I));
+        const int64_t nx = I.n[0], ny = I.n[1], nz = I.n[2];
+        if (d.ntris >= (int64_t(1) << 31) || nx * ny > (int64_t(1) << 27) ||
+            nz * ny * ((nx + 63) / 64) * 64 >= (int64_t(1) << 32))
+            return fail(GPV_ERR_INVALID, "mesh %d exceeds the limits of one run (2^31-1 triangles, 2^27 columns, "
+                        "2^32-1 padded voxels)", m);
+        vox[(size_t)m] = nz * ny * ((nx + 63) / 64) * 64;
+        cols[(size_t)m] = nx * ny;
+        TRY(gpv_predict_footprint(&d, &foot[(size_t)m]));
     }
-    for (int m = 0; m < nmeshes; ++m) out[m] = make_view(J, m);
+    HIP_TRY(hipSetDevice(c->device));
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        free_b = size_t(1) << 36;
+    }
+    gpv_pool_stats ps;
+    c->pool.stats(&ps);
+    int64_t room = (int64_t)free_b + ps.cached_bytes;
+    if (ps.limit_bytes > 0) room = std::min<int64_t>(room, ps.limit_bytes - ps.live_bytes);
+    const int64_t chunk_bytes = std::max<int64_t>(room / 2, 1);
+    // consecutive runs within the per-run limits and half the free memory
+    int lo = 0;
+    while (lo < nmeshes) {
+        int hi = lo;
+        int64_t T = 0, V = 0, C = 0, F = 0;
+        while (hi < nmeshes) {
+            const size_t k = (size_t)hi;
+            const bool first = hi == lo;
+            if (!first && (T + meshes[hi].ntris >= (int64_t(1) << 31) || V + vox[k] >= (int64_t(1) << 32) ||
+                           C + cols[k] > (int64_t(1) << 27) || F + foot[k] > chunk_bytes))
+                break;
+            T += meshes[hi].ntris;
+            V += vox[k];
+            C += cols[k];
+            F += foot[k];
+            ++hi;
+        }
+        const int r = run_batch_range(c, meshes, lo, hi, on_device, out);
+        if (r != GPV_OK) {
+            std::string keep = g_err;
+            for (int m = 0; m < lo; ++m) {
+                gpv_grid_release(c, out[m]);
+                out[m] = nullptr;
+            }
+            g_err = keep;
+            return r;
+        }
+        lo = hi;
+    }
     return GPV_OK;
 }
 

@@ -35,6 +35,9 @@ struct Req {
     std::string err;
     gpv_grid* grid = nullptr;
     int lane = -1, batch_id = -1, batch_size = 0;
+    int retries = 0;                 // re-admissions after a failed batch
+    int64_t batch_predicted = 0;     // predicted bytes of its batch
+    int64_t batch_peak = 0;          // measured peak pool bytes of its batch
 };
 
 int err(int code, const char* fmt, ...)
@@ -69,7 +72,8 @@ struct gpv_server {
     std::deque<Req*> pending;
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
     uint64_t next_id = 1;
-    int64_t charged = 0, peak = 0, completed = 0, batches = 0;
+    int64_t charged = 0, peak = 0, completed = 0, batches = 0, readmitted = 0;
+    int running = 0;   // batches on the device now
     bool stop = false;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
@@ -93,13 +97,27 @@ void gpv_server::lane_main(int lane)
             for (;;) {
                 if (stop && pending.empty()) return;
                 batch.clear();
-                if (!pending.empty()) {
+                if (!pending.empty() && pending.front()->retries > 0) {
+                    // a request of a failed batch runs alone, with no other
+                    // batch on the device, so it has the most memory there is
+                    if (running == 0) {
+                        Req* r = pending.front();
+                        flag = r->on_device;
+                        charged += r->predicted;
+                        peak = std::max(peak, charged);
+                        r->t_admit = now();
+                        r->state = kRunning;
+                        r->lane = lane;
+                        batch.push_back(r);
+                        pending.pop_front();
+                    }
+                } else if (!pending.empty()) {
                     // arrival order; a request that does not fit ends the batch
                     flag = pending.front()->on_device;
                     int64_t tris = 0, vox = 0;
                     while (!pending.empty()) {
                         Req* r = pending.front();
-                        if (r->on_device != flag || (int)batch.size() >= o.max_batch_meshes) break;
+                        if (r->on_device != flag || (int)batch.size() >= o.max_batch_meshes || r->retries > 0) break;
                         if (!batch.empty() &&
                             (tris + r->d.ntris > o.max_batch_tris || vox + r->voxels >= (int64_t(1) << 32)))
                             break;
@@ -120,12 +138,25 @@ void gpv_server::lane_main(int lane)
                 cv_work.wait(lk);
             }
             bid = (int)batches++;
+            ++running;
         }
         const int n = (int)batch.size();
         descs.resize((size_t)n);
         grids.assign((size_t)n, nullptr);
-        for (int i = 0; i < n; ++i) descs[(size_t)i] = batch[(size_t)i]->d;
+        int64_t predicted = 0;
+        for (int i = 0; i < n; ++i) {
+            descs[(size_t)i] = batch[(size_t)i]->d;
+            predicted += batch[(size_t)i]->predicted;
+        }
+        // the batch's own peak: bytes requested from the pool above what the
+        // lane held when it started (earlier results stay resident until
+        // released), the quantity gpv_predict_footprint bounds
+        gpv_pool_stats p0{}, p1{};
+        gpv::pool_reset_peak(c);
+        gpv_ctx_pool_stats(c, &p0);
         const int st = gpv_voxelize_batch(c, descs.data(), n, flag, grids.data());
+        gpv_ctx_pool_stats(c, &p1);
+        const int64_t measured = p1.peak_requested_bytes - p0.requested_bytes;
         const std::string msg = st == GPV_OK ? std::string() : std::string(gpv_last_error());
         std::vector<int64_t> res((size_t)n, 0);
         if (st == GPV_OK)
@@ -137,24 +168,37 @@ void gpv_server::lane_main(int lane)
         const double t = now();
         {
             std::lock_guard<std::mutex> lk(mu);
-            for (int i = 0; i < n; ++i) {
+            --running;
+            for (int i = n - 1; i >= 0; --i) {
                 Req* r = batch[(size_t)i];
-                r->t_done = t;
-                r->status = st;
                 r->batch_id = bid;
                 r->batch_size = n;
+                r->batch_predicted = predicted;
+                r->batch_peak = measured;
+                charged -= r->predicted;
                 if (st == GPV_OK) {
+                    r->t_done = t;
+                    r->status = st;
                     r->grid = grids[(size_t)i];
                     r->resident = res[(size_t)i];
-                    charged += r->resident - r->predicted;
+                    charged += r->resident;
                     r->state = kDone;
+                    ++completed;
+                } else if (r->retries == 0) {
+                    // re-admitted at the front, in arrival order, one at a time
+                    r->retries = 1;
+                    r->state = kPending;
+                    r->lane = -1;
+                    pending.push_front(r);
+                    ++readmitted;
                 } else {
-                    charged -= r->predicted;
+                    r->t_done = t;
+                    r->status = st;
                     r->err = msg;
                     r->state = kFailed;
+                    ++completed;
                 }
             }
-            completed += n;
         }
         cv_done.notify_all();
         cv_work.notify_all();
@@ -162,35 +206,6 @@ void gpv_server::lane_main(int lane)
 }
 
 extern "C" {
-
-int gpv_predict_footprint(const gpv_mesh_desc* d, int64_t* bytes)
-{
-    if (!d || !bytes || d->ntris < 0) return err(GPV_ERR_INVALID, "bad mesh description");
-    gpv_grid_info I;
-    const int st = gpv_grid_derive(d->bmin, d->bmax, d->max_model_size, &d->params, &I);
-    if (st != GPV_OK) return st;
-    const int64_t nx = I.n[0], ny = I.n[1], nz = I.n[2];
-    const int64_t nxw = (nx + 63) / 64;
-    const int64_t W = nz * ny * nxw;
-    const int64_t n1 = nx * ny * nz;
-    const int64_t w2 = I.n2 > 0 ? ((int64_t)I.n2 * I.n2 * I.n2 + 63) / 64 : 0;
-    const int64_t T = d->ntris;
-    const int64_t nmax = std::max(nx, std::max(ny, nz));
-    // boundary voxels of a surface spanning the bbox, twice over for folds
-    const int64_t b_est = std::min(n1, 4 * (nx * ny + ny * nz + nz * nx));
-    const int64_t pairs = std::max<int64_t>(int64_t(1) << 16, 16 * T + 8 * nmax * nmax);
-    const int64_t hits = 8 * T + 4 * nx * ny;
-    const int64_t tables = 4 * (2 * (nx + ny + nz) * (1 + (I.n2 > 0 ? I.n2 : 0)));
-    *bytes = 36 * T + 80 * T            // soup copy and per-triangle set-up
-             + 24 * W                   // boundary, inside, word counts, word prefix
-             + 64 * W                   // boundary flag bytes of the Level1 SAT (transient)
-             + 80 * pairs               // bin pairs, column entries, ranks, lists, column records
-             + 56 * hits                // ray hits and their records
-             + 16 * nx * ny             // column counts and offsets
-             + b_est * (16 + 16 * w2)   // boundary list, bins, Level2 planes
-             + tables;
-    return GPV_OK;
-}
 
 int gpv_server_create(const gpv_server_opts* opts, gpv_server** out)
 {
@@ -205,7 +220,11 @@ int gpv_server_create(const gpv_server_opts* opts, gpv_server** out)
     if (S->o.max_batch_tris <= 0) S->o.max_batch_tris = int64_t(1) << 27;
     for (int i = 0; i < S->o.lanes; ++i) {
         gpv_ctx* c = nullptr;
-        const int st = gpv_ctx_create(S->o.device, 0, &c);
+        int st = gpv_ctx_create(S->o.device, 0, &c);
+        if (st == GPV_OK && S->o.lane_mem_limit > 0) {
+            st = gpv_ctx_set_mem_limit(c, S->o.lane_mem_limit);
+            if (st != GPV_OK) gpv_ctx_destroy(c);
+        }
         if (st != GPV_OK) {
             for (gpv_ctx* p : S->ctx) gpv_ctx_destroy(p);
             delete S;
@@ -283,6 +302,9 @@ int gpv_server_wait(gpv_server* S, uint64_t id, double timeout_s, gpv_grid** gri
         stats->batch_size = r->batch_size;
         stats->lane = r->lane;
         stats->status = r->status;
+        stats->retries = r->retries;
+        stats->batch_predicted_bytes = r->batch_predicted;
+        stats->batch_peak_bytes = r->batch_peak;
     }
     if (r->state == kFailed) return err(r->status, "request %llu: %s", (unsigned long long)id, r->err.c_str());
     if (grid) *grid = r->grid;
@@ -327,6 +349,14 @@ int gpv_server_counters(gpv_server* S, int64_t* completed, int64_t* batches, int
     return GPV_OK;
 }
 
+int gpv_server_readmitted(gpv_server* S, int64_t* readmitted)
+{
+    if (!S || !readmitted) return err(GPV_ERR_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    *readmitted = S->readmitted;
+    return GPV_OK;
+}
+
 int gpv_server_pool_stats(gpv_server* S, gpv_pool_stats* out)
 {
     if (!S || !out) return err(GPV_ERR_INVALID, "NULL argument");
@@ -341,6 +371,11 @@ int gpv_server_pool_stats(gpv_server* S, gpv_pool_stats* out)
         sum.allocations += p.allocations;
         sum.reuses += p.reuses;
         sum.evictions += p.evictions;
+        sum.live_bytes += p.live_bytes;
+        sum.peak_live_bytes += p.peak_live_bytes;
+        sum.limit_bytes += p.limit_bytes;
+        sum.requested_bytes += p.requested_bytes;
+        sum.peak_requested_bytes += p.peak_requested_bytes;
     }
     *out = sum;
     return GPV_OK;

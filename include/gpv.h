@@ -107,6 +107,11 @@ const char *gpv_status_string(int status);
  * them to gpv_voxelize with tris_on_device = 1. */
 int gpv_device_upload(gpv_ctx *ctx, const void *host, int64_t bytes, void **dev);
 int gpv_device_free(gpv_ctx *ctx, void *dev);
+/* Page-locked host memory (hipHostMalloc) for soups that are uploaded
+ * often: copies from it run at the link's full rate and overlap the
+ * device's work.  No counterpart in the reference. */
+int gpv_host_alloc(int64_t bytes, void **host);
+int gpv_host_free(void *host);
 
 /* The context's device memory pool.  No counterpart in the reference, which
  * cudaMallocs and frees around every call; serving loops read it to see that
@@ -121,8 +126,18 @@ typedef struct gpv_pool_stats {
     int64_t allocations;    /* hipMalloc calls so far */
     int64_t reuses;         /* requests served from cached blocks */
     int64_t evictions;      /* cached blocks freed (cap, out of memory, destroy) */
+    int64_t live_bytes;     /* block bytes in use now (size classes, results included) */
+    int64_t peak_live_bytes;/* the most block bytes in use at once so far */
+    int64_t limit_bytes;    /* gpv_ctx_set_mem_limit, 0 = none */
+    int64_t requested_bytes;      /* bytes the blocks in use were requested for */
+    int64_t peak_requested_bytes; /* the most requested bytes in use at once so far */
 } gpv_pool_stats;
 int gpv_ctx_pool_stats(gpv_ctx *ctx, gpv_pool_stats *out);
+/* Caps the device bytes a context's pool may have in use at once (0 = no
+ * cap): an allocation beyond it fails with GPV_ERR_OOM as if the device were
+ * that size, so contexts can share a GPU with other tenants.  No counterpart
+ * in the reference. */
+int gpv_ctx_set_mem_limit(gpv_ctx *ctx, int64_t bytes);
 
 /* Launch shapes of a context, picked by gpv_ctx_autotune or set by hand.  No
  * counterpart in the reference (its launch shapes are fixed,
@@ -225,8 +240,13 @@ typedef struct gpv_mesh_desc {
  * device_views).  The soups are copied into the batch, so the inputs may be
  * reused on return.  Per-mesh ms_stage/ms_total report the batch's stage
  * times.  The batch's device memory returns to the context's pool when the
- * last of its grids is released.  Limits per batch: 2^31-1 triangles,
- * 2^32-1 Level1 voxels (rows padded to 64), 2^32-1 bin entries. */
+ * last of its grids is released.  One run holds at most 2^31-1 triangles,
+ * 2^32-1 padded Level1 voxels (rows padded to 64), 2^27 xy columns, 2^27
+ * boundary voxels and 2^32-1 bin entries: a batch beyond the first three, or
+ * predicted to need more than half the free device memory, is cut into
+ * consecutive runs before it starts, and a run that fails on the others or
+ * on device memory is split in halves and rerun, down to single meshes.
+ * Each mesh must fit one run by itself; on failure no grid is returned. */
 int gpv_voxelize_batch(gpv_ctx *ctx, const gpv_mesh_desc *meshes, int nmeshes, int tris_on_device,
                        gpv_grid **out);
 

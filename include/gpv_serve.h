@@ -20,6 +20,11 @@
  *    has run, the charge drops to the bytes the resident result really holds,
  *    and it is returned by gpv_server_release.  A request larger than the
  *    whole budget is admitted alone when nothing else is charged.
+ *  - A batch that fails (device memory short of the prediction, a per-run
+ *    limit) is not lost: its requests go back to the front of the queue and
+ *    each runs alone, when no other batch is on the device; only a request
+ *    that fails alone is reported failed.  This is the serving analogue of
+ *    the reference's rerun with a larger bin buffer (src/Object.cpp:3198-3214).
  *  - The caller keeps each request's soup valid until the request is done.
  */
 #ifndef GPV_SERVE_H
@@ -41,6 +46,8 @@ typedef struct gpv_server_opts {
     int64_t hbm_budget;       /* admission budget in bytes; <= 0 picks 200e9 */
     int max_batch_meshes;     /* <= 0 picks 256 */
     int64_t max_batch_tris;   /* <= 0 picks 2^27 */
+    int64_t lane_mem_limit;   /* > 0: each lane's context may have at most this many
+                                 device bytes in use (gpv_ctx_set_mem_limit) */
 } gpv_server_opts;
 
 typedef struct gpv_request_stats {
@@ -49,9 +56,18 @@ typedef struct gpv_request_stats {
     int64_t resident_bytes;             /* device bytes the result holds */
     int32_t batch_id, batch_size, lane;
     int32_t status;                     /* GPV_OK or the batch's error */
+    int32_t retries;                    /* 1 if it was re-admitted after its batch failed */
+    int64_t batch_predicted_bytes;      /* sum of the predicted bytes of its batch */
+    int64_t batch_peak_bytes;           /* measured: the most bytes its batch had requested
+                                           from the lane's pool at once, above what the lane
+                                           held when it started */
 } gpv_request_stats;
 
-/* Predicted device bytes of voxelizing one mesh (peak, transient included). */
+/* Predicted device bytes of voxelizing one mesh: every buffer its run
+ * requests, transient ones included, with the run's own bin and hit
+ * capacities and a boundary estimate of twice the voxels on the grid box's
+ * faces.  An upper bound of the run's peak of requested bytes unless the
+ * bins regrow or the surface has more boundary voxels than that. */
 int gpv_predict_footprint(const gpv_mesh_desc *mesh, int64_t *bytes);
 
 int gpv_server_create(const gpv_server_opts *opts, gpv_server **out);
@@ -71,6 +87,8 @@ int gpv_server_release(gpv_server *srv, uint64_t id);
 /* Totals: requests completed, batches run, bytes charged now, peak charge. */
 int gpv_server_counters(gpv_server *srv, int64_t *completed, int64_t *batches, int64_t *charged,
                         int64_t *peak_charged);
+/* Requests re-admitted to run alone after their batch failed. */
+int gpv_server_readmitted(gpv_server *srv, int64_t *readmitted);
 /* The lanes' device memory pools, summed (gpv_ctx_pool_stats per lane). */
 int gpv_server_pool_stats(gpv_server *srv, gpv_pool_stats *out);
 

@@ -677,7 +677,7 @@ typedef struct {
     int64_t nx, ny, nz, n1;
     float *cx, *cy, *cz;     /* L1 SAT centers */
     float *rx, *ry, *rz;     /* L1 ray origins */
-    int mode;                /* ORC_EXACT: literal loops; ORC_FAST: the result-identical forms */
+    int mode;                /* ORC_EXACT: literal loops; ORC_FAST, ORC_PLAN: result-identical forms */
 } ctx_t;
 
 /* Level1 ray parity over ALL triangles (src/Object.cpp:716-779), literal. */
@@ -996,7 +996,7 @@ static void l2_parity_fast(const ctx_t *c, const orc_result *r, const float *ox,
                         }
                         if (lend == 0) continue;
                         int dir = 0;
-                        if (zmono && mt_t_monotone(&hs[h], ozl[0], ozl[lend - 1], &dir)) {
+                        if (c->mode == ORC_FAST && zmono && mt_t_monotone(&hs[h], ozl[0], ozl[lend - 1], &dir)) {
                             int64_t a = 0, bb = lend;
                             if (dir <= 0) {   /* passes on [0, first failure) */
                                 while (a < bb) {

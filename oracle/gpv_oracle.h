@@ -65,7 +65,7 @@ typedef struct orc_result {
     double t_stage[5];    /* seconds: L1 parity, L1 SAT, compaction, L2 parity, L2 SAT */
 } orc_result;
 
-enum { ORC_EXACT = 0, ORC_FAST = 1 };
+enum { ORC_EXACT = 0, ORC_FAST = 1, ORC_PLAN = 2 };
 
 /* Reference geometric predicates, restated. */
 int orc_tri_box_overlap(const float center[3], const float half[3], const float tv[9]);
@@ -109,9 +109,11 @@ int orc_ray_column_bounds(const float tv[9], const float bmin[3], const float bm
 
 /* Full pipeline.  mode ORC_EXACT runs the literal loops (all-triangle
  * parity, every box of every index AABB and bin through the full SAT);
- * ORC_FAST runs the result-identical forms: the column-factorised parity and
- * the SAT sweeps screened by orc_tri_box_factored's predicate groups.
- * threads<=0: all. */
+ * ORC_FAST runs the result-identical forms: the column-factorised parity
+ * (with monotone layer runs at Level2) and the SAT sweeps screened by
+ * orc_tri_box_factored's predicate groups; ORC_PLAN is the CPU baseline's
+ * "fast-identical" composition: the column-factorised parity with a t test
+ * per layer, and the literal SAT loops.  threads<=0: all. */
 int orc_voxelize(const float *tris, int64_t ntris, const orc_grid *g, int mode,
                  int threads, orc_result *out);
 void orc_result_free(orc_result *r);

@@ -1,4 +1,4 @@
-"""Pins the oracle's fast mode against its literal mode.
+"""Pins the oracle's fast modes against its literal mode.
 
 ORC_FAST is what the GPU parity tests compare with at large sizes, so every
 shortcut it takes must give the literal loops' result:
@@ -13,7 +13,9 @@ shortcut it takes must give the literal loops' result:
   each candidate's passing layers by a binary search when t is provably
   monotone along the column, one layer at a time otherwise.
 
-The whole pipeline in both modes must agree on every output (states,
+ORC_PLAN, the CPU baseline's composition (column-factorised parity with a
+t test per layer, literal SAT loops), is checked the same way.  The whole
+pipeline in every mode must agree on every output (states,
 prefix, boundary list, bins, counts) for meshes that exercise all of it:
 lattice-snapped, sliver and face-coplanar triangles at Level2 up to 16, the
 reference's sample mesh, and the CUDA-kernel Level2 candidate rule.
@@ -83,13 +85,14 @@ def test_fast_equals_exact_pipeline(gv, orc, cuda_l2):
         bmin, bmax, mms = gv.mesh_bbox(v)
         g = orc.grid_setup(bmin, bmax, mms, vc, vc2, cuda_l2=cuda_l2)
         a = orc.voxelize(soup, g, orc.ORC_EXACT)
-        b = orc.voxelize(soup, g, orc.ORC_FAST)
         assert a["nb"] > 50, name
-        for k in ("l1", "prefix", "bidx", "l2", "list_off", "list_tri"):
-            assert np.array_equal(a[k], b[k]), f"{name}: {k} differs ({np.count_nonzero(a[k] != b[k])})"
-        for k in ("pairs", "tests_l1", "tests_l2", "l1_inside", "l1_boundary", "l2_inside", "l2_boundary",
-                  "max_tris_per_voxel"):
-            assert a[k] == b[k], f"{name}: {k} {a[k]} != {b[k]}"
+        for mode in (orc.ORC_FAST, orc.ORC_PLAN):
+            b = orc.voxelize(soup, g, mode)
+            for k in ("l1", "prefix", "bidx", "l2", "list_off", "list_tri"):
+                assert np.array_equal(a[k], b[k]), f"{name} mode {mode}: {k} differs ({np.count_nonzero(a[k] != b[k])})"
+            for k in ("pairs", "tests_l1", "tests_l2", "l1_inside", "l1_boundary", "l2_inside", "l2_boundary",
+                      "max_tris_per_voxel"):
+                assert a[k] == b[k], f"{name} mode {mode}: {k} {a[k]} != {b[k]}"
         assert 0 < a["l2_inside"] and 0 < a["l2_boundary"], name
 
 
