@@ -42,8 +42,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # FP32 vector issue without packed instructions (the library is built with
-# packed FP32 off): 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-operations/s
-VALU_PEAK_OPS = 39.3e12
+# packed FP32 off), measured on the library's probe kernel of known VALU
+# count (bench.py --valu-probe; one wave64 instruction per 2 cycles per SIMD);
+# the committed counter summary carries the same figure
+VALU_PEAK_OPS = 77.1e12
 STAGES = ["setup", "l1_sat", "l1_parity", "compaction_bins", "l2_parity", "l2_sat"]
 TESTS_BASIS = ("reference loop counts: sum of index-AABB volumes (Level1) + |bin| * n2^3 (Level2); "
                "the kernels run a pruned subset with the same outcome")
@@ -135,10 +137,9 @@ def bench_stress(gv, ctx, n_meshes, per_run, check_members):
     """Stress members 0..n_meshes-1 (100k-2M triangles, lattice-snapped /
     sliver / face-coplanar) at Level1 512 / Level2 16: gpv_voxelize_batch
     over runs of per_run meshes from pinned host soups, grids released after
-    each run; the plain path mesh by mesh beside it; and the oracle on the
-    sampled members."""
+    each run; the plain path mesh by mesh beside it.  Returns the summary and
+    the batched results of the sampled members (for the CPU leg's check)."""
     from gpview_amd import workloads
-    from oracle import orc
     t0 = time.perf_counter()
     meshes = [workloads.stress_member(gv, i) for i in range(n_meshes)]
     t_gen = time.perf_counter() - t0
@@ -161,18 +162,12 @@ def bench_stress(gv, ctx, n_meshes, per_run, check_members):
     t_plain = time.perf_counter() - t0
     tests = sum(i["tests_l1"] + i["tests_l2"] for i in infos)
     voxels = sum(i["n1"] + i["nb"] * 16 ** 3 for i in infos)
-    checked = []
+    samples = []
     for k in check_members:
-        m = host[k]
-        g = orc.grid_setup(m["bmin"], m["bmax"], m["mms"], 512, 16)
-        want = orc.voxelize(m["soup"], g, orc.ORC_FAST, 0, bins=False)
         gs = ctx.voxelize_batch(host[(k // per_run) * per_run:(k // per_run + 1) * per_run])
-        got = gs[k % per_run].download()
+        samples.append((k, meshes[k], gs[k % per_run].download()))
         for x in gs:
             x.release()
-        ok = all(np.array_equal(got[key], want[key]) for key in ("l1", "prefix", "bidx", "l2"))
-        checked.append({"member": k, "triangles": len(m["soup"]), "nb": int(want["nb"]), "bit_exact": ok})
-        del want, got
     for p in pinned:
         p.close()
     return {
@@ -183,9 +178,10 @@ def bench_stress(gv, ctx, n_meshes, per_run, check_members):
         "batched_s": t_batch, "plain_s": t_plain,
         "batched_tri_box_tests_per_s": tests / t_batch, "batched_voxels_per_s": voxels / t_batch,
         "boundary_voxels": int(sum(i["nb"] for i in infos)),
-        "oracle_sample": checked, "generation_s": t_gen,
-        "note": "soups uploaded from pinned host memory inside the timed region; grids released after each run",
-    }
+        "generation_s": t_gen,
+        "note": "soups uploaded from pinned host memory inside the timed region; grids released after each run; "
+                "sampled members are checked against the oracle in cpu_baseline.stress_sample",
+    }, samples
 
 
 def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48, plain_sample=300):
@@ -300,6 +296,24 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": a.numel() * 4}), flush=True)
 
 
+def valu_probe():
+    """The FP32 issue rate measured on a kernel of known VALU instruction
+    count (gpv_valu_probe): wave instructions / device time, x 64 lanes."""
+    import gpview_amd as gv
+    out = {"note": "gpv_valu_probe: 256-thread blocks, 32 v_add_f32 per loop iteration per thread"}
+    with gv.Context(0) as c:
+        for blocks, iters in ((4096, 4096), (16384, 2048), (65536, 512)):
+            c.valu_probe(blocks, iters)
+            best = None
+            for _ in range(3):
+                ms, wi = c.valu_probe(blocks, iters)
+                best = ms if best is None or ms < best else best
+            out[f"{blocks}x{iters}"] = {"ms": best, "wave_insts": wi, "wave_insts_per_s": wi / (best / 1e3),
+                                        "lane_ops_per_s": wi * 64 / (best / 1e3)}
+    out["peak_lane_ops_per_s"] = max(v["lane_ops_per_s"] for k, v in out.items() if isinstance(v, dict))
+    print(json.dumps(out), flush=True)
+
+
 def kernel_bytes(stage, info, T):
     """Algorithmic HBM bytes of one launch of the stage's main kernel."""
     B, n2, P = info["nb"], info["n2"], info["pairs"]
@@ -315,11 +329,13 @@ def kernel_bytes(stage, info, T):
     return 36 * T + 2 * ((info["n1"] + 7) // 8)
 
 
-def cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads):
+def cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads, mixed=None, stress_samples=()):
     """The oracle on this host: the CPU baseline's composition on the full
-    headline mesh with all cores (value) and one thread, and the literal
+    headline mesh with all cores (value) and one thread; the literal
     composition (all-triangle parity) against it on the 5,120-triangle
-    icosphere."""
+    icosphere; the baseline composition on a stated subset of the mixed
+    stream; and the oracle's fast mode on sampled stress members, whose
+    results double as the bit-exact check of the GPU's batched grids."""
     from gpview_amd import workloads
     g0 = orc.grid_setup(bmin, bmax, mms, 256, 8)
 
@@ -337,6 +353,25 @@ def cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads):
         for th in (threads, 1):
             t, _ = run(s["soup"], g1, mode, th)
             legs[f"icosphere_5120_{name}_{th}t"] = {"seconds": t, "meshes_per_s": 1.0 / t, "cores": th}
+    mixed_leg = None
+    if mixed:
+        c0 = time.perf_counter()
+        for m in mixed:
+            gm = orc.grid_setup(m["bmin"], m["bmax"], m["mms"], m["voxel_count"], m["voxel_count2"])
+            orc.voxelize(m["soup"], gm, orc.ORC_PLAN, threads, bins=False)
+        tm = time.perf_counter() - c0
+        mixed_leg = {"meshes": len(mixed), "triangles": int(sum(len(m["soup"]) for m in mixed)),
+                     "seconds": tm, "meshes_per_s": len(mixed) / tm, "cores": threads,
+                     "sample": f"members 0-{len(mixed) - 1} of the 256-mesh mixed stream, baseline composition"}
+    stress_leg = []
+    for k, m, got in stress_samples:
+        gs = orc.grid_setup(m["bmin"], m["bmax"], m["mms"], 512, 16)
+        ts, ws = run(m["soup"], gs, orc.ORC_FAST, threads)
+        ok = all(np.array_equal(got[key], ws[key]) for key in ("l1", "prefix", "bidx", "l2"))
+        stress_leg.append({"member": k, "triangles": len(m["soup"]), "nb": int(ws["nb"]), "seconds": ts,
+                           "cores": threads, "mode": "oracle fast mode (factored SAT sweeps)",
+                           "gpu_batched_bit_exact": ok})
+        del ws
     tests = want["tests_l1"] + want["tests_l2"]
     out = {"value": 1.0 / t_all, "unit": "meshes/s", "cores": threads, "kind": "port",
            "sample": "the full headline mesh (1 mesh): oracle in the baseline composition (column-factorised "
@@ -345,6 +380,8 @@ def cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads):
            "single_thread": {"value": 1.0 / t_one, "seconds": t_one, "cores": 1,
                              "tri_box_tests_per_s": tests / t_one},
            "icosphere_5120_64_4": legs,
+           "mixed_subset": mixed_leg,
+           "stress_sample": stress_leg,
            "stage_s": dict(zip(["l1_parity", "l1_sat", "compaction", "l2_parity", "l2_sat"], want["t_stage"]))}
     return out, want
 
@@ -368,9 +405,14 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy that calibrates the traffic counters")
     ap.add_argument("--serve-only", action="store_true", help="only the served stream; one JSON line")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only measure the FP32 VALU issue rate on the library's probe kernel")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.valu_probe:
+        valu_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (meshes shard "
@@ -431,11 +473,12 @@ def main():
             gv, ctx, workloads.mixed_stream(gv, args.batch_meshes, seed=7, tri_range=(1000, 20000), level1=(64,),
                                             level2=(4,)), args.batch_steps, 1, False,
             "small-mesh stream (log-uniform 1k-20k triangles, Level1 64, Level2 4)")
+    stress_samples = []
     if args.stress_meshes > 0:
         n = min(args.stress_meshes, 64)
         sizes = [workloads.stress_spec(i)[1] for i in range(n)]
-        sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes))}) if not args.no_check else []
-        stress = bench_stress(gv, ctx, n, args.stress_run, sample)
+        sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes))}) if not args.no_cpu else []
+        stress, stress_samples = bench_stress(gv, ctx, n, args.stress_run, sample)
     ctx.close()
     served_stream, served_sat = served()
 
@@ -470,7 +513,10 @@ def main():
     if not args.no_cpu:
         from oracle import orc
         threads = args.cpu_threads or os.cpu_count()
-        cpu, want = cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads)
+        cpu, want = cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads,
+                                 mixed=workloads.mixed_stream(gv, 16) if args.batch_meshes > 0 else None,
+                                 stress_samples=stress_samples)
+        stress_samples = None
         exact = all(np.array_equal(got[k], want[k]) for k in ("l1", "prefix", "bidx", "l2"))
 
     out = {

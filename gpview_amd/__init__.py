@@ -112,7 +112,7 @@ EXPORTED = [
     "gpv_ctx_create", "gpv_ctx_destroy", "gpv_last_error", "gpv_status_string",
     "gpv_device_upload", "gpv_device_free", "gpv_host_alloc", "gpv_host_free", "gpv_ctx_pool_stats", "gpv_ctx_set_mem_limit", "gpv_ctx_get_tune",
     "gpv_ctx_set_tune",
-    "gpv_ctx_autotune",
+    "gpv_ctx_autotune", "gpv_valu_probe",
     "gpv_mesh_bbox", "gpv_grid_derive", "gpv_grid_centers", "gpv_voxelize", "gpv_voxelize_noalloc",
     "gpv_voxelize_batch",
     "gpv_grid_create",
@@ -151,6 +151,7 @@ def lib():
     L.gpv_host_free.argtypes = [_vp]
     L.gpv_ctx_pool_stats.argtypes = [_vp, P(PoolStats)]
     L.gpv_ctx_set_mem_limit.argtypes = [_vp, c_i64]
+    L.gpv_valu_probe.argtypes = [_vp, c_int, c_int, P(c_float), P(ctypes.c_double)]
     L.gpv_server_readmitted.argtypes = [_vp, P(c_i64)]
     L.gpv_ctx_get_tune.argtypes = [_vp, P(Tune)]
     L.gpv_ctx_set_tune.argtypes = [_vp, P(Tune)]
@@ -356,6 +357,13 @@ class Context:
     def set_mem_limit(self, nbytes):
         """gpv_ctx_set_mem_limit: cap on the pool's bytes in use (0: none)."""
         _check(lib().gpv_ctx_set_mem_limit(self.h, int(nbytes)), "gpv_ctx_set_mem_limit")
+
+    def valu_probe(self, blocks, iters):
+        """gpv_valu_probe: (device ms, VALU wave instructions) of the probe."""
+        ms, wi = ctypes.c_float(), ctypes.c_double()
+        _check(lib().gpv_valu_probe(self.h, int(blocks), int(iters), ctypes.byref(ms), ctypes.byref(wi)),
+               "gpv_valu_probe")
+        return ms.value, wi.value
 
     def get_tune(self):
         """gpv_ctx_get_tune as a dict."""

@@ -24,6 +24,7 @@ SOURCES = [
     "gpv_mesh.cpp",
     "gpv_object.cpp",
     "gpv_serve.cpp",
+    "gpv_probe.hip",
 ]
 
 COMMON = [

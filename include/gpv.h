@@ -156,6 +156,14 @@ typedef struct gpv_tune {
 int gpv_ctx_get_tune(gpv_ctx *ctx, gpv_tune *out);
 int gpv_ctx_set_tune(gpv_ctx *ctx, const gpv_tune *t);
 
+/* Measurement aid, no counterpart in the reference: runs `blocks` blocks of
+ * 256 threads, each thread doing iters * 32 dependent-free FP32 adds
+ * (v_add_f32, no packed forms), and returns the device time of one timed
+ * launch and the number of VALU wave instructions its loop issues.  Their
+ * ratio x 64 lanes is the FP32 vector issue rate the VALU roofline of the
+ * path's kernels is measured against. */
+int gpv_valu_probe(gpv_ctx *ctx, int blocks, int iters, float *ms, double *wave_insts);
+
 /* ---- host-side geometry set-up ----------------------------------------- */
 /* bbox over vertices + 0.1% pad + maxModelSize: Object::ReadObject
  * (src/Object.cpp:441-450, 572-583).  verts: nverts*3 floats (host). */
