@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgpv.so")
+# the in-tree build; GPV_LIBRARY names another build of the same sources
+LIB_PATH = os.environ.get("GPV_LIBRARY") or os.path.join(HERE, "libgpv.so")
 
 GPV_OK, GPV_ERR_INVALID, GPV_ERR_OOM, GPV_ERR_CAPACITY, GPV_ERR_HIP, GPV_ERR_NO_DEVICE, GPV_ERR_IO, \
     GPV_ERR_STATE, GPV_ERR_TOO_LARGE = range(9)
@@ -68,7 +69,9 @@ class PoolStats(ctypes.Structure):
 class Tune(ctypes.Structure):
     """gpv_tune (include/gpv.h)."""
     _fields_ = [("expand_blocks_per_cu", ctypes.c_int), ("l2_group_below", ctypes.c_int),
-                ("sort_items_per_block", ctypes.c_int), ("l2_overlap", ctypes.c_int)]
+                ("sort_items_per_block", ctypes.c_int), ("l2_overlap", ctypes.c_int),
+                ("l2_sat_waves", ctypes.c_int), ("l2_parity_waves", ctypes.c_int),
+                ("expand_stage_tris", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -44,16 +44,19 @@ def hipcc():
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libgpv")
 
 
-def build(verbose=False):
+def build(verbose=False, out=None, defines=(), objdir=None):
+    """Compiles the library to `out` (default gpview_amd/libgpv.so); defines
+    are extra -D macros for alternative builds of the same sources."""
     cc = hipcc()
-    objdir = os.path.join(HERE, "_build")
+    objdir = objdir or os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
+    extra = ["-D" + d for d in defines]
     cmds = []
     objs = []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(objdir, src + ".o")
-        cmds.append([cc, "-x", "hip", "-c", path, "-o", obj, "-pthread"] + COMMON)
+        cmds.append([cc, "-x", "hip", "-c", path, "-o", obj, "-pthread"] + COMMON + extra)
         objs.append(obj)
     from concurrent.futures import ThreadPoolExecutor
 
@@ -64,13 +67,14 @@ def build(verbose=False):
 
     with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1, 8)) as ex:
         list(ex.map(run, cmds))
-    tmp = LIB + ".tmp"
+    lib = out or LIB
+    tmp = lib + ".tmp"
     cmd = [cc, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC", "-pthread"]
     if verbose:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def disassemble():

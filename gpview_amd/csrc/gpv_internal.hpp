@@ -111,13 +111,14 @@ void launch_scan_u32_u64(const uint32_t* in, uint64_t* out, int64_t n, uint64_t*
 void launch_scan_u32_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, unsigned int* max_out,
                          hipStream_t s);
 
+// stage: triangles staged per LDS pass (128 or 256)
 void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const double4* slab1,
                        const uint64_t* off, const GridDev& g0, const Meshes& M, uint8_t* bbytes, Append64 pairs,
-                       Append64 cols, int blocks, hipStream_t s);
+                       Append64 cols, int blocks, int stage, hipStream_t s);
 // boundary bytes (W * 64) -> boundary bitplane (W words)
 void launch_pack_bytes(const uint8_t* by, int64_t W, uint64_t* bits, hipStream_t s);
 void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off, const GridDev& g0,
-                       const Meshes& M, Append64 hits, int blocks, hipStream_t s);
+                       const Meshes& M, Append64 hits, int blocks, int stage, hipStream_t s);
 
 // counting sort helpers (keys hold a batch-wide column index in the high word)
 void launch_count_keys_hi(const uint64_t* key, uint64_t n, uint32_t* cnt, hipStream_t s);
@@ -176,18 +177,19 @@ void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStrea
 // (gidx[B] = group count), gstart[B+1]
 void launch_l2_groups(const uint32_t* boff, int64_t B, const GridDev& g0, const Meshes& M, uint32_t* flag,
                       uint32_t* gidx, uint32_t* scan_tmp, uint32_t* gstart, hipStream_t s);
+// gmax: an upper bound of the group count (grouped form); waves: per block (1, 2, 4)
 void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
                    const uint64_t* pmask, const float4* slab2, const uint32_t* gstart, const uint32_t* ngroups,
-                   int blocks, const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s);
+                   int64_t gmax, int waves, const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s);
 // det/u/v pass bits per (column entry, sub-column): stride words per entry
 void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M, int64_t ncol,
-                     int stride, uint64_t* cpass, hipStream_t s);
+                     int stride, int waves, uint64_t* cpass, hipStream_t s);
 // Level2 parity waves of one mesh: one per boundary voxel, one per four when
 // n2 <= 4, none when Level2 is off
 int64_t l2_parity_waves(int n2, int64_t nb);
 void launch_l2_parity(const int32_t* bidx, int64_t B, int64_t nwaves, const uint32_t* coff, const float4* rec,
-                      const uint64_t* cpass, int stride, const GridDev& g0, const Meshes& M, uint64_t* par2,
-                      hipStream_t s);
+                      const uint64_t* cpass, int stride, int waves, const GridDev& g0, const Meshes& M,
+                      uint64_t* par2, hipStream_t s);
 
 // normals (src/Object.cpp:3219-3253 Level1, 2372-2421 + 2705-2720 Level2)
 void launch_face_normals(const float* tris, int64_t T, float* fn, hipStream_t s);

@@ -558,7 +558,6 @@ void launch_scan_u32_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t*
 // triangles).
 // ---------------------------------------------------------------------------
 constexpr int XB = 256;
-constexpr int XCAP = 256;
 
 // 64-ary search by one wave: largest t in [lo, hi) with off[t] <= x, given
 // off[lo] <= x and (hi == n or off[hi] > x).  ~log64 rounds of dependent
@@ -635,8 +634,9 @@ struct NoAfter {
 
 // Each block owns a contiguous range of items, walked XB at a time, so the
 // triangle searches of consecutive iterations start where the last ended.
-// after(last) runs block-synchronously after every iteration.
-template <typename F, typename A>
+// after(last) runs block-synchronously after every iteration.  XCAP: the
+// triangles one pass stages in LDS (the tuned "triangles per LDS batch").
+template <int XCAP, typename F, typename A>
 __device__ __forceinline__ void expand_items(const float* __restrict__ tris, int64_t T,
                                              const uint64_t* __restrict__ off, F&& body, A&& after)
 {
@@ -707,7 +707,7 @@ __device__ __forceinline__ void expand_items(const float* __restrict__ tris, int
 // (voxel, triangle) bin pair -- key = batch-wide bit index, value = triangle --
 // and the lane's highest hit z becomes the (column, triangle, zmax) entry used
 // by the Level2 parity candidates.
-template <bool kM>
+template <bool kM, int XCAP>
 __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tris, int64_t T,
                                                    const int4* __restrict__ box0,
                                                    const int4* __restrict__ box1,
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
     __syncthreads();
     const LdsAppend lp{s_pk, s_pv, &s_n[0], PB};
     const LdsAppend lc{s_ck, s_cv, &s_n[1], CB};
-    expand_items(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
+    expand_items<XCAP>(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
         const int mesh = kM ? (int)M.tri_mesh[t] : 0;
         const GridDev& g = grid_of<kM>(g0, M, mesh);
         const uint64_t word0 = kM ? M.off[mesh].word0 : 0;
@@ -771,15 +771,26 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
     });
 }
 
+template <int XC>
+static void launch_sat_expand_x(const float* tris, int64_t T, const int4* box0, const int4* box1,
+                                const double4* slab1, const uint64_t* off, const GridDev& g0, const Meshes& M,
+                                uint8_t* b, Append64 pairs, Append64 cols, int blocks, hipStream_t s)
+{
+    if (M.nm > 1)
+        k_sat_expand<true, XC><<<blocks, XB, 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols);
+    else
+        k_sat_expand<false, XC><<<blocks, XB, 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols);
+}
+
 void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const double4* slab1,
                        const uint64_t* off, const GridDev& g0, const Meshes& M, uint8_t* b, Append64 pairs,
-                       Append64 cols, int blocks, hipStream_t s)
+                       Append64 cols, int blocks, int stage, hipStream_t s)
 {
     if (T <= 0) return;
-    if (M.nm > 1)
-        k_sat_expand<true><<<blocks, XB, 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols);
+    if (stage <= 128)
+        launch_sat_expand_x<128>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols, blocks, s);
     else
-        k_sat_expand<false><<<blocks, XB, 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols);
+        launch_sat_expand_x<256>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols, blocks, s);
 }
 
 // boundary flags (one byte per padded voxel, written without atomics by the
@@ -802,7 +813,7 @@ void launch_pack_bytes(const uint8_t* by, int64_t W, uint64_t* bits, hipStream_t
 // Level1 ray parity, first half: one item = (triangle, column inside its
 // conservative bound); the exact det/u/v test keeps the (column, triangle)
 // pairs whose ray the reference's MT accepts at some height.
-template <bool kM>
+template <bool kM, int XCAP>
 __global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tris, int64_t T,
                                                    const int4* __restrict__ rng,
                                                    const uint64_t* __restrict__ off, GridDev g0, Meshes M,
@@ -815,7 +826,7 @@ __global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tri
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const LdsAppend lh{s_hk, nullptr, &s_n, HB};
-    expand_items(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
+    expand_items<XCAP>(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
         const int mesh = kM ? (int)M.tri_mesh[t] : 0;
         const GridDev& g = grid_of<kM>(g0, M, mesh);
         const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
@@ -833,14 +844,24 @@ __global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tri
     });
 }
 
+template <int XC>
+static void launch_ray_expand_x(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off,
+                                const GridDev& g0, const Meshes& M, Append64 hits, int blocks, hipStream_t s)
+{
+    if (M.nm > 1)
+        k_ray_expand<true, XC><<<blocks, XB, 0, s>>>(tris, T, ray_rng, off, g0, M, hits);
+    else
+        k_ray_expand<false, XC><<<blocks, XB, 0, s>>>(tris, T, ray_rng, off, g0, M, hits);
+}
+
 void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off, const GridDev& g0,
-                       const Meshes& M, Append64 hits, int blocks, hipStream_t s)
+                       const Meshes& M, Append64 hits, int blocks, int stage, hipStream_t s)
 {
     if (T <= 0) return;
-    if (M.nm > 1)
-        k_ray_expand<true><<<blocks, XB, 0, s>>>(tris, T, ray_rng, off, g0, M, hits);
+    if (stage <= 128)
+        launch_ray_expand_x<128>(tris, T, ray_rng, off, g0, M, hits, blocks, s);
     else
-        k_ray_expand<false><<<blocks, XB, 0, s>>>(tris, T, ray_rng, off, g0, M, hits);
+        launch_ray_expand_x<256>(tris, T, ray_rng, off, g0, M, hits, blocks, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1431,8 +1452,9 @@ void launch_mesh_bins(const uint32_t* boff, int64_t B, const Meshes& M, hipStrea
 // those queue for the full float SAT in 64-lane batches.  Sub-voxels already
 // set are skipped (the result is an OR).
 // ---------------------------------------------------------------------------
-constexpr int L2_WAVES = 4;
-constexpr int L2_QCAP = 128;   // ring of queued (triangle, sub-voxel) entries per wave
+// ring of queued (triangle, sub-voxel) entries per wave: a round pushes at
+// most 64 entries onto fewer than 64 already queued, so it holds 128
+constexpr int L2_QCAP = 128;
 constexpr int L2_GMAX = kL2GroupMax;   // voxels per group
 constexpr int L2_CMAX = kL2GroupCenters;   // center floats per group: 3 * voxels * n2
 
@@ -1528,8 +1550,8 @@ __device__ __forceinline__ void l2_sat_run(uint32_t ent, bool valid, const L2Lds
     if (hit) atomicOr(&acc[wi], 1ull << (s & 63));
 }
 
-template <bool kM>
-__global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_grp(const float* __restrict__ tris,
+template <bool kM, int W>
+__global__ __launch_bounds__(64 * W) void k_l2_sat_grp(const float* __restrict__ tris,
                                                           const int32_t* __restrict__ bidx,
                                                           const uint32_t* __restrict__ boff,
                                                           const uint32_t* __restrict__ list,
@@ -1539,13 +1561,13 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_grp(const float* __res
                                                           const uint32_t* __restrict__ ngroups, GridDev g0,
                                                           Meshes M, uint64_t* __restrict__ sat2)
 {
-    __shared__ L2Lds s_l[L2_WAVES];
+    __shared__ L2Lds s_l[W];
     const int wv = threadIdx.x >> 6;
     const int lane = __lane_id();
     L2Lds& L = s_l[wv];
     unsigned long long* acc = L.acc;
     const uint32_t G = *ngroups;
-    for (uint32_t grp = blockIdx.x * L2_WAVES + wv; grp < G; grp += gridDim.x * L2_WAVES) {
+    for (uint32_t grp = blockIdx.x * W + wv; grp < G; grp += gridDim.x * W) {
         const uint32_t vb0 = gstart[grp];
         const int nv = (int)(gstart[grp + 1] - vb0);
         const int mesh = kM ? uniform_i((int)M.bmesh[vb0]) : 0;
@@ -1734,8 +1756,8 @@ __device__ __forceinline__ void l2_sat_run_v(uint32_t ent, bool valid, const flo
     if (hit) atomicOr(&acc[s >> 6], 1ull << (s & 63));
 }
 
-template <bool kM>
-__global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_voxel(const float* __restrict__ tris,
+template <bool kM, int W>
+__global__ __launch_bounds__(64 * W) void k_l2_sat_voxel(const float* __restrict__ tris,
                                                           const int32_t* __restrict__ bidx, int64_t B,
                                                           const uint32_t* __restrict__ boff,
                                                           const uint32_t* __restrict__ list,
@@ -1743,22 +1765,22 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_voxel(const float* __r
                                                           const float4* __restrict__ slab2, GridDev g0,
                                                           Meshes M, uint64_t* __restrict__ sat2)
 {
-    __shared__ unsigned long long s_acc[L2_WAVES][64];
-    __shared__ float s_tv[L2_WAVES][64 * 9];
-    __shared__ uint32_t s_q[L2_WAVES][L2_QCAP];
+    __shared__ unsigned long long s_acc[W][64];
+    __shared__ float s_tv[W][64 * 9];
+    __shared__ uint32_t s_q[W][L2_QCAP];
     // per staged triangle: sub-box origin and extents, masks, slab, 1/dp, and
     // the start of its sub-columns in the chunk's flattened (triangle,
     // sub-column) item space
-    __shared__ uint32_t s_dec[L2_WAVES][64];
-    __shared__ unsigned long long s_mk[L2_WAVES][64];
-    __shared__ float4 s_sl[L2_WAVES][64];
-    __shared__ float s_idp[L2_WAVES][64];
-    __shared__ uint32_t s_off[L2_WAVES][64];
-    __shared__ float s_cz[L2_WAVES][16];
-    __shared__ uint8_t s_win[L2_WAVES][64];   // per round: staged entry starting at each item position
+    __shared__ uint32_t s_dec[W][64];
+    __shared__ unsigned long long s_mk[W][64];
+    __shared__ float4 s_sl[W][64];
+    __shared__ float s_idp[W][64];
+    __shared__ uint32_t s_off[W][64];
+    __shared__ float s_cz[W][16];
+    __shared__ uint8_t s_win[W][64];   // per round: staged entry starting at each item position
     const int wv = threadIdx.x >> 6;
     const int lane = __lane_id();
-    const int64_t b = (int64_t)blockIdx.x * L2_WAVES + wv;
+    const int64_t b = (int64_t)blockIdx.x * W + wv;
     if (b >= B) return;
     const int mesh = kM ? uniform_i((int)M.bmesh[b]) : 0;
     const GridDev& g = grid_of<kM>(g0, M, mesh);
@@ -1922,26 +1944,42 @@ __global__ __launch_bounds__(64 * L2_WAVES) void k_l2_sat_voxel(const float* __r
     for (int i = lane; i < w2; i += 64) out[i] = acc[i];
 }
 
-void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
-                   const uint64_t* pmask, const float4* slab2, const uint32_t* gstart, const uint32_t* ngroups,
-                   int blocks, const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s)
+template <int W>
+static void launch_l2_sat_w(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff,
+                            const uint32_t* list, const uint64_t* pmask, const float4* slab2, const uint32_t* gstart,
+                            const uint32_t* ngroups, int64_t gmax, const GridDev& g0, const Meshes& M,
+                            uint64_t* sat2, hipStream_t s)
 {
-    if (B <= 0) return;
-    const dim3 bs(64 * L2_WAVES);
+    const dim3 bs(64 * W);
     if (gstart) {
+        // one wave per group, for an upper bound gmax on the group count
+        const unsigned blocks = (unsigned)std::max<int64_t>(1, (gmax + W - 1) / W);
         if (M.nm > 1)
-            k_l2_sat_grp<true><<<(unsigned)blocks, bs, 0, s>>>(tris, bidx, boff, list, pmask, slab2, gstart, ngroups,
-                                                              g0, M, sat2);
+            k_l2_sat_grp<true, W><<<blocks, bs, 0, s>>>(tris, bidx, boff, list, pmask, slab2, gstart, ngroups, g0, M,
+                                                        sat2);
         else
-            k_l2_sat_grp<false><<<(unsigned)blocks, bs, 0, s>>>(tris, bidx, boff, list, pmask, slab2, gstart, ngroups,
-                                                               g0, M, sat2);
+            k_l2_sat_grp<false, W><<<blocks, bs, 0, s>>>(tris, bidx, boff, list, pmask, slab2, gstart, ngroups, g0, M,
+                                                         sat2);
         return;
     }
-    const unsigned vblocks = (unsigned)((B + L2_WAVES - 1) / L2_WAVES);
+    const unsigned vblocks = (unsigned)((B + W - 1) / W);
     if (M.nm > 1)
-        k_l2_sat_voxel<true><<<vblocks, bs, 0, s>>>(tris, bidx, B, boff, list, pmask, slab2, g0, M, sat2);
+        k_l2_sat_voxel<true, W><<<vblocks, bs, 0, s>>>(tris, bidx, B, boff, list, pmask, slab2, g0, M, sat2);
     else
-        k_l2_sat_voxel<false><<<vblocks, bs, 0, s>>>(tris, bidx, B, boff, list, pmask, slab2, g0, M, sat2);
+        k_l2_sat_voxel<false, W><<<vblocks, bs, 0, s>>>(tris, bidx, B, boff, list, pmask, slab2, g0, M, sat2);
+}
+
+void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint32_t* boff, const uint32_t* list,
+                   const uint64_t* pmask, const float4* slab2, const uint32_t* gstart, const uint32_t* ngroups,
+                   int64_t gmax, int waves, const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s)
+{
+    if (B <= 0) return;
+    if (waves <= 1)
+        launch_l2_sat_w<1>(tris, bidx, B, boff, list, pmask, slab2, gstart, ngroups, gmax, g0, M, sat2, s);
+    else if (waves == 2)
+        launch_l2_sat_w<2>(tris, bidx, B, boff, list, pmask, slab2, gstart, ngroups, gmax, g0, M, sat2, s);
+    else
+        launch_l2_sat_w<4>(tris, bidx, B, boff, list, pmask, slab2, gstart, ngroups, gmax, g0, M, sat2, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -2026,9 +2064,7 @@ __device__ __forceinline__ int mesh_of_col(const MeshOff* __restrict__ off, int 
 // layers [0, kh] (t_decided over that box; -1 if none found).  For a voxel at
 // k1 <= kh the entry then flips every layer of each sub-column it passes
 // without a t test.
-constexpr int CP_WAVES = 4;
-
-template <bool kM>
+template <bool kM, int CP_WAVES>
 __global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass(const uint32_t* __restrict__ coff,
                                                             const float4* __restrict__ rec, GridDev g0, Meshes M,
                                                             int64_t ncol, int stride, uint64_t* __restrict__ cpass)
@@ -2095,18 +2131,25 @@ __global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass(const uint32_t* __re
     }
 }
 
-void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M, int64_t ncol,
-                     int stride, uint64_t* cpass, hipStream_t s)
+template <int CW>
+static void launch_col_pass_w(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M,
+                              int64_t ncol, int stride, uint64_t* cpass, hipStream_t s)
 {
-    if (ncol <= 0) return;
-    const int64_t blocks = (ncol + CP_WAVES - 1) / CP_WAVES;
+    const int64_t blocks = (ncol + CW - 1) / CW;
     if (M.nm > 1)
-        k_col_pass<true><<<(unsigned)blocks, 64 * CP_WAVES, 0, s>>>(coff, rec, g0, M, ncol, stride, cpass);
+        k_col_pass<true, CW><<<(unsigned)blocks, 64 * CW, 0, s>>>(coff, rec, g0, M, ncol, stride, cpass);
     else
-        k_col_pass<false><<<(unsigned)blocks, 64 * CP_WAVES, 0, s>>>(coff, rec, g0, M, ncol, stride, cpass);
+        k_col_pass<false, CW><<<(unsigned)blocks, 64 * CW, 0, s>>>(coff, rec, g0, M, ncol, stride, cpass);
 }
 
-constexpr int L2P_WAVES = 4;
+void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M, int64_t ncol,
+                     int stride, int waves, uint64_t* cpass, hipStream_t s)
+{
+    if (ncol <= 0) return;
+    if (waves <= 1) launch_col_pass_w<1>(coff, rec, g0, M, ncol, stride, cpass, s);
+    else if (waves == 2) launch_col_pass_w<2>(coff, rec, g0, M, ncol, stride, cpass, s);
+    else launch_col_pass_w<4>(coff, rec, g0, M, ncol, stride, cpass, s);
+}
 
 // One boundary voxel (i1, j1, k1) of the Level2 parity.  N2 > 0: the Level2
 // resolution as a compile-time constant (layer loops unrolled, the layer
@@ -2322,7 +2365,7 @@ __device__ __forceinline__ void l2_parity_pack4(int n2rt, const GridDev& g, uint
 
 // Waves: single mesh, one per boundary voxel, or (n2 <= 4) one per four;
 // batch, per mesh from MeshOff::pw0 likewise.
-template <bool kM, int N2>
+template <bool kM, int N2, int L2P_WAVES>
 __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __restrict__ bidx, int64_t B,
                                                               int64_t nwaves, const uint32_t* __restrict__ coff,
                                                               const float4* __restrict__ rec,
@@ -2390,21 +2433,31 @@ int64_t l2_parity_waves(int n2, int64_t nb)
     return n2 <= 4 ? (nb + 3) / 4 : nb;
 }
 
+template <int PW>
+static void launch_l2_parity_w(const int32_t* bidx, int64_t B, int64_t nwaves, const uint32_t* coff,
+                               const float4* rec, const uint64_t* cpass, int stride, const GridDev& g0,
+                               const Meshes& M, uint64_t* par2, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((nwaves + PW - 1) / PW);
+    const dim3 bs(64 * PW);
+    if (M.nm > 1)
+        k_l2_parity<true, 0, PW><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
+    else if (g0.n2 == 8)
+        k_l2_parity<false, 8, PW><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
+    else if (g0.n2 == 4)
+        k_l2_parity<false, 4, PW><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
+    else
+        k_l2_parity<false, 0, PW><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
+}
+
 void launch_l2_parity(const int32_t* bidx, int64_t B, int64_t nwaves, const uint32_t* coff, const float4* rec,
-                      const uint64_t* cpass, int stride, const GridDev& g0, const Meshes& M, uint64_t* par2,
-                      hipStream_t s)
+                      const uint64_t* cpass, int stride, int waves, const GridDev& g0, const Meshes& M,
+                      uint64_t* par2, hipStream_t s)
 {
     if (B <= 0 || nwaves <= 0) return;
-    const unsigned blocks = (unsigned)((nwaves + L2P_WAVES - 1) / L2P_WAVES);
-    const dim3 bs(64 * L2P_WAVES);
-    if (M.nm > 1)
-        k_l2_parity<true, 0><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
-    else if (g0.n2 == 8)
-        k_l2_parity<false, 8><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
-    else if (g0.n2 == 4)
-        k_l2_parity<false, 4><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
-    else
-        k_l2_parity<false, 0><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
+    if (waves <= 1) launch_l2_parity_w<1>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2, s);
+    else if (waves == 2) launch_l2_parity_w<2>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2, s);
+    else launch_l2_parity_w<4>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -222,7 +222,7 @@ struct gpv_ctx {
     char* h_buf = nullptr;   // pinned read-back staging
     size_t h_cap = 0;
     int num_cu = 256;
-    gpv_tune tune{16, 8, 2048, 1};
+    gpv_tune tune{16, 8, 2048, 1, 1, 2, 128};
     std::vector<hipEvent_t> events;   // recycled stage events of finished jobs
     hipEvent_t na_ev[16] = {};        // stage events of gpv_voxelize_noalloc (created with the context)
 };
@@ -789,7 +789,7 @@ int launch_sat_half(Job* J)
     Append64 pa{&J->ctr->pairs, J->pair_key, J->pair_tri, J->pair_cap};
     Append64 co{&J->ctr->colents, J->col_key, J->col_z, J->col_cap};
     launch_sat_expand(J->tris, J->T, J->box0, J->box1, J->slab1, J->sat_off, J->g[0], J->view(), bbytes, pa, co,
-                      expand_blocks(c), s);
+                      expand_blocks(c), c->tune.expand_stage_tris, s);
     launch_pack_bytes(bbytes, J->W, J->bnd, s);
     release(J, bbytes);
     launch_popc_words(J->bnd, J->W, J->wcnt, s);
@@ -808,7 +808,8 @@ int launch_ray_half(Job* J, hipStream_t s)
     HIP_TRY(hipEventRecord(J->ev[2], s));
     launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->hits), 0, 1, s);
     Append64 hi{&J->ctr->hits, J->hit_key, nullptr, J->hit_cap};
-    launch_ray_expand(J->tris, J->T, J->rng, J->ray_off, J->g[0], J->view(), hi, expand_blocks(c), s);
+    launch_ray_expand(J->tris, J->T, J->rng, J->ray_off, J->g[0], J->view(), hi, expand_blocks(c),
+                      c->tune.expand_stage_tris, s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(J->ev[13], s));
     return GPV_OK;
@@ -1062,8 +1063,10 @@ int cpass_stride(const Job* J) { return (J->n2max * J->n2max + 63) / 64 + 1; }
 int l2_parity_kernels(Job* J, hipStream_t s)
 {
     const int stride = cpass_stride(J);
-    launch_col_pass(J->coff, J->crec, J->g[0], J->view(), J->ncol, stride, J->cpass, s);
-    launch_l2_parity(J->bidx, J->B, J->l2p_waves, J->coff, J->crec, J->cpass, stride, J->g[0], J->view(), J->par2,
+    const gpv_tune& tu = J->ctx->tune;
+    launch_col_pass(J->coff, J->crec, J->g[0], J->view(), J->ncol, stride, tu.l2_parity_waves, J->cpass, s);
+    launch_l2_parity(J->bidx, J->B, J->l2p_waves, J->coff, J->crec, J->cpass, stride, tu.l2_parity_waves, J->g[0],
+                     J->view(), J->par2,
                      s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(J->ev[11], s));
@@ -1147,9 +1150,8 @@ int l2_sat_kernels(Job* J)
         }
         gmax = std::min<int64_t>(gmax, J->B);
     }
-    const int blocks = (int)std::max<int64_t>(1, (gmax + 3) / 4);
     launch_l2_sat(J->tris, J->bidx, J->B, J->boff, J->list, J->pmask, J->slab2, grouped ? J->gstart : nullptr,
-                  J->gidx + J->B, blocks, J->g[0], J->view(), J->sat2, s);
+                  J->gidx + J->B, gmax, J->ctx->tune.l2_sat_waves, J->g[0], J->view(), J->sat2, s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(J->ev[7], s));
     return GPV_OK;
@@ -1586,8 +1588,10 @@ int gpv_ctx_get_tune(gpv_ctx* c, gpv_tune* out)
 int gpv_ctx_set_tune(gpv_ctx* c, const gpv_tune* t)
 {
     if (!c || !t) return fail(GPV_ERR_INVALID, "NULL argument");
+    auto wv = [](int w) { return w == 1 || w == 2 || w == 4; };
     if (t->expand_blocks_per_cu < 1 || t->expand_blocks_per_cu > 64 || t->l2_group_below < 0 ||
-        t->sort_items_per_block < 256 || t->sort_items_per_block > (1 << 20))
+        t->sort_items_per_block < 256 || t->sort_items_per_block > (1 << 20) || !wv(t->l2_sat_waves) ||
+        !wv(t->l2_parity_waves) || (t->expand_stage_tris != 128 && t->expand_stage_tris != 256))
         return fail(GPV_ERR_INVALID, "launch shape out of range");
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     c->tune = *t;
@@ -1624,7 +1628,7 @@ int gpv_ctx_autotune(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, 
     int st = tune_trial(c, d, nm, on_device, 1, &warm);
     if (st == GPV_OK) st = tune_trial(c, d, nm, on_device, repeats, &base);
     const std::vector<int> expand = {4, 8, 16, 32}, group = {0, 4, 8, 16, 1 << 20}, items = {1024, 2048, 4096, 8192},
-                           overlap = {0, 1};
+                           overlap = {0, 1}, waves = {1, 2, 4}, stage = {128, 256};
     auto sweep = [&](int gpv_tune::*field, const std::vector<int>& vals) -> int {
         for (int v : vals) {
             if (v == cur.*field) continue;
@@ -1645,6 +1649,9 @@ int gpv_ctx_autotune(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, 
     if (st == GPV_OK) st = sweep(&gpv_tune::l2_group_below, group);
     if (st == GPV_OK) st = sweep(&gpv_tune::sort_items_per_block, items);
     if (st == GPV_OK) st = sweep(&gpv_tune::l2_overlap, overlap);
+    if (st == GPV_OK) st = sweep(&gpv_tune::l2_sat_waves, waves);
+    if (st == GPV_OK) st = sweep(&gpv_tune::l2_parity_waves, waves);
+    if (st == GPV_OK) st = sweep(&gpv_tune::expand_stage_tris, stage);
     if (st != GPV_OK) {
         c->tune = start;
         return st;

@@ -152,6 +152,11 @@ typedef struct gpv_tune {
                                   half of the set-up and the ray expansion beside the SAT half and
                                   the Level1 SAT, the parity fill and the column-entry sort beside
                                   the bins, the Level2 parity beside the Level2 SAT (1) */
+    int l2_sat_waves;          /* waves per block of the Level2 SAT: 1, 2 or 4 (1) */
+    int l2_parity_waves;       /* waves per block of the Level2 parity and its column pass:
+                                  1, 2 or 4 (2) */
+    int expand_stage_tris;     /* triangles per LDS batch of the Level1 SAT and ray expansion:
+                                  128 or 256 (128) */
 } gpv_tune;
 int gpv_ctx_get_tune(gpv_ctx *ctx, gpv_tune *out);
 int gpv_ctx_set_tune(gpv_ctx *ctx, const gpv_tune *t);

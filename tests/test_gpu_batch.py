@@ -250,15 +250,22 @@ def test_autotune_keeps_results(gv):
         assert shapes == c.get_tune() and best_ms > 0
         assert shapes["expand_blocks_per_cu"] in (4, 8, 16, 32) and shapes["l2_overlap"] in (0, 1)
         assert shapes["sort_items_per_block"] in (1024, 2048, 4096, 8192)
+        assert shapes["l2_sat_waves"] in (1, 2, 4) and shapes["l2_parity_waves"] in (1, 2, 4)
+        assert shapes["expand_stage_tris"] in (128, 256)
         for a, b in zip(base, run()):
             assert np.array_equal(a, b)
-        for extreme in ({"l2_group_below": 0, "l2_overlap": 0, "expand_blocks_per_cu": 1, "sort_items_per_block": 256},
-                        {"l2_group_below": 1 << 20, "sort_items_per_block": 1 << 20}):
+        for extreme in ({"l2_group_below": 0, "l2_overlap": 0, "expand_blocks_per_cu": 1, "sort_items_per_block": 256,
+                         "l2_sat_waves": 1, "l2_parity_waves": 1, "expand_stage_tris": 128},
+                        {"l2_group_below": 1 << 20, "sort_items_per_block": 1 << 20, "l2_sat_waves": 4,
+                         "l2_parity_waves": 2, "expand_stage_tris": 256},
+                        {"l2_group_below": 0, "l2_sat_waves": 4, "l2_parity_waves": 4}):
             c.set_tune(**extreme)
             for a, b in zip(base, run()):
                 assert np.array_equal(a, b), extreme
         with pytest.raises(gv.GpvError):
             c.set_tune(sort_items_per_block=1)
+        with pytest.raises(gv.GpvError):
+            c.set_tune(l2_sat_waves=3)
 
 
 def test_bits_download_matches_states(gv, ctx):

@@ -171,3 +171,18 @@ def test_save_raw_bytes(gv, orc, ctx, tmp_path):
     assert names == sorted(os.listdir(got_dir)) and len(names) == 4
     for n in names:
         assert (want_dir / n).read_bytes() == (got_dir / n).read_bytes(), n
+
+
+@pytest.mark.parametrize("sat_w,par_w,stage", [(1, 1, 128), (4, 2, 256), (2, 4, 128)])
+def test_launch_shapes_single_mesh(gv, orc, sat_w, par_w, stage):
+    """Every launch shape the tuner can pick (Level2 SAT and parity waves per
+    block, triangles per LDS batch of the expansions) gives the oracle's
+    grids on the single-mesh kernels (n2 = 8 and 4 specialisations, both
+    Level2 SAT forms)."""
+    with gv.Context(0) as c:
+        for below in (0, 1 << 20):
+            c.set_tune(l2_sat_waves=sat_w, l2_parity_waves=par_w, expand_stage_tris=stage, l2_group_below=below)
+            v, f = gv.mixed(1, 20000, 71)
+            run_case(gv, orc, c, f"shapes {sat_w}/{par_w}/{stage}/{below} torus 64/8", v, f, 64, 8)
+            v, f = gv.mixed(2, 9000, 72)
+            run_case(gv, orc, c, f"shapes {sat_w}/{par_w}/{stage}/{below} box 48/4", v, f, 48, 4)
