@@ -265,23 +265,27 @@ def bench_serve(gv, n_requests, rate, lanes, seed=5, n_library=48, plain_sample=
     }
 
 
-# stage -> its kernel in the single-mesh pipeline (each stage's event pair
-# brackets exactly these launches)
-STAGE_KERNEL = {"l2_sat": "k_l2_sat_voxel<false>", "l2_parity": "k_l2_parity<false, 8>", "l1_sat": "k_sat_expand<false>"}
+# stage -> its kernel family in the single-mesh pipeline (each stage's event
+# pair brackets exactly these launches); the template arguments after the
+# first are the launch shape of the context's tuning
+STAGE_KERNEL = {"l2_sat": "k_l2_sat_voxel<false", "l2_parity": "k_l2_parity<false", "l1_sat": "k_sat_expand<false"}
 PROFILE = os.path.join(ROOT, "profiles", "traffic_torus_knot_plain.json")
 
 
-def measured(kernel):
+def measured(family):
     """HBM traffic (calibrated FETCH_SIZE + WRITE_SIZE) and VALU counters of
-    `kernel` on this workload, from the committed counter summary."""
-    if not kernel or not os.path.exists(PROFILE):
+    the instantiation of kernel `family` that ran on this workload, from the
+    committed counter summary."""
+    if not family or not os.path.exists(PROFILE):
         return None
     with open(PROFILE) as f:
         prof = json.load(f)
-    k = prof.get("kernels", {}).get(kernel)
-    if not k:
+    ks = prof.get("kernels", {})
+    names = [n for n in ks if n.startswith(family + ",") or n.startswith(family + ">")]
+    if not names:
         return None
-    return dict(k, source=os.path.relpath(PROFILE, ROOT), valu_peak=prof.get("valu_peak_lane_ops"))
+    name = max(names, key=lambda n: ks[n].get("avg_us_trace", 0.0))
+    return dict(ks[name], name=name, source=os.path.relpath(PROFILE, ROOT), valu_peak=prof.get("valu_peak_lane_ops"))
 
 
 def copy_calibration():
@@ -493,7 +497,8 @@ def main():
     meas = measured(STAGE_KERNEL.get(dom))
     rl = {"bound": "hbm", "achieved": bytes_dom / t_dom / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
           "frac": bytes_dom / t_dom / 1e9 / HBM_PEAK_GBS,
-          "traffic": meas["traffic_bytes"] if meas else None, "kernel": STAGE_KERNEL.get(dom, dom),
+          "traffic": meas["traffic_bytes"] if meas else None,
+          "kernel": meas["name"] if meas else STAGE_KERNEL.get(dom, dom),
           "algorithmic_bytes": bytes_dom, "ms": sms[dom],
           "traffic_source": meas["source"] if meas else None}
     # The path is not a contraction (no MFMA); its binding resource is VALU

@@ -27,7 +27,10 @@ __device__ __forceinline__ float eps_below() { return __uint_as_float(kEpsBelowB
 // predicates (9 edge axes, 3 box axes, the triangle plane); the order of
 // evaluation cannot change a conjunction, so the box axes run first because
 // they reject most often in a sweep.  Each predicate is the reference's float
-// expression unchanged.
+// expression unchanged, its two projections passed in the order the
+// reference's macro compares them: (p0, p2) for X01/Y02, (p0, p1) for
+// X2/Y1/Z0, (p2, p1) for Z12.  The order decides which of a NaN pair
+// becomes the minimum.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool sat_separated(float pa, float pb, float rad)
 {
@@ -48,9 +51,9 @@ __device__ __forceinline__ bool xy_edges_ok(float cx, float cy, float hx, float 
     const float e0x = v1x - v0x, e0y = v1y - v0y;
     const float e1x = v2x - v1x, e1y = v2y - v1y;
     const float e2x = v0x - v2x, e2y = v0y - v2y;
-    if (sat_separated(e0y * v1x - e0x * v1y, e0y * v2x - e0x * v2y, fabsf(e0y) * hx + fabsf(e0x) * hy)) return false;
+    if (sat_separated(e0y * v2x - e0x * v2y, e0y * v1x - e0x * v1y, fabsf(e0y) * hx + fabsf(e0x) * hy)) return false;
     if (sat_separated(e1y * v0x - e1x * v0y, e1y * v1x - e1x * v1y, fabsf(e1y) * hx + fabsf(e1x) * hy)) return false;
-    if (sat_separated(e2y * v1x - e2x * v1y, e2y * v2x - e2x * v2y, fabsf(e2y) * hx + fabsf(e2x) * hy)) return false;
+    if (sat_separated(e2y * v2x - e2x * v2y, e2y * v1x - e2x * v1y, fabsf(e2y) * hx + fabsf(e2x) * hy)) return false;
     return true;
 }
 
@@ -114,7 +117,7 @@ __device__ __forceinline__ bool tri_box_overlap(float cx, float cy, float cz,
     float fx = fabsf(e0x), fy = fabsf(e0y), fz = fabsf(e0z);
     if (sat_separated(e0z * v0y - e0y * v0z, e0z * v2y - e0y * v2z, fz * hy + fy * hz)) return false;
     if (sat_separated(-e0z * v0x + e0x * v0z, -e0z * v2x + e0x * v2z, fz * hx + fx * hz)) return false;
-    if (kZEdges && sat_separated(e0y * v1x - e0x * v1y, e0y * v2x - e0x * v2y, fy * hx + fx * hy)) return false;
+    if (kZEdges && sat_separated(e0y * v2x - e0x * v2y, e0y * v1x - e0x * v1y, fy * hx + fx * hy)) return false;
     fx = fabsf(e1x); fy = fabsf(e1y); fz = fabsf(e1z);
     if (sat_separated(e1z * v0y - e1y * v0z, e1z * v2y - e1y * v2z, fz * hy + fy * hz)) return false;
     if (sat_separated(-e1z * v0x + e1x * v0z, -e1z * v2x + e1x * v2z, fz * hx + fx * hz)) return false;
@@ -122,7 +125,7 @@ __device__ __forceinline__ bool tri_box_overlap(float cx, float cy, float cz,
     fx = fabsf(e2x); fy = fabsf(e2y); fz = fabsf(e2z);
     if (sat_separated(e2z * v0y - e2y * v0z, e2z * v1y - e2y * v1z, fz * hy + fy * hz)) return false;
     if (sat_separated(-e2z * v0x + e2x * v0z, -e2z * v1x + e2x * v1z, fz * hx + fx * hz)) return false;
-    if (kZEdges && sat_separated(e2y * v1x - e2x * v1y, e2y * v2x - e2x * v2y, fy * hx + fx * hy)) return false;
+    if (kZEdges && sat_separated(e2y * v2x - e2x * v2y, e2y * v1x - e2x * v1y, fy * hx + fx * hy)) return false;
     return true;
 }
 

@@ -84,7 +84,7 @@ struct Counters {
     unsigned long long pairs;       // Level1 SAT (voxel, triangle) pairs
     unsigned long long colents;     // (column, triangle, zmax) entries
     unsigned long long hits;        // (column, triangle) ray hits, Level1
-    unsigned long long pad;
+    unsigned long long bad_input;   // nonzero: a vertex coordinate is not finite or beyond +-2^40
 };
 
 // sets the calling thread's gpv_last_error text and returns code
@@ -99,9 +99,10 @@ void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
 
 void launch_mesh_ids(const MeshOff* off, int nm, int64_t T, uint32_t* tri_mesh, hipStream_t s);
+// part 1 (and 3) also raise *bad_input for a coordinate outside the domain
 void launch_tri_setup(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
                       uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, float4* slab2, int part,
-                      hipStream_t s);
+                      unsigned long long* bad_input, hipStream_t s);
 
 // exclusive scan of n u32 into out[0..n] (out[n] = total); tmp sized by
 // scan_tmp_elems(n).  If max_out != nullptr the max input is atomically

@@ -280,7 +280,8 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
                                                    int4* __restrict__ ray_rng,
                                                    uint32_t* __restrict__ ray_items,
                                                    double4* __restrict__ slab1,
-                                                   float4* __restrict__ slab2)
+                                                   float4* __restrict__ slab2,
+                                                   unsigned long long* __restrict__ bad_input)
 {
     uint64_t tests = 0;
     const int64_t first = (int64_t)blockIdx.x * blockDim.x;
@@ -293,6 +294,12 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
         MtTri m;
         const bool vis = mt_setup(tv, m);
         if constexpr ((kPart & 1) != 0) {
+            // the input domain: finite coordinates within +-2^40, where every
+            // intermediate of the SAT and of the split +z ray test is finite
+            bool bad = false;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) bad = bad || !(fabsf(tv[c]) <= 0x1p40f);
+            if (bad) atomicOr(bad_input, 1ull);
             int lo[3], dd[3];
             const int n[3] = {g.nx, g.ny, g.nz};
             const float mn[3] = {g.bminx, g.bminy, g.bminz};
@@ -368,27 +375,29 @@ __global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ tri
 template <int kPart>
 static void tri_setup_part(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
                            uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, float4* slab2,
-                           hipStream_t s)
+                           unsigned long long* bad_input, hipStream_t s)
 {
     if (M.nm > 1)
         k_tri_setup<true, kPart><<<grid_for(T, 256, 1 << 30), 256, 0, s>>>(tris, T, g0, M, box0, box1, sat_items,
-                                                                            ray_rng, ray_items, slab1, slab2);
+                                                                            ray_rng, ray_items, slab1, slab2,
+                                                                            bad_input);
     else
         k_tri_setup<false, kPart><<<grid_for(T, 256, 1024), 256, 0, s>>>(tris, T, g0, M, box0, box1, sat_items,
-                                                                          ray_rng, ray_items, slab1, slab2);
+                                                                          ray_rng, ray_items, slab1, slab2,
+                                                                          bad_input);
 }
 
 void launch_tri_setup(const float* tris, int64_t T, const GridDev& g0, const Meshes& M, int4* box0, int4* box1,
                       uint32_t* sat_items, int4* ray_rng, uint32_t* ray_items, double4* slab1, float4* slab2,
-                      int part, hipStream_t s)
+                      int part, unsigned long long* bad_input, hipStream_t s)
 {
     if (T <= 0) return;
     if (part == 1)
-        tri_setup_part<1>(tris, T, g0, M, box0, box1, sat_items, ray_rng, ray_items, slab1, slab2, s);
+        tri_setup_part<1>(tris, T, g0, M, box0, box1, sat_items, ray_rng, ray_items, slab1, slab2, bad_input, s);
     else if (part == 2)
-        tri_setup_part<2>(tris, T, g0, M, box0, box1, sat_items, ray_rng, ray_items, slab1, slab2, s);
+        tri_setup_part<2>(tris, T, g0, M, box0, box1, sat_items, ray_rng, ray_items, slab1, slab2, bad_input, s);
     else
-        tri_setup_part<3>(tris, T, g0, M, box0, box1, sat_items, ray_rng, ray_items, slab1, slab2, s);
+        tri_setup_part<3>(tris, T, g0, M, box0, box1, sat_items, ray_rng, ray_items, slab1, slab2, bad_input, s);
 }
 
 // ---------------------------------------------------------------------------

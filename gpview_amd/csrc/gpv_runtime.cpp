@@ -739,9 +739,10 @@ int stage_setup(Job* J)
     hipStream_t s = c->stream;
     HIP_TRY(hipEventRecord(J->ev[0], s));
     launch_fill_u64(reinterpret_cast<uint64_t*>(J->mctr), 0, (int64_t)(sizeof(MeshCtr) / 8) * J->nm, s);
+    launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->bad_input), 0, 1, s);
     if (J->nm > 1) launch_mesh_ids(J->d_off, J->nm, J->T, J->tri_mesh, s);
     launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items,
-                     J->slab1, J->slab2, 3, s);
+                     J->slab1, J->slab2, 3, &J->ctr->bad_input, s);
     TRY(ensure_scan_tmp(J, J->T));
     launch_scan_u32_u64(J->sat_items, J->sat_off, J->T, J->scan64, s);
     launch_scan_u32_u64(J->ray_items, J->ray_off, J->T, J->scan64, s);
@@ -758,17 +759,18 @@ int stage_setup_split(Job* J)
     hipStream_t s = c->stream;
     HIP_TRY(hipEventRecord(J->ev[0], s));
     launch_fill_u64(reinterpret_cast<uint64_t*>(J->mctr), 0, (int64_t)(sizeof(MeshCtr) / 8) * J->nm, s);
+    launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->bad_input), 0, 1, s);
     if (J->nm > 1) launch_mesh_ids(J->d_off, J->nm, J->T, J->tri_mesh, s);
     TRY(ensure_scan_tmp(J, J->T));
     release(J, J->scan_ray);
     TRY(alloc(J, &J->scan_ray, scan_tmp_elems(J->T)));
     HIP_TRY(hipEventRecord(J->ev[15], s));
     launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items,
-                     J->slab1, J->slab2, 1, s);
+                     J->slab1, J->slab2, 1, &J->ctr->bad_input, s);
     launch_scan_u32_u64(J->sat_items, J->sat_off, J->T, J->scan64, s);
     HIP_TRY(hipStreamWaitEvent(c->side, J->ev[15], 0));
     launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items,
-                     J->slab1, J->slab2, 2, c->side);
+                     J->slab1, J->slab2, 2, &J->ctr->bad_input, c->side);
     launch_scan_u32_u64(J->ray_items, J->ray_off, J->T, J->scan_ray, c->side);
     HIP_TRY(hipGetLastError());
     J->setup_done = true;
@@ -831,6 +833,11 @@ int read_counts(Job* J, bool with_b)
             HIP_TRY(hipMemcpyAsync(c->h_buf + kHOff, J->wprefix + J->W, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     }
     HIP_TRY(hipStreamSynchronize(s));
+    // the set-up kernel flags coordinates outside the input domain (NaN and
+    // infinities would reach the index AABBs and the pruning bounds, where
+    // no float expression of the reference decides the outcome)
+    if (reinterpret_cast<const Counters*>(c->h_buf)->bad_input)
+        return fail(GPV_ERR_INVALID, "a vertex coordinate is not finite or exceeds 2^40 in magnitude");
     return GPV_OK;
 }
 

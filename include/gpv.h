@@ -29,7 +29,10 @@ extern "C" {
 /* ---- status codes -------------------------------------------------------- */
 enum gpv_status {
     GPV_OK = 0,
-    GPV_ERR_INVALID = 1,      /* bad argument (null pointer, size, grid limits) */
+    GPV_ERR_INVALID = 1,      /* bad argument (null pointer, size, grid limits, or a
+                                 vertex coordinate that is not finite or exceeds 2^40
+                                 in magnitude: the input domain, checked on the
+                                 device by the per-triangle set-up) */
     GPV_ERR_OOM = 2,          /* device or host allocation failed */
     GPV_ERR_CAPACITY = 3,     /* bin capacity regrow did not converge */
     GPV_ERR_HIP = 4,          /* HIP runtime error (message via gpv_last_error) */

@@ -71,7 +71,9 @@ static int plane_box_overlap(const float nrm[3], const float vert[3], const floa
 
 /* One edge-axis test (the AXISTEST_* macros, src/TriBoxIntersection.cpp:90-133):
  * separated when the projected interval [min(pa,pb), max(pa,pb)] misses
- * [-rad, rad] strictly. */
+ * [-rad, rad] strictly.  (pa, pb) is the macro's compare order: (p0, p2) for
+ * X01/Y02, (p0, p1) for X2/Y1/Z0, (p2, p1) for Z12 (:124); it decides which
+ * of a NaN pair becomes min. */
 static int separated(float pa, float pb, float rad)
 {
     float mn, mx;
@@ -100,7 +102,7 @@ int orc_tri_box_overlap(const float c[3], const float h[3], const float tv[9])
                   fz * h[1] + fy * h[2])) return 0;
     if (separated(-e0[2] * v0[0] + e0[0] * v0[2], -e0[2] * v2[0] + e0[0] * v2[2],
                   fz * h[0] + fx * h[2])) return 0;
-    if (separated(e0[1] * v1[0] - e0[0] * v1[1], e0[1] * v2[0] - e0[0] * v2[1],
+    if (separated(e0[1] * v2[0] - e0[0] * v2[1], e0[1] * v1[0] - e0[0] * v1[1],
                   fy * h[0] + fx * h[1])) return 0;
     /* edge e1: X01, Y02, Z0 (:170-175) */
     fx = fabsf(e1[0]); fy = fabsf(e1[1]); fz = fabsf(e1[2]);
@@ -116,7 +118,7 @@ int orc_tri_box_overlap(const float c[3], const float h[3], const float tv[9])
                   fz * h[1] + fy * h[2])) return 0;
     if (separated(-e2[2] * v0[0] + e2[0] * v0[2], -e2[2] * v1[0] + e2[0] * v1[2],
                   fz * h[0] + fx * h[2])) return 0;
-    if (separated(e2[1] * v1[0] - e2[0] * v1[1], e2[1] * v2[0] - e2[0] * v2[1],
+    if (separated(e2[1] * v2[0] - e2[0] * v2[1], e2[1] * v1[0] - e2[0] * v1[1],
                   fy * h[0] + fx * h[1])) return 0;
     /* AABB axes, FINDMINMAX (:191-200) */
     for (int q = 0; q < 3; q++) {
@@ -193,9 +195,9 @@ static int edges_z_ok(const float *tv, float cx, float cy, const float h[3])
     float v0y = tv[1] - cy, v1y = tv[4] - cy, v2y = tv[7] - cy;
     float e0x = v1x - v0x, e1x = v2x - v1x, e2x = v0x - v2x;
     float e0y = v1y - v0y, e1y = v2y - v1y, e2y = v0y - v2y;
-    if (separated(e0y * v1x - e0x * v1y, e0y * v2x - e0x * v2y, fabsf(e0y) * h[0] + fabsf(e0x) * h[1])) return 0;
+    if (separated(e0y * v2x - e0x * v2y, e0y * v1x - e0x * v1y, fabsf(e0y) * h[0] + fabsf(e0x) * h[1])) return 0;
     if (separated(e1y * v0x - e1x * v0y, e1y * v1x - e1x * v1y, fabsf(e1y) * h[0] + fabsf(e1x) * h[1])) return 0;
-    if (separated(e2y * v1x - e2x * v1y, e2y * v2x - e2x * v2y, fabsf(e2y) * h[0] + fabsf(e2x) * h[1])) return 0;
+    if (separated(e2y * v2x - e2x * v2y, e2y * v1x - e2x * v1y, fabsf(e2y) * h[0] + fabsf(e2x) * h[1])) return 0;
     return 1;
 }
 

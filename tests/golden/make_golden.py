@@ -10,7 +10,9 @@ Run in a tree where /root/reference is mounted:
     make -C oracle && python tests/golden/make_golden.py
 Outputs (committed): tests/golden/sat_cases.npz, tests/golden/mt_cases.npz,
 tests/golden/normal_cases.npz (from includes/FloatVector.h through
-oracle/ref_vec_shim.cpp), tests/golden/manifest.json.
+oracle/ref_vec_shim.cpp), tests/golden/sat_nonfinite.npz (SAT on inputs
+outside the pipeline's domain: NaN, infinities, overflowing products),
+tests/golden/manifest.json.
 """
 import hashlib
 import json
@@ -161,6 +163,39 @@ def mt_cases(rng):
     return cases
 
 
+def nonfinite_cases(rng):
+    """SAT inputs whose projections are NaN or infinite: vertex coordinates
+    replaced by NaN, +-inf or values whose products overflow (inf - inf is
+    NaN), around finite boxes.  The reference's edge macros order the two
+    projections per macro (AXISTEST_Z12 compares p2 before p1), and with a
+    NaN among them that order decides the interval, so these cases fix the
+    operand order of every edge predicate."""
+    cases = {}
+    n = 20000
+    c = _f(rng.uniform(-1, 1, (n, 3)))
+    h = _f(rng.uniform(1e-3, 0.2, (n, 3)))
+    tv = _f(c[:, None, :] + rng.normal(0, 1, (n, 3, 3)) * h[:, None, :] * rng.uniform(0.5, 4, (n, 1, 1)))
+    tv = tv.reshape(n, 9)
+    special = _f([np.nan, np.inf, -np.inf, 1e20, -1e20, 3e38, -3e38])
+    k = rng.integers(1, 4, n)
+    for i in range(n):
+        pos = rng.choice(9, k[i], replace=False)
+        tv[i, pos] = special[rng.integers(0, len(special), k[i])]
+    cases["special"] = (c, h, tv)
+    # one vertex far away: edge components and projections overflow to
+    # +-inf, and differences of them to NaN, on some axes only
+    c = _f(rng.uniform(-1, 1, (n, 3)))
+    h = _f(rng.uniform(1e-3, 0.2, (n, 3)))
+    tv = _f(c[:, None, :] + rng.normal(0, 1, (n, 3, 3)) * h[:, None, :] * 2).reshape(n, 9)
+    v = rng.integers(0, 3, n)
+    ax = rng.integers(0, 3, (n, 2))
+    mag = _f(10 ** rng.uniform(19, 38.5, (n, 2)) * np.where(rng.random((n, 2)) < 0.5, -1, 1))
+    for j in range(2):
+        tv[np.arange(n), 3 * v + ax[:, j]] = mag[:, j]
+    cases["overflow"] = (c, h, tv)
+    return cases
+
+
 def normal_cases(rng, sat):
     """Face normals of every SAT-family triangle (degenerate ones included:
     zero cross products stay zero), VectorNormalize on vectors from 1e-30 to
@@ -203,6 +238,12 @@ def main():
     np.savez_compressed(os.path.join(HERE, "mt_cases.npz"), **out2)
     nc = normal_cases(np.random.default_rng(20261203), sat)
     np.savez_compressed(os.path.join(HERE, "normal_cases.npz"), **nc)
+    nf = nonfinite_cases(np.random.default_rng(20261204))
+    out3 = {}
+    for k, (c, h, tv) in nf.items():
+        out3[f"{k}_c"], out3[f"{k}_h"], out3[f"{k}_tv"] = c, h, tv
+        out3[f"{k}_res"] = orc.tri_box(c, h, tv, use_ref=True)
+    np.savez_compressed(os.path.join(HERE, "sat_nonfinite.npz"), **out3)
     src = "/root/reference/src"
     man = {
         "generator": "tests/golden/make_golden.py",
@@ -219,6 +260,8 @@ def main():
         "sat_hit_rate": {k: float(out[f"{k}_res"].mean()) for k in sat},
         "mt_families": {k: int(len(v[0])) for k, v in mt.items()},
         "mt_hit_rate": {k: float(out2[f"{k}_hit"].mean()) for k in mt},
+        "sat_nonfinite": {"seed": 20261204, "families": {k: int(len(v[0])) for k, v in nf.items()},
+                          "hit_rate": {k: float(out3[f"{k}_res"].mean()) for k in nf}},
     }
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(man, f, indent=1)

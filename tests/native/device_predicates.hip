@@ -1,7 +1,8 @@
 // Test-only harness: runs the device predicates of gpview_amd/csrc/gpv_device.hpp
 // on the GPU over batches of golden cases, so tests/test_gpu_predicates.py can
 // compare them bit for bit with the reference's own TriBoxOverlap and
-// triangle_ray_intersection outputs (tests/golden/{sat,mt}_cases.npz).
+// triangle_ray_intersection outputs (tests/golden/{sat,mt}_cases.npz,
+// tests/golden/sat_nonfinite.npz).
 // Built with the product's flags (no contraction, no fast-math, packed FP32
 // off).  Test tooling only: the product never loads it.
 #include <hip/hip_runtime.h>

@@ -61,6 +61,24 @@ def test_device_sat_golden(dp, fam):
         assert len(bad) == 0, f"{fam}/{name}: {len(bad)} of {len(c)} differ, first case {bad[0]}"
 
 
+@pytest.mark.parametrize("fam", ["special", "overflow"])
+def test_device_sat_nonfinite(dp, fam):
+    """NaN and infinite projections: the pipeline refuses such coordinates,
+    but the device predicates still take the reference's result, because
+    each edge test compares its two projections in the order of the
+    reference's macro."""
+    z = np.load(os.path.join(GOLDEN, "sat_nonfinite.npz"), allow_pickle=False)
+    c, h, tv = _f(z[f"{fam}_c"], 3), _f(z[f"{fam}_h"], 3), _f(z[f"{fam}_tv"], 9)
+    want = z[f"{fam}_res"].astype(np.uint8)
+    out = np.zeros(4 * len(c), np.uint8)
+    say(f"device SAT nonfinite {fam}: {len(c)} cases")
+    assert dp.dp_sat(len(c), c.ravel(), h.ravel(), tv.ravel(), out) == 0
+    out = out.reshape(-1, 4)
+    for k, name in enumerate(VARIANTS):
+        bad = np.nonzero(out[:, k] != want)[0]
+        assert len(bad) == 0, f"{fam}/{name}: {len(bad)} of {len(c)} differ, first case {bad[0]}"
+
+
 @pytest.mark.parametrize("fam", MT_ZRAY_FAMILIES)
 def test_device_mt_golden(dp, fam):
     z = np.load(os.path.join(GOLDEN, "mt_cases.npz"), allow_pickle=False)

@@ -9,6 +9,10 @@
 - Each served batch's measured peak of device bytes stays within the sum of
   its requests' predicted footprints (gpv_predict_footprint), the number
   admission charges against the budget.
+- A vertex coordinate outside the input domain (not finite, or beyond 2^40
+  in magnitude) fails the call with GPV_ERR_INVALID, alone or in a batch,
+  holds nothing afterwards and leaves the context usable; 2^40 itself is
+  inside.
 """
 import numpy as np
 import pytest
@@ -171,3 +175,34 @@ def test_serve_peak_within_predicted_footprint(gv):
     say(f"{len(batches)} batches, peak/predicted max {max(ratios):.3f} median {np.median(ratios):.3f}")
     for bid, (peak, pred, n) in batches.items():
         assert 0 < peak <= pred, f"batch {bid} of {n}: measured {peak} > predicted {pred}"
+
+
+def test_out_of_domain_coordinates_refused(gv):
+    from gpview_amd import workloads
+    v, f = gv.icosphere(3, 0.4, (0.5, 0.5, 0.5), 0.02, seed=11)
+    good = workloads.mesh_entry(gv, v, f, 48, 4)
+    with gv.Context(0) as c:
+        ref = alone(c, good)
+        for bad_value in (np.nan, np.inf, -np.inf, np.float32(2.0 ** 41), np.float32(-3e38)):
+            m = dict(good)
+            soup = good["soup"].copy()
+            soup[77, 4] = bad_value
+            m["soup"] = soup
+            say(f"out of domain: {bad_value}")
+            with pytest.raises(gv.GpvError) as e:
+                alone(c, m)
+            assert e.value.status == gv.GPV_ERR_INVALID
+            assert c.pool_stats()["live_bytes"] == 0
+            batch = [good, good, good, m, good]
+            with pytest.raises(gv.GpvError) as e:
+                c.voxelize_batch(batch)
+            assert e.value.status == gv.GPV_ERR_INVALID
+            assert c.pool_stats()["live_bytes"] == 0
+            same(f"after {bad_value}", alone(c, good), ref)
+        # the domain's edge: a far vertex at exactly 2^40 is voxelized
+        m = dict(good)
+        soup = good["soup"].copy()
+        soup[77, 4] = np.float32(2.0 ** 40)
+        m["soup"] = soup
+        edge = alone(c, m)
+        assert edge[1]["nb"] > 0

@@ -4,7 +4,8 @@
    outputs of the reference's own TriBoxOverlap / triangle_ray_intersection
    (src/TriBoxIntersection.cpp, src/TriRayIntersection.cpp compiled
    unmodified, see tests/golden/make_golden.py).  The oracle must match every
-   case bit for bit (hit flag and the float t).
+   case bit for bit (hit flag and the float t).  sat_nonfinite.npz does the
+   same for SAT inputs with NaN and infinite projections.
 2. Live cross-check against oracle/_ref when the reference tree is mounted.
 3. Mathematics the golden set leaves open: the float thresholds standing in
    for the double 1e-6 compares, the column factorisation of the +z ray, and
@@ -33,6 +34,22 @@ def test_sat_golden(orc, fam):
     want = z[f"{fam}_res"]
     assert len(want) >= 10000
     assert 0.05 < want.mean() < 0.95, "family must exercise both outcomes"
+    assert np.array_equal(got, want), f"{fam}: {np.count_nonzero(got != want)} mismatches"
+
+
+@pytest.mark.parametrize("fam", ["special", "overflow"])
+@pytest.mark.parametrize("factored", [False, True])
+def test_sat_nonfinite_golden(orc, fam, factored):
+    """Inputs outside the pipeline's domain (NaN, infinities, overflowing
+    products), where the reference's per-macro order of the two projections
+    decides the outcome (AXISTEST_Z12 compares p2 before p1): the oracle's
+    literal and factored SAT take the reference's result on every case."""
+    z = _load("sat_nonfinite.npz")
+    got = orc.tri_box(z[f"{fam}_c"], z[f"{fam}_h"], z[f"{fam}_tv"], factored=factored)
+    want = z[f"{fam}_res"]
+    assert len(want) >= 10000 and 0.05 < want.mean() < 0.95
+    tv = z[f"{fam}_tv"]
+    assert np.all(np.any(~(np.abs(tv) <= 2.0 ** 40), axis=1)), "every case lies outside the pipeline's domain"
     assert np.array_equal(got, want), f"{fam}: {np.count_nonzero(got != want)} mismatches"
 
 
