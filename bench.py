@@ -333,6 +333,11 @@ def kernel_bytes(stage, info, T):
     return 36 * T + 2 * ((info["n1"] + 7) // 8)
 
 
+def progress(msg):
+    """One line per finished leg on stderr, so a long run shows its steps."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads, mixed=None, stress_samples=()):
     """The oracle on this host: the CPU baseline's composition on the full
     headline mesh with all cores (value) and one thread; the literal
@@ -349,7 +354,9 @@ def cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads, mixed=None, stress_sam
         return time.perf_counter() - c0, r
 
     t_all, want = run(soup, g0, orc.ORC_PLAN, threads)
+    progress(f"cpu baseline, {threads} threads: {t_all:.2f} s")
     t_one, _ = run(soup, g0, orc.ORC_PLAN, 1)
+    progress(f"cpu baseline, 1 thread: {t_one:.2f} s")
     s = workloads.plain_small(gv)
     g1 = orc.grid_setup(s["bmin"], s["bmax"], s["mms"], 64, 4)
     legs = {}
@@ -357,6 +364,7 @@ def cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads, mixed=None, stress_sam
         for th in (threads, 1):
             t, _ = run(s["soup"], g1, mode, th)
             legs[f"icosphere_5120_{name}_{th}t"] = {"seconds": t, "meshes_per_s": 1.0 / t, "cores": th}
+            progress(f"cpu icosphere {name}, {th} threads: {t:.2f} s")
     mixed_leg = None
     if mixed:
         c0 = time.perf_counter()
@@ -372,6 +380,7 @@ def cpu_baseline(gv, orc, soup, bmin, bmax, mms, threads, mixed=None, stress_sam
         gs = orc.grid_setup(m["bmin"], m["bmax"], m["mms"], 512, 16)
         ts, ws = run(m["soup"], gs, orc.ORC_FAST, threads)
         ok = all(np.array_equal(got[key], ws[key]) for key in ("l1", "prefix", "bidx", "l2"))
+        progress(f"cpu stress member {k}: {ts:.2f} s, bit-exact {ok}")
         stress_leg.append({"member": k, "triangles": len(m["soup"]), "nb": int(ws["nb"]), "seconds": ts,
                            "cores": threads, "mode": "oracle fast mode (factored SAT sweeps)",
                            "gpu_batched_bit_exact": ok})
@@ -460,6 +469,7 @@ def main():
         infos.append(g.info())
         g.release()
     wall = time.perf_counter() - t0
+    progress(f"headline: {args.steps} steps, {wall / args.steps * 1e3:.3f} ms per step")
 
     # one more run outside the timed region, kept for the correctness check
     g = step()
@@ -477,21 +487,29 @@ def main():
             gv, ctx, workloads.mixed_stream(gv, args.batch_meshes, seed=7, tri_range=(1000, 20000), level1=(64,),
                                             level2=(4,)), args.batch_steps, 1, False,
             "small-mesh stream (log-uniform 1k-20k triangles, Level1 64, Level2 4)")
+        progress(f"batched: {batched['batched_meshes_per_s']:.0f} meshes/s (plain {batched['plain_meshes_per_s']:.0f})")
     stress_samples = []
     if args.stress_meshes > 0:
         n = min(args.stress_meshes, 64)
         sizes = [workloads.stress_spec(i)[1] for i in range(n)]
         sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes))}) if not args.no_cpu else []
         stress, stress_samples = bench_stress(gv, ctx, n, args.stress_run, sample)
+        progress(f"stress: {stress['batched_meshes_per_s']:.1f} meshes/s")
     ctx.close()
     served_stream, served_sat = served()
+    if served_stream:
+        progress(f"served: {served_stream['meshes_per_s']:.0f} meshes/s at {served_stream['rate_per_s']:.0f}/s")
 
     meshes_per_s = args.steps / wall
     tests = info["tests_l1"] + info["tests_l2"]
     n2 = info["n2"]
     voxels = info["n1"] + info["nb"] * n2 ** 3
     sms = stage_ms(infos)
-    dom = max(STAGES, key=lambda s: sms[s])
+    # the dominant kernel: the Level2 SAT has the most device time in the
+    # committed trace of this workload, and its stage's event pair brackets
+    # exactly its one launch (the Level2 parity stage spans two kernels and
+    # the column sort's wait, so its time is not one kernel's)
+    dom = "l2_sat"
     t_dom = sms[dom] / 1e3
     bytes_dom = kernel_bytes(dom, info, T)
     meas = measured(STAGE_KERNEL.get(dom))
@@ -500,7 +518,9 @@ def main():
           "traffic": meas["traffic_bytes"] if meas else None,
           "kernel": meas["name"] if meas else STAGE_KERNEL.get(dom, dom),
           "algorithmic_bytes": bytes_dom, "ms": sms[dom],
-          "traffic_source": meas["source"] if meas else None}
+          "traffic_source": meas["source"] if meas else None,
+          "trace_avg_us": meas.get("avg_us_trace") if meas else None,
+          "alone_us": meas.get("avg_us_alone") if meas else None}
     # The path is not a contraction (no MFMA); its binding resource is VALU
     # issue: VALU wave instructions of the same kernel and workload (SQ
     # counters, committed summary) x 64 lanes / this run's launch time,
