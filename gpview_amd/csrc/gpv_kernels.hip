@@ -1765,26 +1765,23 @@ __device__ __forceinline__ void l2_sat_run_v(uint32_t ent, bool valid, const flo
     if (hit) atomicOr(&acc[s >> 6], 1ull << (s & 63));
 }
 
+// Eight waves per SIMD: at most 64 VGPRs and 4.7 KB of LDS per wave (the
+// staged entries' masks and item offsets stay in their lanes' registers and
+// are read across lanes).
 template <bool kM, int W>
-__global__ __launch_bounds__(64 * W) void k_l2_sat_voxel(const float* __restrict__ tris,
-                                                          const int32_t* __restrict__ bidx, int64_t B,
-                                                          const uint32_t* __restrict__ boff,
-                                                          const uint32_t* __restrict__ list,
-                                                          const uint64_t* __restrict__ pmask,
-                                                          const float4* __restrict__ slab2, GridDev g0,
-                                                          Meshes M, uint64_t* __restrict__ sat2)
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(8))) void k_l2_sat_voxel(
+    const float* __restrict__ tris, const int32_t* __restrict__ bidx, int64_t B, const uint32_t* __restrict__ boff,
+    const uint32_t* __restrict__ list, const uint64_t* __restrict__ pmask, const float4* __restrict__ slab2,
+    GridDev g0, Meshes M, uint64_t* __restrict__ sat2)
 {
     __shared__ unsigned long long s_acc[W][64];
     __shared__ float s_tv[W][64 * 9];
     __shared__ uint32_t s_q[W][L2_QCAP];
-    // per staged triangle: sub-box origin and extents, masks, slab, 1/dp, and
-    // the start of its sub-columns in the chunk's flattened (triangle,
-    // sub-column) item space
+    // per staged triangle: sub-box origin and extents, and its slab (its
+    // masks and the start of its sub-columns in the chunk's flattened
+    // (triangle, sub-column) item space stay in the staging lane)
     __shared__ uint32_t s_dec[W][64];
-    __shared__ unsigned long long s_mk[W][64];
     __shared__ float4 s_sl[W][64];
-    __shared__ float s_idp[W][64];
-    __shared__ uint32_t s_off[W][64];
     __shared__ float s_cz[W][16];
     __shared__ uint8_t s_win[W][64];   // per round: staged entry starting at each item position
     const int wv = threadIdx.x >> 6;
@@ -1815,10 +1812,7 @@ __global__ __launch_bounds__(64 * W) void k_l2_sat_voxel(const float* __restrict
     float* stv = s_tv[wv];
     uint32_t* sq = s_q[wv];
     uint32_t* dec = s_dec[wv];
-    unsigned long long* smk = s_mk[wv];
     float4* ssl = s_sl[wv];
-    float* sidp = s_idp[wv];
-    uint32_t* soff = s_off[wv];
     uint8_t* win = s_win[wv];
     __builtin_amdgcn_wave_barrier();
     uint32_t qh = 0, qt = 0;   // ring head/tail (uniform)
@@ -1837,6 +1831,7 @@ __global__ __launch_bounds__(64 * W) void k_l2_sat_voxel(const float* __restrict
         // its sub-box decode go to LDS
         const uint32_t i = base + lane;
         uint32_t P = 0;
+        uint64_t mk_reg = 0;
         if (i < e1) {
             const uint64_t t = list[i];
 #pragma unroll
@@ -1855,15 +1850,13 @@ __global__ __launch_bounds__(64 * W) void k_l2_sat_voxel(const float* __restrict
                     (__popc(mx) == dp && __popc(my) == dq && __popc(mz) == dr) ? 1u : 0u;
                 dec[lane] = (uint32_t)pa | ((uint32_t)qa << 4) | ((uint32_t)ra << 8) | ((uint32_t)rb << 12) |
                             ((uint32_t)dp << 16) | (boxbit << 24);
-                smk[lane] = mk;
+                mk_reg = mk;
                 ssl[lane] = slab2[t];
-                sidp[lane] = 1.0f / (float)dp;
                 P = (uint32_t)(dp * dq);
             }
         }
         const uint32_t incl = wave_incl_scan<uint32_t>(P);
         const uint32_t so = incl - P;   // this lane's entry: items [so, so + P)
-        soff[lane] = so;
         const uint32_t S = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         __builtin_amdgcn_wave_barrier();
         // the chunk's (triangle, sub-column) items, 64 per round
@@ -1880,6 +1873,10 @@ __global__ __launch_bounds__(64 * W) void k_l2_sat_voxel(const float* __restrict
             const uint64_t marks = __ballot(wj != 0xffu) & ((2ull << lane) - 1ull);
             const int top = marks ? 63 - __clzll((long long)marks) : 0;
             const int jw = __shfl((int)wj, top);
+            // the item's entry fields, read across lanes by every lane
+            const uint32_t so_j = (uint32_t)__shfl((int)so, jw);
+            const uint32_t mklo_j = (uint32_t)__shfl((int)(uint32_t)mk_reg, jw);
+            const uint32_t mkhi_j = (uint32_t)__shfl((int)(uint32_t)(mk_reg >> 32), jw);
             __builtin_amdgcn_wave_barrier();
             const uint32_t l = l0 + (uint32_t)lane;
             uint32_t keep = 0;
@@ -1894,14 +1891,13 @@ __global__ __launch_bounds__(64 * W) void k_l2_sat_voxel(const float* __restrict
                 rb = (d >> 12) & 15;
                 const int dp = (d >> 16) & 255;
                 boxbit = (d >> 24) & 1u;
-                const int lj = (int)(l - soff[j]);
+                const int lj = (int)(l - so_j);
                 // exact small-integer division: (lj + 0.5)/dp is >= 1/32 from an integer
-                const int rem = (int)(((float)lj + 0.5f) * sidp[j]);
+                const int rem = (int)(((float)lj + 0.5f) * kInvSmall[dp]);
                 p = pa + lj - rem * dp;
                 q = qa + rem;
-                const uint64_t mk = smk[j];
-                const uint32_t mx = (uint32_t)(mk & 0xffffu), my = (uint32_t)((mk >> 16) & 0xffffu);
-                mz = (uint32_t)((mk >> 32) & 0xffffu);
+                const uint32_t mx = mklo_j & 0xffffu, my = mklo_j >> 16;
+                mz = mkhi_j & 0xffffu;
                 in = ((mx >> p) & (my >> q) & 1u) != 0;
             }
             // every lane takes part in the cross-lane reads
