@@ -957,7 +957,8 @@ struct ColPlace {
         if (!ok) return;
         rec[3 * (uint64_t)pos + 0] = make_float4(m.e1x, m.e1y, m.e1z, m.e2x);
         rec[3 * (uint64_t)pos + 1] = make_float4(m.e2y, m.e2z, m.inv_det, tv[0]);
-        rec[3 * (uint64_t)pos + 2] = make_float4(tv[1], tv[2], __int_as_float((int)it.z), __uint_as_float(t));
+        // the last word: the entry's batch-wide column (read by the packed column pass)
+        rec[3 * (uint64_t)pos + 2] = make_float4(tv[1], tv[2], __int_as_float((int)it.z), __uint_as_float(it.x));
     }
 };
 
@@ -2136,24 +2137,90 @@ __global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass(const uint32_t* __re
     }
 }
 
+// The same pass for one mesh with the column entries packed 64 to a wave
+// across columns (one wave per column leaves most lanes idle when columns
+// hold a few dozen entries).  Each lane takes its column from its record
+// and its sub-column origins from the tables; the bits and header written
+// per entry are those of k_col_pass.
+template <int CP_WAVES>
+__global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass_packed(const float4* __restrict__ rec, GridDev g,
+                                                                   uint32_t nent, int stride,
+                                                                   uint64_t* __restrict__ cpass)
+{
+    const uint64_t wbase = ((uint64_t)blockIdx.x * CP_WAVES + (threadIdx.x >> 6)) * 64;
+    const int n2 = g.n2;
+    if (wbase >= nent || n2 == 0) return;
+    const uint32_t e = (uint32_t)wbase + (uint32_t)__lane_id();
+    const bool own = e < nent;
+    const uint32_t ei = own ? e : nent - 1;
+    const ColRec r = load_colrec(rec, ei);
+    const uint32_t col = __float_as_uint(rec[3 * (uint64_t)ei + 2].w);
+    const int i1 = (int)(col % (uint32_t)g.nx), j1 = (int)(col / (uint32_t)g.nx);
+    const float* rx = g.r2x + (int64_t)i1 * n2;
+    const float* ry = g.r2y + (int64_t)j1 * n2;
+    uint64_t* dst = cpass + (uint64_t)ei * stride;
+    {
+        const int zc = g.l2cuda ? g.nz - 1 : r.zmax;
+        int kh = -1;
+        if (g.r2mono != 0) {
+            const float oxlo = rx[0], oxhi = rx[n2 - 1], oylo = ry[0], oyhi = ry[n2 - 1];
+            const float oz0 = g.r2z[0];
+#pragma unroll
+            for (int d = 8; d >= 1; d >>= 1) {
+                const int k = r.zmax - d;
+                if (k >= 0 && t_decided(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, r.v1x, r.v1y, r.v1z, oxlo, oxhi,
+                                        oylo, oyhi, oz0, g.r2z[(int64_t)k * n2 + n2 - 1], eps_above()) > 0)
+                    kh = k;
+            }
+        }
+        if (own) dst[stride - 1] = (uint64_t)(uint32_t)(kh + 1) | ((uint64_t)(uint32_t)(zc + 1) << 32);
+    }
+    // the lane's sub-column x origins in registers
+    float oxr[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) oxr[p] = p < n2 ? rx[p] : 0.0f;
+    uint64_t w = 0;
+    int k = 0, slot = 0;
+    for (int q = 0; q < n2; ++q) {
+        const float oy = ry[q];
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            if (p < n2) {
+                float tx, ty;
+                w |= (uint64_t)colrec_xy(r, oxr[p], oy, tx, ty) << k;
+                if (++k == 64) {
+                    if (own) dst[slot] = w;
+                    ++slot;
+                    w = 0;
+                    k = 0;
+                }
+            }
+        }
+    }
+    if (k && own) dst[slot] = w;
+}
+
 template <int CW>
 static void launch_col_pass_w(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M,
-                              int64_t ncol, int stride, uint64_t* cpass, hipStream_t s)
+                              int64_t ncol, uint64_t nent, int stride, uint64_t* cpass, hipStream_t s)
 {
-    const int64_t blocks = (ncol + CW - 1) / CW;
-    if (M.nm > 1)
+    if (M.nm > 1) {
+        const int64_t blocks = (ncol + CW - 1) / CW;
         k_col_pass<true, CW><<<(unsigned)blocks, 64 * CW, 0, s>>>(coff, rec, g0, M, ncol, stride, cpass);
-    else
-        k_col_pass<false, CW><<<(unsigned)blocks, 64 * CW, 0, s>>>(coff, rec, g0, M, ncol, stride, cpass);
+    } else {
+        const uint64_t waves = (nent + 63) / 64;
+        const uint64_t blocks = (waves + CW - 1) / CW;
+        k_col_pass_packed<CW><<<(unsigned)blocks, 64 * CW, 0, s>>>(rec, g0, (uint32_t)nent, stride, cpass);
+    }
 }
 
 void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M, int64_t ncol,
-                     int stride, int waves, uint64_t* cpass, hipStream_t s)
+                     uint64_t nent, int stride, int waves, uint64_t* cpass, hipStream_t s)
 {
-    if (ncol <= 0) return;
-    if (waves <= 1) launch_col_pass_w<1>(coff, rec, g0, M, ncol, stride, cpass, s);
-    else if (waves == 2) launch_col_pass_w<2>(coff, rec, g0, M, ncol, stride, cpass, s);
-    else launch_col_pass_w<4>(coff, rec, g0, M, ncol, stride, cpass, s);
+    if (ncol <= 0 || nent == 0) return;
+    if (waves <= 1) launch_col_pass_w<1>(coff, rec, g0, M, ncol, nent, stride, cpass, s);
+    else if (waves == 2) launch_col_pass_w<2>(coff, rec, g0, M, ncol, nent, stride, cpass, s);
+    else launch_col_pass_w<4>(coff, rec, g0, M, ncol, nent, stride, cpass, s);
 }
 
 // One boundary voxel (i1, j1, k1) of the Level2 parity.  N2 > 0: the Level2
