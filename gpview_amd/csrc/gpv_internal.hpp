@@ -183,9 +183,8 @@ void launch_l2_sat(const float* tris, const int32_t* bidx, int64_t B, const uint
                    const uint64_t* pmask, const float4* slab2, const uint32_t* gstart, const uint32_t* ngroups,
                    int64_t gmax, int waves, const GridDev& g0, const Meshes& M, uint64_t* sat2, hipStream_t s);
 // det/u/v pass bits per (column entry, sub-column): stride words per entry
-void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M, int64_t ncol,
-                     uint64_t nent,
-                     int stride, int waves, uint64_t* cpass, hipStream_t s);
+void launch_col_pass(const float4* rec, const GridDev& g0, const Meshes& M, uint64_t nent, int stride, int waves,
+                     uint64_t* cpass, hipStream_t s);
 // Level2 parity waves of one mesh: one per boundary voxel, one per four when
 // n2 <= 4, none when Level2 is off
 int64_t l2_parity_waves(int n2, int64_t nb);

@@ -2058,8 +2058,8 @@ __device__ __forceinline__ int mesh_of_col(const MeshOff* __restrict__ off, int 
 
 // The z-independent half of the MT test (det/u/v, src/TriRayIntersection.cpp:
 // 78-131 with D = +z) depends on the column entry and the sub-column's (x, y)
-// only, so it runs once per (column entry, sub-column) here: one wave per xy
-// column, lane = column entry, looping over the column's n2^2 sub-columns
+// only, so it runs once per (column entry, sub-column) here: one lane per
+// column entry, looping over its column's n2^2 sub-columns
 // (sc = q*n2 + p); entry e's pass bits go to cpass[e * stride + sc / 64], bit
 // sc % 64.  Every boundary voxel of the column then reads them instead of
 // repeating the test.  The entry's last word, cpass[e * stride + stride - 1],
@@ -2070,95 +2070,20 @@ __device__ __forceinline__ int mesh_of_col(const MeshOff* __restrict__ off, int 
 // layers [0, kh] (t_decided over that box; -1 if none found).  For a voxel at
 // k1 <= kh the entry then flips every layer of each sub-column it passes
 // without a t test.
-template <bool kM, int CP_WAVES>
-__global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass(const uint32_t* __restrict__ coff,
-                                                            const float4* __restrict__ rec, GridDev g0, Meshes M,
-                                                            int64_t ncol, int stride, uint64_t* __restrict__ cpass)
+// The column entries are packed 64 to a wave across columns (one wave per
+// column would leave most lanes idle: columns hold a few dozen entries).
+// Each lane takes its column from its record and its sub-column origins from
+// the tables.  In a batch a wave's entries can span meshes (they are in
+// column order, and each mesh's columns are contiguous): the wave takes its
+// meshes one after another, each with the lanes whose columns it holds.
+__device__ __forceinline__ void col_pass_entry(const GridDev& g, uint32_t col, const ColRec& r, bool own,
+                                               uint64_t* __restrict__ dst, int stride)
 {
-    const int64_t c = (int64_t)blockIdx.x * CP_WAVES + (threadIdx.x >> 6);
-    if (c >= ncol) return;
-    const uint32_t e0 = coff[c], e1 = coff[c + 1];
-    if (e0 == e1) return;
-    const int lane = __lane_id();
-    const int mesh = kM ? uniform_i(mesh_of_col(M.off, M.nm, (uint32_t)c)) : 0;
-    const GridDev& g = grid_of<kM>(g0, M, mesh);
     const int n2 = g.n2;
     if (n2 == 0) return;
-    const uint32_t col = (uint32_t)c - (kM ? M.off[mesh].col0 : 0u);
-    const int i1 = (int)(col % (uint32_t)g.nx), j1 = (int)(col / (uint32_t)g.nx);
-    // sub-column origins: x of sub-index p in lane p, y of q in lane q
-    const float rxl = lane < n2 ? g.r2x[(int64_t)i1 * n2 + lane] : 0.0f;
-    const float ryl = lane < n2 ? g.r2y[(int64_t)j1 * n2 + lane] : 0.0f;
-    // the column's box of ray origins (tables non-decreasing)
-    const bool tbox = g.r2mono != 0;
-    const float oxlo = g.r2x[(int64_t)i1 * n2], oxhi = g.r2x[(int64_t)i1 * n2 + n2 - 1];
-    const float oylo = g.r2y[(int64_t)j1 * n2], oyhi = g.r2y[(int64_t)j1 * n2 + n2 - 1];
-    const float oz0 = g.r2z[0];
-    for (uint32_t base = e0; base < e1; base += 64) {
-        const uint32_t e = base + (uint32_t)lane;
-        const bool own = e < e1;
-        const ColRec r = load_colrec(rec, own ? e : e0);
-        uint64_t* dst = cpass + (uint64_t)e * stride;
-        {
-            const int zc = g.l2cuda ? g.nz - 1 : r.zmax;
-            // kh: the largest of the layers zmax - 1, - 2, - 4, - 8 whose box
-            // [layer 0, that layer] t_decided passes (below its lowest layer a
-            // triangle lies above every origin; the few probes are independent).
-            // A probe that returns +1 is itself the proof for its box.
-            int kh = -1;
-            if (tbox) {
-#pragma unroll
-                for (int d = 8; d >= 1; d >>= 1) {
-                    const int k = r.zmax - d;
-                    if (k >= 0 && t_decided(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, r.v1x, r.v1y, r.v1z, oxlo,
-                                            oxhi, oylo, oyhi, oz0, g.r2z[(int64_t)k * n2 + n2 - 1], eps_above()) > 0)
-                        kh = k;
-                }
-            }
-            if (own) dst[stride - 1] = (uint64_t)(uint32_t)(kh + 1) | ((uint64_t)(uint32_t)(zc + 1) << 32);
-        }
-        uint64_t w = 0;
-        int k = 0, slot = 0;
-        for (int q = 0; q < n2; ++q) {
-            const float oy = lane_f(ryl, q);
-            for (int p = 0; p < n2; ++p) {
-                const float ox = lane_f(rxl, p);
-                float tx, ty;
-                w |= (uint64_t)colrec_xy(r, ox, oy, tx, ty) << k;
-                if (++k == 64) {
-                    if (own) dst[slot] = w;
-                    ++slot;
-                    w = 0;
-                    k = 0;
-                }
-            }
-        }
-        if (k && own) dst[slot] = w;
-    }
-}
-
-// The same pass for one mesh with the column entries packed 64 to a wave
-// across columns (one wave per column leaves most lanes idle when columns
-// hold a few dozen entries).  Each lane takes its column from its record
-// and its sub-column origins from the tables; the bits and header written
-// per entry are those of k_col_pass.
-template <int CP_WAVES>
-__global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass_packed(const float4* __restrict__ rec, GridDev g,
-                                                                   uint32_t nent, int stride,
-                                                                   uint64_t* __restrict__ cpass)
-{
-    const uint64_t wbase = ((uint64_t)blockIdx.x * CP_WAVES + (threadIdx.x >> 6)) * 64;
-    const int n2 = g.n2;
-    if (wbase >= nent || n2 == 0) return;
-    const uint32_t e = (uint32_t)wbase + (uint32_t)__lane_id();
-    const bool own = e < nent;
-    const uint32_t ei = own ? e : nent - 1;
-    const ColRec r = load_colrec(rec, ei);
-    const uint32_t col = __float_as_uint(rec[3 * (uint64_t)ei + 2].w);
     const int i1 = (int)(col % (uint32_t)g.nx), j1 = (int)(col / (uint32_t)g.nx);
     const float* rx = g.r2x + (int64_t)i1 * n2;
     const float* ry = g.r2y + (int64_t)j1 * n2;
-    uint64_t* dst = cpass + (uint64_t)ei * stride;
     {
         const int zc = g.l2cuda ? g.nz - 1 : r.zmax;
         int kh = -1;
@@ -2200,27 +2125,52 @@ __global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass_packed(const float4*
     if (k && own) dst[slot] = w;
 }
 
-template <int CW>
-static void launch_col_pass_w(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M,
-                              int64_t ncol, uint64_t nent, int stride, uint64_t* cpass, hipStream_t s)
+template <bool kM, int CP_WAVES>
+__global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass(const float4* __restrict__ rec, GridDev g0,
+                                                                   Meshes M, uint32_t nent, int stride,
+                                                                   uint64_t* __restrict__ cpass)
 {
-    if (M.nm > 1) {
-        const int64_t blocks = (ncol + CW - 1) / CW;
-        k_col_pass<true, CW><<<(unsigned)blocks, 64 * CW, 0, s>>>(coff, rec, g0, M, ncol, stride, cpass);
+    const uint64_t wbase = ((uint64_t)blockIdx.x * CP_WAVES + (threadIdx.x >> 6)) * 64;
+    if (wbase >= nent) return;
+    const int lane = __lane_id();
+    const uint32_t e = (uint32_t)wbase + (uint32_t)lane;
+    const bool own = e < nent;
+    const uint32_t ei = own ? e : nent - 1;
+    const ColRec r = load_colrec(rec, ei);
+    const uint32_t colb = __float_as_uint(rec[3 * (uint64_t)ei + 2].w);   // batch-wide column
+    uint64_t* dst = cpass + (uint64_t)ei * stride;
+    if constexpr (!kM) {
+        col_pass_entry(g0, colb, r, own, dst, stride);
     } else {
-        const uint64_t waves = (nent + 63) / 64;
-        const uint64_t blocks = (waves + CW - 1) / CW;
-        k_col_pass_packed<CW><<<(unsigned)blocks, 64 * CW, 0, s>>>(rec, g0, (uint32_t)nent, stride, cpass);
+        const int lastl = nent - 1 - wbase < 63 ? (int)(nent - 1 - wbase) : 63;
+        const uint32_t cf = (uint32_t)__shfl((int)colb, 0), cl = (uint32_t)__shfl((int)colb, lastl);
+        const int mf = uniform_i(mesh_of_col(M.off, M.nm, cf)), ml = uniform_i(mesh_of_col(M.off, M.nm, cl));
+        for (int m = mf; m <= ml; ++m) {
+            const uint32_t c0 = M.off[m].col0, c1 = M.off[m + 1].col0;
+            if (colb >= c0 && colb < c1) col_pass_entry(M.grids[m], colb - c0, r, own, dst, stride);
+        }
     }
 }
 
-void launch_col_pass(const uint32_t* coff, const float4* rec, const GridDev& g0, const Meshes& M, int64_t ncol,
-                     uint64_t nent, int stride, int waves, uint64_t* cpass, hipStream_t s)
+template <int CW>
+static void launch_col_pass_w(const float4* rec, const GridDev& g0, const Meshes& M, uint64_t nent, int stride,
+                              uint64_t* cpass, hipStream_t s)
 {
-    if (ncol <= 0 || nent == 0) return;
-    if (waves <= 1) launch_col_pass_w<1>(coff, rec, g0, M, ncol, nent, stride, cpass, s);
-    else if (waves == 2) launch_col_pass_w<2>(coff, rec, g0, M, ncol, nent, stride, cpass, s);
-    else launch_col_pass_w<4>(coff, rec, g0, M, ncol, nent, stride, cpass, s);
+    const uint64_t waves = (nent + 63) / 64;
+    const uint64_t blocks = (waves + CW - 1) / CW;
+    if (M.nm > 1)
+        k_col_pass<true, CW><<<(unsigned)blocks, 64 * CW, 0, s>>>(rec, g0, M, (uint32_t)nent, stride, cpass);
+    else
+        k_col_pass<false, CW><<<(unsigned)blocks, 64 * CW, 0, s>>>(rec, g0, M, (uint32_t)nent, stride, cpass);
+}
+
+void launch_col_pass(const float4* rec, const GridDev& g0, const Meshes& M, uint64_t nent, int stride, int waves,
+                     uint64_t* cpass, hipStream_t s)
+{
+    if (nent == 0) return;
+    if (waves <= 1) launch_col_pass_w<1>(rec, g0, M, nent, stride, cpass, s);
+    else if (waves == 2) launch_col_pass_w<2>(rec, g0, M, nent, stride, cpass, s);
+    else launch_col_pass_w<4>(rec, g0, M, nent, stride, cpass, s);
 }
 
 // One boundary voxel (i1, j1, k1) of the Level2 parity.  N2 > 0: the Level2

@@ -1071,8 +1071,7 @@ int l2_parity_kernels(Job* J, hipStream_t s)
 {
     const int stride = cpass_stride(J);
     const gpv_tune& tu = J->ctx->tune;
-    launch_col_pass(J->coff, J->crec, J->g[0], J->view(), J->ncol, J->n_cols, stride, tu.l2_parity_waves, J->cpass,
-                    s);
+    launch_col_pass(J->crec, J->g[0], J->view(), J->n_cols, stride, tu.l2_parity_waves, J->cpass, s);
     launch_l2_parity(J->bidx, J->B, J->l2p_waves, J->coff, J->crec, J->cpass, stride, tu.l2_parity_waves, J->g[0],
                      J->view(), J->par2,
                      s);
