@@ -137,6 +137,27 @@ __device__ __forceinline__ int mesh_of_word(const MeshOff* __restrict__ off, int
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Largest m in [0, nm) with key(off[m]) <= x for a wave-uniform x (given
+// key(off[0]) <= x), by the whole wave: 64 meshes per round of independent
+// loads, so two rounds of latency up to 4096 meshes instead of log2(nm).
+template <typename K>
+__device__ __forceinline__ int wave_mesh_search(const MeshOff* __restrict__ off, int nm, uint64_t x, K key)
+{
+    const int lane = __lane_id();
+    int lo = 0, hi = nm;   // key(off[lo]) <= x; the answer lies in [lo, hi)
+    while (hi - lo > 1) {
+        const int step = (hi - lo + 63) / 64;
+        const int idx = lo + lane * step;
+        const bool le = idx < hi && (uint64_t)key(off[idx]) <= x;
+        const uint64_t m = __ballot(le);
+        const int L = 63 - __clzll((long long)m);
+        const int nlo = lo + L * step;
+        hi = nlo + step < hi ? nlo + step : hi;
+        lo = nlo;
+    }
+    return uniform_i(lo);
+}
+
 // ---------------------------------------------------------------------------
 // fills (used instead of memset so that the same code can sit in a graph)
 // ---------------------------------------------------------------------------
@@ -969,19 +990,7 @@ struct ColPlace {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int wave_mesh_of_task(const MeshOff* __restrict__ off, int nm, uint64_t task)
 {
-    const int lane = __lane_id();
-    int lo = 0, hi = nm;   // off[lo].ptask0 <= task; the answer lies in [lo, hi)
-    while (hi - lo > 1) {
-        const int step = (hi - lo + 63) / 64;
-        const int idx = lo + lane * step;
-        const bool le = idx < hi && (uint64_t)off[idx].ptask0 <= task;
-        const uint64_t m = __ballot(le);
-        const int L = 63 - __clzll((long long)m);
-        const int nlo = lo + L * step;
-        hi = nlo + step < hi ? nlo + step : hi;
-        lo = nlo;
-    }
-    return uniform_i(lo);
+    return wave_mesh_search(off, nm, task, [](const MeshOff& o) { return (uint64_t)o.ptask0; });
 }
 
 template <bool kM>
@@ -2044,18 +2053,6 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t v)
     return v;
 }
 
-// the mesh whose columns hold global column c (c uniform)
-__device__ __forceinline__ int mesh_of_col(const MeshOff* __restrict__ off, int nm, uint32_t c)
-{
-    int lo = 0, hi = nm;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (off[mid].col0 <= c) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // The z-independent half of the MT test (det/u/v, src/TriRayIntersection.cpp:
 // 78-131 with D = +z) depends on the column entry and the sub-column's (x, y)
 // only, so it runs once per (column entry, sub-column) here: one lane per
@@ -2144,7 +2141,8 @@ __global__ __launch_bounds__(64 * CP_WAVES) void k_col_pass(const float4* __rest
     } else {
         const int lastl = nent - 1 - wbase < 63 ? (int)(nent - 1 - wbase) : 63;
         const uint32_t cf = (uint32_t)__shfl((int)colb, 0), cl = (uint32_t)__shfl((int)colb, lastl);
-        const int mf = uniform_i(mesh_of_col(M.off, M.nm, cf)), ml = uniform_i(mesh_of_col(M.off, M.nm, cl));
+        auto col0 = [](const MeshOff& o) { return (uint64_t)o.col0; };
+        const int mf = wave_mesh_search(M.off, M.nm, cf, col0), ml = wave_mesh_search(M.off, M.nm, cl, col0);
         for (int m = mf; m <= ml; ++m) {
             const uint32_t c0 = M.off[m].col0, c1 = M.off[m + 1].col0;
             if (colb >= c0 && colb < c1) col_pass_entry(M.grids[m], colb - c0, r, own, dst, stride);
@@ -2400,15 +2398,8 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
     const int64_t w = (int64_t)blockIdx.x * L2P_WAVES + wv;
     if (w >= nwaves) return;
     int mesh = 0;
-    if constexpr (kM) {
-        int lo = 0, hi = M.nm;   // the mesh whose waves hold w (w uniform)
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if ((int64_t)M.off[mid].pw0 <= w) lo = mid;
-            else hi = mid;
-        }
-        mesh = uniform_i(lo);
-    }
+    if constexpr (kM)   // the mesh whose waves hold w
+        mesh = wave_mesh_search(M.off, M.nm, (uint64_t)w, [](const MeshOff& o) { return (uint64_t)o.pw0; });
     const GridDev& g = grid_of<kM>(g0, M, mesh);
     const int n2 = g.n2, w2 = g.w2, gnx = g.nx, gny = g.ny;
     if (kM && n2 == 0) return;
