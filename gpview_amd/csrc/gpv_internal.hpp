@@ -188,8 +188,9 @@ void launch_col_pass(const float4* rec, const GridDev& g0, const Meshes& M, uint
 // Level2 parity waves of one mesh: one per boundary voxel, one per four when
 // n2 <= 4, none when Level2 is off
 int64_t l2_parity_waves(int n2, int64_t nb);
+// n2max: the largest Level2 resolution of the run
 void launch_l2_parity(const int32_t* bidx, int64_t B, int64_t nwaves, const uint32_t* coff, const float4* rec,
-                      const uint64_t* cpass, int stride, int waves, const GridDev& g0, const Meshes& M,
+                      const uint64_t* cpass, int stride, int n2max, int waves, const GridDev& g0, const Meshes& M,
                       uint64_t* par2, hipStream_t s);
 
 // normals (src/Object.cpp:3219-3253 Level1, 2372-2421 + 2705-2720 Level2)

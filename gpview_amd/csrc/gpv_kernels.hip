@@ -2171,14 +2171,35 @@ void launch_col_pass(const float4* rec, const GridDev& g0, const Meshes& M, uint
     else launch_col_pass_w<4>(rec, g0, M, nent, stride, cpass, s);
 }
 
+// How the t test of a column record moves with the ray origin's z, for every
+// sub-column: 0 it passes on a prefix of rising z (or is constant), 1 on a
+// suffix, 2 not known to be monotone.  Rounding is monotone, so each float
+// operation of mt_t_hit is monotone in a monotone operand: tz rises with z,
+// qx = ty e1z - tz e1y moves with -sgn(e1y), qy = tz e1x - tx e1z with
+// sgn(e1x), qz is constant; e2x qx and e2y qy then move with -sgn(e2x)
+// sgn(e1y) and sgn(e2y) sgn(e1x), their left-to-right sum (plus the constant
+// e2z qz) is monotone when those do not oppose, and the product by inv flips
+// it for inv < 0.  Every intermediate is finite: the input domain (vertex
+// coordinates within +-2^40, checked by the set-up kernel) bounds the edges,
+// origins and differences by 2^42 and |inv| by 1e6, so no product or sum
+// reaches the float range.  The directions do not depend on (tx, ty).
+__device__ __forceinline__ int sgnf(float x) { return (x > 0.0f) - (x < 0.0f); }
+
+__device__ __forceinline__ int t_run_kind(const ColRec& r)
+{
+    const int da = -sgnf(r.e2x) * sgnf(r.e1y), db = sgnf(r.e2y) * sgnf(r.e1x);
+    if (da * db < 0) return 2;
+    return (da ? da : db) * sgnf(r.inv) > 0 ? 1 : 0;
+}
+
 // One boundary voxel (i1, j1, k1) of the Level2 parity.  N2 > 0: the Level2
 // resolution as a compile-time constant (layer loops unrolled, the layer
 // origins held uniform); N2 == 0: any resolution n2 <= 16.
-template <int N2>
+template <int N2, bool kRuns = false>
 __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev& g, int i1, int j1, int k1,
                                                 uint32_t e0, uint32_t e1, const float4* __restrict__ rec,
                                                 const uint64_t* __restrict__ cpass, int stride, ColRec* cr,
-                                                unsigned long long* acc, uint64_t* __restrict__ out)
+                                                unsigned long long* acc, float* soz, uint64_t* __restrict__ out)
 {
     const int lane = __lane_id();
     const int n2 = N2 > 0 ? N2 : n2_rt;
@@ -2202,6 +2223,11 @@ __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev
     float oz[N2 > 0 ? N2 : 1];
 #pragma unroll
     for (int rr = 0; rr < (N2 > 0 ? N2 : 1); ++rr) oz[rr] = lane_f(ozl, rr);
+    // resolutions without a fixed layer loop: the layer origins also in LDS,
+    // for per-lane searches along the layers (tables non-decreasing)
+    const bool zruns = kRuns && N2 == 0 && g.r2mono != 0 && n2 >= 8;
+    if (kRuns && N2 == 0 && lane < n2) soz[lane] = ozl;
+    __builtin_amdgcn_wave_barrier();
     const uint64_t layers = (1ull << n2) - 1;
     uint64_t bits = 0;   // bit (mm*n2 + r) for sub-column lane + 64*mm
     for (uint32_t base = e0; base < e1; base += 64) {
@@ -2220,7 +2246,11 @@ __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev
                 for (int mm = 0; mm < NMX; ++mm)
                     if (mm < nm) row[mm] = cpass[(uint64_t)i * stride + mm];
                 hit = k1c <= kh;
-                if (!hit) cr[lane] = load_colrec(rec, i);
+                if (!hit) {
+                    ColRec r = load_colrec(rec, i);
+                    if (kRuns) r.zmax = t_run_kind(r);   // the slot is free here: the header holds zc
+                    cr[lane] = r;
+                }
             }
         }
         const uint64_t all_hit = __ballot(hit);
@@ -2246,10 +2276,24 @@ __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev
                                                  oz[rr])
                               << rr;
                 } else {
-                    for (int rr = 0; rr < n2; ++rr)
-                        hm |= (uint32_t)mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z,
-                                                 lane_f(ozl, rr))
-                              << rr;
+                    if (kRuns && zruns && r.zmax < 2) {
+                        // the passing layers are one run at an end: find its edge
+                        const bool pre = r.zmax == 0;
+                        int lo = 0, hi = n2;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            const bool t = mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z,
+                                                    soz[mid]);
+                            if (t == pre) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        hm = pre ? (uint32_t)((1ull << lo) - 1) : (uint32_t)(layers & ~((1ull << lo) - 1));
+                    } else {
+                        for (int rr = 0; rr < n2; ++rr)
+                            hm |= (uint32_t)mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z,
+                                                     lane_f(ozl, rr))
+                                  << rr;
+                    }
                 }
                 bits ^= (uint64_t)hm << (mm * n2);
             }
@@ -2385,7 +2429,10 @@ __device__ __forceinline__ void l2_parity_pack4(int n2rt, const GridDev& g, uint
 
 // Waves: single mesh, one per boundary voxel, or (n2 <= 4) one per four;
 // batch, per mesh from MeshOff::pw0 likewise.
-template <bool kM, int N2, int L2P_WAVES>
+// kRuns: some mesh has a resolution above 8 without a fixed layer loop,
+// whose passing layers are found by a search along the run (the other
+// instantiations leave that path out and keep its registers)
+template <bool kM, int N2, int L2P_WAVES, bool kRuns>
 __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __restrict__ bidx, int64_t B,
                                                               int64_t nwaves, const uint32_t* __restrict__ coff,
                                                               const float4* __restrict__ rec,
@@ -2394,6 +2441,7 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
 {
     __shared__ unsigned long long s_acc[L2P_WAVES][64];
     __shared__ ColRec s_rec[L2P_WAVES][64];
+    __shared__ float s_oz[L2P_WAVES][16];
     const int wv = threadIdx.x >> 6;
     const int64_t w = (int64_t)blockIdx.x * L2P_WAVES + wv;
     if (w >= nwaves) return;
@@ -2431,11 +2479,13 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
         // a batch mixes resolutions: the common one gets its unrolled body
         // (the branch is uniform per wave)
         if (n2 == 8)
-            l2_parity_voxel<8>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
+            l2_parity_voxel<8>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], s_oz[wv], out);
         else
-            l2_parity_voxel<0>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
+            l2_parity_voxel<0, kRuns>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], s_oz[wv],
+                                      out);
     } else if constexpr (N2 != 4) {
-        l2_parity_voxel<N2>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], out);
+        l2_parity_voxel<N2, kRuns>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv],
+                                   s_oz[wv], out);
     }
 }
 
@@ -2446,31 +2496,45 @@ int64_t l2_parity_waves(int n2, int64_t nb)
     return n2 <= 4 ? (nb + 3) / 4 : nb;
 }
 
-template <int PW>
-static void launch_l2_parity_w(const int32_t* bidx, int64_t B, int64_t nwaves, const uint32_t* coff,
+template <int PW, bool kRuns>
+static void launch_l2_parity_r(const int32_t* bidx, int64_t B, int64_t nwaves, const uint32_t* coff,
                                const float4* rec, const uint64_t* cpass, int stride, const GridDev& g0,
                                const Meshes& M, uint64_t* par2, hipStream_t s)
 {
     const unsigned blocks = (unsigned)((nwaves + PW - 1) / PW);
     const dim3 bs(64 * PW);
     if (M.nm > 1)
-        k_l2_parity<true, 0, PW><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
+        k_l2_parity<true, 0, PW, kRuns><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
     else if (g0.n2 == 8)
-        k_l2_parity<false, 8, PW><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
+        k_l2_parity<false, 8, PW, false><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M,
+                                                                par2);
     else if (g0.n2 == 4)
-        k_l2_parity<false, 4, PW><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
+        k_l2_parity<false, 4, PW, false><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M,
+                                                                par2);
     else
-        k_l2_parity<false, 0, PW><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2);
+        k_l2_parity<false, 0, PW, kRuns><<<blocks, bs, 0, s>>>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M,
+                                                                par2);
+}
+
+template <int PW>
+static void launch_l2_parity_w(const int32_t* bidx, int64_t B, int64_t nwaves, const uint32_t* coff,
+                               const float4* rec, const uint64_t* cpass, int stride, int n2max, const GridDev& g0,
+                               const Meshes& M, uint64_t* par2, hipStream_t s)
+{
+    if (n2max > 8)
+        launch_l2_parity_r<PW, true>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2, s);
+    else
+        launch_l2_parity_r<PW, false>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2, s);
 }
 
 void launch_l2_parity(const int32_t* bidx, int64_t B, int64_t nwaves, const uint32_t* coff, const float4* rec,
-                      const uint64_t* cpass, int stride, int waves, const GridDev& g0, const Meshes& M,
+                      const uint64_t* cpass, int stride, int n2max, int waves, const GridDev& g0, const Meshes& M,
                       uint64_t* par2, hipStream_t s)
 {
     if (B <= 0 || nwaves <= 0) return;
-    if (waves <= 1) launch_l2_parity_w<1>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2, s);
-    else if (waves == 2) launch_l2_parity_w<2>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2, s);
-    else launch_l2_parity_w<4>(bidx, B, nwaves, coff, rec, cpass, stride, g0, M, par2, s);
+    if (waves <= 1) launch_l2_parity_w<1>(bidx, B, nwaves, coff, rec, cpass, stride, n2max, g0, M, par2, s);
+    else if (waves == 2) launch_l2_parity_w<2>(bidx, B, nwaves, coff, rec, cpass, stride, n2max, g0, M, par2, s);
+    else launch_l2_parity_w<4>(bidx, B, nwaves, coff, rec, cpass, stride, n2max, g0, M, par2, s);
 }
 
 // ---------------------------------------------------------------------------

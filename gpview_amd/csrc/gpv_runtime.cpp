@@ -1072,9 +1072,8 @@ int l2_parity_kernels(Job* J, hipStream_t s)
     const int stride = cpass_stride(J);
     const gpv_tune& tu = J->ctx->tune;
     launch_col_pass(J->crec, J->g[0], J->view(), J->n_cols, stride, tu.l2_parity_waves, J->cpass, s);
-    launch_l2_parity(J->bidx, J->B, J->l2p_waves, J->coff, J->crec, J->cpass, stride, tu.l2_parity_waves, J->g[0],
-                     J->view(), J->par2,
-                     s);
+    launch_l2_parity(J->bidx, J->B, J->l2p_waves, J->coff, J->crec, J->cpass, stride, J->n2max, tu.l2_parity_waves,
+                     J->g[0], J->view(), J->par2, s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(J->ev[11], s));
     return GPV_OK;
@@ -1123,7 +1122,12 @@ void release_l2_groups(Job* J)
 // whether the Level2 SAT shares waves among voxels: few bin entries per voxel
 bool l2_grouped(const Job* J)
 {
-    return J->B > 0 && (int64_t)J->n_pairs < (int64_t)J->ctx->tune.l2_group_below * J->B;
+    if (J->B <= 0 || (int64_t)J->n_pairs >= (int64_t)J->ctx->tune.l2_group_below * J->B) return false;
+    // a group of one voxel (n2 = 16: its 64 words fill the wave's LDS) is
+    // the per-voxel kernel at a lower occupancy
+    for (int m = 0; m < J->nm; ++m)
+        if (J->g[m].n2 > 0 && l2_group_voxels(J->g[m].w2, J->g[m].n2) > 1) return true;
+    return false;
 }
 
 // group formation of the Level2 SAT (main stream, group arrays allocated)

@@ -149,7 +149,8 @@ typedef struct gpv_tune {
     int expand_blocks_per_cu;  /* persistent blocks per CU of the Level1 SAT and ray expansion (16) */
     int l2_group_below;        /* the Level2 SAT shares a wave among consecutive boundary voxels
                                   when the bins average fewer entries than this per voxel (8);
-                                  0 never, a large value always */
+                                  0 never, a large value always (except where a group could
+                                  hold only one voxel: Level2 16 in every mesh of the run) */
     int sort_items_per_block;  /* bin entries per block in the counting sorts (2048) */
     int l2_overlap;            /* run independent stages on the context's second stream: the ray
                                   half of the set-up and the ray expansion beside the SAT half and

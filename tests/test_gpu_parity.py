@@ -61,7 +61,7 @@ def test_cessna_fine(gv, orc, ctx, vc, vc2):
 @pytest.mark.parametrize("kind,target,vc,vc2,seed", [
     (0, 20000, 64, 8, 10), (1, 30000, 128, 4, 11), (2, 20000, 64, 8, 12), (2, 60000, 128, 16, 13),
     (1, 3000, 256, 4, 14), (0, 5000, 100, 5, 15), (2, 4000, 48, 3, 16), (0, 3000, 40, 1, 17),
-    (1, 5000, 72, 2, 18),
+    (1, 5000, 72, 2, 18), (0, 20000, 64, 12, 19), (2, 15000, 56, 9, 20),
 ])
 def test_mixed_kinds(gv, orc, ctx, kind, target, vc, vc2, seed):
     v, f = gv.mixed(kind, target, seed)
