@@ -199,6 +199,26 @@ def test_out_of_domain_coordinates_refused(gv):
             assert e.value.status == gv.GPV_ERR_INVALID
             assert c.pool_stats()["live_bytes"] == 0
             same(f"after {bad_value}", alone(c, good), ref)
+        # the no-allocation entry point refuses it too, and runs the next mesh
+        nx, ny, nz = ref[1]["n"]
+        w1 = nz * ny * ((nx + 63) // 64)
+        nb_cap = ref[1]["nb"] + 8
+        w2 = (ref[1]["n2"] ** 3 + 63) // 64
+        bufs = [c.upload(np.zeros(n, np.uint8)) for n in (w1 * 8, w1 * 8, nb_cap * 4, nb_cap * w2 * 8,
+                                                           nb_cap * w2 * 8)]
+        soup = good["soup"].copy()
+        soup[5, 0] = np.nan
+        dbad, dgood = c.upload(soup), c.upload(good["soup"])
+        try:
+            args = (good["bmin"], good["bmax"], good["mms"], 48, 4, True, bufs[0], bufs[1], w1, bufs[2], bufs[3],
+                    bufs[4], nb_cap)
+            with pytest.raises(gv.GpvError) as e:
+                c.voxelize_noalloc(dbad, len(soup), *args)
+            assert e.value.status == gv.GPV_ERR_INVALID
+            assert c.voxelize_noalloc(dgood, len(soup), *args)["nb"] == ref[1]["nb"]
+        finally:
+            for p in bufs + [dbad, dgood]:
+                c.free(p)
         # the domain's edge: a far vertex at exactly 2^40 is voxelized
         m = dict(good)
         soup = good["soup"].copy()
