@@ -168,6 +168,28 @@ def test_invalid_arguments(gv):
 
 @pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
                     reason="needs hipcc")
+def test_c_headers_compile_as_c99(tmp_path):
+    """include/gpv.h, gpv_mesh.h and gpv_serve.h are plain C: a C99 caller
+    written like the integration notes (stage sequence, batch, serving loop,
+    no-allocation entry point) compiles with warnings as errors and links
+    against libgpv (tests/native/c_abi_usage.c)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        pytest.skip("no C compiler")
+    src = os.path.join(ROOT, "tests", "native", "c_abi_usage.c")
+    inc = os.path.join(ROOT, "include")
+    obj = str(tmp_path / "c_abi_usage.o")
+    subprocess.run([cc, "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror", "-I", inc, "-c", src, "-o", obj],
+                   check=True)
+    libdir = os.path.join(ROOT, "gpview_amd")
+    if os.path.exists(os.path.join(libdir, "libgpv.so")):
+        out = str(tmp_path / "libcheck.so")
+        subprocess.run([cc, "-shared", "-o", out, obj, "-L", libdir, "-lgpv", "-Wl,--no-undefined",
+                        "-Wl,-rpath," + libdir], check=True)
+
+
 def test_codegen_exactness():
     """Our own gfx950 ISA: f32 denormals kept, f32 division is the
     correctly rounded v_div_scale/v_div_fmas/v_div_fixup expansion, f32 sqrt
