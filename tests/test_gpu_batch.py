@@ -87,6 +87,33 @@ def test_batch_small_mix_vs_oracle(gv, orc, ctx):
         check_info(f"mesh {i}", info, want)
 
 
+def test_batch_high_level2_mix_vs_oracle(gv, orc, ctx):
+    """Level2 resolutions above 8 beside small ones in one run: groups of one
+    voxel (n2 = 16) and of many (n2 = 4) in the grouped Level2 SAT, the
+    Level2 parity's run search (n2 = 9, 12, 16) next to its fixed-loop and
+    four-voxel paths, and column-pass waves whose entries span meshes of
+    different resolutions.  Also the same meshes with the per-voxel Level2
+    SAT (grouping off)."""
+    specs = [(0, 6000, 40, 16), (1, 5000, 32, 4), (2, 6000, 36, 12), (0, 3000, 24, 9), (1, 4000, 28, 8),
+             (2, 2000, 20, 2), (0, 4000, 30, 16)]
+    meshes = []
+    for i, (kind, target, vc, vc2) in enumerate(specs):
+        v, f = gv.mixed(kind, target, 700 + i)
+        meshes.append(mesh_entry(gv, v, f, vc, vc2))
+    want = [oracle_of(orc, m) for m in meshes]
+    before = ctx.get_tune()
+    try:
+        for below in (1 << 20, 0):
+            ctx.set_tune(l2_group_below=below)
+            say(f"batch high Level2 mix: {len(meshes)} meshes, l2_group_below {below}")
+            res = batch_results(ctx, meshes)
+            for i, (m, (got, info)) in enumerate(zip(meshes, res)):
+                compare_results(f"mesh {i} (below {below})", got, want[i][0], want[i][1])
+                check_info(f"mesh {i} (below {below})", info, want[i][0])
+    finally:
+        ctx.set_tune(**before)
+
+
 def test_batch_order_independent(gv, ctx):
     """Each mesh's grid is the same in any batch position (alone vs in the mix)."""
     meshes = small_mix(gv, seed=300)
