@@ -585,9 +585,16 @@ void launch_scan_u32_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t*
 // are staged in LDS (offsets + vertices) with one coalesced copy, and each
 // lane finds its triangle by a binary search in LDS.  Falls back to global
 // memory when a block spans more than XCAP triangles (long runs of empty
-// triangles).
+// triangles).  Batches use blocks of 512 threads (the many short item runs
+// of small meshes amortise each pass's searches and staging over twice the
+// items: Level1 SAT of the mixed-size stream 11.9 -> 8.7 ms); one mesh uses
+// 256 (with 512 its Level1 SAT beside the ray expansion took 0.21 -> 0.25 ms).
 // ---------------------------------------------------------------------------
-constexpr int XB = 256;
+template <bool kM>
+constexpr int expand_threads()
+{
+    return kM ? 512 : 256;
+}
 
 // 64-ary search by one wave: largest t in [lo, hi) with off[t] <= x, given
 // off[lo] <= x and (hi == n or off[hi] > x).  ~log64 rounds of dependent
@@ -666,7 +673,7 @@ struct NoAfter {
 // triangle searches of consecutive iterations start where the last ended.
 // after(last) runs block-synchronously after every iteration.  XCAP: the
 // triangles one pass stages in LDS (the tuned "triangles per LDS batch").
-template <int XCAP, typename F, typename A>
+template <int XB, int XCAP, typename F, typename A>
 __device__ __forceinline__ void expand_items(const float* __restrict__ tris, int64_t T,
                                              const uint64_t* __restrict__ off, F&& body, A&& after)
 {
@@ -738,7 +745,7 @@ __device__ __forceinline__ void expand_items(const float* __restrict__ tris, int
 // and the lane's highest hit z becomes the (column, triangle, zmax) entry used
 // by the Level2 parity candidates.
 template <bool kM, int XCAP>
-__global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tris, int64_t T,
+__global__ __launch_bounds__(expand_threads<kM>()) void k_sat_expand(const float* __restrict__ tris, int64_t T,
                                                    const int4* __restrict__ box0,
                                                    const int4* __restrict__ box1,
                                                    const double4* __restrict__ slab1,
@@ -757,7 +764,7 @@ __global__ __launch_bounds__(XB) void k_sat_expand(const float* __restrict__ tri
     __syncthreads();
     const LdsAppend lp{s_pk, s_pv, &s_n[0], PB};
     const LdsAppend lc{s_ck, s_cv, &s_n[1], CB};
-    expand_items<XCAP>(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
+    expand_items<expand_threads<kM>(), XCAP>(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
         const int mesh = kM ? (int)M.tri_mesh[t] : 0;
         const GridDev& g = grid_of<kM>(g0, M, mesh);
         const uint64_t word0 = kM ? M.off[mesh].word0 : 0;
@@ -807,9 +814,11 @@ static void launch_sat_expand_x(const float* tris, int64_t T, const int4* box0, 
                                 uint8_t* b, Append64 pairs, Append64 cols, int blocks, hipStream_t s)
 {
     if (M.nm > 1)
-        k_sat_expand<true, XC><<<blocks, XB, 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols);
+        k_sat_expand<true, XC><<<blocks, expand_threads<true>(), 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b,
+                                                                        pairs, cols);
     else
-        k_sat_expand<false, XC><<<blocks, XB, 0, s>>>(tris, T, box0, box1, slab1, off, g0, M, b, pairs, cols);
+        k_sat_expand<false, XC><<<blocks, expand_threads<false>(), 0, s>>>(tris, T, box0, box1, slab1, off, g0, M,
+                                                                          b, pairs, cols);
 }
 
 void launch_sat_expand(const float* tris, int64_t T, const int4* box0, const int4* box1, const double4* slab1,
@@ -844,7 +853,7 @@ void launch_pack_bytes(const uint8_t* by, int64_t W, uint64_t* bits, hipStream_t
 // conservative bound); the exact det/u/v test keeps the (column, triangle)
 // pairs whose ray the reference's MT accepts at some height.
 template <bool kM, int XCAP>
-__global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tris, int64_t T,
+__global__ __launch_bounds__(expand_threads<kM>()) void k_ray_expand(const float* __restrict__ tris, int64_t T,
                                                    const int4* __restrict__ rng,
                                                    const uint64_t* __restrict__ off, GridDev g0, Meshes M,
                                                    Append64 hits)
@@ -856,7 +865,7 @@ __global__ __launch_bounds__(XB) void k_ray_expand(const float* __restrict__ tri
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const LdsAppend lh{s_hk, nullptr, &s_n, HB};
-    expand_items<XCAP>(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
+    expand_items<expand_threads<kM>(), XCAP>(tris, T, off, [&](int64_t t, uint64_t l, const float* tv) {
         const int mesh = kM ? (int)M.tri_mesh[t] : 0;
         const GridDev& g = grid_of<kM>(g0, M, mesh);
         const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
@@ -879,9 +888,9 @@ static void launch_ray_expand_x(const float* tris, int64_t T, const int4* ray_rn
                                 const GridDev& g0, const Meshes& M, Append64 hits, int blocks, hipStream_t s)
 {
     if (M.nm > 1)
-        k_ray_expand<true, XC><<<blocks, XB, 0, s>>>(tris, T, ray_rng, off, g0, M, hits);
+        k_ray_expand<true, XC><<<blocks, expand_threads<true>(), 0, s>>>(tris, T, ray_rng, off, g0, M, hits);
     else
-        k_ray_expand<false, XC><<<blocks, XB, 0, s>>>(tris, T, ray_rng, off, g0, M, hits);
+        k_ray_expand<false, XC><<<blocks, expand_threads<false>(), 0, s>>>(tris, T, ray_rng, off, g0, M, hits);
 }
 
 void launch_ray_expand(const float* tris, int64_t T, const int4* ray_rng, const uint64_t* off, const GridDev& g0,
