@@ -2188,14 +2188,19 @@ void launch_col_pass(const float4* rec, const GridDev& g0, const Meshes& M, uint
 // sgn(e1x), qz is constant; e2x qx and e2y qy then move with -sgn(e2x)
 // sgn(e1y) and sgn(e2y) sgn(e1x), their left-to-right sum (plus the constant
 // e2z qz) is monotone when those do not oppose, and the product by inv flips
-// it for inv < 0.  Every intermediate is finite: the input domain (vertex
-// coordinates within +-2^40, checked by the set-up kernel) bounds the edges,
-// origins and differences by 2^42 and |inv| by 1e6, so no product or sum
-// reaches the float range.  The directions do not depend on (tx, ty).
+// it for inv < 0.  That needs every intermediate finite: with the edges,
+// |tx|, |ty| and |v1z| within 2^42, the ray origins within 2^41 and |inv|
+// within 1e7 no product or sum leaves the float range (2^110 at most); the
+// record's part is checked here, the origins' where the search is used.
+// The directions do not depend on (tx, ty).
 __device__ __forceinline__ int sgnf(float x) { return (x > 0.0f) - (x < 0.0f); }
 
 __device__ __forceinline__ int t_run_kind(const ColRec& r)
 {
+    const float lim = 0x1p42f;
+    if (!(fabsf(r.e1x) <= lim && fabsf(r.e1y) <= lim && fabsf(r.e1z) <= lim && fabsf(r.e2x) <= lim &&
+          fabsf(r.e2y) <= lim && fabsf(r.e2z) <= lim && fabsf(r.v1z) <= lim && fabsf(r.inv) <= 1e7f))
+        return 2;
     const int da = -sgnf(r.e2x) * sgnf(r.e1y), db = sgnf(r.e2y) * sgnf(r.e1x);
     if (da * db < 0) return 2;
     return (da ? da : db) * sgnf(r.inv) > 0 ? 1 : 0;
@@ -2234,7 +2239,8 @@ __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev
     for (int rr = 0; rr < (N2 > 0 ? N2 : 1); ++rr) oz[rr] = lane_f(ozl, rr);
     // resolutions without a fixed layer loop: the layer origins also in LDS,
     // for per-lane searches along the layers (tables non-decreasing)
-    const bool zruns = kRuns && N2 == 0 && g.r2mono != 0 && n2 >= 8;
+    const bool zruns = kRuns && N2 == 0 && g.r2mono != 0 && n2 >= 8 && fabsf(lane_f(ozl, 0)) <= 0x1p41f &&
+                       fabsf(lane_f(ozl, n2 - 1)) <= 0x1p41f;
     if (kRuns && N2 == 0 && lane < n2) soz[lane] = ozl;
     __builtin_amdgcn_wave_barrier();
     const uint64_t layers = (1ull << n2) - 1;
@@ -2285,7 +2291,7 @@ __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev
                                                  oz[rr])
                               << rr;
                 } else {
-                    if (kRuns && zruns && r.zmax < 2) {
+                    if (kRuns && zruns && r.zmax < 2 && fabsf(tx) <= 0x1p42f && fabsf(ty) <= 0x1p42f) {
                         // the passing layers are one run at an end: find its edge
                         const bool pre = r.zmax == 0;
                         int lo = 0, hi = n2;
