@@ -2208,7 +2208,9 @@ __device__ __forceinline__ int t_run_kind(const ColRec& r)
 
 // One boundary voxel (i1, j1, k1) of the Level2 parity.  N2 > 0: the Level2
 // resolution as a compile-time constant (layer loops unrolled, the layer
-// origins held uniform); N2 == 0: any resolution n2 <= 16.
+// origins held uniform); N2 == 0: any resolution n2 <= 16.  The kernels use
+// it for the resolutions other than 8 (two voxels per wave,
+// l2_parity_pack2_8) and at most 4 (four per wave, l2_parity_pack4).
 template <int N2, bool kRuns = false>
 __device__ __forceinline__ void l2_parity_voxel(int n2_rt, int w2, const GridDev& g, int i1, int j1, int k1,
                                                 uint32_t e0, uint32_t e1, const float4* __restrict__ rec,
@@ -2444,11 +2446,99 @@ __device__ __forceinline__ void l2_parity_pack4(int n2rt, const GridDev& g, uint
 
 // Waves: single mesh, one per boundary voxel, or (n2 <= 4) one per four;
 // batch, per mesh from MeshOff::pw0 likewise.
+// Two boundary voxels of a Level2-8 mesh per wave.  Lanes 32h .. 32h + 31
+// load voxel h's column entries, 32 per round; one 64x64 transpose of their
+// pass-bit rows leaves lane l with sub-column l of both voxels (bit j < 32:
+// voxel 0's slot j passes there; bit j >= 32: voxel 1's slot j - 32), so the
+// two voxels' dependent loads and tests overlap in one wave and share its
+// transposes.  Voxel h's words go to out[8h .. 8h + 7] (word r holds layer r
+// of its 64 sub-columns, bit = sub-column).
+__device__ __forceinline__ void l2_parity_pack2_8(const GridDev& g, uint32_t col0, int64_t bfirst, int nv,
+                                                  const int32_t* __restrict__ bidx, const uint32_t* __restrict__ coff,
+                                                  const float4* __restrict__ rec, const uint64_t* __restrict__ cpass,
+                                                  int stride, ColRec* cr, uint64_t* __restrict__ out)
+{
+    const int lane = __lane_id();
+    const int h = lane >> 5, slot = lane & 31;
+    const int nxy = g.nx * g.ny;
+    int k1v[2];
+    uint32_t e0v[2], e1v[2];
+    float ox[2], oy[2], oz[2][8];
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+        const int vox = uniform_i(bidx[bfirst + (hh < nv ? hh : 0)]);
+        const int k1 = vox / nxy;
+        const int ij = vox - k1 * nxy;
+        const int j1 = ij / g.nx;
+        const int i1 = ij - j1 * g.nx;
+        k1v[hh] = k1;
+        e0v[hh] = (uint32_t)uniform_i((int)coff[col0 + ij]);
+        e1v[hh] = hh < nv ? (uint32_t)uniform_i((int)coff[col0 + ij + 1]) : e0v[hh];
+        ox[hh] = g.r2x[(int64_t)i1 * 8 + (lane & 7)];
+        oy[hh] = g.r2y[(int64_t)j1 * 8 + (lane >> 3)];
+#pragma unroll
+        for (int rr = 0; rr < 8; ++rr)
+            oz[hh][rr] = __int_as_float(uniform_i(__float_as_int(g.r2z[(int64_t)k1 * 8 + rr])));
+    }
+    const int k1 = h ? k1v[1] : k1v[0];
+    const uint32_t e0 = h ? e0v[1] : e0v[0], e1 = h ? e1v[1] : e1v[0];
+    const uint32_t n0 = (e1v[0] - e0v[0] + 31) >> 5, n1 = (e1v[1] - e0v[1] + 31) >> 5;
+    const uint32_t rounds = n0 > n1 ? n0 : n1;
+    uint32_t par[2] = {0u, 0u};   // bit r: parity at layer r of sub-column lane of voxel h
+    for (uint32_t rd = 0; rd < rounds; ++rd) {
+        const uint32_t i = e0 + rd * 32 + (uint32_t)slot;
+        uint64_t row = 0;
+        bool hit = false;   // every layer's t passes: no t test, no record
+        if (i < e1) {
+            // header (k_col_pass): candidate while k1 <= zc
+            const uint64_t hw = cpass[(uint64_t)i * stride + stride - 1];
+            const int kh = (int)(uint32_t)hw - 1, zc = (int)(uint32_t)(hw >> 32) - 1;
+            if (k1 <= zc) {
+                row = cpass[(uint64_t)i * stride];
+                hit = k1 <= kh;
+                if (!hit) cr[lane] = load_colrec(rec, i);
+            }
+        }
+        const uint64_t all_hit = __ballot(hit);
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t pm = wave_transpose64(row);
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            const uint64_t mh = pm & (hh ? 0xffffffff00000000ull : 0x00000000ffffffffull);
+            if (__popcll(mh & all_hit) & 1) par[hh] ^= 0xffu;
+            for (uint64_t m = mh & ~all_hit; m; m &= m - 1) {
+                const int j = __ffsll((unsigned long long)m) - 1;
+                const ColRec r = cr[j];
+                const float tx = ox[hh] - r.v1x, ty = oy[hh] - r.v1y;
+                uint32_t hm = 0;
+#pragma unroll
+                for (int rr = 0; rr < 8; ++rr)
+                    hm |= (uint32_t)mt_t_hit(r.e1x, r.e1y, r.e1z, r.e2x, r.e2y, r.e2z, r.inv, tx, ty, r.v1z, oz[hh][rr])
+                          << rr;
+                par[hh] ^= hm;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    uint64_t mine = 0;
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint64_t wd = __ballot((par[hh] >> r) & 1u);
+            if (lane == hh * 8 + r) mine = wd;
+        }
+    }
+    if (lane < 8 * nv) out[lane] = mine;
+}
+
 // kRuns: some mesh has a resolution above 8 without a fixed layer loop,
 // whose passing layers are found by a search along the run (the other
 // instantiations leave that path out and keep its registers)
+// Eight waves per SIMD (at most 64 VGPRs) for the latency of each voxel's
+// chain of dependent loads.
 template <bool kM, int N2, int L2P_WAVES, bool kRuns>
-__global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __restrict__ bidx, int64_t B,
+__global__ __launch_bounds__(64 * L2P_WAVES) __attribute__((amdgpu_waves_per_eu(8))) void k_l2_parity(const int32_t* __restrict__ bidx, int64_t B,
                                                               int64_t nwaves, const uint32_t* __restrict__ coff,
                                                               const float4* __restrict__ rec,
                                                               const uint64_t* __restrict__ cpass, int stride,
@@ -2471,6 +2561,13 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
     const int64_t lw = w - (kM ? (int64_t)M.off[mesh].pw0 : 0);
     const uint64_t l2w0 = kM ? M.off[mesh].l2w0 : 0;
     const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
+    if ((N2 == 0 || N2 == 8) && n2 == 8) {
+        const int64_t bfirst = mb0 + 2 * lw;
+        const int nv = (int)(mb1 - bfirst < 2 ? mb1 - bfirst : 2);
+        l2_parity_pack2_8(g, col0, bfirst, nv, bidx, coff, rec, cpass, stride, s_rec[wv],
+                          par2 + l2w0 + (uint64_t)(bfirst - mb0) * 8);
+        return;
+    }
     if ((N2 == 0 || N2 == 4) && n2 <= 4) {
         const int64_t bfirst = mb0 + 4 * lw;
         const int nv = (int)(mb1 - bfirst < 4 ? mb1 - bfirst : 4);
@@ -2490,25 +2587,17 @@ __global__ __launch_bounds__(64 * L2P_WAVES) void k_l2_parity(const int32_t* __r
     const int j1 = ij / gnx;
     const int i1 = ij - j1 * gnx;
     const uint32_t e0 = coff[col0 + ij], e1 = coff[col0 + ij + 1];
-    if constexpr (kM && N2 == 0) {
-        // a batch mixes resolutions: the common one gets its unrolled body
-        // (the branch is uniform per wave)
-        if (n2 == 8)
-            l2_parity_voxel<8>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], s_oz[wv], out);
-        else
-            l2_parity_voxel<0, kRuns>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], s_oz[wv],
-                                      out);
-    } else if constexpr (N2 != 4) {
-        l2_parity_voxel<N2, kRuns>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv],
-                                   s_oz[wv], out);
-    }
+    // the other resolutions (Level2 8 and at most 4 returned above)
+    if constexpr (N2 == 0)
+        l2_parity_voxel<0, kRuns>(n2, w2, g, i1, j1, k1, e0, e1, rec, cpass, stride, s_rec[wv], s_acc[wv], s_oz[wv],
+                                  out);
 }
 
 // waves of the Level2 parity for nb boundary voxels at resolution n2
 int64_t l2_parity_waves(int n2, int64_t nb)
 {
     if (n2 <= 0) return 0;
-    return n2 <= 4 ? (nb + 3) / 4 : nb;
+    return n2 <= 4 ? (nb + 3) / 4 : n2 == 8 ? (nb + 1) / 2 : nb;
 }
 
 template <int PW, bool kRuns>
