@@ -30,7 +30,8 @@ class GpvError(RuntimeError):
 class Params(ctypes.Structure):
     """gpv_params (include/gpv.h)."""
     _fields_ = [("voxel_count", ctypes.c_int), ("voxel_count2", ctypes.c_int), ("level2", ctypes.c_int),
-                ("normals", ctypes.c_int), ("initial_capacity", ctypes.c_int64), ("cuda_level2", ctypes.c_int)]
+                ("normals", ctypes.c_int), ("initial_capacity", ctypes.c_int64), ("cuda_level2", ctypes.c_int),
+                ("l1_stencil", ctypes.c_int)]
 
 
 class GridInfo(ctypes.Structure):
@@ -395,13 +396,15 @@ class Context:
         return t.as_dict(), ms.value
 
     def voxelize(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0,
-                 device_ptr=None, normals=False, cuda_level2=False):
+                 device_ptr=None, normals=False, cuda_level2=False, l1_stencil=False):
         """Object::PerformVoxelization on a flat soup; returns a Grid.
-        cuda_level2: the Level2 arithmetic of the reference's CUDA kernels."""
+        cuda_level2: the Level2 arithmetic of the reference's CUDA kernels;
+        l1_stencil: Level1 inside/outside by the GL-stencil-equivalent fill
+        (ClassifyInOut2x) instead of ray parity."""
         soup = np.ascontiguousarray(soup, np.float32).reshape(-1, 9)
         p = Params(voxel_count=int(voxel_count), voxel_count2=int(voxel_count2), level2=1 if level2 else 0,
                    normals=1 if normals else 0, initial_capacity=int(initial_capacity),
-                   cuda_level2=1 if cuda_level2 else 0)
+                   cuda_level2=1 if cuda_level2 else 0, l1_stencil=1 if l1_stencil else 0)
         g = _vp()
         ptr = device_ptr if device_ptr is not None else soup.ctypes.data
         _check(lib().gpv_voxelize(self.h, ptr, len(soup), 1 if device_ptr is not None else 0,
@@ -440,11 +443,12 @@ class Context:
         return [Grid(self, _vp(out[i])) for i in range(n)]
 
     def create(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0,
-               normals=False):
+               normals=False, l1_stencil=False):
         """gpv_grid_create: the staged form (call the stage methods on the Grid)."""
         soup = np.ascontiguousarray(soup, np.float32).reshape(-1, 9)
         p = Params(voxel_count=int(voxel_count), voxel_count2=int(voxel_count2), level2=1 if level2 else 0,
-                   normals=1 if normals else 0, initial_capacity=int(initial_capacity))
+                   normals=1 if normals else 0, initial_capacity=int(initial_capacity),
+                   l1_stencil=1 if l1_stencil else 0)
         g = _vp()
         _check(lib().gpv_grid_create(self.h, soup.ctypes.data, len(soup), 0, np.asarray(bmin, np.float32),
                                      np.asarray(bmax, np.float32), float(mms), ctypes.byref(p),
@@ -491,7 +495,8 @@ def mesh_desc(m, device_ptr=None):
     d.params = Params(voxel_count=int(m["voxel_count"]), voxel_count2=int(m.get("voxel_count2", 4)),
                       level2=1 if m.get("level2", True) else 0, normals=1 if m.get("normals", False) else 0,
                       initial_capacity=int(m.get("initial_capacity", 0)),
-                      cuda_level2=1 if m.get("cuda_level2", False) else 0)
+                      cuda_level2=1 if m.get("cuda_level2", False) else 0,
+                      l1_stencil=1 if m.get("l1_stencil", False) else 0)
     return d, soup
 
 

@@ -20,7 +20,7 @@ struct GridDev {
     int normals;           // this mesh asks for Level1/Level2 normals
     int l2cuda;            // Level2 in the reference's CUDA arithmetic (gpv_params::cuda_level2)
     int r2mono;            // each Level2 ray-origin table non-decreasing over its whole length
-    int pad;
+    int l1st;              // Level1 inside/outside by the GL-stencil-equivalent fill (gpv_params::l1_stencil)
     float bminx, bminy, bminz, bmaxx, bmaxy, bmaxz;
     float h1x, h1y, h1z;   // Level1 half extents
     float h2x, h2y, h2z;   // Level2 half extents
@@ -156,6 +156,12 @@ void launch_bin_cols(const float* tris, const uint64_t* key, const uint32_t* zma
 // Level1 parity tasks: L1P_Z consecutive layers of one 64-column row strip
 constexpr int L1P_Z = 16;
 inline int64_t l1_parity_tasks(const GridDev& g) { return (int64_t)((g.nz + L1P_Z - 1) / L1P_Z) * g.ny * g.nxw; }
+// GL-stencil-equivalent Level1 fill of one mesh (soup tris[T], grid g):
+// overwrites the mesh's Level1 inside bits ins[nz*ny*nxw].  Scratch: cnt[T],
+// off[T+1], scan_tmp[scan_tmp_elems(T)], tog[l1_stencil_words(g)].
+size_t l1_stencil_words(const GridDev& g);
+void launch_l1_stencil(const float* tris, int64_t T, const GridDev& g, uint32_t* cnt, uint64_t* off,
+                       uint64_t* scan_tmp, uint64_t* tog, uint64_t* ins, hipStream_t s);
 void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0, const Meshes& M, int64_t NT,
                       uint64_t* ins, hipStream_t s);
 

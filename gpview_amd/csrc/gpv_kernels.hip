@@ -1056,6 +1056,211 @@ void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0
 }
 
 // ---------------------------------------------------------------------------
+// Level1 inside/outside by the GL-stencil-equivalent fill (opt-in,
+// gpv_params::l1_stencil; ClassifyInOut2x, src/Object.cpp:1446-1601).  Per
+// slice s the reference draws the mesh orthographically over the bbox into a
+// 2nx x 2ny stencil, clipped to z <= zs, counts the fragments per sample,
+// and calls a voxel inside when more than 2 of its 4 samples hold an odd
+// count.  Which samples a triangle covers is the GL driver's choice; the
+// rule here is fixed: vertices snapped to 1/256 sample in window
+// coordinates, sample (px, py) at (256 px + 128, 256 py + 128), edge
+// functions of the counter-clockwise snapped triangle positive, or zero on a
+// top or left edge (shared edges count once), no sample for a zero snapped
+// area; a covered sample counts in slice s when the height of the triangle's
+// plane (float vertices, double arithmetic) over the sample's world point is
+// at most the slice's ray-origin z.  A covered sample toggles bit (s0, py,
+// px) of a [nz][2ny][2nx] bit set at its first slice s0; the running XOR
+// along s is then the stencil's parity, and the 2 x 2 vote writes the
+// Level1 inside bits.  Counts are exact (GL's 8-bit stencil saturates).
+// ---------------------------------------------------------------------------
+struct StTri {
+    long long X[3], Y[3];             // snapped window coordinates, counter-clockwise
+    int pxlo, pxhi, pylo, pyhi;       // samples whose centers lie in the snapped bbox
+    double v0x, v0y, v0z, nx, ny, nz; // first vertex and plane normal
+};
+
+__device__ __forceinline__ long long st_snap(float x, float lo, float hi, int samples)
+{
+    double w = ((double)x - (double)lo) / ((double)hi - (double)lo) * (double)samples;
+    // vertices far outside the bbox: bounded so that edge functions stay in 64 bits
+    w = w < -4194304.0 ? -4194304.0 : (w > 4194304.0 ? 4194304.0 : w);
+    return (long long)rint(w * 256.0);
+}
+
+__device__ __forceinline__ long long st_floor256(long long a) { return a >= 0 ? a / 256 : -((255 - a) / 256); }
+
+// false when the triangle covers no sample in any slice
+__device__ bool st_prepare(const float* __restrict__ tv, const GridDev& g, StTri& s)
+{
+    const int W = 2 * g.nx, H = 2 * g.ny;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        s.X[i] = st_snap(tv[3 * i], g.bminx, g.bmaxx, W);
+        s.Y[i] = st_snap(tv[3 * i + 1], g.bminy, g.bmaxy, H);
+    }
+    const long long A = (s.X[1] - s.X[0]) * (s.Y[2] - s.Y[0]) - (s.Y[1] - s.Y[0]) * (s.X[2] - s.X[0]);
+    if (A == 0) return false;
+    if (A < 0) {
+        long long t = s.X[1]; s.X[1] = s.X[2]; s.X[2] = t;
+        t = s.Y[1]; s.Y[1] = s.Y[2]; s.Y[2] = t;
+    }
+    s.v0x = (double)tv[0]; s.v0y = (double)tv[1]; s.v0z = (double)tv[2];
+    const double e1x = (double)tv[3] - s.v0x, e1y = (double)tv[4] - s.v0y, e1z = (double)tv[5] - s.v0z;
+    const double e2x = (double)tv[6] - s.v0x, e2y = (double)tv[7] - s.v0y, e2z = (double)tv[8] - s.v0z;
+    s.nx = e1y * e2z - e1z * e2y;
+    s.ny = e1z * e2x - e1x * e2z;
+    s.nz = e1x * e2y - e1y * e2x;
+    if (s.nz == 0.0) return false;
+    long long mnx = s.X[0], mxx = s.X[0], mny = s.Y[0], mxy = s.Y[0];
+#pragma unroll
+    for (int i = 1; i < 3; ++i) {
+        mnx = s.X[i] < mnx ? s.X[i] : mnx;
+        mxx = s.X[i] > mxx ? s.X[i] : mxx;
+        mny = s.Y[i] < mny ? s.Y[i] : mny;
+        mxy = s.Y[i] > mxy ? s.Y[i] : mxy;
+    }
+    const long long xlo = st_floor256(mnx - 128 + 255), xhi = st_floor256(mxx - 128);
+    const long long ylo = st_floor256(mny - 128 + 255), yhi = st_floor256(mxy - 128);
+    s.pxlo = (int)(xlo < 0 ? 0 : xlo);
+    s.pylo = (int)(ylo < 0 ? 0 : ylo);
+    s.pxhi = (int)(xhi > W - 1 ? W - 1 : xhi);
+    s.pyhi = (int)(yhi > H - 1 ? H - 1 : yhi);
+    return s.pxlo <= s.pxhi && s.pylo <= s.pyhi;
+}
+
+__device__ __forceinline__ bool st_covers(const StTri& s, int px, int py)
+{
+    const long long sx = 256ll * px + 128, sy = 256ll * py + 128;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const int a = e, b = e == 2 ? 0 : e + 1;
+        const long long dx = s.X[b] - s.X[a], dy = s.Y[b] - s.Y[a];
+        const long long E = dx * (sy - s.Y[a]) - dy * (sx - s.X[a]);
+        if (E < 0 || (E == 0 && !(dy < 0 || (dy == 0 && dx < 0)))) return false;
+    }
+    return true;
+}
+
+// first slice whose clip plane keeps the triangle's point over the sample
+// (z_T <= zs); nz when none does
+__device__ __forceinline__ int st_slice(const StTri& s, const GridDev& g, int px, int py)
+{
+    const double l = (double)g.bminx, r = (double)g.bmaxx, b = (double)g.bminy, t = (double)g.bmaxy;
+    const double x = l + ((double)px + 0.5) * ((r - l) / (double)(2 * g.nx));
+    const double y = b + ((double)py + 0.5) * ((t - b) / (double)(2 * g.ny));
+    const double zt = s.v0z - (s.nx * (x - s.v0x) + s.ny * (y - s.v0y)) / s.nz;
+    int lo = 0, hi = g.nz;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((double)g.r1z[mid] < zt) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_st_count(const float* __restrict__ tris, int64_t T, GridDev g,
+                                                  uint32_t* __restrict__ cnt)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x) {
+        StTri s;
+        cnt[t] = st_prepare(tris + 9 * t, g, s)
+                     ? (uint32_t)(s.pxhi - s.pxlo + 1) * (uint32_t)(s.pyhi - s.pylo + 1)
+                     : 0u;
+    }
+}
+
+// the (triangle, sample) items, ST_RUN consecutive ones per thread: one
+// search for the run's first triangle, then forward
+constexpr int ST_RUN = 8;
+__global__ __launch_bounds__(256) void k_st_toggle(const float* __restrict__ tris, int64_t T, GridDev g,
+                                                   const uint64_t* __restrict__ off, int wpr,
+                                                   unsigned long long* __restrict__ tog)
+{
+    const uint64_t N = off[T];
+    const int H = 2 * g.ny;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * ST_RUN;
+    for (uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * ST_RUN; i0 < N; i0 += stride) {
+        const uint64_t i1 = i0 + ST_RUN < N ? i0 + ST_RUN : N;
+        int64_t t = find_segment(off, T, i0);
+        StTri s;
+        st_prepare(tris + 9 * t, g, s);
+        for (uint64_t i = i0; i < i1; ++i) {
+            if (off[t + 1] <= i) {
+                do ++t; while (off[t + 1] <= i);
+                st_prepare(tris + 9 * t, g, s);
+            }
+            const uint32_t li = (uint32_t)(i - off[t]);
+            const uint32_t bw = (uint32_t)(s.pxhi - s.pxlo + 1);
+            const int py = s.pylo + (int)(li / bw), px = s.pxlo + (int)(li % bw);
+            if (!st_covers(s, px, py)) continue;
+            const int k = st_slice(s, g, px, py);
+            if (k < g.nz)
+                atomicXor(&tog[((int64_t)k * H + py) * wpr + (px >> 6)], 1ull << (px & 63));
+        }
+    }
+}
+
+// running XOR along the slices, per 64-sample word of a sample row
+__global__ __launch_bounds__(256) void k_st_fill(unsigned long long* __restrict__ tog, int nz, int64_t plane)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane; q += (int64_t)gridDim.x * blockDim.x) {
+        unsigned long long acc = 0;
+        for (int k = 0; k < nz; ++k) {
+            acc ^= tog[k * plane + q];
+            tog[k * plane + q] = acc;
+        }
+    }
+}
+
+// one wave per Level1 word (slice k, row j, 64 voxels): more than 2 of the
+// voxel's samples (2x + {0,1}, 2j + {0,1}) odd
+__global__ __launch_bounds__(256) void k_st_vote(const unsigned long long* __restrict__ tog, GridDev g, int wpr,
+                                                 uint64_t* __restrict__ ins)
+{
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nxw = g.nxw, ny = g.ny, nx = g.nx, H = 2 * g.ny;
+    if (w >= (int64_t)g.nz * ny * nxw) return;
+    const int lane = __lane_id();
+    const int xw = (int)(w % nxw);
+    const int64_t kj = w / nxw;
+    const int j = (int)(kj % ny), k = (int)(kj / ny);
+    const int x = xw * 64 + lane;
+    bool inside = false;
+    if (x < nx) {
+        const unsigned long long* r0 = tog + ((int64_t)k * H + 2 * j) * wpr;
+        const int sx = 2 * x;
+        const unsigned long long a = r0[sx >> 6], b = r0[wpr + (sx >> 6)];
+        const int sh = sx & 63;
+        const int odd = (int)((a >> sh) & 1ull) + (int)((a >> (sh + 1)) & 1ull) + (int)((b >> sh) & 1ull) +
+                        (int)((b >> (sh + 1)) & 1ull);
+        inside = odd > 2;
+    }
+    const uint64_t m = __ballot(inside);
+    if (lane == 0) ins[w] = m;
+}
+
+size_t l1_stencil_words(const GridDev& g)
+{
+    return (size_t)g.nz * (size_t)(2 * g.ny) * (size_t)((2 * g.nx + 63) / 64);
+}
+
+void launch_l1_stencil(const float* tris, int64_t T, const GridDev& g, uint32_t* cnt, uint64_t* off,
+                       uint64_t* scan_tmp, uint64_t* tog, uint64_t* ins, hipStream_t s)
+{
+    const int wpr = (2 * g.nx + 63) / 64;
+    const int64_t plane = (int64_t)(2 * g.ny) * wpr;
+    unsigned long long* tg = reinterpret_cast<unsigned long long*>(tog);
+    launch_fill_u64(tog, 0, (int64_t)l1_stencil_words(g), s);
+    if (T > 0) {
+        k_st_count<<<grid_for(T, 256, 16384), 256, 0, s>>>(tris, T, g, cnt);
+        launch_scan_u32_u64(cnt, off, T, scan_tmp, s);
+        k_st_toggle<<<4096, 256, 0, s>>>(tris, T, g, off, wpr, tg);
+    }
+    k_st_fill<<<grid_for(plane, 256, 16384), 256, 0, s>>>(tg, g.nz, plane);
+    const int64_t words = (int64_t)g.nz * g.ny * g.nxw;
+    k_st_vote<<<(unsigned)((words * 64 + 255) / 256), 256, 0, s>>>(tg, g, wpr, ins);
+}
+
+// ---------------------------------------------------------------------------
 // boundary compaction (src/Object.cpp:3270-3279): popcount per word ->
 // exclusive scan -> ascending linear indices of the boundary voxels
 // ---------------------------------------------------------------------------

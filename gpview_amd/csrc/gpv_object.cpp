@@ -84,7 +84,8 @@ void Object::PerformVoxelization(GLParameters* glParam, int bufferSize)
     if ((int64_t)flatCPUTriangleData.size() != totalNumTriangles * 9) CreateFlatTriangleData();
     // the reference always computes the normals it saves (src/Object.cpp:3219-3253, 2404-2421)
     gpv_params p{glParam->voxelCount, glParam->voxelCount2, glParam->level2Voxels ? 1 : 0, 1,
-                 bufferSize > 0 ? (int64_t)bufferSize : 0, glParam->cudaClassifyTessellation ? 1 : 0};
+                 bufferSize > 0 ? (int64_t)bufferSize : 0, glParam->cudaClassifyTessellation ? 1 : 0,
+                 glParam->stencilInOut ? 1 : 0};
     int st = gpv_voxelize(ctx_, flatCPUTriangleData.data(), totalNumTriangles, 0, bBoxMin, bBoxMax, maxModelSize,
                           &p, &grid_);
     if (st != GPV_OK) device_error("PerformVoxelization", st);

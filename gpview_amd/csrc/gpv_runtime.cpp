@@ -298,6 +298,11 @@ struct Job {
     int64_t l2p_waves = 0;   // Level2 parity waves (l2_parity_plan)
     uint32_t *hcnt = nullptr, *hoff = nullptr;
     float4* hrec = nullptr;
+    // GL-stencil-equivalent Level1 fill (meshes with params.l1_stencil):
+    // per-triangle sample counts and offsets, and the slice toggle planes
+    bool any_stencil = false;
+    uint32_t* st_cnt = nullptr;
+    uint64_t *st_off = nullptr, *st_scan = nullptr, *st_tog = nullptr;
     uint64_t *sat2 = nullptr, *par2 = nullptr;
     int n2max = 0;
     // normals (meshes with params.normals): face normals, sorted bins, and the
@@ -564,6 +569,8 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         TRY(derive_grid(d[m].bmin, d[m].bmax, d[m].max_model_size, &d[m].params, &info[m]));
         const int64_t nx = info[m].n[0], ny = info[m].n[1], nz = info[m].n[2];
         if (nx * ny >= (int64_t(1) << 31)) return fail(GPV_ERR_INVALID, "mesh %d: too many columns", m);
+        if (d[m].params.l1_stencil && (nx > (int64_t(1) << 19) || ny > (int64_t(1) << 19) || nx * ny >= (int64_t(1) << 29)))
+            return fail(GPV_ERR_INVALID, "mesh %d: the stencil fill takes at most 2^19 divisions per axis and 2^29 columns", m);
         info[m].n1 = nx * ny * nz;
         T += d[m].ntris;
         W += nz * ny * ((nx + 63) / 64);
@@ -641,6 +648,8 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         g = grid_dev(J->info[m]);
         g.normals = d[m].params.normals ? 1 : 0;
         g.l2cuda = d[m].params.cuda_level2 ? 1 : 0;
+        g.l1st = d[m].params.l1_stencil ? 1 : 0;
+        J->any_stencil = J->any_stencil || g.l1st;
         J->info[m].has_normals = g.normals;
         J->any_normals = J->any_normals || g.normals;
         J->n2max = std::max(J->n2max, g.n2);
@@ -918,7 +927,26 @@ int alloc_parity(Job* J)
     TRY(alloc(J, &J->hcnt, (size_t)J->ncol));
     TRY(alloc(J, &J->hoff, (size_t)J->ncol + 1));
     TRY(alloc(J, &J->hrec, (size_t)std::max<uint64_t>(J->n_hits, 1) * 3));
-    return alloc(J, &J->scan_side, scan_tmp_elems(J->ncol));
+    TRY(alloc(J, &J->scan_side, scan_tmp_elems(J->ncol)));
+    if (J->any_stencil) {
+        // one set for the stencil meshes in turn (stream-ordered), sized for the largest
+        release(J, J->st_cnt);
+        release(J, J->st_off);
+        release(J, J->st_scan);
+        release(J, J->st_tog);
+        int64_t tmax = 1;
+        size_t words = 1;
+        for (int m = 0; m < J->nm; ++m) {
+            if (!J->g[m].l1st) continue;
+            tmax = std::max<int64_t>(tmax, (int64_t)J->off[m + 1].tri0 - (int64_t)J->off[m].tri0);
+            words = std::max(words, l1_stencil_words(J->g[m]));
+        }
+        TRY(alloc(J, &J->st_cnt, (size_t)tmax));
+        TRY(alloc(J, &J->st_off, (size_t)tmax + 1));
+        TRY(alloc(J, &J->st_scan, scan_tmp_elems(tmax)));
+        TRY(alloc(J, &J->st_tog, words));
+    }
+    return GPV_OK;
 }
 
 // buffers of the column-entry sort when it runs on the side stream
@@ -955,6 +983,11 @@ int parity_kernels(Job* J, hipStream_t s)
     launch_scan_u32_u32(J->hcnt, J->hoff, J->ncol, J->scan_side, nullptr, s);
     launch_scatter_hits(J->tris, J->hit_key, J->n_hits, J->g[0], J->view(), J->hoff, J->hcnt, J->hrec, s);
     launch_l1_parity(J->hrec, J->hoff, J->g[0], J->view(), (int64_t)J->off[J->nm].ptask0, J->ins, s);
+    // meshes with the stencil fill: their inside bits, overwritten in turn
+    for (int m = 0; m < J->nm && J->any_stencil; ++m)
+        if (J->g[m].l1st)
+            launch_l1_stencil(J->tris + (size_t)J->off[m].tri0 * 9, (int64_t)J->off[m + 1].tri0 - J->off[m].tri0,
+                              J->g[m], J->st_cnt, J->st_off, J->st_scan, J->st_tog, J->ins + J->off[m].word0, s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(J->ev[14], s));
     return GPV_OK;
@@ -1292,6 +1325,7 @@ int job_finish(Job* J)
     release(J, J->list); release(J, J->coff); release(J, J->crec); release(J, J->hcnt);
     release(J, J->cs_cnt); release(J, J->cs_bucket);
     release(J, J->hoff); release(J, J->hrec); release(J, J->scan_side); release(J, J->tri_mesh);
+    release(J, J->st_cnt); release(J, J->st_off); release(J, J->st_scan); release(J, J->st_tog);
     release(J, J->bmesh);
     release(J, J->d_grids); release(J, J->d_off); release(J, J->own_soup); release(J, J->mctr);
     release(J, J->fnorm); release(J, J->slist);
@@ -1392,6 +1426,7 @@ int64_t predict_run_bytes(const gpv_mesh_desc& d, const gpv_grid_info& I)
     b += 16 * B * w2;                                            // Level2 planes
     b += n2 > 0 ? 8 * P * ((n2 * n2 + 63) / 64 + 1) + 12 * (B + 1) : 0;   // Level2 parity bits, SAT groups
     if (d.params.normals) b += 12 * T + 4 * P + 12 * B + 12 * B * n2 * n2 * n2;
+    if (d.params.l1_stencil) b += 12 * T + 8 + scan(T) + 8 * nz * 2 * ny * ((2 * nx + 63) / 64);   // stencil fill
     return b;
 }
 

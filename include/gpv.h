@@ -67,6 +67,21 @@ typedef struct gpv_params {
                                  the normals keep the CPU semantics.  Not pinned
                                  against the CUDA build (it needs nvcc and an
                                  NVIDIA GPU). */
+    int l1_stencil;           /* 0: Level1 inside/outside by +z ray parity at the
+                                 voxel centers (the CPU composition, the
+                                 contract).  1: by the reference's active GL path,
+                                 ClassifyInOut2x (src/Object.cpp:1446-1601):
+                                 per slice, the fragment count of the mesh clipped
+                                 to z <= the slice's center plane on a 2nx x 2ny
+                                 sample grid, inside when more than 2 of a
+                                 voxel's 4 samples are odd.  Sample coverage
+                                 follows a fixed rule in place of the GL
+                                 driver's: vertices snapped to 1/256 sample,
+                                 top-left rule, slice test on the triangle's
+                                 plane in double; counts do not saturate.  Not
+                                 pinned against a GL run (it needs a GL context).
+                                 At most 2^19 divisions per axis and 2^29
+                                 columns. */
 } gpv_params;
 
 /* ---- grid description (VoxelData, includes/Utilities.h:79-154) ----------- */

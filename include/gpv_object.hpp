@@ -33,6 +33,10 @@ struct GLParameters {
     // the Level2 arithmetic of the reference's CUDACLASSIFYTESSELLATION build
     // (src/Object.cpp:3342-3347; gpv_params::cuda_level2)
     bool cudaClassifyTessellation = false;
+    // Level1 inside/outside by the GL stencil fill the reference's INOUT build
+    // runs (ClassifyInOut2x, src/Object.cpp:3156-3159; gpv_params::l1_stencil)
+    // instead of the CPU composition's ray parity
+    bool stencilInOut = false;
 };
 
 class Object {

@@ -669,6 +669,160 @@ static int lower_idx(const float *tab, int n, double x)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Level1 GL-stencil-equivalent fill (src/Object.cpp:1446-1601): the rule of  */
+/* orc_stencil_counts in gpv_oracle.h                                         */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int64_t X[3], Y[3];               /* snapped window coordinates, counter-clockwise */
+    int64_t pxlo, pxhi, pylo, pyhi;   /* samples whose centers lie in the snapped bbox */
+    double v0[3];                     /* first vertex */
+    double nx, ny, nz;                /* plane normal (e1 x e2 of the float vertices) */
+} st_tri;
+
+#define ST_MAX_SAMPLES (1 << 20)
+
+static int64_t floor_div(int64_t a, int64_t b) /* b > 0 */
+{
+    int64_t q = a / b;
+    return (a % b != 0 && a < 0) ? q - 1 : q;
+}
+
+/* window coordinate of a vertex coordinate, in 1/256 sample units */
+static int64_t st_snap(float x, float lo, float hi, int samples)
+{
+    double w = ((double)x - (double)lo) / ((double)hi - (double)lo) * (double)samples;
+    /* vertices far outside the bbox: bounded so that edge functions stay in 64 bits */
+    w = w < -4194304.0 ? -4194304.0 : (w > 4194304.0 ? 4194304.0 : w);
+    return (int64_t)rint(w * 256.0);
+}
+
+/* 0 when the triangle covers no sample in any slice */
+static int st_prepare(const float *tv, const orc_grid *g, st_tri *s)
+{
+    const int W = 2 * g->n[0], H = 2 * g->n[1];
+    for (int i = 0; i < 3; i++) {
+        s->X[i] = st_snap(tv[3 * i], g->bmin[0], g->bmax[0], W);
+        s->Y[i] = st_snap(tv[3 * i + 1], g->bmin[1], g->bmax[1], H);
+    }
+    int64_t A = (s->X[1] - s->X[0]) * (s->Y[2] - s->Y[0]) - (s->Y[1] - s->Y[0]) * (s->X[2] - s->X[0]);
+    if (A == 0) return 0;
+    if (A < 0) {
+        int64_t t = s->X[1]; s->X[1] = s->X[2]; s->X[2] = t;
+        t = s->Y[1]; s->Y[1] = s->Y[2]; s->Y[2] = t;
+    }
+    double e1[3], e2[3];
+    for (int q = 0; q < 3; q++) {
+        s->v0[q] = (double)tv[q];
+        e1[q] = (double)tv[3 + q] - (double)tv[q];
+        e2[q] = (double)tv[6 + q] - (double)tv[q];
+    }
+    s->nx = e1[1] * e2[2] - e1[2] * e2[1];
+    s->ny = e1[2] * e2[0] - e1[0] * e2[2];
+    s->nz = e1[0] * e2[1] - e1[1] * e2[0];
+    if (s->nz == 0.0) return 0;
+    int64_t mnx = s->X[0], mxx = s->X[0], mny = s->Y[0], mxy = s->Y[0];
+    for (int i = 1; i < 3; i++) {
+        if (s->X[i] < mnx) mnx = s->X[i];
+        if (s->X[i] > mxx) mxx = s->X[i];
+        if (s->Y[i] < mny) mny = s->Y[i];
+        if (s->Y[i] > mxy) mxy = s->Y[i];
+    }
+    s->pxlo = floor_div(mnx - 128 + 255, 256);
+    s->pxhi = floor_div(mxx - 128, 256);
+    s->pylo = floor_div(mny - 128 + 255, 256);
+    s->pyhi = floor_div(mxy - 128, 256);
+    if (s->pxlo < 0) s->pxlo = 0;
+    if (s->pylo < 0) s->pylo = 0;
+    if (s->pxhi > W - 1) s->pxhi = W - 1;
+    if (s->pyhi > H - 1) s->pyhi = H - 1;
+    return s->pxlo <= s->pxhi && s->pylo <= s->pyhi;
+}
+
+/* edge functions of the snapped triangle at the sample, top-left rule */
+static int st_covers(const st_tri *s, int64_t px, int64_t py)
+{
+    const int64_t sx = 256 * px + 128, sy = 256 * py + 128;
+    for (int e = 0; e < 3; e++) {
+        const int a = e, b = e == 2 ? 0 : e + 1;
+        const int64_t dx = s->X[b] - s->X[a], dy = s->Y[b] - s->Y[a];
+        const int64_t E = dx * (sy - s->Y[a]) - dy * (sx - s->X[a]);
+        if (E < 0) return 0;
+        if (E == 0 && !(dy < 0 || (dy == 0 && dx < 0))) return 0;
+    }
+    return 1;
+}
+
+static int lower_idx(const float *tab, int n, double x);
+
+/* first slice whose clip plane keeps the triangle's point over the sample
+ * (z_T <= zs); nz when none does */
+static int st_slice(const st_tri *s, const orc_grid *g, const float *rz, int64_t px, int64_t py)
+{
+    const double l = g->bmin[0], r = g->bmax[0], b = g->bmin[1], t = g->bmax[1];
+    const double x = l + ((double)px + 0.5) * ((r - l) / (double)(2 * g->n[0]));
+    const double y = b + ((double)py + 0.5) * ((t - b) / (double)(2 * g->n[1]));
+    const double zt = s->v0[2] - (s->nx * (x - s->v0[0]) + s->ny * (y - s->v0[1])) / s->nz;
+    return lower_idx(rz, g->n[2], zt);
+}
+
+int orc_stencil_counts(const float *tris, int64_t T, const orc_grid *g, int slice, int32_t *counts)
+{
+    const int64_t W = 2 * (int64_t)g->n[0], H = 2 * (int64_t)g->n[1];
+    if (W > ST_MAX_SAMPLES || H > ST_MAX_SAMPLES) return -1;
+    memset(counts, 0, sizeof(int32_t) * (size_t)(W * H));
+    float *rz = (float *)xcalloc((size_t)g->n[2], sizeof(float));
+    orc_l1_ray_origins(g, 2, rz);
+    for (int64_t t = 0; t < T; t++) {
+        st_tri s;
+        if (!st_prepare(tris + 9 * t, g, &s)) continue;
+        for (int64_t py = s.pylo; py <= s.pyhi; py++)
+            for (int64_t px = s.pxlo; px <= s.pxhi; px++)
+                if (st_covers(&s, px, py) && st_slice(&s, g, rz, px, py) <= slice) counts[py * W + px]++;
+    }
+    free(rz);
+    return 0;
+}
+
+/* every covered sample toggles its column from its first slice up; the
+ * running parity per column is then the stencil's parity per slice, and
+ * the 2 x 2 samples of a voxel vote (src/Object.cpp:1580-1601) */
+int orc_stencil_inside(const float *tris, int64_t T, const orc_grid *g, uint8_t *inside)
+{
+    const int64_t nx = g->n[0], ny = g->n[1], nz = g->n[2], W = 2 * nx, H = 2 * ny;
+    if (W > ST_MAX_SAMPLES || H > ST_MAX_SAMPLES) return -1;
+    float *rz = (float *)xcalloc((size_t)nz, sizeof(float));
+    orc_l1_ray_origins(g, 2, rz);
+    uint8_t *tog = (uint8_t *)xcalloc((size_t)(nz * H * W), 1);
+    for (int64_t t = 0; t < T; t++) {
+        st_tri s;
+        if (!st_prepare(tris + 9 * t, g, &s)) continue;
+        for (int64_t py = s.pylo; py <= s.pyhi; py++)
+            for (int64_t px = s.pxlo; px <= s.pxhi; px++) {
+                if (!st_covers(&s, px, py)) continue;
+                const int k = st_slice(&s, g, rz, px, py);
+                if (k < nz) tog[((int64_t)k * H + py) * W + px] ^= 1;
+            }
+    }
+    for (int64_t q = 0; q < H * W; q++) {
+        uint8_t p = 0;
+        for (int64_t k = 0; k < nz; k++) {
+            p ^= tog[k * H * W + q];
+            tog[k * H * W + q] = p;
+        }
+    }
+    for (int64_t k = 0; k < nz; k++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t i = 0; i < nx; i++) {
+                const uint8_t *a = tog + (k * H + 2 * j) * W + 2 * i;
+                const int odd = a[0] + a[1] + a[W] + a[W + 1];
+                inside[(k * ny + j) * nx + i] = (uint8_t)(odd > 2);   /* > numSamples^2 / 2.0 */
+            }
+    free(tog);
+    free(rz);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* pipeline                                                                  */
 /* ------------------------------------------------------------------------ */
 typedef struct {
@@ -1119,7 +1273,17 @@ int orc_voxelize(const float *tris, int64_t T, const orc_grid *g, int mode, int 
 
     double t0 = omp_get_wtime();
     uint8_t *par = (uint8_t *)xcalloc((size_t)c.n1, 1);
-    if (mode == ORC_EXACT) l1_parity_exact(&c, par); else l1_parity_fast(&c, par);
+    if (g->l1_stencil) {
+        if (orc_stencil_inside(tris, T, g, par) != 0) {
+            free(par);
+            for (int q = 0; q < 6; q++) free(tab[q]);
+            return -2;
+        }
+    } else if (mode == ORC_EXACT) {
+        l1_parity_exact(&c, par);
+    } else {
+        l1_parity_fast(&c, par);
+    }
     double t1 = omp_get_wtime();
     l1_sat(&c, r);
     double t2 = omp_get_wtime();

@@ -11,6 +11,7 @@
  *   bbox + offset         src/Object.cpp:441-450, 572-583 (OFF: 257-296)
  *   grid derivation       src/Object.cpp:3094-3136, includes/Utilities.h:313-316
  *   Level1 ray parity     src/Object.cpp:716-779  + src/TriRayIntersection.cpp:78-131
+ *   Level1 stencil fill   src/Object.cpp:1446-1601 (opt-in, orc_grid.l1_stencil)
  *   Level1 SAT + bins     src/Object.cpp:2256-2342, 2681-2748 + src/TriBoxIntersection.cpp:64-210
  *   compaction / prefix   src/Object.cpp:3258-3284
  *   Level2 ray parity     src/Object.cpp:1111-1207
@@ -45,6 +46,8 @@ typedef struct orc_grid {
                              kernels (cuda/CUDAClassifyTessellation.cu:403-504):
                              float centers/origins (2p+1)*ext2 + mid - ext1 and
                              parity over the whole xy column's triangle list */
+    int l1_stencil;       /* 1: Level1 inside/outside by the GL-stencil-equivalent
+                             fill (orc_stencil_counts) instead of ray parity */
 } orc_grid;
 
 typedef struct orc_result {
@@ -106,6 +109,31 @@ int orc_vertex_index(float v, float mn, float mx, int n);
  * the det test for every ray (then no column can count it). */
 int orc_ray_column_bounds(const float tv[9], const float bmin[3], const float bmax[3],
                           double safety, double lo[2], double hi[2]);
+
+/* The GL stencil fill of ClassifyInOut2x (src/Object.cpp:1446-1601) with a
+ * fixed rasterization rule.  The reference draws the mesh orthographically
+ * over the bbox into a 2nx x 2ny stencil, clipped to z <= zs for slice
+ * s (zs = (s + 0.5) g + min), counting every fragment, and calls a voxel
+ * inside when more than 2 of its 4 samples hold an odd count.  Which samples
+ * a triangle covers is up to the GL driver; this rule fixes it:
+ *   window x = ((double)x - l) / ((double)r - l) * W (W = 2 nx; y likewise
+ *   with H = 2 ny), snapped to 1/256 pixel (rint(x * 256)); sample (px, py)
+ *   sits at (256 px + 128, 256 py + 128); a triangle with zero snapped area
+ *   covers nothing; otherwise a sample is covered when every edge function
+ *   of the counter-clockwise-ordered snapped triangle is positive, or zero
+ *   on a top or left edge (dy < 0, or dy == 0 and dx < 0), so samples on
+ *   shared edges count once;
+ *   a covered sample counts in slice s when z_T <= zs(s), z_T the height of
+ *   the triangle's plane (float vertices, double arithmetic, no fused ops)
+ *   at the sample's world point l + (px + 0.5) ((double)r - l) / W, and zs(s)
+ *   the Level1 ray-origin table; triangles whose plane is vertical (normal
+ *   z exactly 0) count nowhere;
+ *   counts are exact (GL's 8-bit stencil saturates at 255).
+ * counts[py * W + px] receives the fragment count of slice s; returns 0, or
+ * -1 when the grid is too large for the rule (W or H above 2^20). */
+int orc_stencil_counts(const float *tris, int64_t ntris, const orc_grid *g, int slice, int32_t *counts);
+/* Level1 inside (1) / outside (0) per voxel by that fill: [z][y][x]. */
+int orc_stencil_inside(const float *tris, int64_t ntris, const orc_grid *g, uint8_t *inside);
 
 /* Full pipeline.  mode ORC_EXACT runs the literal loops (all-triangle
  * parity, every box of every index AABB and bin through the full SAT);

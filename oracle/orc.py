@@ -29,6 +29,7 @@ class Grid(ctypes.Structure):
         ("g2", ctypes.c_float * 3),
         ("ext2", ctypes.c_float * 3),
         ("cuda_l2", ctypes.c_int),
+        ("l1_stencil", ctypes.c_int),
     ]
 
 
@@ -89,7 +90,13 @@ def lib():
         L.orc_ray_column_bounds.restype = ctypes.c_int
         L.orc_voxelize.argtypes = [_f32p, ctypes.c_int64, ctypes.POINTER(Grid), ctypes.c_int,
                                    ctypes.c_int, ctypes.POINTER(_Result)]
+        L.orc_voxelize.restype = ctypes.c_int
         L.orc_result_free.argtypes = [ctypes.POINTER(_Result)]
+        L.orc_stencil_counts.argtypes = [_f32p, ctypes.c_int64, ctypes.POINTER(Grid), ctypes.c_int,
+                                         np.ctypeslib.ndpointer(dtype=np.int32)]
+        L.orc_stencil_counts.restype = ctypes.c_int
+        L.orc_stencil_inside.argtypes = [_f32p, ctypes.c_int64, ctypes.POINTER(Grid), _u8p]
+        L.orc_stencil_inside.restype = ctypes.c_int
         L.orc_face_normal_batch.argtypes = [ctypes.c_int64, _f32p, _f32p]
         L.orc_normalize_batch.argtypes = [ctypes.c_int64, _f32p, _f32p]
         L.orc_average_normals.argtypes = [ctypes.c_int64, np.ctypeslib.ndpointer(np.int64), _f32p, _f32p]
@@ -160,14 +167,34 @@ def bbox(verts):
     return bmin, bmax, np.float32(mms.value)
 
 
-def grid_setup(bmin, bmax, mms, voxel_count, voxel_count2, cuda_l2=False):
+def grid_setup(bmin, bmax, mms, voxel_count, voxel_count2, cuda_l2=False, l1_stencil=False):
     """orc_grid_setup; cuda_l2 selects the Level2 arithmetic of the
-    reference's CUDA kernels (orc_grid.cuda_l2)."""
+    reference's CUDA kernels (orc_grid.cuda_l2), l1_stencil the
+    GL-stencil-equivalent Level1 fill (orc_grid.l1_stencil)."""
     g = Grid()
     lib().orc_grid_setup(_f32(bmin), _f32(bmax), float(mms), int(voxel_count), int(voxel_count2),
                          ctypes.byref(g))
     g.cuda_l2 = 1 if cuda_l2 else 0
+    g.l1_stencil = 1 if l1_stencil else 0
     return g
+
+
+def stencil_counts(tris, g, slice_):
+    """orc_stencil_counts: fragment count per sample [2ny, 2nx] of one slice
+    of the GL-stencil-equivalent fill."""
+    tris = _f32(tris).reshape(-1)
+    out = np.zeros(4 * g.n[0] * g.n[1], np.int32)
+    assert lib().orc_stencil_counts(tris, len(tris) // 9, ctypes.byref(g), int(slice_), out) == 0
+    return out.reshape(2 * g.n[1], 2 * g.n[0])
+
+
+def stencil_inside(tris, g):
+    """orc_stencil_inside: Level1 inside (1) / outside (0) [nz, ny, nx] by
+    the GL-stencil-equivalent fill."""
+    tris = _f32(tris).reshape(-1)
+    out = np.zeros(g.n[0] * g.n[1] * g.n[2], np.uint8)
+    assert lib().orc_stencil_inside(tris, len(tris) // 9, ctypes.byref(g), out) == 0
+    return out.reshape(g.n[2], g.n[1], g.n[0])
 
 
 def grid_for(tris, voxel_count, voxel_count2):
@@ -236,7 +263,9 @@ def voxelize(tris, g, mode=ORC_FAST, threads=0, normals=False, bins=True):
     tris = _f32(tris).reshape(-1)
     T = len(tris) // 9
     r = _Result()
-    lib().orc_voxelize(tris, T, ctypes.byref(g), int(mode), int(threads), ctypes.byref(r))
+    rc = lib().orc_voxelize(tris, T, ctypes.byref(g), int(mode), int(threads), ctypes.byref(r))
+    if rc != 0:
+        raise ValueError(f"orc_voxelize failed ({rc})")
     n23 = g.n2 ** 3 if g.n2 > 0 else 0
     try:
         out = {

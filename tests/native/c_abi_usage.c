@@ -19,7 +19,7 @@ int voxelize_staged(const float *soup, int64_t ntris, const float *verts, int64_
     int st = gpv_ctx_create(-1, 0, &ctx);
     if (st != GPV_OK) return st;
     st = gpv_mesh_bbox(verts, nverts, bmin, bmax, &mms);
-    gpv_params p = {256, 8, 1, 1, 0, 0};
+    gpv_params p = {256, 8, 1, 1, 0, 0, 0};
     if (st == GPV_OK) st = gpv_grid_create(ctx, soup, ntris, 0, bmin, bmax, mms, &p, &grid);
     if (st == GPV_OK) st = gpv_classify_inout(ctx, grid);
     do {
@@ -58,7 +58,7 @@ int voxelize_without_allocation(gpv_ctx *ctx, const float *d_soup, int64_t ntris
                                 const float bmax[3], float mms, uint64_t *l1b, uint64_t *l1i, int32_t *bidx,
                                 uint64_t *l2b, uint64_t *l2i, int64_t nb_cap)
 {
-    gpv_params p = {256, 8, 1, 0, 0, 0};
+    gpv_params p = {256, 8, 1, 0, 0, 0, 0};
     gpv_grid_info info;
     int st = gpv_grid_derive(bmin, bmax, mms, &p, &info);
     if (st != GPV_OK) return st;
