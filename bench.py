@@ -13,6 +13,8 @@ Prints ONE JSON line with, beside the headline:
   outcome is the same, so the figure is reference-equivalent;
 - pcie_inclusive: the headline step with the soup uploaded from
   page-locked host memory in every step;
+- stencil_fill: the headline step with Level1 inside/outside from the
+  GL-stencil-equivalent fill (gpv_params.l1_stencil) instead of ray parity;
 - roofline (HBM) and roofline_valu of the dominant kernel, from HIP events
   of this run and the committed counter summary of the same kernel;
 - cpu_baseline: the oracle in the CPU baseline's composition (column-
@@ -487,12 +489,34 @@ def main():
     pcie = {"meshes_per_s": n_pc / t_pc, "ms_per_step": t_pc / n_pc * 1e3, "steps": n_pc,
             "soup_bytes": int(soup.nbytes), "note": "soup uploaded from page-locked host memory in every step"}
 
+    # the same step with Level1 inside/outside from the GL-stencil-equivalent
+    # fill (gpv_params.l1_stencil) in place of the ray parity
+    def stencil_step():
+        return ctx.voxelize(soup, bmin, bmax, mms, 256, 8, True, device_ptr=dsoup, l1_stencil=True)
+
+    for _ in range(2):
+        stencil_step().release()
+    n_st = max(5, args.steps // 2)
+    st_infos = []
+    t2 = time.perf_counter()
+    for _ in range(n_st):
+        gs = stencil_step()
+        st_infos.append(gs.info())
+        gs.release()
+    t_st = time.perf_counter() - t2
+
     # one more run outside the timed region, kept for the correctness check
     g = step()
     info = g.info()
     got = g.download()
     g.release()
     ctx.free(dsoup)
+    stencil = {"meshes_per_s": n_st / t_st, "ms_per_step": t_st / n_st * 1e3, "steps": n_st,
+               "l1_parity_stage_ms": float(np.median([i["ms_stage"][2] for i in st_infos])),
+               "l1_inside": int(st_infos[-1]["l1_inside"]), "l1_inside_ray_parity": int(info["l1_inside"]),
+               "note": "gpv_params.l1_stencil: Level1 inside/outside by the GL-stencil-equivalent fill "
+                       "(ClassifyInOut2x) on the headline mesh, resident soup; the headline uses the ray parity"}
+    progress(f"stencil fill: {stencil['ms_per_step']:.3f} ms per step")
     batched = batched_small = stress = None
     if args.batch_meshes > 0:
         batched = bench_batched(gv, ctx, workloads.mixed_stream(gv, args.batch_meshes), args.batch_steps, 1,
@@ -586,6 +610,7 @@ def main():
         "tests_l1": info["tests_l1"], "tests_l2": info["tests_l2"], "pairs": info["pairs"],
         "bit_exact_vs_cpu": exact,
         "pcie_inclusive": pcie,
+        "stencil_fill": stencil,
         "roofline": rl,
         "roofline_valu": valu,
         "cpu_baseline": cpu,

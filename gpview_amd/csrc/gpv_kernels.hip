@@ -1012,6 +1012,7 @@ __global__ __launch_bounds__(256) void k_l1_parity(const float4* __restrict__ re
     const int lane = __lane_id();
     const int mesh = kM ? wave_mesh_of_task(M.off, M.nm, (uint64_t)task) : 0;
     const GridDev& g = grid_of<kM>(g0, M, mesh);
+    if (g.l1st) return;   // the stencil fill writes this mesh's inside bits
     const int64_t tl = task - (int64_t)(kM ? M.off[mesh].ptask0 : 0);
     const uint64_t word0 = kM ? M.off[mesh].word0 : 0;
     const uint32_t col0 = kM ? M.off[mesh].col0 : 0;
@@ -1072,6 +1073,8 @@ void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0
 // px) of a [nz][2ny][2nx] bit set at its first slice s0; the running XOR
 // along s is then the stencil's parity, and the 2 x 2 vote writes the
 // Level1 inside bits.  Counts are exact (GL's 8-bit stencil saturates).
+// The kernels read only the soup and the grid, so the pipeline runs them
+// beside the Level1 SAT.
 // ---------------------------------------------------------------------------
 struct StTri {
     long long X[3], Y[3];             // snapped window coordinates, counter-clockwise
@@ -1199,43 +1202,46 @@ __global__ __launch_bounds__(256) void k_st_toggle(const float* __restrict__ tri
     }
 }
 
-// running XOR along the slices, per 64-sample word of a sample row
-__global__ __launch_bounds__(256) void k_st_fill(unsigned long long* __restrict__ tog, int nz, int64_t plane)
-{
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane; q += (int64_t)gridDim.x * blockDim.x) {
-        unsigned long long acc = 0;
-        for (int k = 0; k < nz; ++k) {
-            acc ^= tog[k * plane + q];
-            tog[k * plane + q] = acc;
-        }
-    }
-}
-
-// one wave per Level1 word (slice k, row j, 64 voxels): more than 2 of the
-// voxel's samples (2x + {0,1}, 2j + {0,1}) odd
+// One wave per Level1 row word (row j, 64 voxels) walks the slices: the
+// running XOR of the toggles of sample rows 2j and 2j + 1 is the stencil's
+// parity at each slice, and a voxel is inside when more than 2 of its
+// samples (2x + {0,1}, 2j + {0,1}) are odd.  The slices' toggle words are
+// loaded ST_ZU at a time ahead of their use.
+constexpr int ST_ZU = 8;
 __global__ __launch_bounds__(256) void k_st_vote(const unsigned long long* __restrict__ tog, GridDev g, int wpr,
                                                  uint64_t* __restrict__ ins)
 {
     const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nxw = g.nxw, ny = g.ny, nx = g.nx, H = 2 * g.ny;
-    if (w >= (int64_t)g.nz * ny * nxw) return;
+    const int nxw = g.nxw, ny = g.ny, nx = g.nx, nz = g.nz, H = 2 * g.ny;
+    if (w >= (int64_t)ny * nxw) return;
     const int lane = __lane_id();
-    const int xw = (int)(w % nxw);
-    const int64_t kj = w / nxw;
-    const int j = (int)(kj % ny), k = (int)(kj / ny);
+    const int xw = (int)(w % nxw), j = (int)(w / nxw);
     const int x = xw * 64 + lane;
-    bool inside = false;
-    if (x < nx) {
-        const unsigned long long* r0 = tog + ((int64_t)k * H + 2 * j) * wpr;
-        const int sx = 2 * x;
-        const unsigned long long a = r0[sx >> 6], b = r0[wpr + (sx >> 6)];
-        const int sh = sx & 63;
-        const int odd = (int)((a >> sh) & 1ull) + (int)((a >> (sh + 1)) & 1ull) + (int)((b >> sh) & 1ull) +
-                        (int)((b >> (sh + 1)) & 1ull);
-        inside = odd > 2;
+    const int sx = 2 * x, sh = sx & 63;
+    // lanes past the row read a word of the row (their votes are masked)
+    const int wi = x < nx ? sx >> 6 : 0;
+    const int64_t plane = (int64_t)H * wpr;
+    const unsigned long long* r0 = tog + (int64_t)(2 * j) * wpr + wi;
+    unsigned long long a = 0, b = 0;
+    for (int k0 = 0; k0 < nz; k0 += ST_ZU) {
+        unsigned long long ta[ST_ZU], tb[ST_ZU];
+#pragma unroll
+        for (int u = 0; u < ST_ZU; ++u) {
+            const int k = k0 + u < nz ? k0 + u : nz - 1;
+            ta[u] = r0[k * plane];
+            tb[u] = r0[k * plane + wpr];
+        }
+#pragma unroll
+        for (int u = 0; u < ST_ZU; ++u) {
+            if (k0 + u >= nz) break;
+            a ^= ta[u];
+            b ^= tb[u];
+            const int odd = (int)((a >> sh) & 1ull) + (int)((a >> (sh + 1)) & 1ull) + (int)((b >> sh) & 1ull) +
+                            (int)((b >> (sh + 1)) & 1ull);
+            const uint64_t m = __ballot(x < nx && odd > 2);
+            if (lane == 0) ins[((int64_t)(k0 + u) * ny + j) * nxw + xw] = m;
+        }
     }
-    const uint64_t m = __ballot(inside);
-    if (lane == 0) ins[w] = m;
 }
 
 size_t l1_stencil_words(const GridDev& g)
@@ -1247,7 +1253,6 @@ void launch_l1_stencil(const float* tris, int64_t T, const GridDev& g, uint32_t*
                        uint64_t* scan_tmp, uint64_t* tog, uint64_t* ins, hipStream_t s)
 {
     const int wpr = (2 * g.nx + 63) / 64;
-    const int64_t plane = (int64_t)(2 * g.ny) * wpr;
     unsigned long long* tg = reinterpret_cast<unsigned long long*>(tog);
     launch_fill_u64(tog, 0, (int64_t)l1_stencil_words(g), s);
     if (T > 0) {
@@ -1255,9 +1260,8 @@ void launch_l1_stencil(const float* tris, int64_t T, const GridDev& g, uint32_t*
         launch_scan_u32_u64(cnt, off, T, scan_tmp, s);
         k_st_toggle<<<4096, 256, 0, s>>>(tris, T, g, off, wpr, tg);
     }
-    k_st_fill<<<grid_for(plane, 256, 16384), 256, 0, s>>>(tg, g.nz, plane);
-    const int64_t words = (int64_t)g.nz * g.ny * g.nxw;
-    k_st_vote<<<(unsigned)((words * 64 + 255) / 256), 256, 0, s>>>(tg, g, wpr, ins);
+    const int64_t rows = (int64_t)g.ny * g.nxw;   // one wave each
+    k_st_vote<<<(unsigned)((rows * 64 + 255) / 256), 256, 0, s>>>(tg, g, wpr, ins);
 }
 
 // ---------------------------------------------------------------------------

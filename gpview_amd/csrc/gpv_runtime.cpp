@@ -300,7 +300,7 @@ struct Job {
     float4* hrec = nullptr;
     // GL-stencil-equivalent Level1 fill (meshes with params.l1_stencil):
     // per-triangle sample counts and offsets, and the slice toggle planes
-    bool any_stencil = false;
+    bool any_stencil = false, all_stencil = false, stencil_done = false;
     uint32_t* st_cnt = nullptr;
     uint64_t *st_off = nullptr, *st_scan = nullptr, *st_tog = nullptr;
     uint64_t *sat2 = nullptr, *par2 = nullptr;
@@ -650,6 +650,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         g.l2cuda = d[m].params.cuda_level2 ? 1 : 0;
         g.l1st = d[m].params.l1_stencil ? 1 : 0;
         J->any_stencil = J->any_stencil || g.l1st;
+        J->all_stencil = (m == 0 || J->all_stencil) && g.l1st;
         J->info[m].has_normals = g.normals;
         J->any_normals = J->any_normals || g.normals;
         J->n2max = std::max(J->n2max, g.n2);
@@ -736,6 +737,20 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         (r = alloc(J, &J->col_key, J->col_cap)) || (r = alloc(J, &J->col_z, J->col_cap)) ||
         (r = alloc(J, &J->hit_key, J->hit_cap)))
         return bail(r);
+    if (J->any_stencil) {
+        // the stencil fill's scratch: one set for the stencil meshes in turn
+        // (stream-ordered), sized for the largest; it needs no count
+        int64_t tmax = 1;
+        size_t words = 1;
+        for (int m = 0; m < nm; ++m) {
+            if (!J->g[m].l1st) continue;
+            tmax = std::max<int64_t>(tmax, d[m].ntris);
+            words = std::max(words, l1_stencil_words(J->g[m]));
+        }
+        if ((r = alloc(J, &J->st_cnt, (size_t)tmax)) || (r = alloc(J, &J->st_off, (size_t)tmax + 1)) ||
+            (r = alloc(J, &J->st_scan, scan_tmp_elems(tmax))) || (r = alloc(J, &J->st_tog, words)))
+            return bail(r);
+    }
     *out = J;
     return GPV_OK;
 }
@@ -778,9 +793,11 @@ int stage_setup_split(Job* J)
                      J->slab1, J->slab2, 1, &J->ctr->bad_input, s);
     launch_scan_u32_u64(J->sat_items, J->sat_off, J->T, J->scan64, s);
     HIP_TRY(hipStreamWaitEvent(c->side, J->ev[15], 0));
-    launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items,
-                     J->slab1, J->slab2, 2, &J->ctr->bad_input, c->side);
-    launch_scan_u32_u64(J->ray_items, J->ray_off, J->T, J->scan_ray, c->side);
+    if (!J->all_stencil) {   // the ray half feeds only the ray parity
+        launch_tri_setup(J->tris, J->T, J->g[0], J->view(), J->box0, J->box1, J->sat_items, J->rng, J->ray_items,
+                         J->slab1, J->slab2, 2, &J->ctr->bad_input, c->side);
+        launch_scan_u32_u64(J->ray_items, J->ray_off, J->T, J->scan_ray, c->side);
+    }
     HIP_TRY(hipGetLastError());
     J->setup_done = true;
     return GPV_OK;
@@ -812,11 +829,21 @@ int launch_sat_half(Job* J)
     return GPV_OK;
 }
 
+int launch_stencil(Job* J, hipStream_t s);
+
 // Level1 ray expansion on stream s (allocates nothing)
 int launch_ray_half(Job* J, hipStream_t s)
 {
     gpv_ctx* c = J->ctx;
     HIP_TRY(hipEventRecord(J->ev[2], s));
+    if (J->all_stencil) {
+        // no ray parity, hence no hits (the counter is cleared on the stream
+        // of the count read-back): the stencil fill takes the ray half's place
+        launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->hits), 0, 1, J->ctx->stream);
+        TRY(launch_stencil(J, s));
+        HIP_TRY(hipEventRecord(J->ev[13], s));
+        return GPV_OK;
+    }
     launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->hits), 0, 1, s);
     Append64 hi{&J->ctr->hits, J->hit_key, nullptr, J->hit_cap};
     launch_ray_expand(J->tris, J->T, J->rng, J->ray_off, J->g[0], J->view(), hi, expand_blocks(c),
@@ -927,25 +954,22 @@ int alloc_parity(Job* J)
     TRY(alloc(J, &J->hcnt, (size_t)J->ncol));
     TRY(alloc(J, &J->hoff, (size_t)J->ncol + 1));
     TRY(alloc(J, &J->hrec, (size_t)std::max<uint64_t>(J->n_hits, 1) * 3));
-    TRY(alloc(J, &J->scan_side, scan_tmp_elems(J->ncol)));
-    if (J->any_stencil) {
-        // one set for the stencil meshes in turn (stream-ordered), sized for the largest
-        release(J, J->st_cnt);
-        release(J, J->st_off);
-        release(J, J->st_scan);
-        release(J, J->st_tog);
-        int64_t tmax = 1;
-        size_t words = 1;
-        for (int m = 0; m < J->nm; ++m) {
-            if (!J->g[m].l1st) continue;
-            tmax = std::max<int64_t>(tmax, (int64_t)J->off[m + 1].tri0 - (int64_t)J->off[m].tri0);
-            words = std::max(words, l1_stencil_words(J->g[m]));
-        }
-        TRY(alloc(J, &J->st_cnt, (size_t)tmax));
-        TRY(alloc(J, &J->st_off, (size_t)tmax + 1));
-        TRY(alloc(J, &J->st_scan, scan_tmp_elems(tmax)));
-        TRY(alloc(J, &J->st_tog, words));
-    }
+    return alloc(J, &J->scan_side, scan_tmp_elems(J->ncol));
+}
+
+// the GL-stencil-equivalent Level1 fill of every mesh that asks for it, on
+// stream s: it reads only the soup and the grid, so the fused pipeline runs
+// it beside the Level1 SAT; it writes the inside bits of those meshes,
+// which the ray-parity fill leaves alone
+int launch_stencil(Job* J, hipStream_t s)
+{
+    if (!J->any_stencil || J->stencil_done) return GPV_OK;
+    for (int m = 0; m < J->nm; ++m)
+        if (J->g[m].l1st)
+            launch_l1_stencil(J->tris + (size_t)J->off[m].tri0 * 9, (int64_t)J->off[m + 1].tri0 - J->off[m].tri0,
+                              J->g[m], J->st_cnt, J->st_off, J->st_scan, J->st_tog, J->ins + J->off[m].word0, s);
+    HIP_TRY(hipGetLastError());
+    J->stencil_done = true;
     return GPV_OK;
 }
 
@@ -978,16 +1002,14 @@ int col_sort_kernels(Job* J, hipStream_t s, uint32_t* scan)
 // the parity fill on stream s: hits by column, their MT records, the fill
 int parity_kernels(Job* J, hipStream_t s)
 {
-    launch_fill_u32(J->hcnt, 0, J->ncol, s);
-    launch_count_keys_hi(J->hit_key, J->n_hits, J->hcnt, s);
-    launch_scan_u32_u32(J->hcnt, J->hoff, J->ncol, J->scan_side, nullptr, s);
-    launch_scatter_hits(J->tris, J->hit_key, J->n_hits, J->g[0], J->view(), J->hoff, J->hcnt, J->hrec, s);
-    launch_l1_parity(J->hrec, J->hoff, J->g[0], J->view(), (int64_t)J->off[J->nm].ptask0, J->ins, s);
-    // meshes with the stencil fill: their inside bits, overwritten in turn
-    for (int m = 0; m < J->nm && J->any_stencil; ++m)
-        if (J->g[m].l1st)
-            launch_l1_stencil(J->tris + (size_t)J->off[m].tri0 * 9, (int64_t)J->off[m + 1].tri0 - J->off[m].tri0,
-                              J->g[m], J->st_cnt, J->st_off, J->st_scan, J->st_tog, J->ins + J->off[m].word0, s);
+    if (!J->all_stencil) {   // the ray parity skips the stencil meshes
+        launch_fill_u32(J->hcnt, 0, J->ncol, s);
+        launch_count_keys_hi(J->hit_key, J->n_hits, J->hcnt, s);
+        launch_scan_u32_u32(J->hcnt, J->hoff, J->ncol, J->scan_side, nullptr, s);
+        launch_scatter_hits(J->tris, J->hit_key, J->n_hits, J->g[0], J->view(), J->hoff, J->hcnt, J->hrec, s);
+        launch_l1_parity(J->hrec, J->hoff, J->g[0], J->view(), (int64_t)J->off[J->nm].ptask0, J->ins, s);
+    }
+    TRY(launch_stencil(J, s));   // already launched beside the Level1 SAT in the fused pipeline
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(J->ev[14], s));
     return GPV_OK;
@@ -1349,7 +1371,11 @@ int job_run(Job* J)
         TRY(stage_setup_split(J));
         TRY(launch_ray_half(J, c->side));
         TRY(launch_sat_half(J));
-        HIP_TRY(hipStreamWaitEvent(c->stream, J->ev[13], 0));
+        // the count read-back waits for the ray expansion's hits; in a batch
+        // that mixes the two fills the stencil follows it on the side stream,
+        // after that point (with only stencil meshes it has run in its place)
+        if (!J->all_stencil) HIP_TRY(hipStreamWaitEvent(c->stream, J->ev[13], 0));
+        TRY(launch_stencil(J, c->side));
     } else {
         TRY(stage_setup(J));
         TRY(launch_sat_half(J));
