@@ -413,10 +413,11 @@ class Context:
         return Grid(self, g)
 
     def voxelize_noalloc(self, device_ptr, ntris, bmin, bmax, mms, voxel_count, voxel_count2, level2, l1b, l1i,
-                         l1_words, bidx, l2b, l2i, nb_cap):
+                         l1_words, bidx, l2b, l2i, nb_cap, l1_stencil=False):
         """gpv_voxelize_noalloc: device soup in, results into the caller's
         device buffers (pointers); returns the grid info dict."""
-        p = Params(voxel_count=int(voxel_count), voxel_count2=int(voxel_count2), level2=1 if level2 else 0)
+        p = Params(voxel_count=int(voxel_count), voxel_count2=int(voxel_count2), level2=1 if level2 else 0,
+                   l1_stencil=1 if l1_stencil else 0)
         info = GridInfo()
         _check(lib().gpv_voxelize_noalloc(self.h, device_ptr, int(ntris), np.asarray(bmin, np.float32),
                                           np.asarray(bmax, np.float32), float(mms), ctypes.byref(p), l1b, l1i,

@@ -36,11 +36,11 @@ def garbage(ctx, nbytes):
     return ctx.upload(np.full(max(nbytes, 8), 0xA5, np.uint8))
 
 
-def run_pair(gv, ctx, name, verts, faces, vc, vc2, level2=True):
+def run_pair(gv, ctx, name, verts, faces, vc, vc2, level2=True, l1_stencil=False):
     soup = gv.flatten(verts, faces)
     bmin, bmax, mms = gv.mesh_bbox(verts)
     say(f"noalloc {name}: plain T={len(soup)} {vc}/{vc2}")
-    g = ctx.voxelize(soup, bmin, bmax, mms, vc, vc2, level2)
+    g = ctx.voxelize(soup, bmin, bmax, mms, vc, vc2, level2, l1_stencil=l1_stencil)
     try:
         want_bits = g.download_bits()
         want = g.download()
@@ -58,7 +58,7 @@ def run_pair(gv, ctx, name, verts, faces, vc, vc2, level2=True):
     try:
         say(f"noalloc {name}: no-allocation entry point")
         info = ctx.voxelize_noalloc(dsoup, len(soup), bmin, bmax, mms, vc, vc2, level2, bufs[0], bufs[1], w1,
-                                    bufs[2], bufs[3], bufs[4], nb_cap)
+                                    bufs[2], bufs[3], bufs[4], nb_cap, l1_stencil=l1_stencil)
         nb = info["nb"]
         assert nb == winfo["nb"], f"{name}: nb {nb} != {winfo['nb']}"
         got = (d2h(bufs[0], w1 * 8, np.uint64), d2h(bufs[1], w1 * 8, np.uint64),
@@ -91,6 +91,15 @@ def test_noalloc_level2_off_and_stress(gv, ctx):
     run_pair(gv, ctx, "stress-box-64-8", vs, fs, 64, 8)
     v, f = gv.icosphere(3)
     run_pair(gv, ctx, "level2-off", v, f, 48, 4, level2=False)
+
+
+def test_noalloc_stencil_fill(gv, ctx):
+    """The GL-stencil-equivalent Level1 fill from the scratch array, on a knot
+    whose stencil and ray-parity fills differ (ray-invisible triangles)."""
+    v, f = gv.torus_knot(p=3, q=7, segments=1024, ring=64, tube=0.3, jitter=0.01, extent=0.9, seed=2)
+    a = run_pair(gv, ctx, "knot-131k-128-4-stencil", v, f, 128, 4, l1_stencil=True)
+    b = run_pair(gv, ctx, "knot-131k-128-4", v, f, 128, 4)
+    assert a["l1_inside"] != b["l1_inside"]
 
 
 def test_noalloc_rejects_what_does_not_fit(gv, ctx):
