@@ -115,3 +115,31 @@ def test_stencil_headline_torus_knot(gv, orc, ctx):
     m = workloads.plain_large(gv)
     got, want = stencil_case(gv, orc, ctx, "knot-256-8", m["soup"], m["bmin"], m["bmax"], m["mms"], 256, 8)
     assert want["l1_inside"] > 800000
+
+
+def test_stencil_batch_edge_cases(gv, orc, ctx):
+    """The stencil fill on the batch edge cases: the eleven-mesh mix (Level2
+    off for one, a single triangle, an empty soup), a mesh of vertical
+    triangles only (no sample covered) and a lone horizontal square (open)."""
+    from test_gpu_batch import small_mix
+    meshes = small_mix(gv)
+    vert = np.array([[0.2, 0.2, 0.1, 0.7, 0.6, 0.1, 0.2, 0.2, 0.9],
+                     [0.7, 0.6, 0.1, 0.7, 0.6, 0.9, 0.2, 0.2, 0.9]], np.float32)
+    sq = square(2.3, 2.3, 29.7, 29.7, 0.5)
+    for soup in (vert, sq):
+        meshes.append({"soup": soup, "bmin": UNIT[0], "bmax": UNIT[1], "mms": UNIT[2], "voxel_count": 16,
+                       "voxel_count2": 4, "level2": True, "initial_capacity": 0})
+    for m in meshes:
+        m["l1_stencil"] = True
+    say(f"stencil edge-case batch: {len(meshes)} meshes")
+    grids = ctx.voxelize_batch(meshes)
+    try:
+        for i, (m, gr) in enumerate(zip(meshes, grids)):
+            g = orc.grid_setup(m["bmin"], m["bmax"], m["mms"], m["voxel_count"],
+                               m["voxel_count2"] if m["level2"] else 0, l1_stencil=True)
+            want = orc.voxelize(m["soup"], g, orc.ORC_FAST)
+            compare_results(f"edge member {i}", gr.download(), want, g.n2)
+            assert gr.info()["l1_inside"] == want["l1_inside"]
+    finally:
+        for gr in grids:
+            gr.release()
