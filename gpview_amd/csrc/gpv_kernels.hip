@@ -3155,7 +3155,24 @@ __global__ void k_count_states(const uint64_t* __restrict__ bnd, const uint64_t*
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = w_lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < w_hi; i += stride)
         a += (uint64_t)__popcll(ins[i] & ~bnd[i]);
-    for (uint64_t i = l_lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < l_hi; i += stride) {
+    // the Level2 planes four words per lane at a time, their loads issued
+    // together (the blocks are few: they add into one counter each)
+    constexpr int U = 4;
+    uint64_t i = l_lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (U - 1) * stride < l_hi; i += U * stride) {
+        uint64_t p[U], q[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            p[u] = par2[i + u * stride];
+            q[u] = sat2[i + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            b += (uint64_t)__popcll(p[u] & ~q[u]);
+            c += (uint64_t)__popcll(q[u]);
+        }
+    }
+    for (; i < l_hi; i += stride) {
         b += (uint64_t)__popcll(par2[i] & ~sat2[i]);
         c += (uint64_t)__popcll(sat2[i]);
     }
