@@ -218,6 +218,7 @@ struct gpv_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;   // second stream: Level2 parity beside the Level2 SAT
+    hipStream_t copy = nullptr;   // uploads of a batch's host soups ahead of its runs (created on first use)
     Pool pool;
     char* h_buf = nullptr;   // pinned read-back staging
     size_t h_cap = 0;
@@ -1501,7 +1502,7 @@ int run_batch_range(gpv_ctx* c, const gpv_mesh_desc* d, int lo, int hi, int on_d
         g_err = keep;
     }
     const bool split = r == GPV_ERR_OOM || r == GPV_ERR_CAPACITY || r == GPV_ERR_INVALID;
-    if (hi - lo < 2 || !split) return r;
+    if (hi - lo < 2 || !split) return r;   // (a failed run's own soup copy is redone by its halves)
     const int mid = lo + (hi - lo) / 2;
     TRY(run_batch_range(c, d, lo, mid, on_device, out));
     r = run_batch_range(c, d, mid, hi, on_device, out);
@@ -1514,6 +1515,80 @@ int run_batch_range(gpv_ctx* c, const gpv_mesh_desc* d, int lo, int hi, int on_d
         g_err = keep;
     }
     return r;
+}
+
+// batch uploads ahead of their runs: host soups of runs [lo, hi) copied in
+// mesh order into one device buffer per run on the context's copy stream,
+// an event after each run's copies, and the descriptors pointing there
+constexpr int kStagePieces = 2;
+constexpr int64_t kStageMinBytes = int64_t(64) << 20;
+
+struct StagedSoups {
+    std::vector<void*> buf;
+    std::vector<hipEvent_t> ev;
+    std::vector<gpv_mesh_desc> desc;
+};
+
+void unstage_soups(gpv_ctx* c, StagedSoups* st)
+{
+    if (c->copy && !st->buf.empty()) (void)hipStreamSynchronize(c->copy);
+    // the runs that read the buffers have finished (every run ends with a
+    // host synchronisation), so the blocks go back to the pool
+    for (void* p : st->buf) c->pool.put(p);
+    for (hipEvent_t e : st->ev) (void)hipEventDestroy(e);
+    st->buf.clear();
+    st->ev.clear();
+    st->desc.clear();
+}
+
+int stage_soups(gpv_ctx* c, const gpv_mesh_desc* d, int nm, const std::vector<std::pair<int, int>>& runs,
+                StagedSoups* st)
+{
+    if (!c->copy && hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        c->copy = nullptr;
+        return GPV_ERR_HIP;
+    }
+    // pool blocks are reused in the context stream's order; the copy stream
+    // writes its blocks out of that order, so nothing may be pending there
+    if (hipStreamSynchronize(c->stream) != hipSuccess || hipStreamSynchronize(c->side) != hipSuccess) {
+        (void)hipGetLastError();
+        return GPV_ERR_HIP;
+    }
+    st->desc.assign(d, d + nm);
+    for (const auto& rg : runs) {
+        int64_t bytes = 0;
+        for (int m = rg.first; m < rg.second; ++m) bytes += d[m].ntris * 36;
+        void* p = nullptr;
+        hipEvent_t e = nullptr;
+        if (c->pool.get((size_t)std::max<int64_t>(bytes, 1), &p) != hipSuccess) {
+            (void)hipGetLastError();
+            unstage_soups(c, st);
+            return GPV_ERR_OOM;
+        }
+        st->buf.push_back(p);
+        int64_t at = 0;
+        for (int m = rg.first; m < rg.second; ++m) {
+            const int64_t b = d[m].ntris * 36;
+            if (b > 0 && hipMemcpyAsync(static_cast<char*>(p) + at, d[m].tris, (size_t)b, hipMemcpyHostToDevice,
+                                        c->copy) != hipSuccess) {
+                (void)hipGetLastError();
+                unstage_soups(c, st);
+                return GPV_ERR_HIP;
+            }
+            st->desc[(size_t)m].tris = reinterpret_cast<const float*>(static_cast<char*>(p) + at);
+            at += b;
+        }
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(e, c->copy) != hipSuccess) {
+            (void)hipGetLastError();
+            if (e) (void)hipEventDestroy(e);
+            unstage_soups(c, st);
+            return GPV_ERR_HIP;
+        }
+        st->ev.push_back(e);
+    }
+    return GPV_OK;
 }
 
 }  // namespace
@@ -1596,6 +1671,7 @@ int gpv_ctx_destroy(gpv_ctx* c)
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->side);
+        if (c->copy) (void)hipStreamSynchronize(c->copy);
         for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
         c->events.clear();
         for (hipEvent_t e : c->na_ev)
@@ -1604,6 +1680,7 @@ int gpv_ctx_destroy(gpv_ctx* c)
         if (c->h_buf) (void)hipHostFree(c->h_buf);
         (void)hipStreamDestroy(c->stream);
         (void)hipStreamDestroy(c->side);
+        if (c->copy) (void)hipStreamDestroy(c->copy);
     }
     delete c;
     return GPV_OK;
@@ -2031,36 +2108,60 @@ int gpv_voxelize_batch(gpv_ctx* c, const gpv_mesh_desc* meshes, int nmeshes, int
     int64_t room = (int64_t)free_b + ps.cached_bytes;
     if (ps.limit_bytes > 0) room = std::min<int64_t>(room, ps.limit_bytes - ps.live_bytes);
     const int64_t chunk_bytes = std::max<int64_t>(room / 2, 1);
+    // Host soups of a large batch go up in kStagePieces runs: every run's
+    // soups are uploaded ahead on the copy stream and each run waits only
+    // for its own, so the next run's upload proceeds under this run's kernels.
+    int64_t soup_bytes = 0;
+    for (int m = 0; m < nmeshes; ++m) soup_bytes += meshes[m].ntris * 36;
+    const int pieces = !on_device && nmeshes >= 2 && soup_bytes >= kStageMinBytes ? kStagePieces : 1;
+    const int64_t piece_bytes = (soup_bytes + pieces - 1) / pieces;
     // consecutive runs within the per-run limits and half the free memory
-    int lo = 0;
-    while (lo < nmeshes) {
+    std::vector<std::pair<int, int>> runs;
+    for (int lo = 0; lo < nmeshes;) {
         int hi = lo;
-        int64_t T = 0, V = 0, C = 0, F = 0;
+        int64_t T = 0, V = 0, C = 0, F = 0, S = 0;
         while (hi < nmeshes) {
             const size_t k = (size_t)hi;
             const bool first = hi == lo;
             if (!first && (T + meshes[hi].ntris >= (int64_t(1) << 31) || V + vox[k] >= (int64_t(1) << 32) ||
-                           C + cols[k] > (int64_t(1) << 27) || F + foot[k] > chunk_bytes))
+                           C + cols[k] > (int64_t(1) << 27) || F + foot[k] > chunk_bytes ||
+                           (pieces > 1 && S + meshes[hi].ntris * 36 > piece_bytes)))
                 break;
             T += meshes[hi].ntris;
             V += vox[k];
             C += cols[k];
             F += foot[k];
+            S += meshes[hi].ntris * 36;
             ++hi;
         }
-        const int r = run_batch_range(c, meshes, lo, hi, on_device, out);
+        runs.emplace_back(lo, hi);
+        lo = hi;
+    }
+    StagedSoups st;
+    const gpv_mesh_desc* src = meshes;
+    int src_on_device = on_device;
+    if (pieces > 1 && runs.size() > 1 && stage_soups(c, meshes, nmeshes, runs, &st) == GPV_OK) {
+        src = st.desc.data();
+        src_on_device = 1;
+    }
+    int r = GPV_OK;
+    for (size_t q = 0; q < runs.size() && r == GPV_OK; ++q) {
+        if (!st.ev.empty() && hipStreamWaitEvent(c->stream, st.ev[q], 0) != hipSuccess) {
+            r = fail(GPV_ERR_HIP, "stream wait on a staged upload failed");
+            break;
+        }
+        r = run_batch_range(c, src, runs[q].first, runs[q].second, src_on_device, out);
         if (r != GPV_OK) {
             std::string keep = g_err;
-            for (int m = 0; m < lo; ++m) {
+            for (int m = 0; m < runs[q].first; ++m) {
                 gpv_grid_release(c, out[m]);
                 out[m] = nullptr;
             }
             g_err = keep;
-            return r;
         }
-        lo = hi;
     }
-    return GPV_OK;
+    unstage_soups(c, &st);
+    return r;
 }
 
 int gpv_grid_get_info(const gpv_grid* G, gpv_grid_info* info)

@@ -345,3 +345,31 @@ def test_stress_level1_512_batch_equals_alone(gv, ctx):
     finally:
         for g in gs:
             g.release()
+
+
+def test_batch_staged_uploads_equal_resident(gv, ctx):
+    """Host soups of a batch above 64 MB go up in two runs, the second
+    uploaded under the first's kernels; the grids equal those of the same
+    batch from resident soups."""
+    meshes = []
+    for i, (kind, target) in enumerate([(1, 900000), (2, 700000), (1, 500000), (1, 300000), (0, 40000)]):
+        v, f = gv.mixed(kind, target, 900 + i)
+        meshes.append(mesh_entry(gv, v, f, 96 + 32 * (i % 2), 4 + 4 * (i % 2)))
+    total = sum(m["soup"].nbytes for m in meshes)
+    assert total > (64 << 20)
+    pinned = [gv.pinned_copy(m["soup"]) for m in meshes]
+    host = [dict(m, soup=p.array) for m, p in zip(meshes, pinned)]
+    ptrs = [ctx.upload(m["soup"]) for m in meshes]
+    try:
+        say(f"staged batch: {len(meshes)} meshes, {total >> 20} MiB of soup")
+        a = batch_results(ctx, host)
+        b = batch_results(ctx, meshes, device_ptrs=ptrs)
+    finally:
+        for p in ptrs:
+            ctx.free(p)
+        for p in pinned:
+            p.close()
+    for i, ((ga, ia), (gb, ib)) in enumerate(zip(a, b)):
+        for k in ("l1", "prefix", "bidx", "l2"):
+            assert np.array_equal(ga[k], gb[k]), f"member {i}: {k}"
+        check_info(f"member {i}", ia, ib)
