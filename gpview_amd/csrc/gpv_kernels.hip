@@ -1057,6 +1057,27 @@ void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0
 }
 
 // ---------------------------------------------------------------------------
+// a batch's device soups gathered into the batch's soup in one launch (one
+// grid row per mesh, each row's blocks striding over the mesh's floats)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gather_soups(const SoupPiece* __restrict__ pc, int m0,
+                                                      float* __restrict__ dst)
+{
+    const SoupPiece p = pc[m0 + blockIdx.y];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += stride)
+        dst[p.at + i] = p.src[i];
+}
+
+void launch_gather_soups(const SoupPiece* pc, int nm, float* dst, hipStream_t s)
+{
+    for (int m0 = 0; m0 < nm; m0 += 65535) {
+        const int rows = nm - m0 < 65535 ? nm - m0 : 65535;
+        k_gather_soups<<<dim3(64, (unsigned)rows), 256, 0, s>>>(pc, m0, dst);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Level1 inside/outside by the GL-stencil-equivalent fill (opt-in,
 // gpv_params::l1_stencil; ClassifyInOut2x, src/Object.cpp:1446-1601).  Per
 // slice s the reference draws the mesh orthographically over the bbox into a

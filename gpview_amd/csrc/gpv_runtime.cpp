@@ -251,6 +251,8 @@ struct Job {
     int64_t T = 0;
     const float* tris = nullptr;
     float* own_soup = nullptr;
+    SoupPiece* soup_pieces = nullptr;   // the gather table of a batch of device soups
+    std::vector<SoupPiece> h_pieces;    // its host copy (the source of an asynchronous upload)
     float* tables = nullptr;
     GridDev* d_grids = nullptr;
     MeshOff* d_off = nullptr;
@@ -688,17 +690,44 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         g.r2x = tb + o[9]; g.r2y = tb + o[10]; g.r2z = tb + o[11];
     }
 
-    // the soup
+    // the soup; device soups that already lie back to back in mesh order (a
+    // staged batch upload) are used in place (empty meshes may point anywhere;
+    // the first non-empty one starts the soup, its first triangle being 0)
+    const float* base = nullptr;
+    bool contiguous = on_device != 0 && nm > 1;
+    for (int m = 0; m < nm && contiguous; ++m) {
+        if (d[m].ntris == 0) continue;
+        if (!base) base = d[m].tris;
+        else contiguous = d[m].tris == base + (size_t)J->off[m].tri0 * 9;
+    }
     if (nm == 1 && on_device) {
         J->tris = d[0].tris;
+    } else if (contiguous && base) {
+        J->tris = base;
+    } else if (T > 0 && on_device) {
+        // device soups in one gather launch instead of a copy per mesh
+        float* s = nullptr;
+        SoupPiece* pc = nullptr;
+        if ((r = alloc(J, &s, (size_t)T * 9)) != GPV_OK || (r = alloc(J, &pc, (size_t)nm)) != GPV_OK) return bail(r);
+        std::vector<SoupPiece>& hp = J->h_pieces;
+        hp.resize((size_t)nm);
+        for (int m = 0; m < nm; ++m)
+            hp[(size_t)m] = SoupPiece{d[m].tris, (uint64_t)J->off[m].tri0 * 9, (uint64_t)d[m].ntris * 9};
+        if (hipMemcpyAsync(pc, hp.data(), sizeof(SoupPiece) * (size_t)nm, hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess)
+            return bail(fail(GPV_ERR_HIP, "soup table upload failed"));
+        launch_gather_soups(pc, nm, s, c->stream);
+        if (hipGetLastError() != hipSuccess) return bail(fail(GPV_ERR_HIP, "soup gather failed"));
+        J->tris = s;
+        J->own_soup = s;
+        J->soup_pieces = pc;
     } else if (T > 0) {
         float* s = nullptr;
         if ((r = alloc(J, &s, (size_t)T * 9)) != GPV_OK) return bail(r);
-        const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
         for (int m = 0; m < nm; ++m)
             if (d[m].ntris > 0 &&
                 hipMemcpyAsync(s + (size_t)J->off[m].tri0 * 9, d[m].tris, (size_t)d[m].ntris * 9 * sizeof(float),
-                               kind, c->stream) != hipSuccess)
+                               hipMemcpyHostToDevice, c->stream) != hipSuccess)
                 return bail(fail(GPV_ERR_HIP, "soup copy of mesh %d failed", m));
         J->tris = s;
         J->own_soup = s;
@@ -1351,6 +1380,7 @@ int job_finish(Job* J)
     release(J, J->st_cnt); release(J, J->st_off); release(J, J->st_scan); release(J, J->st_tog);
     release(J, J->bmesh);
     release(J, J->d_grids); release(J, J->d_off); release(J, J->own_soup); release(J, J->mctr);
+    release(J, J->soup_pieces);
     release(J, J->fnorm); release(J, J->slist);
     J->tris = nullptr;
     J->finished = true;
@@ -2114,24 +2144,33 @@ int gpv_voxelize_batch(gpv_ctx* c, const gpv_mesh_desc* meshes, int nmeshes, int
     int64_t soup_bytes = 0;
     for (int m = 0; m < nmeshes; ++m) soup_bytes += meshes[m].ntris * 36;
     const int pieces = !on_device && nmeshes >= 2 && soup_bytes >= kStageMinBytes ? kStagePieces : 1;
-    const int64_t piece_bytes = (soup_bytes + pieces - 1) / pieces;
+    const int64_t piece_bytes = std::max<int64_t>(1, (soup_bytes + pieces - 1) / pieces);
+    // the piece of a mesh: where the middle of its soup falls
+    std::vector<int> piece_of((size_t)nmeshes, 0);
+    {
+        int64_t at = 0;
+        for (int m = 0; m < nmeshes && pieces > 1; ++m) {
+            const int64_t b = meshes[m].ntris * 36;
+            piece_of[(size_t)m] = (int)std::min<int64_t>(pieces - 1, (at + b / 2) / piece_bytes);
+            at += b;
+        }
+    }
     // consecutive runs within the per-run limits and half the free memory
     std::vector<std::pair<int, int>> runs;
     for (int lo = 0; lo < nmeshes;) {
         int hi = lo;
-        int64_t T = 0, V = 0, C = 0, F = 0, S = 0;
+        int64_t T = 0, V = 0, C = 0, F = 0;
         while (hi < nmeshes) {
             const size_t k = (size_t)hi;
             const bool first = hi == lo;
             if (!first && (T + meshes[hi].ntris >= (int64_t(1) << 31) || V + vox[k] >= (int64_t(1) << 32) ||
                            C + cols[k] > (int64_t(1) << 27) || F + foot[k] > chunk_bytes ||
-                           (pieces > 1 && S + meshes[hi].ntris * 36 > piece_bytes)))
+                           piece_of[k] != piece_of[(size_t)lo]))
                 break;
             T += meshes[hi].ntris;
             V += vox[k];
             C += cols[k];
             F += foot[k];
-            S += meshes[hi].ntris * 36;
             ++hi;
         }
         runs.emplace_back(lo, hi);
