@@ -222,6 +222,8 @@ struct gpv_ctx {
     Pool pool;
     char* h_buf = nullptr;   // pinned read-back staging
     size_t h_cap = 0;
+    float* tab_buf = nullptr;   // pinned staging of a job's center tables
+    size_t tab_cap = 0;
     int num_cu = 256;
     gpv_tune tune{16, 8, 2048, 1, 1, 2, 128};
     std::vector<hipEvent_t> events;   // recycled stage events of finished jobs
@@ -246,8 +248,8 @@ struct Job {
     int nm = 1;
     std::vector<gpv_grid_info> info;   // [nm]
     std::vector<GridDev> g;            // [nm], device table pointers
+    std::vector<float> h_tab;          // center tables of a no-allocation call (ordinary host memory)
     std::vector<MeshOff> off;          // [nm + 1]
-    std::vector<float> h_tab;
     int64_t T = 0;
     const float* tris = nullptr;
     float* own_soup = nullptr;
@@ -442,7 +444,8 @@ int derive_grid(const float bmin[3], const float bmax[3], float mms, const gpv_p
 // Returns whether each Level2 ray-origin table is non-decreasing over its
 // whole length (the parity kernels then bound a range of origins by its
 // first and last entries).
-bool build_tables(const gpv_grid_info& I, bool cuda_l2, std::vector<float>& tab, size_t off[12])
+// the twelve tables' offsets; returns the floats they take (plus one)
+size_t table_layout(const gpv_grid_info& I, size_t off[12])
 {
     const int n2 = I.n2 > 0 ? I.n2 : 0;
     size_t pos = 0;
@@ -450,7 +453,12 @@ bool build_tables(const gpv_grid_info& I, bool cuda_l2, std::vector<float>& tab,
     for (int a = 0; a < 3; a++) { off[3 + a] = pos; pos += I.n[a]; }
     for (int a = 0; a < 3; a++) { off[6 + a] = pos; pos += (size_t)I.n[a] * n2; }
     for (int a = 0; a < 3; a++) { off[9 + a] = pos; pos += (size_t)I.n[a] * n2; }
-    tab.assign(pos + 1, 0.0f);
+    return pos + 1;
+}
+
+bool fill_tables(const gpv_grid_info& I, bool cuda_l2, float* tab, const size_t off[12])
+{
+    const int n2 = I.n2 > 0 ? I.n2 : 0;
     for (int a = 0; a < 3; a++) {
         const float ext1 = (float)((double)I.g[a] / 2.0);
         for (int p = 0; p < I.n[a]; p++) {
@@ -481,6 +489,12 @@ bool build_tables(const gpv_grid_info& I, bool cuda_l2, std::vector<float>& tab,
     for (int a = 0; a < 3; a++)
         for (size_t k = 1; k < (size_t)I.n[a] * n2; k++) mono = mono && tab[off[9 + a] + k - 1] <= tab[off[9 + a] + k];
     return mono;
+}
+
+bool build_tables(const gpv_grid_info& I, bool cuda_l2, std::vector<float>& tab, size_t off[12])
+{
+    tab.assign(table_layout(I, off), 0.0f);
+    return fill_tables(I, cuda_l2, tab.data(), off);
 }
 
 int expand_blocks(const gpv_ctx* c) { return std::max(1, c->num_cu) * std::max(1, c->tune.expand_blocks_per_cu); }
@@ -644,6 +658,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
     J->g.resize((size_t)nm);
     J->off.assign((size_t)nm + 1, MeshOff{});
     std::vector<size_t> tab_at((size_t)nm * 12);
+    size_t ntab = 0;
     uint64_t word0 = 0;
     uint32_t col0 = 0, tri0 = 0, ptask0 = 0;
     for (int m = 0; m < nm; ++m) {
@@ -665,20 +680,48 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         word0 += (uint64_t)g.nz * g.ny * g.nxw;
         col0 += (uint32_t)(g.nx * g.ny);
         tri0 += (uint32_t)d[m].ntris;
-        std::vector<float> tab;
         size_t o[12];
-        g.r2mono = build_tables(J->info[m], g.l2cuda != 0, tab, o) ? 1 : 0;
-        for (int k = 0; k < 12; ++k) tab_at[(size_t)m * 12 + k] = J->h_tab.size() + o[k];
-        J->h_tab.insert(J->h_tab.end(), tab.begin(), tab.end());
+        const size_t n = table_layout(J->info[m], o);
+        for (int k = 0; k < 12; ++k) tab_at[(size_t)m * 12 + k] = ntab + o[k];
+        ntab += n;
+    }
+    // the tables are written straight into the context's page-locked
+    // staging, so their upload is asynchronous (the previous job on this
+    // context has finished); the no-allocation entry point, which may not
+    // grow that staging, uses ordinary host memory instead
+    float* tab_src = nullptr;
+    if (J->arena) {
+        J->h_tab.assign(ntab, 0.0f);
+        tab_src = J->h_tab.data();
+    } else {
+        if (ntab * sizeof(float) > c->tab_cap) {
+            const size_t want = std::max<size_t>(ntab * sizeof(float), 2 * c->tab_cap);
+            if (c->tab_buf) (void)hipHostFree(c->tab_buf);
+            c->tab_buf = nullptr;
+            c->tab_cap = 0;
+            if (hipHostMalloc(reinterpret_cast<void**>(&c->tab_buf), want, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                c->tab_buf = nullptr;
+                return bail(fail(GPV_ERR_OOM, "page-locked table staging of %zu bytes failed", want));
+            }
+            c->tab_cap = want;
+        }
+        tab_src = c->tab_buf;
+    }
+    for (int m = 0; m < nm; ++m) {
+        GridDev& g = J->g[m];
+        size_t o[12];
+        table_layout(J->info[m], o);
+        const size_t base = tab_at[(size_t)m * 12];   // the first table starts at offset 0 of the mesh's block
+        g.r2mono = fill_tables(J->info[m], g.l2cuda != 0, tab_src + base, o) ? 1 : 0;
     }
     J->off[nm].word0 = word0;
     J->off[nm].col0 = col0;
     J->off[nm].tri0 = tri0;
     J->off[nm].ptask0 = ptask0;
     int r;
-    if ((r = alloc(J, &J->tables, J->h_tab.size())) != GPV_OK) return bail(r);
-    if (hipMemcpyAsync(J->tables, J->h_tab.data(), J->h_tab.size() * sizeof(float), hipMemcpyHostToDevice,
-                       c->stream) != hipSuccess)
+    if ((r = alloc(J, &J->tables, ntab)) != GPV_OK) return bail(r);
+    if (hipMemcpyAsync(J->tables, tab_src, ntab * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return bail(fail(GPV_ERR_HIP, "table upload failed"));
     for (int m = 0; m < nm; ++m) {
         const float* tb = J->tables;
@@ -1708,6 +1751,7 @@ int gpv_ctx_destroy(gpv_ctx* c)
             if (e) (void)hipEventDestroy(e);
         c->pool.trim();
         if (c->h_buf) (void)hipHostFree(c->h_buf);
+        if (c->tab_buf) (void)hipHostFree(c->tab_buf);
         (void)hipStreamDestroy(c->stream);
         (void)hipStreamDestroy(c->side);
         if (c->copy) (void)hipStreamDestroy(c->copy);
