@@ -84,6 +84,23 @@ class MeshDesc(ctypes.Structure):
                 ("bmax", ctypes.c_float * 3), ("max_model_size", ctypes.c_float), ("params", Params)]
 
 
+def _desc_dtype():
+    """numpy mirror of gpv_mesh_desc (offsets from the ctypes layout), so a
+    batch's descriptors are filled a field at a time."""
+    P = Params
+    names = ["tris", "ntris", "bmin", "bmax", "max_model_size"] + ["p_" + f for f, _ in P._fields_]
+    fmts = ["<u8", "<i8", ("<f4", 3), ("<f4", 3), "<f4"]
+    offs = [getattr(MeshDesc, f).offset for f in ("tris", "ntris", "bmin", "bmax", "max_model_size")]
+    base = MeshDesc.params.offset
+    for f, t in P._fields_:
+        fmts.append("<i8" if t is ctypes.c_int64 else "<i4")
+        offs.append(base + getattr(P, f).offset)
+    return np.dtype({"names": names, "formats": fmts, "offsets": offs, "itemsize": ctypes.sizeof(MeshDesc)})
+
+
+_DESC_DTYPE = None
+
+
 class ServerOpts(ctypes.Structure):
     """gpv_server_opts (include/gpv_serve.h)."""
     _fields_ = [("device", ctypes.c_int), ("lanes", ctypes.c_int), ("hbm_budget", ctypes.c_int64),
@@ -432,15 +449,10 @@ class Context:
         soups are only used for their triangle counts).  Returns one Grid per
         mesh."""
         n = len(meshes)
-        descs = (MeshDesc * n)()
-        keep = []
-        for i, m in enumerate(meshes):
-            d, soup = mesh_desc(m, device_ptrs[i] if device_ptrs is not None else None)
-            descs[i] = d
-            keep.append(soup)
+        descs, keep = mesh_descs(meshes, device_ptrs)
         out = (_vp * n)()
-        _check(lib().gpv_voxelize_batch(self.h, descs, n, 1 if device_ptrs is not None else 0, out),
-               "gpv_voxelize_batch")
+        _check(lib().gpv_voxelize_batch(self.h, descs.ctypes.data_as(ctypes.POINTER(MeshDesc)), n,
+                                        1 if device_ptrs is not None else 0, out), "gpv_voxelize_batch")
         return [Grid(self, _vp(out[i])) for i in range(n)]
 
     def create(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0,
@@ -481,6 +493,34 @@ def pinned_copy(a):
     p = PinnedArray(a.shape, a.dtype)
     p.array[...] = a
     return p
+
+
+def mesh_descs(meshes, device_ptrs=None):
+    """gpv_mesh_desc array of a sequence of mesh dicts, filled field by field
+    (numpy mirror of the struct); returns (array, soups) -- keep the soups
+    alive while the array is in use."""
+    global _DESC_DTYPE
+    if _DESC_DTYPE is None:
+        _DESC_DTYPE = _desc_dtype()
+    n = len(meshes)
+    a = np.zeros(n, _DESC_DTYPE)
+    soups = [np.ascontiguousarray(m["soup"], np.float32).reshape(-1, 9) for m in meshes]
+    if device_ptrs is not None:
+        a["tris"] = [int(p) for p in device_ptrs]
+    else:
+        a["tris"] = [s.__array_interface__["data"][0] for s in soups]
+    a["ntris"] = [len(s) for s in soups]
+    a["bmin"] = [m["bmin"] for m in meshes]
+    a["bmax"] = [m["bmax"] for m in meshes]
+    a["max_model_size"] = [m["mms"] for m in meshes]
+    a["p_voxel_count"] = [m["voxel_count"] for m in meshes]
+    a["p_voxel_count2"] = [m.get("voxel_count2", 4) for m in meshes]
+    a["p_level2"] = [1 if m.get("level2", True) else 0 for m in meshes]
+    a["p_normals"] = [1 if m.get("normals", False) else 0 for m in meshes]
+    a["p_initial_capacity"] = [m.get("initial_capacity", 0) for m in meshes]
+    a["p_cuda_level2"] = [1 if m.get("cuda_level2", False) else 0 for m in meshes]
+    a["p_l1_stencil"] = [1 if m.get("l1_stencil", False) else 0 for m in meshes]
+    return a, soups
 
 
 def mesh_desc(m, device_ptr=None):
