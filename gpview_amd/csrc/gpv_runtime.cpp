@@ -1593,7 +1593,7 @@ int run_batch_range(gpv_ctx* c, const gpv_mesh_desc* d, int lo, int hi, int on_d
 // batch uploads ahead of their runs: host soups of runs [lo, hi) copied in
 // mesh order into one device buffer per run on the context's copy stream,
 // an event after each run's copies, and the descriptors pointing there
-constexpr int kStagePieces = 2;
+constexpr int kStageFirst = 4;   // the first run takes 1/kStageFirst of the soup
 constexpr int64_t kStageMinBytes = int64_t(64) << 20;
 
 struct StagedSoups {
@@ -2182,20 +2182,22 @@ int gpv_voxelize_batch(gpv_ctx* c, const gpv_mesh_desc* meshes, int nmeshes, int
     int64_t room = (int64_t)free_b + ps.cached_bytes;
     if (ps.limit_bytes > 0) room = std::min<int64_t>(room, ps.limit_bytes - ps.live_bytes);
     const int64_t chunk_bytes = std::max<int64_t>(room / 2, 1);
-    // Host soups of a large batch go up in kStagePieces runs: every run's
-    // soups are uploaded ahead on the copy stream and each run waits only
-    // for its own, so the next run's upload proceeds under this run's kernels.
+    // Host soups of a large batch go up in two runs: both runs' soups are
+    // uploaded ahead on the copy stream and each run waits only for its own,
+    // so the second upload proceeds under the first run's kernels.
     int64_t soup_bytes = 0;
     for (int m = 0; m < nmeshes; ++m) soup_bytes += meshes[m].ntris * 36;
-    const int pieces = !on_device && nmeshes >= 2 && soup_bytes >= kStageMinBytes ? kStagePieces : 1;
-    const int64_t piece_bytes = std::max<int64_t>(1, (soup_bytes + pieces - 1) / pieces);
-    // the piece of a mesh: where the middle of its soup falls
+    const int pieces = !on_device && nmeshes >= 2 && soup_bytes >= kStageMinBytes ? 2 : 1;
+    // the piece of a mesh: where the middle of its soup falls; the first
+    // piece is the first kStageFirst of the soup, so its upload (the one
+    // nothing overlaps) is short and its run outlasts the rest's upload
     std::vector<int> piece_of((size_t)nmeshes, 0);
     {
+        const int64_t first = std::max<int64_t>(1, soup_bytes / kStageFirst);
         int64_t at = 0;
         for (int m = 0; m < nmeshes && pieces > 1; ++m) {
             const int64_t b = meshes[m].ntris * 36;
-            piece_of[(size_t)m] = (int)std::min<int64_t>(pieces - 1, (at + b / 2) / piece_bytes);
+            piece_of[(size_t)m] = at + b / 2 < first ? 0 : 1;
             at += b;
         }
     }
