@@ -1594,7 +1594,7 @@ int run_batch_range(gpv_ctx* c, const gpv_mesh_desc* d, int lo, int hi, int on_d
 // mesh order into one device buffer per run on the context's copy stream,
 // an event after each run's copies, and the descriptors pointing there
 constexpr int kStageFirst = 4;   // the first run takes 1/kStageFirst of the soup
-constexpr int64_t kStageMinBytes = int64_t(64) << 20;
+constexpr int64_t kStageMinBytes = int64_t(2) << 20;
 
 struct StagedSoups {
     std::vector<void*> buf;
@@ -2182,7 +2182,7 @@ int gpv_voxelize_batch(gpv_ctx* c, const gpv_mesh_desc* meshes, int nmeshes, int
     int64_t room = (int64_t)free_b + ps.cached_bytes;
     if (ps.limit_bytes > 0) room = std::min<int64_t>(room, ps.limit_bytes - ps.live_bytes);
     const int64_t chunk_bytes = std::max<int64_t>(room / 2, 1);
-    // Host soups of a large batch go up in two runs: both runs' soups are
+    // Host soups of a batch above 2 MiB go up in two runs: both runs' soups are
     // uploaded ahead on the copy stream and each run waits only for its own,
     // so the second upload proceeds under the first run's kernels.
     int64_t soup_bytes = 0;

@@ -347,16 +347,28 @@ def test_stress_level1_512_batch_equals_alone(gv, ctx):
             g.release()
 
 
-def test_batch_staged_uploads_equal_resident(gv, ctx):
-    """Host soups of a batch above 64 MB go up in two runs, the second
+STAGED_BATCHES = {
+    "large": [(1, 900000), (2, 700000), (1, 500000), (1, 300000), (0, 40000)],
+    # just above the 2 MiB staging threshold, the small meshes at the cut
+    "small": [(0, 30000), (1, 40000), (2, 20000), (0, 900), (1, 500), (2, 3000)],
+}
+
+
+@pytest.mark.parametrize("which", sorted(STAGED_BATCHES))
+def test_batch_staged_uploads_equal_resident(gv, ctx, which):
+    """Host soups of a batch above 2 MiB go up in two runs, the second
     uploaded under the first's kernels; the grids equal those of the same
     batch from resident soups."""
     meshes = []
-    for i, (kind, target) in enumerate([(1, 900000), (2, 700000), (1, 500000), (1, 300000), (0, 40000)]):
+    for i, (kind, target) in enumerate(STAGED_BATCHES[which]):
         v, f = gv.mixed(kind, target, 900 + i)
         meshes.append(mesh_entry(gv, v, f, 96 + 32 * (i % 2), 4 + 4 * (i % 2)))
+    if which == "small":
+        # an empty soup and a single triangle where the first run ends
+        meshes.insert(1, dict(meshes[0], soup=meshes[0]["soup"][:0]))
+        meshes.insert(2, dict(meshes[0], soup=meshes[0]["soup"][:1]))
     total = sum(m["soup"].nbytes for m in meshes)
-    assert total > (64 << 20)
+    assert total > (2 << 20)
     pinned = [gv.pinned_copy(m["soup"]) for m in meshes]
     host = [dict(m, soup=p.array) for m, p in zip(meshes, pinned)]
     ptrs = [ctx.upload(m["soup"]) for m in meshes]
