@@ -156,13 +156,14 @@ void launch_bin_cols(const float* tris, const uint64_t* key, const uint32_t* zma
 // Level1 parity tasks: L1P_Z consecutive layers of one 64-column row strip
 constexpr int L1P_Z = 16;
 inline int64_t l1_parity_tasks(const GridDev& g) { return (int64_t)((g.nz + L1P_Z - 1) / L1P_Z) * g.ny * g.nxw; }
-// one mesh's device soup (n floats) and its place in the batch's soup
+// one mesh's soup (n floats, device memory or page-locked host memory mapped
+// into the device's address space) and its place in the batch's soup
 struct SoupPiece {
     const float* src;
     uint64_t at;
     uint64_t n;
 };
-void launch_gather_soups(const SoupPiece* pc, int nm, float* dst, hipStream_t s);
+void launch_gather_soups(const SoupPiece* pc, int nm, float* dst, int blocks, hipStream_t s);
 
 // GL-stencil-equivalent Level1 fill of one mesh (soup tris[T], grid g):
 // overwrites the mesh's Level1 inside bits ins[nz*ny*nxw].  Scratch: cnt[T],

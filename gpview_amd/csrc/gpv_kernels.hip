@@ -1069,11 +1069,11 @@ __global__ __launch_bounds__(256) void k_gather_soups(const SoupPiece* __restric
         dst[p.at + i] = p.src[i];
 }
 
-void launch_gather_soups(const SoupPiece* pc, int nm, float* dst, hipStream_t s)
+void launch_gather_soups(const SoupPiece* pc, int nm, float* dst, int blocks, hipStream_t s)
 {
     for (int m0 = 0; m0 < nm; m0 += 65535) {
         const int rows = nm - m0 < 65535 ? nm - m0 : 65535;
-        k_gather_soups<<<dim3(64, (unsigned)rows), 256, 0, s>>>(pc, m0, dst);
+        k_gather_soups<<<dim3((unsigned)blocks, (unsigned)rows), 256, 0, s>>>(pc, m0, dst);
     }
 }
 

@@ -759,7 +759,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         if (hipMemcpyAsync(pc, hp.data(), sizeof(SoupPiece) * (size_t)nm, hipMemcpyHostToDevice, c->stream) !=
             hipSuccess)
             return bail(fail(GPV_ERR_HIP, "soup table upload failed"));
-        launch_gather_soups(pc, nm, s, c->stream);
+        launch_gather_soups(pc, nm, s, 64, c->stream);
         if (hipGetLastError() != hipSuccess) return bail(fail(GPV_ERR_HIP, "soup gather failed"));
         J->tris = s;
         J->own_soup = s;
@@ -1600,7 +1600,36 @@ struct StagedSoups {
     std::vector<void*> buf;
     std::vector<hipEvent_t> ev;
     std::vector<gpv_mesh_desc> desc;
+    std::vector<std::vector<SoupPiece>> pieces;   // gather tables, read by pending copies
 };
+
+constexpr int kMappedGatherBlocks = 16;   // gather workgroups per mesh for mapped host soups
+constexpr int64_t kMappedMaxBytes = int64_t(256) << 10;   // larger soups go by DMA copy
+
+// The device address of host soup [p, p + bytes) when it lies in one
+// page-locked allocation mapped into the device's address space at the same
+// address (hipHostMalloc memory on this platform), so that a kernel can read
+// it across the bus; null for pageable or any other memory.
+const float* mapped_host_soup(const float* p, int64_t bytes)
+{
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (a.type != hipMemoryTypeHost || a.devicePointer != (const void*)p || a.hostPointer != (const void*)p)
+        return nullptr;
+    hipDeviceptr_t start = nullptr;
+    size_t size = 0;
+    if (hipPointerGetAttribute(&start, HIP_POINTER_ATTRIBUTE_RANGE_START_ADDR, (hipDeviceptr_t)p) != hipSuccess ||
+        hipPointerGetAttribute(&size, HIP_POINTER_ATTRIBUTE_RANGE_SIZE, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    const uintptr_t lo = reinterpret_cast<uintptr_t>(start), q = reinterpret_cast<uintptr_t>(p);
+    if (q < lo || q - lo > size || (uint64_t)(size - (q - lo)) < (uint64_t)bytes) return nullptr;
+    return p;
+}
 
 void unstage_soups(gpv_ctx* c, StagedSoups* st)
 {
@@ -1612,6 +1641,7 @@ void unstage_soups(gpv_ctx* c, StagedSoups* st)
     st->buf.clear();
     st->ev.clear();
     st->desc.clear();
+    st->pieces.clear();
 }
 
 int stage_soups(gpv_ctx* c, const gpv_mesh_desc* d, int nm, const std::vector<std::pair<int, int>>& runs,
@@ -1629,28 +1659,59 @@ int stage_soups(gpv_ctx* c, const gpv_mesh_desc* d, int nm, const std::vector<st
         return GPV_ERR_HIP;
     }
     st->desc.assign(d, d + nm);
+    st->pieces.reserve(runs.size());
     for (const auto& rg : runs) {
+        // small soups in mapped page-locked memory are gathered by one kernel
+        // reading across the bus (a copy per mesh costs more than the bytes
+        // of a small soup); every other soup is one DMA copy
+        std::vector<SoupPiece> pc;
+        std::vector<int64_t> at((size_t)(rg.second - rg.first));
+        std::vector<char> gathered((size_t)(rg.second - rg.first), 0);
         int64_t bytes = 0;
-        for (int m = rg.first; m < rg.second; ++m) bytes += d[m].ntris * 36;
+        for (int m = rg.first; m < rg.second; ++m) {
+            const int64_t b = d[m].ntris * 36;
+            const float* src = b > 0 && b <= kMappedMaxBytes ? mapped_host_soup(d[m].tris, b) : nullptr;
+            if (src) pc.push_back(SoupPiece{src, (uint64_t)bytes / 4, (uint64_t)b / 4});
+            gathered[(size_t)(m - rg.first)] = src != nullptr;
+            at[(size_t)(m - rg.first)] = bytes;
+            bytes += b;
+        }
+        const size_t tab_at = ((size_t)bytes + 255) & ~size_t(255);
+        const size_t alloc = pc.empty() ? (size_t)std::max<int64_t>(bytes, 1) : tab_at + sizeof(SoupPiece) * pc.size();
         void* p = nullptr;
         hipEvent_t e = nullptr;
-        if (c->pool.get((size_t)std::max<int64_t>(bytes, 1), &p) != hipSuccess) {
+        if (c->pool.get(alloc, &p) != hipSuccess) {
             (void)hipGetLastError();
             unstage_soups(c, st);
             return GPV_ERR_OOM;
         }
         st->buf.push_back(p);
-        int64_t at = 0;
+        char* const dst = static_cast<char*>(p);
         for (int m = rg.first; m < rg.second; ++m) {
-            const int64_t b = d[m].ntris * 36;
-            if (b > 0 && hipMemcpyAsync(static_cast<char*>(p) + at, d[m].tris, (size_t)b, hipMemcpyHostToDevice,
-                                        c->copy) != hipSuccess) {
+            const int64_t b = d[m].ntris * 36, o = at[(size_t)(m - rg.first)];
+            if (b > 0 && !gathered[(size_t)(m - rg.first)] &&
+                hipMemcpyAsync(dst + o, d[m].tris, (size_t)b, hipMemcpyHostToDevice, c->copy) != hipSuccess) {
                 (void)hipGetLastError();
                 unstage_soups(c, st);
                 return GPV_ERR_HIP;
             }
-            st->desc[(size_t)m].tris = reinterpret_cast<const float*>(static_cast<char*>(p) + at);
-            at += b;
+            st->desc[(size_t)m].tris = reinterpret_cast<const float*>(dst + o);
+        }
+        if (!pc.empty()) {
+            const int n = (int)pc.size();
+            st->pieces.push_back(std::move(pc));
+            SoupPiece* tab = reinterpret_cast<SoupPiece*>(dst + tab_at);
+            if (hipMemcpyAsync(tab, st->pieces.back().data(), sizeof(SoupPiece) * (size_t)n, hipMemcpyHostToDevice,
+                               c->copy) != hipSuccess) {
+                (void)hipGetLastError();
+                unstage_soups(c, st);
+                return GPV_ERR_HIP;
+            }
+            launch_gather_soups(tab, n, reinterpret_cast<float*>(dst), kMappedGatherBlocks, c->copy);
+            if (hipGetLastError() != hipSuccess) {
+                unstage_soups(c, st);
+                return GPV_ERR_HIP;
+            }
         }
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ||
             hipEventRecord(e, c->copy) != hipSuccess) {

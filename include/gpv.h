@@ -270,7 +270,10 @@ typedef struct gpv_mesh_desc {
  * each grid holds exactly what gpv_voxelize of that mesh alone would hold
  * and works with every result call below (get_info, download, save_raw,
  * device_views).  The soups are copied into the batch, so the inputs may be
- * reused on return.  Per-mesh ms_stage/ms_total report the batch's stage
+ * reused on return: host soups of more than 2 MiB in total go in two runs,
+ * the second's upload under the first's kernels, and soups of up to 256 KiB
+ * in page-locked memory (gpv_host_alloc) are read across the bus by one
+ * gather kernel per run instead of a copy each.  Per-mesh ms_stage/ms_total report the batch's stage
  * times.  The batch's device memory returns to the context's pool when the
  * last of its grids is released.  One run holds at most 2^31-1 triangles,
  * 2^32-1 padded Level1 voxels (rows padded to 64), 2^27 xy columns, 2^27

@@ -351,26 +351,36 @@ STAGED_BATCHES = {
     "large": [(1, 900000), (2, 700000), (1, 500000), (1, 300000), (0, 40000)],
     # just above the 2 MiB staging threshold, the small meshes at the cut
     "small": [(0, 30000), (1, 40000), (2, 20000), (0, 900), (1, 500), (2, 3000)],
+    "small_mixed_memory": [(0, 30000), (1, 40000), (2, 20000), (0, 900), (1, 500), (2, 3000)],
 }
 
 
 @pytest.mark.parametrize("which", sorted(STAGED_BATCHES))
 def test_batch_staged_uploads_equal_resident(gv, ctx, which):
     """Host soups of a batch above 2 MiB go up in two runs, the second
-    uploaded under the first's kernels; the grids equal those of the same
-    batch from resident soups."""
+    uploaded under the first's kernels (soups of up to 256 KiB in
+    page-locked memory by one gather kernel reading across the bus, the rest
+    by a copy each); the grids equal those of the same batch from resident
+    soups."""
     meshes = []
     for i, (kind, target) in enumerate(STAGED_BATCHES[which]):
         v, f = gv.mixed(kind, target, 900 + i)
         meshes.append(mesh_entry(gv, v, f, 96 + 32 * (i % 2), 4 + 4 * (i % 2)))
-    if which == "small":
+    if which.startswith("small"):
         # an empty soup and a single triangle where the first run ends
         meshes.insert(1, dict(meshes[0], soup=meshes[0]["soup"][:0]))
         meshes.insert(2, dict(meshes[0], soup=meshes[0]["soup"][:1]))
-    total = sum(m["soup"].nbytes for m in meshes)
-    assert total > (2 << 20)
     pinned = [gv.pinned_copy(m["soup"]) for m in meshes]
     host = [dict(m, soup=p.array) for m, p in zip(meshes, pinned)]
+    if which == "small_mixed_memory":
+        # pageable soups between page-locked ones, and a page-locked soup
+        # that starts inside its allocation and ends at its end
+        host = [dict(m, soup=m["soup"]) if i % 2 else h for i, (m, h) in enumerate(zip(meshes, host))]
+        small = min((k for k, m in enumerate(meshes) if len(m["soup"]) > 100), key=lambda k: len(meshes[k]["soup"]))
+        meshes.append(dict(meshes[small], soup=meshes[small]["soup"][7:]))
+        host.append(dict(meshes[small], soup=pinned[small].array[7:]))
+    total = sum(m["soup"].nbytes for m in meshes)
+    assert total > (2 << 20)
     ptrs = [ctx.upload(m["soup"]) for m in meshes]
     try:
         say(f"staged batch: {len(meshes)} meshes, {total >> 20} MiB of soup")
