@@ -163,7 +163,7 @@ struct SoupPiece {
     uint64_t at;
     uint64_t n;
 };
-void launch_gather_soups(const SoupPiece* pc, int nm, float* dst, int blocks, hipStream_t s);
+void launch_gather_soups(const SoupPiece* pc, int nm, float* dst, int blocks, bool host, hipStream_t s);
 
 // GL-stencil-equivalent Level1 fill of one mesh (soup tris[T], grid g):
 // overwrites the mesh's Level1 inside bits ins[nz*ny*nxw].  Scratch: cnt[T],

@@ -1060,20 +1060,31 @@ void launch_l1_parity(const float4* rec, const uint32_t* hoff, const GridDev& g0
 // a batch's device soups gathered into the batch's soup in one launch (one
 // grid row per mesh, each row's blocks striding over the mesh's floats)
 // ---------------------------------------------------------------------------
+// kHost: the sources are page-locked host memory, read with system-scope
+// loads so that no line a cache kept from an earlier call is returned (the
+// caller may have rewritten the buffer since)
+template <bool kHost>
 __global__ __launch_bounds__(256) void k_gather_soups(const SoupPiece* __restrict__ pc, int m0,
                                                       float* __restrict__ dst)
 {
     const SoupPiece p = pc[m0 + blockIdx.y];
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += stride)
-        dst[p.at + i] = p.src[i];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += stride) {
+        if constexpr (kHost)
+            dst[p.at + i] = __hip_atomic_load(p.src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        else
+            dst[p.at + i] = p.src[i];
+    }
 }
 
-void launch_gather_soups(const SoupPiece* pc, int nm, float* dst, int blocks, hipStream_t s)
+void launch_gather_soups(const SoupPiece* pc, int nm, float* dst, int blocks, bool host, hipStream_t s)
 {
     for (int m0 = 0; m0 < nm; m0 += 65535) {
         const int rows = nm - m0 < 65535 ? nm - m0 : 65535;
-        k_gather_soups<<<dim3((unsigned)blocks, (unsigned)rows), 256, 0, s>>>(pc, m0, dst);
+        if (host)
+            k_gather_soups<true><<<dim3((unsigned)blocks, (unsigned)rows), 256, 0, s>>>(pc, m0, dst);
+        else
+            k_gather_soups<false><<<dim3((unsigned)blocks, (unsigned)rows), 256, 0, s>>>(pc, m0, dst);
     }
 }
 

@@ -759,7 +759,7 @@ int job_create(gpv_ctx* c, const gpv_mesh_desc* d, int nm, int on_device, Job** 
         if (hipMemcpyAsync(pc, hp.data(), sizeof(SoupPiece) * (size_t)nm, hipMemcpyHostToDevice, c->stream) !=
             hipSuccess)
             return bail(fail(GPV_ERR_HIP, "soup table upload failed"));
-        launch_gather_soups(pc, nm, s, 64, c->stream);
+        launch_gather_soups(pc, nm, s, 64, false, c->stream);
         if (hipGetLastError() != hipSuccess) return bail(fail(GPV_ERR_HIP, "soup gather failed"));
         J->tris = s;
         J->own_soup = s;
@@ -1707,7 +1707,7 @@ int stage_soups(gpv_ctx* c, const gpv_mesh_desc* d, int nm, const std::vector<st
                 unstage_soups(c, st);
                 return GPV_ERR_HIP;
             }
-            launch_gather_soups(tab, n, reinterpret_cast<float*>(dst), kMappedGatherBlocks, c->copy);
+            launch_gather_soups(tab, n, reinterpret_cast<float*>(dst), kMappedGatherBlocks, true, c->copy);
             if (hipGetLastError() != hipSuccess) {
                 unstage_soups(c, st);
                 return GPV_ERR_HIP;

@@ -395,3 +395,48 @@ def test_batch_staged_uploads_equal_resident(gv, ctx, which):
         for k in ("l1", "prefix", "bidx", "l2"):
             assert np.array_equal(ga[k], gb[k]), f"member {i}: {k}"
         check_info(f"member {i}", ia, ib)
+
+
+def test_batch_rewritten_pinned_soups(gv, ctx):
+    """Page-locked soups rewritten in place between two batch calls: the
+    second call reads the new contents (the gather of small soups must not
+    return lines cached from the first)."""
+    meshes = []
+    for i, (kind, target) in enumerate(STAGED_BATCHES["small"]):
+        v, f = gv.mixed(kind, target, 950 + i)
+        meshes.append(mesh_entry(gv, v, f, 64, 4))
+    pinned = [gv.pinned_copy(m["soup"]) for m in meshes]
+    host = [dict(m, soup=p.array) for m, p in zip(meshes, pinned)]
+
+    def resident(ms):
+        ptrs = [ctx.upload(m["soup"]) for m in ms]
+        try:
+            return batch_results(ctx, ms, device_ptrs=ptrs)
+        finally:
+            for q in ptrs:
+                ctx.free(q)
+
+    try:
+        say("rewritten pinned soups: first call")
+        first = batch_results(ctx, host)
+        want_first = resident(meshes)
+        # shrink every mesh about its box center, in the same buffers
+        shrunk = []
+        for m, p in zip(meshes, pinned):
+            c = (np.asarray(m["bmin"], np.float32) + np.asarray(m["bmax"], np.float32)) * np.float32(0.5)
+            t = m["soup"].reshape(-1, 3, 3)
+            new = (c + (t - c) * np.float32(0.75)).astype(np.float32).reshape(-1, 9)
+            p.array[...] = new
+            shrunk.append(dict(m, soup=new))
+        say("rewritten pinned soups: second call")
+        second = batch_results(ctx, host)
+        want_second = resident(shrunk)
+    finally:
+        for p in pinned:
+            p.close()
+    for name, got, want in (("first", first, want_first), ("second", second, want_second)):
+        for i, ((ga, ia), (gb, ib)) in enumerate(zip(got, want)):
+            for k in ("l1", "prefix", "bidx", "l2"):
+                assert np.array_equal(ga[k], gb[k]), f"{name} call, member {i}: {k}"
+            check_info(f"{name} call, member {i}", ia, ib)
+    assert any(not np.array_equal(a[0]["l1"], b[0]["l1"]) for a, b in zip(first, second))
