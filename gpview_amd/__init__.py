@@ -453,7 +453,7 @@ class Context:
         out = (_vp * n)()
         _check(lib().gpv_voxelize_batch(self.h, descs.ctypes.data_as(ctypes.POINTER(MeshDesc)), n,
                                         1 if device_ptrs is not None else 0, out), "gpv_voxelize_batch")
-        return [Grid(self, _vp(out[i])) for i in range(n)]
+        return [Grid(self, _vp(h)) for h in out]
 
     def create(self, soup, bmin, bmax, mms, voxel_count, voxel_count2=4, level2=True, initial_capacity=0,
                normals=False, l1_stencil=False):
@@ -495,6 +495,16 @@ def pinned_copy(a):
     return p
 
 
+def _data_ptr(a):
+    """Address of a contiguous array's data (through the buffer protocol,
+    which is cheaper than __array_interface__; that one for read-only and
+    empty arrays)."""
+    try:
+        return ctypes.addressof(ctypes.c_char.from_buffer(a))
+    except (TypeError, ValueError, BufferError):
+        return a.__array_interface__["data"][0]
+
+
 def mesh_descs(meshes, device_ptrs=None):
     """gpv_mesh_desc array of a sequence of mesh dicts, filled field by field
     (numpy mirror of the struct); returns (array, soups) -- keep the soups
@@ -504,12 +514,19 @@ def mesh_descs(meshes, device_ptrs=None):
         _DESC_DTYPE = _desc_dtype()
     n = len(meshes)
     a = np.zeros(n, _DESC_DTYPE)
-    soups = [np.ascontiguousarray(m["soup"], np.float32).reshape(-1, 9) for m in meshes]
+    soups = []
+    for m in meshes:
+        s = m["soup"]
+        if not (type(s) is np.ndarray and s.dtype == np.float32 and s.flags.c_contiguous):
+            s = np.ascontiguousarray(s, np.float32)
+        if s.size % 9:
+            raise ValueError(f"a soup of {s.size} floats is not a whole number of triangles")
+        soups.append(s)
     if device_ptrs is not None:
         a["tris"] = [int(p) for p in device_ptrs]
     else:
-        a["tris"] = [s.__array_interface__["data"][0] for s in soups]
-    a["ntris"] = [len(s) for s in soups]
+        a["tris"] = [_data_ptr(s) for s in soups]
+    a["ntris"] = [s.size // 9 for s in soups]
     a["bmin"] = [m["bmin"] for m in meshes]
     a["bmax"] = [m["bmax"] for m in meshes]
     a["max_model_size"] = [m["mms"] for m in meshes]

@@ -27,6 +27,34 @@ def test_exports_every_declared_symbol(gv):
     assert sorted(declared) == sorted(gv.EXPORTED)
 
 
+def test_batch_descriptors_match_struct(gv):
+    """The batch's numpy descriptor array holds, byte for byte, the
+    gpv_mesh_desc structs that the ctypes mirror builds for each mesh, also
+    for read-only, empty, float64 and strided soups and for device pointers."""
+    import ctypes
+    ro = np.arange(18, dtype=np.float32)
+    ro.setflags(write=False)
+    soups = [np.arange(27, dtype=np.float32).reshape(3, 9), ro, np.zeros((0, 9), np.float32),
+             np.arange(27, dtype=np.float64).reshape(3, 9), np.arange(36, dtype=np.float32).reshape(2, 18)[:, :9]]
+    extra = [{}, {"voxel_count2": 8, "level2": False}, {"normals": True, "initial_capacity": 1 << 40},
+             {"cuda_level2": True}, {"l1_stencil": True}]
+    meshes = [dict(soup=x, bmin=(0.25 * i, -1.0, 2.0), bmax=np.array([3.0, 4.0 + i, 5.0], np.float32), mms=6.5 + i,
+                   voxel_count=16 + i, **e) for i, (x, e) in enumerate(zip(soups, extra))]
+    for ptrs in (None, [4096 * (i + 1) for i in range(len(meshes))]):
+        a, keep = gv.mesh_descs(meshes, ptrs)
+        assert a.dtype.itemsize == ctypes.sizeof(gv.MeshDesc)
+        for i, m in enumerate(meshes):
+            d, soup = gv.mesh_desc(m, None if ptrs is None else ptrs[i])
+            if ptrs is None:
+                # the ctypes mirror converted its own copy; the batch keeps its soup
+                assert a["tris"][i] == keep[i].__array_interface__["data"][0]
+                assert np.array_equal(keep[i].reshape(-1, 9), soup)
+                d.tris = int(a["tris"][i])
+            assert a[i:i + 1].tobytes() == ctypes.string_at(ctypes.addressof(d), ctypes.sizeof(d)), i
+    with pytest.raises(ValueError):
+        gv.mesh_descs([dict(meshes[0], soup=np.zeros(10, np.float32))])
+
+
 def test_status_strings(gv):
     L = gv.lib()
     for s in range(9):
