@@ -1551,13 +1551,17 @@ __global__ __launch_bounds__(TS_NT) void k_ts_place(const uint4* __restrict__ bu
     }
 }
 
-// tile size 2^shift (>= TS_NT) so that at most 16384 tiles cover nkeys keys;
-// nkeys <= 2^27 keeps a tile's counters within 32 KiB of LDS
+// tile size 2^shift, 2^8 (TS_NT) to 2^13 keys: at most 8192 tiles below 2^26
+// keys, 16384 at 2^27.  The count and bucket passes hold a counter per tile
+// in every block's LDS (32 KiB at 8192 tiles: five blocks per CU, where
+// 16384 tiles allowed two), the place pass one per key of its tile (at most
+// 32 KiB).
 int ts_shift(uint64_t nkeys)
 {
     int lg = 0;
     while ((1ull << lg) < nkeys) ++lg;
-    return lg - 14 > 8 ? lg - 14 : 8;
+    const int sh = lg - 13;
+    return sh < 8 ? 8 : sh > 13 ? 13 : sh;
 }
 
 template <typename Src, typename Place>
