@@ -181,8 +181,9 @@ void launch_boundary_index(const uint64_t* bnd, const uint32_t* wprefix, const G
                            int64_t W, int32_t* bidx, hipStream_t s);
 // pairs: key = batch-wide boundary bit index, val = triangle
 // bin pairs -> CSR by boundary rank (boff[B + 1], list, Level2 box-axis
-// masks); B <= 2^27
-void launch_bin_pairs(const uint64_t* bits, const uint32_t* tri, uint64_t n, const uint64_t* bnd,
+// masks); B <= 2^27.  The pairs' voxel bits are replaced by their boundary
+// ranks (ranked: they hold ranks already)
+void launch_bin_pairs(uint64_t* bits, bool ranked, const uint32_t* tri, uint64_t n, const uint64_t* bnd,
                       const uint32_t* wprefix, uint32_t B, const TileSortTmp& tmp, int items_per_block,
                       uint32_t* boff, unsigned int* max_out, uint32_t* list, const float* tris,
                       const int32_t* bidx, const GridDev& g0, const Meshes& M, uint64_t* pmask, hipStream_t s);

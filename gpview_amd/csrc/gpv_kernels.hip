@@ -964,6 +964,8 @@ struct ColSrc {
     const uint64_t* key;
     const uint32_t* zmax;
     __device__ uint32_t sort_key(uint64_t i) const { return (uint32_t)(key[i] >> 32); }
+    __device__ uint32_t count_key(uint64_t i) const { return sort_key(i); }
+    __device__ void keep_key(uint64_t, uint32_t) const {}
     // (column, triangle, zmax)
     __device__ uint4 item(uint64_t i) const
     {
@@ -1418,7 +1420,10 @@ __device__ __forceinline__ void ts_range(uint64_t n, uint64_t& lo, uint64_t& hi)
 }
 
 // LDS histogram of the tile ids of items [lo, hi) of this block
-template <typename Src>
+// kCount: the count pass, which takes each item's key from count_key and
+// hands it to keep_key (a source may keep it for the later passes); the
+// bucket pass reads sort_key
+template <bool kCount, typename Src>
 __device__ __forceinline__ void ts_histogram(const Src& src, uint64_t lo, uint64_t hi, int shift, uint32_t* s_h)
 {
     for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)TS_U * TS_NT) {
@@ -1426,11 +1431,16 @@ __device__ __forceinline__ void ts_histogram(const Src& src, uint64_t lo, uint64
 #pragma unroll
         for (int u = 0; u < TS_U; ++u) {
             const uint64_t i = i0 + (uint64_t)u * TS_NT;
-            k[u] = src.sort_key(i < hi ? i : i0);
+            k[u] = kCount ? src.count_key(i < hi ? i : i0) : src.sort_key(i < hi ? i : i0);
         }
 #pragma unroll
-        for (int u = 0; u < TS_U; ++u)
-            if (i0 + (uint64_t)u * TS_NT < hi) atomicAdd(&s_h[k[u] >> shift], 1u);
+        for (int u = 0; u < TS_U; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * TS_NT;
+            if (i < hi) {
+                atomicAdd(&s_h[k[u] >> shift], 1u);
+                if constexpr (kCount) src.keep_key(i, k[u]);
+            }
+        }
     }
 }
 
@@ -1443,7 +1453,7 @@ __global__ __launch_bounds__(TS_NT) void k_ts_count(Src src, uint64_t n, int shi
     __syncthreads();
     uint64_t lo, hi;
     ts_range(n, lo, hi);
-    ts_histogram(src, lo, hi, shift, s_h);
+    ts_histogram<true>(src, lo, hi, shift, s_h);
     __syncthreads();
     for (int t = threadIdx.x; t < ntiles; t += TS_NT)
         if (s_h[t]) atomicAdd(&tile_total[t], s_h[t]);
@@ -1461,7 +1471,7 @@ __global__ __launch_bounds__(TS_NT) void k_ts_bucket(Src src, uint64_t n, int sh
     __syncthreads();
     uint64_t lo, hi;
     ts_range(n, lo, hi);
-    ts_histogram(src, lo, hi, shift, s_h);
+    ts_histogram<false>(src, lo, hi, shift, s_h);
     __syncthreads();
     for (int t = threadIdx.x; t < ntiles; t += TS_NT)
         if (s_h[t]) s_h[t] = tile_off[t] + atomicAdd(&tile_fill[t], s_h[t]);
@@ -1603,7 +1613,11 @@ size_t ts_tiles(uint64_t nkeys)
 // y, z; packed x | y << 16 | z << 32.
 template <bool kM>
 struct PairSrc {
-    const uint64_t* bits;
+    // per pair, its voxel's bit in the Level1 grid; the count pass writes the
+    // voxel's boundary rank over it (ranked: a repeated compaction, whose
+    // pairs hold ranks already), so the bucket pass reads the rank directly
+    uint64_t* bits;
+    bool ranked;
     const uint32_t* tri;
     const uint64_t* bnd;
     const uint32_t* wprefix;
@@ -1612,10 +1626,18 @@ struct PairSrc {
     GridDev g0;
     Meshes M;
     bool masks;
-    __device__ uint32_t sort_key(uint64_t i) const { return rank_of(bnd, wprefix, bits[i]); }
+    __device__ uint32_t count_key(uint64_t i) const
+    {
+        return ranked ? (uint32_t)bits[i] : rank_of(bnd, wprefix, bits[i]);
+    }
+    __device__ void keep_key(uint64_t i, uint32_t k) const
+    {
+        if (!ranked) bits[i] = k;
+    }
+    __device__ uint32_t sort_key(uint64_t i) const { return (uint32_t)bits[i]; }
     __device__ uint4 item(uint64_t i) const
     {
-        const uint32_t b = rank_of(bnd, wprefix, bits[i]);
+        const uint32_t b = (uint32_t)bits[i];
         const uint32_t t = tri[i];
         const int mesh = kM ? (int)M.bmesh[b] : 0;
         const GridDev& g = grid_of<kM>(g0, M, mesh);
@@ -1660,17 +1682,17 @@ void launch_bin_cols(const float* tris, const uint64_t* key, const uint32_t* zma
     ts_sort(ColSrc{key, zmax}, n, ncol, tmp, items_per_block, coff, nullptr, ColPlace{tris, rec}, s);
 }
 
-void launch_bin_pairs(const uint64_t* bits, const uint32_t* tri, uint64_t n, const uint64_t* bnd,
+void launch_bin_pairs(uint64_t* bits, bool ranked, const uint32_t* tri, uint64_t n, const uint64_t* bnd,
                       const uint32_t* wprefix, uint32_t B, const TileSortTmp& tmp, int items_per_block,
                       uint32_t* boff, unsigned int* max_out, uint32_t* list, const float* tris,
                       const int32_t* bidx, const GridDev& g0, const Meshes& M, uint64_t* pmask, hipStream_t s)
 {
     const PairPlace place{list, pmask};
     if (M.nm > 1)
-        ts_sort(PairSrc<true>{bits, tri, bnd, wprefix, tris, bidx, g0, M, pmask != nullptr}, n, B, tmp,
+        ts_sort(PairSrc<true>{bits, ranked, tri, bnd, wprefix, tris, bidx, g0, M, pmask != nullptr}, n, B, tmp,
                 items_per_block, boff, max_out, place, s);
     else
-        ts_sort(PairSrc<false>{bits, tri, bnd, wprefix, tris, bidx, g0, M, pmask != nullptr}, n, B, tmp,
+        ts_sort(PairSrc<false>{bits, ranked, tri, bnd, wprefix, tris, bidx, g0, M, pmask != nullptr}, n, B, tmp,
                 items_per_block, boff, max_out, place, s);
 }
 

@@ -275,9 +275,10 @@ struct Job {
     int64_t W = 0, ncol = 0;
     uint64_t *bnd = nullptr, *ins = nullptr;
     // append buffers
-    uint64_t* pair_key = nullptr;
+    uint64_t* pair_key = nullptr;     // per pair, its voxel's bit; its boundary rank once binned
     uint32_t* pair_tri = nullptr;
     uint64_t pair_cap = 0;
+    bool pairs_ranked = false;        // pair_key holds boundary ranks (a compaction has binned the pairs)
     uint64_t* col_key = nullptr;
     uint32_t* col_z = nullptr;
     uint64_t col_cap = 0;
@@ -889,6 +890,7 @@ int launch_sat_half(Job* J)
     launch_fill_u64(reinterpret_cast<uint64_t*>(&J->ctr->pairs), 0, 2, s);   // pairs, colents
     Append64 pa{&J->ctr->pairs, J->pair_key, J->pair_tri, J->pair_cap};
     Append64 co{&J->ctr->colents, J->col_key, J->col_z, J->col_cap};
+    J->pairs_ranked = false;   // the sweep appends voxel bits
     launch_sat_expand(J->tris, J->T, J->box0, J->box1, J->slab1, J->sat_off, J->g[0], J->view(), bbytes, pa, co,
                       expand_blocks(c), c->tune.expand_stage_tris, s);
     launch_pack_bytes(bbytes, J->W, J->bnd, s);
@@ -1138,9 +1140,10 @@ int launch_compact(Job* J)
     ts.scan = J->scan32;
     ts.bucket = ts_bucket;
     const int ipb = std::max(256, J->ctx->tune.sort_items_per_block);
-    launch_bin_pairs(J->pair_key, J->pair_tri, J->n_pairs, J->bnd, J->wprefix, (uint32_t)J->B, ts, ipb, J->boff,
-                     J->nm > 1 ? nullptr : &J->mctr[0].max_per_voxel, J->list, J->tris, J->bidx, J->g[0],
-                     J->view(), J->pmask, s);
+    launch_bin_pairs(J->pair_key, J->pairs_ranked, J->pair_tri, J->n_pairs, J->bnd, J->wprefix, (uint32_t)J->B, ts,
+                     ipb, J->boff, J->nm > 1 ? nullptr : &J->mctr[0].max_per_voxel, J->list, J->tris, J->bidx,
+                     J->g[0], J->view(), J->pmask, s);
+    J->pairs_ranked = true;
     if (J->nm > 1) launch_mesh_bins(J->boff, J->B, J->view(), s);
     if (!J->cols_side) {
         release(J, J->coff);
